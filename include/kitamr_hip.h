@@ -1,0 +1,194 @@
+/*
+ * kitamr_hip.h -- C ABI of the MI355X (gfx950) backend for KitAMR.jl's per-step
+ * phase-space update: slope-limited reconstruction, the upwind DVM/CAIDVM face flux
+ * (with interpolation across cells that carry different adapted velocity grids), the
+ * Newton-Cotes moment sums, the Shakhov g+ build and the implicit relaxation update.
+ *
+ * The entry points replace the three per-step calls of the reference solver loop
+ *     slope!(p4est,ka); flux!(p4est,ka); iterate!(p4est,ka)     src/Solver/Solver.jl:65-67
+ * plus the data hand-over a Julia `ccall` shim performs after every
+ * `amr_recover!` (src/Solver/AMR.jl:54-70) and before every host consumer of the state
+ * (`check!`, `save_result`, AMR criteria).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative KAH_E* code on failure; the
+ *     message is kept in the context (kah_last_error). Nothing throws or aborts.
+ *   - Host buffers belong to the caller and are copied in/out; calls return only when
+ *     the caller's buffers may be reused. Device buffers belong to the library.
+ *   - A context is bound to one thread and one device. There are no globals.
+ *   - All indices are 0-based. Array layouts follow the reference's Julia column-major
+ *     per-cell arrays, concatenated over cells in cell order:
+ *        df   of cell c : df  [off_c*NDF     + k*n_c + i]          (VsData.df[n,NDF])
+ *        sdf  of cell c : sdf [off_c*NDF*DIM + (d*NDF+k)*n_c + i]  (VsData.sdf[n,NDF,DIM])
+ *        vs_mid cell c  : mid [off_c*DIM     + d*n_c + i]          (VsData.midpoint[n,DIM])
+ *     with off_c = vs_offset[c], n_c = vs_offset[c+1]-off_c.
+ */
+#ifndef KITAMR_HIP_H
+#define KITAMR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enumerations ------------------------------------------------------------- */
+/* numerical flux, src/Abstract/Types.jl:85-91 (DVM, CAIDVM) */
+enum { KAH_FLUX_CAIDVM = 0, KAH_FLUX_DVM = 1 };
+/* time marching, src/Abstract/Types.jl:96-102 (CAIDVM_Marching, Euler) */
+enum { KAH_MARCH_CAIDVM = 0, KAH_MARCH_EULER = 1 };
+/* storage / arithmetic form of f on the device */
+enum { KAH_PREC_FP64 = 0,    /* f, sdf, flux in FP64 (parity form)                      */
+       KAH_PREC_FP32F = 1 }; /* f, sdf stored and advected in FP32, moments/w in FP64    */
+/* domain boundary conditions, src/Abstract/Types.jl:125-153 */
+enum { KAH_BC_MAXWELLIAN = 0, KAH_BC_SUPERSONIC_INFLOW = 1, KAH_BC_UNIFORM_OUTFLOW = 2,
+       KAH_BC_INTERPOLATED_OUTFLOW = 3 };
+/* neighbour states, src/Physical_space/Types.jl:1-22 / src/Mesh/Neighbor.jl:3-135:
+ *   0 boundary, 1 same size, 2^(DIM-1) half size (finer), -1 double size (coarser) */
+
+/* ---- error codes ---------------------------------------------------------------- */
+enum { KAH_OK = 0, KAH_EINVAL = -1, KAH_ENOMEM = -2, KAH_EDEVICE = -3, KAH_ESTATE = -4,
+       KAH_EUNSUPPORTED = -5 };
+
+/* ---- parameters ------------------------------------------------------------------ */
+typedef struct kah_params {
+  int32_t dim;          /* 2 or 3                                                      */
+  int32_t ndf;          /* 2 for DIM=2 (h,b: "2D2F"), 1 for DIM=3 ("3D1F")            */
+  int32_t flux_kind;    /* KAH_FLUX_*                Solver.flux                       */
+  int32_t marching;     /* KAH_MARCH_*               Solver.time_marching              */
+  int32_t precision;    /* KAH_PREC_*                                                  */
+  int32_t st_check_interval; /* Solver.ST_CHECK_INTERVAL (residual sampling)           */
+  int32_t device;       /* HIP device ordinal                                          */
+  int32_t reserved;
+  double K;             /* Gas.K      internal degrees of freedom                      */
+  double gamma;         /* Gas.γ                                                       */
+  double Pr;            /* Gas.Pr                                                      */
+  double mu_ref;        /* Gas.μᵣ as constructed by Julia (src/Gas/Types.jl:46)        */
+  double omega;         /* Gas.ω                                                       */
+} kah_params;
+
+/* ---- flattened topology (one call after every adapt / recover) ------------------ */
+typedef struct kah_topology {
+  /* Cell records. Records [0, n_cells) are the local PsData cells (fluid: bound_enc>=0,
+   * solid cells of an immersed boundary: bound_enc<0). Records [n_cells, n_cells+n_images)
+   * are periodic images (src/Boundary/Period.jl:1-18): shallow copies of cell
+   * image_src[k] that carry a shifted midpoint and share the source's velocity data,
+   * flux and state. */
+  int64_t n_cells;
+  int64_t n_images;
+  const int64_t* image_src;   /* [n_images]                                           */
+  const double*  cell_mid;    /* [(n_cells+n_images)*dim]  PsData.midpoint             */
+  const double*  cell_ds;     /* [(n_cells+n_images)*dim]  PsData.ds                   */
+  const int32_t* bound_enc;   /* [n_cells]                 PsData.bound_enc            */
+  const int32_t* ps_level;    /* [n_cells]  physical level, src/Flux/Slope.jl:825-829  */
+
+  /* Velocity grids (VsData), concatenated in cell order. When vs_uniform is 1 every cell
+   * carries the same grid (AMR_VS_MAXLEVEL == 0 builds identical grids for all cells,
+   * src/Velocity_space/Initialize.jl:9-37): vs_level/vs_mid/vs_weight then hold that
+   * one grid of n0 = vs_offset[1] cells and vs_offset[c] must equal c*n0. */
+  const int64_t* vs_offset;   /* [n_cells+1]                                           */
+  const int8_t*  vs_level;    /* [P]        VsData.level        ([n0] if vs_uniform)   */
+  const double*  vs_mid;      /* [P*dim]    VsData.midpoint     ([n0*dim])             */
+  const double*  vs_weight;   /* [P]        VsData.weight       ([n0])                 */
+  int32_t vs_uniform;
+  int32_t reserved0;
+
+  /* Neighbours of each cell per face id (2*dim faces: xmin,xmax,ymin,ymax,zmin,zmax),
+   * PsData.neighbor (src/Physical_space/Types.jl:15-22). nb_cell lists real cell ids
+   * (fine neighbours in p4est sub-face order). Solid cells may leave this empty. */
+  const int32_t* nb_state;    /* [n_cells*2*dim]                                       */
+  const int64_t* nb_offset;   /* [n_cells*2*dim+1]                                     */
+  const int64_t* nb_cell;     /* [nb_offset[n_cells*2*dim]]                            */
+
+  /* Flux pairs: the reference's face list (src/Solver/Initialize.jl:5-228) with every
+   * HangingFace/BackHangingFace expanded into its FluxData sub-faces, in face order.
+   * here is always a local fluid cell record; there is a cell or image record, or -1
+   * for a DomainFace. area is signed: rot*face_area (src/Flux/Flux.jl:64-111). */
+  int64_t n_pairs;
+  const int64_t* pair_here;   /* [n_pairs]                                             */
+  const int64_t* pair_there;  /* [n_pairs]                                             */
+  const int32_t* pair_dir;    /* [n_pairs]  0-based normal direction                   */
+  const double*  pair_rot;    /* [n_pairs]  (-1)^(faceid-1) of the here side           */
+  const double*  pair_area;   /* [n_pairs]                                             */
+  const double*  pair_mid;    /* [n_pairs*dim] face (sub-face) midpoint                */
+  const int32_t* pair_bc;     /* [n_pairs]  index into bc_* for domain faces, else -1  */
+
+  /* Domain boundary conditions (Domain.bc, src/Boundary/Types.jl:12-33). */
+  int32_t n_bc;
+  int32_t vs_maxlevel;        /* Solver.AMR_VS_MAXLEVEL (key width)                    */
+  const int32_t* bc_kind;     /* [n_bc]  KAH_BC_*                                      */
+  const double*  bc_prim;     /* [n_bc*(dim+2)]  prim of the boundary state            */
+
+  /* Velocity root grid (Configure.quadrature / vs_trees_num), used to key the cells of
+   * adapted grids (src/Velocity_space/Neighbor.jl:9-24). */
+  int32_t vs_trees_num[3];
+  int32_t reserved;
+  double  vs_lo[3];
+  double  vs_hi[3];
+} kah_topology;
+
+typedef struct kah_ctx kah_ctx;
+
+/* ---- lifecycle -------------------------------------------------------------------- */
+/* Replaces the KA/KInfo set-up of initialize(config), src/Solver/Initialize.jl:554-573 */
+int  kah_create(const kah_params* params, kah_ctx** out);
+/* Destroys a context and frees its device memory; accepts NULL. */
+void kah_destroy(kah_ctx* ctx);
+/* Message of the last failing call on ctx (never NULL). */
+const char* kah_last_error(const kah_ctx* ctx);
+
+/* Replaces the flattening point after initialize_faces!/update_faces!
+ * (src/Solver/Initialize.jl:5-228, src/Solver/AMR.jl:54-70). Drops any previous state. */
+int kah_set_topology(kah_ctx* ctx, const kah_topology* topo);
+
+/* Uploads the cell state: w and prim [n_cells*(dim+2)], df [P*ndf] (layout above).
+ * Entries of solid cells are copied as given (they are read as neighbours only).
+ * Zeroes the flux accumulators (PsData.flux, VsData.flux). */
+int kah_set_state(kah_ctx* ctx, const double* w, const double* prim, const double* df);
+
+/* Downloads any of w, prim, qf [n_cells*(dim+2) | n_cells*dim], sw [n_cells*(dim+2)*dim]
+ * (PsData.sw, the macro slope of src/Flux/Slope.jl:1022-1036), df [P*ndf] and
+ * sdf [P*ndf*dim]. Pass NULL to skip an array. */
+int kah_get_state(kah_ctx* ctx, double* w, double* prim, double* qf, double* sw,
+                  double* df, double* sdf);
+
+/* Downloads the flux accumulators PsData.flux [n_cells*(dim+2)] and VsData.flux [P*ndf]
+ * (NULL skips). They hold the face fluxes between kah_flux and kah_iterate and are zero
+ * after kah_iterate / kah_step, as after the reference's iterate!. */
+int kah_get_flux(kah_ctx* ctx, double* flux, double* vs_flux);
+
+/* ---- the per-step path --------------------------------------------------------------- */
+/* slope!(p4est,ka), src/Flux/Slope.jl:1047-1070: level-ordered velocity-space slopes
+ * with the transverse correction on coarse-neighbour faces. dt is status.Δt of the step
+ * (limit_Δt!, src/Theory/Iterate.jl:27-39); it is needed here because the per-face
+ * macroscopic half fluxes are formed together with the slopes. */
+int kah_slope(kah_ctx* ctx, double dt);
+/* flux!(p4est,ka), src/Flux/Flux.jl:458-487 (non-IB faces + domain faces). Completes the
+ * face fluxes into PsData.flux / VsData.flux. Must follow kah_slope with the same dt. */
+int kah_flux(kah_ctx* ctx, double dt);
+/* iterate!(p4est,ka), src/Theory/Iterate.jl:5-16 with the selected marching scheme.
+ * step is status.step before the increment; when step % ST_CHECK_INTERVAL == 0 the
+ * residual sums of src/Solver/Finalize.jl:5-11 are returned in sum_res/sum_avg
+ * [dim+2] (may be NULL); otherwise they are zeroed. */
+int kah_iterate(kah_ctx* ctx, double dt, int64_t step, double* sum_res, double* sum_avg);
+/* slope + flux + iterate of one step in one call (fused flux/iterate kernels). Returns
+ * without waiting for the device unless the residual is sampled this step; host buffers
+ * are not touched before that point. */
+int kah_step(kah_ctx* ctx, double dt, int64_t step, double* sum_res, double* sum_avg);
+
+/* Waits for all work queued by kah_step on the context's stream. */
+int kah_sync(kah_ctx* ctx);
+
+/* ---- introspection -------------------------------------------------------------------- */
+/* Number of phase-space updates per step: sum of vs_num over fluid cells. */
+int64_t kah_phase_cells(const kah_ctx* ctx);
+/* Device bytes held by the context (state + topology). */
+int64_t kah_device_bytes(const kah_ctx* ctx);
+/* Average duration in ms of each kernel class over the calls since the last reset,
+ * measured with HIP events on the library's stream: out[0] slope, out[1] flux+iterate. */
+int kah_kernel_times(kah_ctx* ctx, int reset, double* out_ms, int64_t* out_calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KITAMR_HIP_H */

@@ -1,0 +1,863 @@
+// kah_api.cpp -- the C ABI of include/kitamr_hip.h: context, topology derivation, state
+// transfer and the per-step launches.
+//
+// kah_set_topology turns the reference-shaped topology (neighbour lists with p4est states,
+// the face list as flux pairs) into the per-cell descriptors the kernels gather from:
+//   - a slope descriptor per (cell, direction), which resolves the reference's dispatch on
+//     the neighbour-state pair (src/Flux/Slope.jl:458-771) and, for cells finer than the
+//     coarsest level, the transverse-projection cases (src/Flux/Slope.jl:849-945);
+//   - the half fluxes each cell computes (the moments of its own upwind micro flux through a
+//     face, src/Flux/CAIDVM.jl:119) and the face sides each cell gathers in K2, in
+//     face-list order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kitamr_hip.h"
+#include "kah_internal.h"
+#include "kah_kernels.h"
+
+using namespace kah;
+
+namespace {
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct LevelBin {
+  int level;
+  int maxh;
+  int n;
+  DBuf cells;
+};
+
+struct EvPair {
+  hipEvent_t a, b;
+  int cls;
+};
+
+}  // namespace
+
+struct kah_ctx {
+  kah_params P{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool have_topo = false, have_state = false;
+  int D = 2, NDF = 2, NW = 4;
+  int64_t nc = 0, np = 0, npairs = 0;
+  bool uni = true;
+  int n0 = 0;
+  int lmax = 0;
+  int max_n = 0;
+  size_t lds_bytes = 0;
+  int64_t phase_cells = 0;
+  std::vector<int32_t> fluid;
+  std::vector<int32_t> benc;
+  std::vector<LevelBin*> bins;
+  DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey;
+  DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
+  DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw;
+  int cur = 0;
+  int64_t dev_bytes = 0;
+  bool timing = false;
+  std::vector<EvPair> events;
+  double t_sum[2] = {0, 0};
+  int64_t t_cnt[2] = {0, 0};
+
+  ~kah_ctx() {
+    for (auto* b : bins) delete b;
+    for (auto& e : events) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return KAH_EDEVICE;
+  }
+  int alloc(DBuf& b, size_t bytes, const char* what) {
+    b.release();
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+      b.p = nullptr;
+      err = std::string("hipMalloc(") + what + ", " + std::to_string(bytes) +
+            " B): " + hipGetErrorString(e);
+      return e == hipErrorOutOfMemory ? KAH_ENOMEM : KAH_EDEVICE;
+    }
+    b.bytes = bytes;
+    dev_bytes += int64_t(bytes);
+    return 0;
+  }
+  template <class T>
+  int upload(DBuf& b, const T* src, size_t count, const char* what) {
+    int rc = alloc(b, count * sizeof(T), what);
+    if (rc) return rc;
+    if (count == 0) return 0;
+    return hip(hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice), what);
+  }
+  int zero(DBuf& b) {
+    if (!b.p) return 0;
+    return hip(hipMemsetAsync(b.p, 0, b.bytes, stream), "hipMemsetAsync");
+  }
+};
+
+namespace {
+
+#define KAH_TRY(x)            \
+  do {                        \
+    int _rc = (x);            \
+    if (_rc) return _rc;      \
+  } while (0)
+
+// Key of a velocity cell (src/Velocity_space/Neighbor.jl:61-124): root lattice index times
+// 2^(D*Lmax) plus the Morton code of the cell's lower corner inside its root.
+uint64_t cell_key(int D, int L, int lmax, const double* mid, const double* lo,
+                  const double* root_ds, const int32_t* ntree) {
+  uint64_t root_lin = 0, stride = 1, morton = 0;
+  for (int d = 0; d < D; ++d) {
+    double hf = root_ds[d] / std::ldexp(1.0, lmax);
+    double cell = hf * double(int64_t(1) << (lmax - L));
+    int64_t g = std::llround((mid[d] - 0.5 * cell - lo[d]) / hf);
+    root_lin += uint64_t(g >> lmax) * stride;
+    stride *= uint64_t(ntree[d]);
+    int64_t q = g & ((int64_t(1) << lmax) - 1);
+    for (int b = 0; b < lmax; ++b) morton |= uint64_t((q >> b) & 1) << (b * D + d);
+  }
+  return (root_lin << (D * lmax)) | morton;
+}
+
+struct Builder {
+  kah_ctx& C;
+  const kah_topology& T;
+  int D, NW;
+  int64_t nc, nrec;
+  std::vector<SlopeDesc> sl;
+  std::vector<SlopeNb> slnb;
+  std::vector<std::vector<HalfDesc>> hal;
+  std::vector<std::vector<EntryDesc>> ent;
+
+  Builder(kah_ctx& c, const kah_topology& t) : C(c), T(t), D(c.D), NW(c.NW) {
+    nc = t.n_cells;
+    nrec = t.n_cells + t.n_images;
+  }
+  int64_t src(int64_t rec) const { return rec < nc ? rec : T.image_src[rec - nc]; }
+  const double* mid(int64_t rec) const { return &T.cell_mid[rec * D]; }
+  const double* dsr(int64_t rec) const { return &T.cell_ds[rec * D]; }
+  int32_t benc(int64_t rec) const { return T.bound_enc[src(rec)]; }
+  int state(int64_t c, int f) const { return T.nb_state[c * 2 * D + f]; }
+  int64_t nbn(int64_t c, int f) const {
+    return T.nb_offset[c * 2 * D + f + 1] - T.nb_offset[c * 2 * D + f];
+  }
+  int64_t nb(int64_t c, int f, int64_t j) const { return T.nb_cell[T.nb_offset[c * 2 * D + f] + j]; }
+
+  void add_side(SlopeDesc& s, int side, int64_t c, int f, int dir, bool proj) {
+    s.nb_beg[side] = int32_t(slnb.size());
+    s.nb_cnt[side] = int32_t(nbn(c, f));
+    for (int64_t j = 0; j < nbn(c, f); ++j) {
+      SlopeNb e{};
+      e.cell = int32_t(nb(c, f, j));
+      e.proj = proj ? 1 : 0;
+      for (int t = 0; t < 3; ++t) e.dm[t] = 0.0;
+      if (proj)
+        for (int t = 0; t < D; ++t) e.dm[t] = (t == dir) ? 0.0 : (mid(c)[t] - mid(e.cell)[t]);
+      slnb.push_back(e);
+    }
+  }
+  // _ps_has_transverse_offset, src/Flux/Slope.jl:177-199
+  bool offset(int64_t c, int f, int dir) const {
+    int64_t n = nbn(c, f);
+    if (n == 0) return false;
+    for (int t = 0; t < D; ++t) {
+      if (t == dir) continue;
+      double avg = 0.0;
+      for (int64_t j = 0; j < n; ++j) avg += mid(nb(c, f, j))[t];
+      avg /= double(n);
+      if (avg != mid(c)[t]) return true;
+    }
+    return false;
+  }
+  void inner(SlopeDesc& s, int64_t c, int dir, double dsL, double dsR, bool pL, bool pR) {
+    s.mode = SL_INNER;
+    s.ds[0] = dsL;
+    s.ds[1] = dsR;
+    add_side(s, 0, c, 2 * dir, dir, pL);
+    add_side(s, 1, c, 2 * dir + 1, dir, pR);
+  }
+  void bound(SlopeDesc& s, int64_t c, int f, int dir, double ds, bool proj) {
+    s.mode = SL_BOUND;
+    s.ds[0] = ds;
+    s.ds[1] = 0.0;
+    add_side(s, 0, c, f, dir, proj);
+    s.nb_beg[1] = 0;
+    s.nb_cnt[1] = 0;
+  }
+
+  // update_slope!(Val(sL), Val(sR), ...), src/Flux/Slope.jl:458-771
+  int plain(int64_t c, int dir, SlopeDesc& s) {
+    int fL = 2 * dir, fR = 2 * dir + 1;
+    int sL = state(c, fL), sR = state(c, fR);
+    double ds = dsr(c)[dir], xm = mid(c)[dir];
+    int F = 1 << (D - 1);
+    bool FL = sL == F, FR = sR == F;
+    if (sL == 1 && sR == 1) {
+      bool Ls = benc(nb(c, fL, 0)) < 0, Rs = benc(nb(c, fR, 0)) < 0;
+      if (Ls && Rs) s.mode = SL_ZERO;
+      else if (Ls) bound(s, c, fR, dir, xm - mid(nb(c, fR, 0))[dir], false);
+      else if (Rs) bound(s, c, fL, dir, xm - mid(nb(c, fL, 0))[dir], false);
+      else inner(s, c, dir, ds, -ds, false, false);
+    } else if (sL == 1 && FR) inner(s, c, dir, ds, -0.75 * ds, false, false);
+    else if (FL && sR == 1) inner(s, c, dir, 0.75 * ds, -ds, false, false);
+    else if (FL && FR) inner(s, c, dir, 0.75 * ds, -0.75 * ds, false, false);
+    else if (sL == -1 && sR == -1) inner(s, c, dir, 1.5 * ds, -1.5 * ds, false, false);
+    else if (sL == 1 && sR == -1) inner(s, c, dir, ds, -1.5 * ds, false, false);
+    else if (sL == -1 && sR == 1) inner(s, c, dir, 1.5 * ds, -ds, false, false);
+    else if (FL && sR == -1) inner(s, c, dir, 0.75 * ds, -1.5 * ds, false, false);
+    else if (sL == -1 && FR) inner(s, c, dir, 1.5 * ds, -0.75 * ds, false, false);
+    else if (sL == 0 && sR != 0) bound(s, c, fR, dir, xm - mid(nb(c, fR, 0))[dir], false);
+    else if (sR == 0 && sL != 0) bound(s, c, fL, dir, xm - mid(nb(c, fL, 0))[dir], false);
+    else
+      return C.fail(KAH_EINVAL, "update_slope!: no method for neighbour states (" +
+                                    std::to_string(sL) + "," + std::to_string(sR) +
+                                    ") of cell " + std::to_string(c));
+    return 0;
+  }
+  // per-direction outcome of update_slope_transverse_level!, src/Flux/Slope.jl:849-945
+  int transverse(int64_t c, int dir, SlopeDesc& s) {
+    int fL = 2 * dir, fR = 2 * dir + 1;
+    int sL = state(c, fL), sR = state(c, fR);
+    double xm = mid(c)[dir];
+    if (sL != 0 && sR != 0) {
+      int64_t L0 = nb(c, fL, 0), R0 = nb(c, fR, 0);
+      double dsL = xm - mid(L0)[dir], dsR = xm - mid(R0)[dir];
+      if (benc(L0) < 0) bound(s, c, fR, dir, dsR, true);
+      else if (benc(R0) < 0) bound(s, c, fL, dir, dsL, true);
+      else inner(s, c, dir, dsL, dsR, offset(c, fL, dir), offset(c, fR, dir));
+    } else if (sR == 0 && sL == -1) {
+      int64_t L0 = nb(c, fL, 0);
+      if (benc(L0) < 0) s.mode = SL_SKIP;
+      else bound(s, c, fL, dir, xm - mid(L0)[dir], true);
+    } else {  // sL == 0 && sR == -1
+      int64_t R0 = nb(c, fR, 0);
+      if (benc(R0) < 0) s.mode = SL_SKIP;
+      else bound(s, c, fR, dir, xm - mid(R0)[dir], true);
+    }
+    return 0;
+  }
+
+  HalfDesc half(int32_t slot, int32_t data, int8_t mode, int8_t sel, int dir, double rot,
+                int64_t rec, const double* fm, int32_t bc) {
+    HalfDesc h{};
+    h.slot = slot;
+    h.data_cell = data;
+    h.mode = mode;
+    h.sel = sel;
+    h.dir = int8_t(dir);
+    h.bc = bc;
+    h.rot = rot;
+    for (int t = 0; t < D; ++t) {
+      h.xrec[t] = mid(rec)[t];
+      h.dsrec[t] = dsr(rec)[t];
+      h.fmid[t] = fm[t];
+    }
+    return h;
+  }
+  EntryDesc entry(int32_t pair, int8_t side, int8_t om, int8_t tm, int32_t oth, int dir,
+                  double rot, double area, const double* fm, int64_t own_rec, int64_t oth_rec,
+                  int32_t bc) {
+    EntryDesc e{};
+    e.pair = pair;
+    e.side = side;
+    e.own_mode = om;
+    e.oth_mode = tm;
+    e.oth_cell = oth;
+    e.dir = int8_t(dir);
+    e.bc = bc;
+    e.rot = rot;
+    e.area = area;
+    for (int t = 0; t < D; ++t) {
+      e.fmid[t] = fm[t];
+      e.xown[t] = mid(own_rec)[t];
+      e.dsown[t] = dsr(own_rec)[t];
+      e.xoth[t] = oth_rec >= 0 ? mid(oth_rec)[t] : 0.0;
+      e.dsoth[t] = oth_rec >= 0 ? dsr(oth_rec)[t] : 0.0;
+    }
+    return e;
+  }
+
+  int build() {
+    const bool caidvm = C.P.flux_kind == KAH_FLUX_CAIDVM;
+    sl.assign(size_t(nc * D), SlopeDesc{});
+    hal.assign(size_t(nc), {});
+    ent.assign(size_t(nc), {});
+    int Lmin = 1 << 30;
+    for (int64_t c = 0; c < nc; ++c)
+      if (T.bound_enc[c] >= 0) Lmin = std::min(Lmin, int(T.ps_level[c]));
+    for (int64_t c = 0; c < nc; ++c) {
+      if (T.bound_enc[c] < 0) continue;
+      bool fine = T.ps_level[c] > Lmin;
+      for (int d = 0; d < D; ++d) {
+        SlopeDesc& s = sl[c * D + d];
+        s.mode = SL_ZERO;
+        int sL = state(c, 2 * d), sR = state(c, 2 * d + 1);
+        if (fine && D > 1 && (sL == -1 || sR == -1)) KAH_TRY(transverse(c, d, s));
+        else KAH_TRY(plain(c, d, s));
+      }
+    }
+    for (int64_t p = 0; p < T.n_pairs; ++p) {
+      int64_t h = T.pair_here[p], trec = T.pair_there[p];
+      if (h < 0 || h >= nc || T.bound_enc[h] < 0)
+        return C.fail(KAH_EINVAL, "pair " + std::to_string(p) + ": here is not a local fluid cell");
+      int dir = T.pair_dir[p];
+      double rot = T.pair_rot[p], area = T.pair_area[p];
+      const double* fm = &T.pair_mid[p * D];
+      int32_t s0 = int32_t(2 * p), s1 = int32_t(2 * p + 1);
+      if (trec < 0) {
+        int32_t bc = T.pair_bc[p];
+        if (bc < 0 || bc >= T.n_bc) return C.fail(KAH_EINVAL, "domain pair without a boundary condition");
+        int kind = T.bc_kind[bc];
+        int8_t m0, m1, mo;
+        switch (kind) {
+          case KAH_BC_UNIFORM_OUTFLOW: m0 = MI_PLAIN; m1 = MI_PLAIN; mo = MI_SELF_PLAIN; break;
+          case KAH_BC_SUPERSONIC_INFLOW: m0 = MI_UNLIMITED; m1 = MI_MAXWELL; mo = MI_MAXWELL; break;
+          case KAH_BC_MAXWELLIAN:
+            if (!caidvm)
+              return C.fail(KAH_EUNSUPPORTED,
+                            "UndefVarError: `here_weight` not defined (DVM Maxwellian domain "
+                            "flux, src/Flux/DVM.jl:12)");
+            m0 = MI_UNLIMITED; m1 = MI_WALL; mo = MI_WALL; break;
+          case KAH_BC_INTERPOLATED_OUTFLOW:
+            if (D != 2)
+              return C.fail(KAH_EUNSUPPORTED, "InterpolatedOutflow is defined for DIM=2 only");
+            m0 = MI_UNLIMITED; m1 = MI_INTERP; mo = MI_INTERP; break;
+          default:
+            return C.fail(KAH_EINVAL, "unknown boundary condition kind " + std::to_string(kind));
+        }
+        if (caidvm) {
+          hal[h].push_back(half(s0, int32_t(h), m0, 0, dir, rot, h, fm, bc));
+          hal[h].push_back(half(s1, int32_t(h), m1, 1, dir, rot, h, fm, bc));
+        }
+        ent[h].push_back(entry(int32_t(p), 0, m0, mo, -1, dir, rot, area, fm, h, -1, bc));
+      } else {
+        if (trec >= nrec) return C.fail(KAH_EINVAL, "pair there out of range");
+        int64_t t = src(trec);
+        bool solid = T.bound_enc[t] < 0;
+        if (solid && !C.uni)
+          return C.fail(KAH_EUNSUPPORTED,
+                        "faces with solid cells need a shared velocity grid in this version");
+        int8_t mh = solid ? MI_UNLIMITED : MI_LIMITED;
+        int8_t mt = solid ? (caidvm ? MI_PLAIN : MI_UNLIMITED) : MI_LIMITED;
+        if (caidvm) {
+          hal[h].push_back(half(s0, int32_t(h), mh, 0, dir, rot, h, fm, -1));
+          if (solid) hal[h].push_back(half(s1, int32_t(t), MI_PLAIN, 1, dir, rot, trec, fm, -1));
+          else hal[t].push_back(half(s1, int32_t(t), MI_LIMITED, 1, dir, rot, trec, fm, -1));
+        }
+        ent[h].push_back(entry(int32_t(p), 0, mh, mt, int32_t(t), dir, rot, area, fm, h, trec, -1));
+        if (!solid)
+          ent[t].push_back(entry(int32_t(p), 1, MI_LIMITED, MI_LIMITED, int32_t(h), dir, rot, area,
+                                 fm, trec, h, -1));
+      }
+    }
+    return 0;
+  }
+};
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+int kah_create(const kah_params* params, kah_ctx** out) {
+  if (!out) return KAH_EINVAL;
+  *out = nullptr;
+  if (!params) return KAH_EINVAL;
+  if (!((params->dim == 2 && params->ndf == 2) || (params->dim == 3 && params->ndf == 1)))
+    return KAH_EUNSUPPORTED;
+  if (params->flux_kind != KAH_FLUX_CAIDVM && params->flux_kind != KAH_FLUX_DVM)
+    return KAH_EINVAL;
+  if (params->marching != KAH_MARCH_CAIDVM && params->marching != KAH_MARCH_EULER)
+    return KAH_EINVAL;
+  if (params->precision != KAH_PREC_FP64) return KAH_EUNSUPPORTED;
+  kah_ctx* C = new kah_ctx();
+  C->P = *params;
+  if (C->P.st_check_interval <= 0) C->P.st_check_interval = 100;
+  C->D = params->dim;
+  C->NDF = params->ndf;
+  C->NW = params->dim + 2;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    delete C;
+    return KAH_EDEVICE;
+  }
+  if (params->device < 0 || params->device >= ndev || hipSetDevice(params->device) != hipSuccess ||
+      hipStreamCreate(&C->stream) != hipSuccess) {
+    delete C;
+    return KAH_EDEVICE;
+  }
+  *out = C;
+  return KAH_OK;
+}
+
+void kah_destroy(kah_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->P.device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+const char* kah_last_error(const kah_ctx* ctx) {
+  if (!ctx) return "null kah_ctx";
+  return ctx->err.c_str();
+}
+
+int kah_set_topology(kah_ctx* C, const kah_topology* T) {
+  if (!C) return KAH_EINVAL;
+  if (!T) return C->fail(KAH_EINVAL, "null topology");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  C->have_topo = C->have_state = false;
+  const int D = C->D;
+  const int64_t nc = T->n_cells;
+  if (nc <= 0) return C->fail(KAH_EINVAL, "n_cells must be positive");
+  if (nc >= (int64_t(1) << 31)) return C->fail(KAH_EUNSUPPORTED, "more than 2^31 cells");
+  if (!T->cell_mid || !T->cell_ds || !T->bound_enc || !T->ps_level || !T->vs_offset ||
+      !T->vs_level || !T->vs_mid || !T->vs_weight || !T->nb_state || !T->nb_offset ||
+      (T->n_images > 0 && !T->image_src) ||
+      (T->n_pairs > 0 && (!T->pair_here || !T->pair_there || !T->pair_dir || !T->pair_rot ||
+                          !T->pair_area || !T->pair_mid || !T->pair_bc)))
+    return C->fail(KAH_EINVAL, "null array in topology");
+  for (int64_t c = 0; c < nc; ++c)
+    if (T->vs_offset[c + 1] <= T->vs_offset[c])
+      return C->fail(KAH_EINVAL, "empty or unordered velocity grid of cell " + std::to_string(c));
+  C->nc = nc;
+  C->np = T->vs_offset[nc];
+  C->npairs = T->n_pairs;
+  C->uni = T->vs_uniform != 0;
+  C->n0 = int(T->vs_offset[1] - T->vs_offset[0]);
+  C->lmax = C->uni ? 0 : T->vs_maxlevel;
+  C->max_n = 0;
+  for (int64_t c = 0; c < nc; ++c) {
+    int64_t n = T->vs_offset[c + 1] - T->vs_offset[c];
+    if (n >= (int64_t(1) << 31)) return C->fail(KAH_EUNSUPPORTED, "velocity grid too large");
+    C->max_n = std::max<int>(C->max_n, int(n));
+    if (C->uni && T->vs_offset[c] != c * int64_t(C->n0))
+      return C->fail(KAH_EINVAL, "vs_uniform requires vs_offset[c] == c*n0");
+  }
+  for (int64_t k = 0; k < T->n_images; ++k)
+    if (T->image_src[k] < 0 || T->image_src[k] >= nc)
+      return C->fail(KAH_EINVAL, "image source out of range");
+  const int64_t nb_total = T->nb_offset[nc * 2 * D];
+  for (int64_t x = 0; x < nb_total; ++x)
+    if (T->nb_cell[x] < 0 || T->nb_cell[x] >= nc)
+      return C->fail(KAH_EINVAL, "neighbour id out of range");
+  C->benc.assign(T->bound_enc, T->bound_enc + nc);
+
+  // velocity grids: upload, and keys for adapted grids
+  const int64_t ng = C->uni ? C->n0 : C->np;
+  std::vector<uint32_t> keys;
+  if (!C->uni) {
+    int lmax = T->vs_maxlevel;
+    double root_ds[3];
+    int64_t roots = 1;
+    for (int d = 0; d < D; ++d) {
+      if (T->vs_trees_num[d] <= 0 || !(T->vs_hi[d] > T->vs_lo[d]))
+        return C->fail(KAH_EINVAL, "velocity root grid (vs_trees_num / vs_lo / vs_hi) not set");
+      root_ds[d] = 2.0 * ((T->vs_hi[d] - T->vs_lo[d]) / (2.0 * T->vs_trees_num[d]));
+      roots *= T->vs_trees_num[d];
+    }
+    int rootbits = 0;
+    while ((int64_t(1) << rootbits) < roots) ++rootbits;
+    if (rootbits + D * lmax > 32)
+      return C->fail(KAH_EUNSUPPORTED, "velocity cell keys exceed 32 bits");
+    keys.resize(size_t(ng));
+    for (int64_t c = 0; c < nc; ++c) {
+      int64_t off = T->vs_offset[c], n = T->vs_offset[c + 1] - off;
+      for (int64_t i = 0; i < n; ++i) {
+        double m[3];
+        for (int d = 0; d < D; ++d) m[d] = T->vs_mid[off * D + d * n + i];
+        int L = T->vs_level[off + i];
+        if (L < 0 || L > lmax) return C->fail(KAH_EINVAL, "velocity level out of range");
+        uint64_t k = cell_key(D, L, lmax, m, T->vs_lo, root_ds, T->vs_trees_num);
+        keys[size_t(off + i)] = uint32_t(k);
+        if (i > 0 && keys[size_t(off + i)] <= keys[size_t(off + i - 1)])
+          return C->fail(KAH_EINVAL, "velocity grid of cell " + std::to_string(c) +
+                                         " is not in key order");
+      }
+    }
+  }
+  C->dev_bytes = 0;
+  KAH_TRY(C->upload(C->d_voff, T->vs_offset, size_t(nc + 1), "vs_offset"));
+  KAH_TRY(C->upload(C->d_gu, T->vs_mid, size_t(ng * D), "vs_mid"));
+  KAH_TRY(C->upload(C->d_gw, T->vs_weight, size_t(ng), "vs_weight"));
+  KAH_TRY(C->upload(C->d_glev, T->vs_level, size_t(ng), "vs_level"));
+  if (!C->uni) KAH_TRY(C->upload(C->d_gkey, keys.data(), keys.size(), "vs_key"));
+  else C->d_gkey.release();
+
+  // descriptors
+  Builder B(*C, *T);
+  KAH_TRY(B.build());
+  std::vector<int32_t> hoff(size_t(nc + 1), 0), eoff(size_t(nc + 1), 0);
+  std::vector<HalfDesc> halves;
+  std::vector<EntryDesc> ents;
+  for (int64_t c = 0; c < nc; ++c) {
+    hoff[c] = int32_t(halves.size());
+    eoff[c] = int32_t(ents.size());
+    if (B.hal[c].size() > 24)
+      return C->fail(KAH_EUNSUPPORTED, "cell " + std::to_string(c) + " owns more than 24 half fluxes");
+    halves.insert(halves.end(), B.hal[c].begin(), B.hal[c].end());
+    ents.insert(ents.end(), B.ent[c].begin(), B.ent[c].end());
+  }
+  hoff[nc] = int32_t(halves.size());
+  eoff[nc] = int32_t(ents.size());
+  KAH_TRY(C->upload(C->d_sl, B.sl.data(), B.sl.size(), "slope descriptors"));
+  KAH_TRY(C->upload(C->d_slnb, B.slnb.data(), B.slnb.size(), "slope neighbours"));
+  KAH_TRY(C->upload(C->d_hoff, hoff.data(), hoff.size(), "half offsets"));
+  KAH_TRY(C->upload(C->d_halves, halves.data(), halves.size(), "half descriptors"));
+  KAH_TRY(C->upload(C->d_eoff, eoff.data(), eoff.size(), "entry offsets"));
+  KAH_TRY(C->upload(C->d_ents, ents.data(), ents.size(), "entry descriptors"));
+  KAH_TRY(C->upload(C->d_recds, T->cell_ds, size_t(nc * D), "cell ds"));
+  std::vector<double> bc(size_t(std::max(T->n_bc, 1) * C->NW), 0.0);
+  if (T->n_bc > 0) std::memcpy(bc.data(), T->bc_prim, sizeof(double) * T->n_bc * C->NW);
+  KAH_TRY(C->upload(C->d_bc, bc.data(), bc.size(), "bc prim"));
+
+  // fluid cells, per level and per half-count bin
+  C->fluid.clear();
+  C->phase_cells = 0;
+  for (auto* b : C->bins) delete b;
+  C->bins.clear();
+  int Lmin = 1 << 30, Lmax = -1;
+  for (int64_t c = 0; c < nc; ++c) {
+    if (T->bound_enc[c] < 0) continue;
+    C->fluid.push_back(int32_t(c));
+    C->phase_cells += T->vs_offset[c + 1] - T->vs_offset[c];
+    Lmin = std::min(Lmin, int(T->ps_level[c]));
+    Lmax = std::max(Lmax, int(T->ps_level[c]));
+  }
+  for (int L = Lmin; L <= Lmax; ++L) {
+    const int caps[3] = {8, 16, 24};
+    std::vector<int32_t> lists[3];
+    for (int32_t c : C->fluid) {
+      if (T->ps_level[c] != L) continue;
+      int nh = hoff[c + 1] - hoff[c];
+      int b = nh <= 8 ? 0 : (nh <= 16 ? 1 : 2);
+      lists[b].push_back(c);
+    }
+    for (int b = 0; b < 3; ++b) {
+      if (lists[b].empty()) continue;
+      LevelBin* lb = new LevelBin();
+      lb->level = L;
+      lb->maxh = caps[b];
+      lb->n = int(lists[b].size());
+      C->bins.push_back(lb);
+      KAH_TRY(C->upload(lb->cells, lists[b].data(), lists[b].size(), "level cells"));
+    }
+  }
+  KAH_TRY(C->upload(C->d_fluid, C->fluid.data(), C->fluid.size(), "fluid cells"));
+
+  // state buffers
+  const int64_t NDF = C->NDF, NW = C->NW;
+  KAH_TRY(C->alloc(C->d_df[0], sizeof(double) * C->np * NDF, "df"));
+  KAH_TRY(C->alloc(C->d_df[1], sizeof(double) * C->np * NDF, "df"));
+  KAH_TRY(C->alloc(C->d_sdf, sizeof(double) * C->np * NDF * D, "sdf"));
+  KAH_TRY(C->alloc(C->d_psflux, sizeof(double) * nc * NW, "flux"));
+  KAH_TRY(C->alloc(C->d_w, sizeof(double) * nc * NW, "w"));
+  KAH_TRY(C->alloc(C->d_prim, sizeof(double) * nc * NW, "prim"));
+  KAH_TRY(C->alloc(C->d_qf, sizeof(double) * nc * D, "qf"));
+  KAH_TRY(C->alloc(C->d_half, sizeof(double) * std::max<int64_t>(C->npairs, 1) * 2 * NW, "half fluxes"));
+  KAH_TRY(C->alloc(C->d_rho, sizeof(double) * std::max<int64_t>(C->npairs, 1), "wall density"));
+  KAH_TRY(C->alloc(C->d_res, sizeof(double) * nc * 2 * NW, "residual"));
+  C->d_vflux.release();
+  C->lds_bytes = sizeof(double) * size_t(NDF + 1) * size_t(C->max_n);
+  if (C->lds_bytes > 64 * 1024) {
+    C->lds_bytes = 0;
+    KAH_TRY(C->alloc(C->d_scratch, sizeof(double) * C->np * (NDF + 1), "scratch"));
+  } else {
+    C->d_scratch.release();
+  }
+  KAH_TRY(C->zero(C->d_sdf));
+  KAH_TRY(C->zero(C->d_psflux));
+  KAH_TRY(C->zero(C->d_half));
+  KAH_TRY(C->zero(C->d_rho));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "topology upload"));
+  C->cur = 0;
+  C->have_topo = true;
+  return KAH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+DevArgs make_args(kah_ctx* C, double dt) {
+  DevArgs A{};
+  A.uni_n = C->n0;
+  A.lmax = C->lmax;
+  A.voff = C->d_voff.as<int64_t>();
+  A.gu = C->d_gu.as<double>();
+  A.gw = C->d_gw.as<double>();
+  A.glev = C->d_glev.as<int8_t>();
+  A.gkey = C->d_gkey.as<uint32_t>();
+  A.df = C->d_df[C->cur].as<double>();
+  A.df_out = C->d_df[1 - C->cur].as<double>();
+  A.sdf = C->d_sdf.as<double>();
+  A.vflux = C->d_vflux.as<double>();
+  A.psflux = C->d_psflux.as<double>();
+  A.w = C->d_w.as<double>();
+  A.prim = C->d_prim.as<double>();
+  A.qf = C->d_qf.as<double>();
+  A.half_fw = C->d_half.as<double>();
+  A.rho_w = C->d_rho.as<double>();
+  A.res = C->d_res.as<double>();
+  A.scratch = C->d_scratch.as<double>();
+  A.sl = C->d_sl.as<SlopeDesc>();
+  A.slnb = C->d_slnb.as<SlopeNb>();
+  A.half_off = C->d_hoff.as<int32_t>();
+  A.halves = C->d_halves.as<HalfDesc>();
+  A.ent_off = C->d_eoff.as<int32_t>();
+  A.ents = C->d_ents.as<EntryDesc>();
+  A.rec_ds = C->d_recds.as<double>();
+  A.bc_prim = C->d_bc.as<double>();
+  A.K = C->P.K;
+  A.gamma = C->P.gamma;
+  A.Pr = C->P.Pr;
+  A.mu = C->P.mu_ref;
+  A.omega = C->P.omega;
+  A.dt = dt;
+  A.flux_kind = C->P.flux_kind;
+  return A;
+}
+
+int check_ready(kah_ctx* C, double dt) {
+  if (!C->have_topo) return C->fail(KAH_ESTATE, "no topology: call kah_set_topology first");
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return C->fail(KAH_EINVAL, "dt must be positive and finite");
+  return C->hip(hipSetDevice(C->P.device), "hipSetDevice");
+}
+
+int ev_begin(kah_ctx* C, EvPair& e, int cls) {
+  e.cls = cls;
+  KAH_TRY(C->hip(hipEventCreate(&e.a), "hipEventCreate"));
+  KAH_TRY(C->hip(hipEventCreate(&e.b), "hipEventCreate"));
+  return C->hip(hipEventRecord(e.a, C->stream), "hipEventRecord");
+}
+int ev_end(kah_ctx* C, EvPair& e) {
+  KAH_TRY(C->hip(hipEventRecord(e.b, C->stream), "hipEventRecord"));
+  C->events.push_back(e);
+  return 0;
+}
+
+int run_slope(kah_ctx* C, const DevArgs& A) {
+  EvPair e{};
+  if (C->timing) KAH_TRY(ev_begin(C, e, 0));
+  for (LevelBin* b : C->bins)
+    KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, b->cells.as<int32_t>(), b->n, b->maxh, C->stream),
+                   "slope kernel launch"));
+  if (C->timing) KAH_TRY(ev_end(C, e));
+  return 0;
+}
+
+int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
+  EvPair e{};
+  if (C->timing) KAH_TRY(ev_begin(C, e, 1));
+  int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER : MARCH_CAIDVM;
+  KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
+                                   mode, march, C->lds_bytes, C->stream),
+                 "flux/relaxation kernel launch"));
+  if (C->timing) KAH_TRY(ev_end(C, e));
+  return 0;
+}
+
+int finish_iterate(kah_ctx* C, int64_t step, double* sum_res, double* sum_avg) {
+  const int NW = C->NW;
+  C->cur = 1 - C->cur;
+  bool check = (step % C->P.st_check_interval) == 0;
+  if (!check || (!sum_res && !sum_avg)) {
+    for (int q = 0; q < NW; ++q) {
+      if (sum_res) sum_res[q] = 0.0;
+      if (sum_avg) sum_avg[q] = 0.0;
+    }
+    return 0;
+  }
+  std::vector<double> r(size_t(C->nc * 2 * NW));
+  KAH_TRY(C->hip(hipMemcpyAsync(r.data(), C->d_res.p, r.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, C->stream), "residual download"));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "residual download"));
+  for (int q = 0; q < NW; ++q) {
+    double sr = 0.0, sa = 0.0;
+    for (int32_t c : C->fluid) {
+      sr += r[size_t(c) * 2 * NW + q];
+      sa += r[size_t(c) * 2 * NW + NW + q];
+    }
+    if (sum_res) sum_res[q] = sr;
+    if (sum_avg) sum_avg[q] = sa;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kah_set_state(kah_ctx* C, const double* w, const double* prim, const double* df) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_topo) return C->fail(KAH_ESTATE, "no topology: call kah_set_topology first");
+  if (!w || !prim || !df) return C->fail(KAH_EINVAL, "null state array");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  const int64_t NW = C->NW;
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_w.p, w, sizeof(double) * C->nc * NW, hipMemcpyHostToDevice,
+                                C->stream), "w upload"));
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_prim.p, prim, sizeof(double) * C->nc * NW,
+                                hipMemcpyHostToDevice, C->stream), "prim upload"));
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_df[C->cur].p, df, sizeof(double) * C->np * C->NDF,
+                                hipMemcpyHostToDevice, C->stream), "df upload"));
+  KAH_TRY(C->zero(C->d_psflux));
+  KAH_TRY(C->zero(C->d_vflux));
+  KAH_TRY(C->zero(C->d_qf));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "state upload"));
+  C->have_state = true;
+  return KAH_OK;
+}
+
+int kah_get_state(kah_ctx* C, double* w, double* prim, double* qf, double* sw, double* df,
+                  double* sdf) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  const int64_t NW = C->NW, D = C->D;
+  if (sw) {
+    if (!C->d_sw.p) KAH_TRY(C->alloc(C->d_sw, sizeof(double) * C->nc * NW * D, "sw"));
+    KAH_TRY(C->zero(C->d_sw));
+    DevArgs A = make_args(C, 0.0);
+    KAH_TRY(C->hip(launch_macro_slope(A, C->D, C->uni, C->d_fluid.as<int32_t>(),
+                                      int(C->fluid.size()), C->d_sw.as<double>(), C->stream),
+                   "macro slope launch"));
+    KAH_TRY(C->hip(hipMemcpyAsync(sw, C->d_sw.p, sizeof(double) * C->nc * NW * D,
+                                  hipMemcpyDeviceToHost, C->stream), "sw download"));
+  }
+  auto dl = [&](double* dst, const DBuf& b, size_t count, const char* what) -> int {
+    if (!dst) return 0;
+    return C->hip(hipMemcpyAsync(dst, b.p, count * sizeof(double), hipMemcpyDeviceToHost, C->stream), what);
+  };
+  KAH_TRY(dl(w, C->d_w, size_t(C->nc * NW), "w download"));
+  KAH_TRY(dl(prim, C->d_prim, size_t(C->nc * NW), "prim download"));
+  KAH_TRY(dl(qf, C->d_qf, size_t(C->nc * D), "qf download"));
+  KAH_TRY(dl(df, C->d_df[C->cur], size_t(C->np * C->NDF), "df download"));
+  KAH_TRY(dl(sdf, C->d_sdf, size_t(C->np * C->NDF * D), "sdf download"));
+  return C->hip(hipStreamSynchronize(C->stream), "state download");
+}
+
+int kah_get_flux(kah_ctx* C, double* flux, double* vs_flux) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  if (flux)
+    KAH_TRY(C->hip(hipMemcpyAsync(flux, C->d_psflux.p, sizeof(double) * C->nc * C->NW,
+                                  hipMemcpyDeviceToHost, C->stream), "flux download"));
+  if (vs_flux) {
+    if (C->d_vflux.p)
+      KAH_TRY(C->hip(hipMemcpyAsync(vs_flux, C->d_vflux.p, sizeof(double) * C->np * C->NDF,
+                                    hipMemcpyDeviceToHost, C->stream), "vs flux download"));
+    else
+      std::memset(vs_flux, 0, sizeof(double) * C->np * C->NDF);
+  }
+  return C->hip(hipStreamSynchronize(C->stream), "flux download");
+}
+
+int kah_slope(kah_ctx* C, double dt) {
+  if (!C) return KAH_EINVAL;
+  KAH_TRY(check_ready(C, dt));
+  DevArgs A = make_args(C, dt);
+  KAH_TRY(run_slope(C, A));
+  return C->hip(hipStreamSynchronize(C->stream), "slope");
+}
+
+int kah_flux(kah_ctx* C, double dt) {
+  if (!C) return KAH_EINVAL;
+  KAH_TRY(check_ready(C, dt));
+  if (!C->d_vflux.p) {
+    KAH_TRY(C->alloc(C->d_vflux, sizeof(double) * C->np * C->NDF, "vs flux"));
+    KAH_TRY(C->zero(C->d_vflux));
+  }
+  DevArgs A = make_args(C, dt);
+  KAH_TRY(run_flux_relax(C, A, MODE_FLUX));
+  return C->hip(hipStreamSynchronize(C->stream), "flux");
+}
+
+int kah_iterate(kah_ctx* C, double dt, int64_t step, double* sum_res, double* sum_avg) {
+  if (!C) return KAH_EINVAL;
+  KAH_TRY(check_ready(C, dt));
+  if (!C->d_vflux.p) {
+    KAH_TRY(C->alloc(C->d_vflux, sizeof(double) * C->np * C->NDF, "vs flux"));
+    KAH_TRY(C->zero(C->d_vflux));
+  }
+  DevArgs A = make_args(C, dt);
+  KAH_TRY(run_flux_relax(C, A, MODE_ITER));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "iterate"));
+  return finish_iterate(C, step, sum_res, sum_avg);
+}
+
+int kah_step(kah_ctx* C, double dt, int64_t step, double* sum_res, double* sum_avg) {
+  if (!C) return KAH_EINVAL;
+  KAH_TRY(check_ready(C, dt));
+  DevArgs A = make_args(C, dt);
+  KAH_TRY(run_slope(C, A));
+  KAH_TRY(run_flux_relax(C, A, MODE_FUSED));
+  bool check = (step % C->P.st_check_interval) == 0 && (sum_res || sum_avg);
+  if (check) KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "step"));
+  else KAH_TRY(C->hip(hipGetLastError(), "step"));
+  return finish_iterate(C, step, sum_res, sum_avg);
+}
+
+int kah_sync(kah_ctx* C) {
+  if (!C) return KAH_EINVAL;
+  return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int64_t kah_phase_cells(const kah_ctx* C) { return C ? C->phase_cells : 0; }
+int64_t kah_device_bytes(const kah_ctx* C) { return C ? C->dev_bytes : 0; }
+
+int kah_kernel_times(kah_ctx* C, int reset, double* out_ms, int64_t* out_calls) {
+  if (!C) return KAH_EINVAL;
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "kernel times"));
+  for (auto& e : C->events) {
+    float ms = 0.f;
+    KAH_TRY(C->hip(hipEventElapsedTime(&ms, e.a, e.b), "hipEventElapsedTime"));
+    C->t_sum[e.cls] += ms;
+    C->t_cnt[e.cls] += 1;
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  C->events.clear();
+  for (int k = 0; k < 2; ++k) {
+    if (out_ms) out_ms[k] = C->t_cnt[k] ? C->t_sum[k] / double(C->t_cnt[k]) : 0.0;
+    if (out_calls) out_calls[k] = C->t_cnt[k];
+  }
+  if (reset) {
+    C->t_sum[0] = C->t_sum[1] = 0.0;
+    C->t_cnt[0] = C->t_cnt[1] = 0;
+    C->timing = true;
+  }
+  return KAH_OK;
+}
+
+}  // extern "C"

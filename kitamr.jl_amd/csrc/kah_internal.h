@@ -1,0 +1,179 @@
+// kah_internal.h -- device-side descriptors and point math shared by the kernels and the
+// host code that derives the descriptors from a kah_topology.
+//
+// Dataflow of one step on the device (single GPU, all state resident in HBM):
+//   K1  slope   (one launch per physical level, coarse to fine): per fluid cell, the
+//       velocity-space slopes sdf in every direction, and the macroscopic "half fluxes":
+//       the moments of the upwind micro flux that this cell's own reconstruction sends
+//       through each of its faces.
+//   K2  flux+relax (one launch): per fluid cell, the micro flux of every face gathered per
+//       velocity cell (own upwind cells from its own reconstruction, incoming cells from
+//       the neighbour's), the macroscopic flux from the two half fluxes of each face, and
+//       the implicit relaxation update, writing f of the next step into the other buffer.
+// A face's macroscopic flux is the sum of two half fluxes that each side reads from the
+// same buffer, so the two cells see bit-identical, opposite face fluxes (mass is conserved
+// to round-off), and each cell accumulates its face contributions in face-list order, the
+// order of the reference's scatter.
+#pragma once
+#include <stdint.h>
+
+namespace kah {
+
+constexpr double EPS = 1e-12;                        // src/Abstract/Types.jl:3
+constexpr double MACH_EPS = 2.220446049250313e-16;   // eps(Float64)
+constexpr double PI = 3.141592653589793;
+
+// slope modes per (cell, direction)
+enum : int8_t { SL_ZERO = 0, SL_INNER = 1, SL_BOUND = 2, SL_SKIP = 3 };
+// micro-flux modes of one face side
+enum : int8_t {
+  MI_LIMITED = 0,     // positivity-preserving reconstruction, CAIDVM.jl:127-141
+  MI_UNLIMITED = 1,   // (f + dx.sdf) * vn
+  MI_PLAIN = 2,       // f * vn (no reconstruction)
+  MI_MAXWELL = 3,     // Maxwellian of a boundary state * vn (inflow)
+  MI_WALL = 4,        // Maxwellian with the zero-mass-flux wall density * vn
+  MI_INTERP = 5,      // InterpolatedOutflow ghost reconstruction * vn
+  MI_SELF_PLAIN = 6,  // UniformOutflow: the cell's own f * vn on the incoming set
+};
+
+struct SlopeNb {        // one neighbour in a slope stencil
+  int32_t cell;         // data cell
+  int32_t proj;         // 1: project with the neighbour's transverse slopes
+  double dm[3];         // transverse offsets (dm[dir] = 0)
+};
+
+struct SlopeDesc {      // per (fluid cell, direction)
+  int8_t mode;
+  int8_t pad[3];
+  int32_t nb_beg[2];    // side 0 = L (or the one side of a BOUND), side 1 = R
+  int32_t nb_cnt[2];
+  int32_t pad2;
+  double ds[2];         // dsL, dsR
+};
+
+struct HalfDesc {       // one half flux computed by its owner cell in K1
+  int32_t slot;         // output row in half_fw (pair*2 + side)
+  int32_t data_cell;    // cell whose f/sdf feed the micro flux
+  int8_t mode;          // MI_*
+  int8_t sel;           // 0: rot*u[dir] <= 0 ; 1: rot*u[dir] > 0
+  int8_t dir;
+  int8_t pad;
+  int32_t bc;           // boundary condition (domain halves) or -1
+  double rot;
+  double xrec[3];       // cell-record midpoint used for the reconstruction
+  double dsrec[3];      // cell-record size used by the limiter
+  double fmid[3];       // face midpoint
+};
+
+struct EntryDesc {      // one face side of a fluid cell, gathered in K2
+  int32_t pair;         // half fluxes at half_fw[pair*2 + 0/1]
+  int32_t oth_cell;     // data cell of the other side (-1 for domain faces)
+  int8_t side;          // 0: the cell is "here" (adds), 1: "there" (subtracts)
+  int8_t own_mode;      // MI_* of the cell's own upwind set
+  int8_t oth_mode;      // MI_* of the incoming set
+  int8_t dir;
+  int32_t bc;
+  double rot;
+  double area;          // signed face area, rot*A (or rot*A/2^(DIM-1))
+  double fmid[3];
+  double xown[3], dsown[3];
+  double xoth[3], dsoth[3];
+};
+
+// ---------------------------------------------------------------- point math (FP64)
+// Every expression keeps the reference's association order; the library is compiled
+// without floating-point contraction so each operation rounds as written.
+
+__host__ __device__ inline double jl_sign(double x) {
+  return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x);
+}
+__host__ __device__ inline double jl_min(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return b < a ? b : a;
+}
+// minmod, src/Flux/Slope.jl:20-24
+__host__ __device__ inline double minmod(double sL, double sR) {
+  double SL = fabs(sL), SR = fabs(sR);
+  return 0.5 * (jl_sign(sL) + jl_sign(sR)) * jl_min(SL, SR);
+}
+
+// get_prim_2D / get_prim_3D, lib/KitCore/2D.jl:9-16, 3D.jl:12-20
+template <int D>
+__host__ __device__ inline void get_prim(const double* w, double g, double* p) {
+  if constexpr (D == 2) {
+    p[0] = w[0];
+    p[1] = w[1] / w[0];
+    p[2] = w[2] / w[0];
+    p[3] = 0.5 * w[0] / (g - 1.0) / (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+  } else {
+    p[0] = w[0];
+    p[1] = w[1] / w[0];
+    p[2] = w[2] / w[0];
+    p[3] = w[3] / w[0];
+    p[4] = 0.5 * w[0] / (g - 1.0) /
+           (w[4] - 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) / w[0]);
+  }
+}
+
+// discrete_maxwell_2D2F / 3D1F point forms, lib/KitCore/2D2F.jl:15-21, 3D1F.jl:14-23
+template <int D>
+__host__ __device__ inline void maxwell(const double* u, const double* p, double K,
+                                        double* out) {
+  if constexpr (D == 2) {
+    double a = u[0] - p[1], b = u[1] - p[2];
+    double c2 = a * a + b * b;
+    double h = p[0] * (p[3] / PI) * exp(-p[3] * c2);
+    out[0] = h;
+    out[1] = h * K / (2.0 * p[3]);
+  } else {
+    double a = u[0] - p[1], b = u[1] - p[2], c = u[2] - p[3];
+    double c2 = a * a + b * b + c * c;
+    out[0] = p[0] * pow(p[4] / PI, 1.5) * exp(-p[4] * c2);
+  }
+}
+
+// shakhov_part_2D2F / 3D1F point forms, lib/KitCore/2D2F.jl:22-44, 3D1F.jl:24-40
+template <int D>
+__host__ __device__ inline void shakhov(const double* u, const double* F, const double* p,
+                                        const double* q, double Pr, double K, double* out) {
+  if constexpr (D == 2) {
+    double cu = u[0] - p[1], cv = u[1] - p[2];
+    double c2 = cu * cu + cv * cv;
+    double cq = cu * q[0] + cv * q[1];
+    out[0] = 0.8 * (1 - Pr) * (p[3] * p[3]) / p[0] * cq * (2 * p[3] * c2 + K - 5) * F[0];
+    out[1] = 0.8 * (1 - Pr) * (p[3] * p[3]) / p[0] * cq * (2 * p[3] * c2 + K - 3) * F[1];
+  } else {
+    double cu = u[0] - p[1], cv = u[1] - p[2], cw = u[2] - p[3];
+    double c2 = cu * cu + cv * cv + cw * cw;
+    double cq = cu * q[0] + cv * q[1] + cw * q[2];
+    out[0] = 0.8 * (1 - Pr) * (p[4] * p[4]) / p[0] * cq * (2 * p[4] * c2 - 5) * F[0];
+  }
+}
+
+// get_τ, src/Gas/Model.jl:14
+template <int D>
+__host__ __device__ inline double get_tau(const double* prim, double mu, double omega) {
+  return mu * 2.0 * pow(prim[D + 1], 1 - omega) / prim[0];
+}
+
+// Contribution of one velocity cell to micro_to_macro (2D2F.jl:119-126, 3D1F.jl:109-123);
+// the 1/2 of the energy row is applied after the sum.
+template <int D, int NDF>
+__host__ __device__ inline void moment_terms(double wt, const double* u, const double* f,
+                                             double* m) {
+  if constexpr (D == 2) {
+    m[0] = wt * f[0];
+    m[1] = wt * u[0] * f[0];
+    m[2] = wt * u[1] * f[0];
+    m[3] = wt * ((u[0] * u[0] + u[1] * u[1]) * f[0] + f[1]);
+  } else {
+    m[0] = wt * f[0];
+    m[1] = wt * u[0] * f[0];
+    m[2] = wt * u[1] * f[0];
+    m[3] = wt * u[2] * f[0];
+    m[4] = wt * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * f[0];
+  }
+}
+
+}  // namespace kah

@@ -1,0 +1,275 @@
+"""ctypes binding of the C ABI in include/kitamr_hip.h.
+
+This is the same binding a Julia ``ccall`` shim performs: plain pointers and sizes, the
+caller owning its host arrays. The library is loaded from ``kitamr.jl_amd/lib`` and the
+load fails loudly when it is missing; nothing here falls back to a host implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_ROOT / "lib" / "libkitamr_hip.so"
+
+FLUX = {"CAIDVM": 0, "DVM": 1}
+MARCH = {"CAIDVM_Marching": 0, "Euler": 1}
+PREC = {"fp64": 0, "fp32f": 1}
+BC = {"Maxwellian": 0, "SuperSonicInflow": 1, "UniformOutflow": 2, "InterpolatedOutflow": 3}
+ERRORS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "device error",
+          -4: "bad call order", -5: "unsupported"}
+
+EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "kah_set_state",
+           "kah_get_state", "kah_get_flux", "kah_slope", "kah_flux", "kah_iterate", "kah_step",
+           "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times"]
+
+
+class KahParams(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("ndf", C.c_int32), ("flux_kind", C.c_int32),
+                ("marching", C.c_int32), ("precision", C.c_int32),
+                ("st_check_interval", C.c_int32), ("device", C.c_int32),
+                ("reserved", C.c_int32), ("K", C.c_double), ("gamma", C.c_double),
+                ("Pr", C.c_double), ("mu_ref", C.c_double), ("omega", C.c_double)]
+
+
+_P = C.c_void_p
+
+
+class KahTopology(C.Structure):
+    _fields_ = [("n_cells", C.c_int64), ("n_images", C.c_int64), ("image_src", _P),
+                ("cell_mid", _P), ("cell_ds", _P), ("bound_enc", _P), ("ps_level", _P),
+                ("vs_offset", _P), ("vs_level", _P), ("vs_mid", _P), ("vs_weight", _P),
+                ("vs_uniform", C.c_int32), ("reserved0", C.c_int32),
+                ("nb_state", _P), ("nb_offset", _P), ("nb_cell", _P),
+                ("n_pairs", C.c_int64), ("pair_here", _P), ("pair_there", _P),
+                ("pair_dir", _P), ("pair_rot", _P), ("pair_area", _P), ("pair_mid", _P),
+                ("pair_bc", _P), ("n_bc", C.c_int32), ("vs_maxlevel", C.c_int32),
+                ("bc_kind", _P), ("bc_prim", _P), ("vs_trees_num", C.c_int32 * 3),
+                ("reserved", C.c_int32), ("vs_lo", C.c_double * 3), ("vs_hi", C.c_double * 3)]
+
+
+_TOPO_ARRAYS = {
+    "image_src": np.int64, "cell_mid": np.float64, "cell_ds": np.float64,
+    "bound_enc": np.int32, "ps_level": np.int32, "vs_offset": np.int64, "vs_level": np.int8,
+    "vs_mid": np.float64, "vs_weight": np.float64, "nb_state": np.int32,
+    "nb_offset": np.int64, "nb_cell": np.int64, "pair_here": np.int64,
+    "pair_there": np.int64, "pair_dir": np.int32, "pair_rot": np.float64,
+    "pair_area": np.float64, "pair_mid": np.float64, "pair_bc": np.int32,
+    "bc_kind": np.int32, "bc_prim": np.float64,
+}
+
+
+def make_topology_struct(topo):
+    """Fill a KahTopology from a dict of arrays; returns (struct, keepalive list)."""
+    keep = []
+    t = KahTopology()
+    for name, dt in _TOPO_ARRAYS.items():
+        a = topo.get(name)
+        if a is None:
+            setattr(t, name, None)
+            continue
+        a = np.ascontiguousarray(np.asarray(a, dtype=dt).reshape(-1))
+        keep.append(a)
+        setattr(t, name, a.ctypes.data if a.size else None)
+    t.n_cells = int(topo["n_cells"])
+    t.n_images = int(topo.get("n_images", 0))
+    t.n_pairs = int(topo["n_pairs"])
+    t.n_bc = int(len(topo.get("bc_kind", [])))
+    t.vs_uniform = int(topo.get("vs_uniform", 0))
+    t.vs_maxlevel = int(topo.get("vs_maxlevel", 0))
+    vtn = list(topo.get("vs_trees_num", [0, 0, 0])) + [0, 0, 0]
+    lo = list(topo.get("vs_lo", [0.0, 0.0, 0.0])) + [0.0, 0.0, 0.0]
+    hi = list(topo.get("vs_hi", [0.0, 0.0, 0.0])) + [0.0, 0.0, 0.0]
+    for d in range(3):
+        t.vs_trees_num[d] = int(vtn[d])
+        t.vs_lo[d] = float(lo[d])
+        t.vs_hi[d] = float(hi[d])
+    return t, keep
+
+
+def make_params(dim, ndf, flux="CAIDVM", marching="CAIDVM_Marching", precision="fp64",
+                K=1.0, gamma=5.0 / 3.0, Pr=2.0 / 3.0, mu_ref=0.0, omega=0.5,
+                st_check_interval=100, device=0):
+    p = KahParams()
+    p.dim, p.ndf = dim, ndf
+    p.flux_kind = FLUX[flux]
+    p.marching = MARCH[marching]
+    p.precision = PREC[precision]
+    p.st_check_interval = st_check_interval
+    p.device = device
+    p.K, p.gamma, p.Pr, p.mu_ref, p.omega = K, gamma, Pr, mu_ref, omega
+    return p
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libkitamr_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = C.CDLL(str(p))
+    d, i32, i64 = C.c_double, C.c_int32, C.c_int64
+    dp = C.POINTER(C.c_double)
+    lib.kah_create.argtypes = [C.POINTER(KahParams), C.POINTER(C.c_void_p)]
+    lib.kah_create.restype = C.c_int
+    lib.kah_destroy.argtypes = [C.c_void_p]
+    lib.kah_destroy.restype = None
+    lib.kah_last_error.argtypes = [C.c_void_p]
+    lib.kah_last_error.restype = C.c_char_p
+    lib.kah_set_topology.argtypes = [C.c_void_p, C.POINTER(KahTopology)]
+    lib.kah_set_state.argtypes = [C.c_void_p, dp, dp, dp]
+    lib.kah_get_state.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp]
+    lib.kah_get_flux.argtypes = [C.c_void_p, dp, dp]
+    lib.kah_slope.argtypes = [C.c_void_p, d]
+    lib.kah_flux.argtypes = [C.c_void_p, d]
+    lib.kah_iterate.argtypes = [C.c_void_p, d, i64, dp, dp]
+    lib.kah_step.argtypes = [C.c_void_p, d, i64, dp, dp]
+    lib.kah_sync.argtypes = [C.c_void_p]
+    lib.kah_phase_cells.argtypes = [C.c_void_p]
+    lib.kah_phase_cells.restype = i64
+    lib.kah_device_bytes.argtypes = [C.c_void_p]
+    lib.kah_device_bytes.restype = i64
+    lib.kah_kernel_times.argtypes = [C.c_void_p, C.c_int, dp, C.POINTER(i64)]
+    for name in ["kah_set_topology", "kah_set_state", "kah_get_state", "kah_get_flux",
+                 "kah_slope", "kah_flux", "kah_iterate", "kah_step", "kah_sync",
+                 "kah_kernel_times"]:
+        getattr(lib, name).restype = C.c_int
+    del i32
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class KahError(RuntimeError):
+    pass
+
+
+class HipContext:
+    """One device context: owns the device state of one flow case."""
+
+    def __init__(self, params, lib=None):
+        self.lib = lib or load_library()
+        self.params = params
+        self.dim, self.ndf = params.dim, params.ndf
+        h = C.c_void_p()
+        rc = self.lib.kah_create(C.byref(params), C.byref(h))
+        if rc != 0:
+            raise KahError(f"kah_create failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+        self.nc = 0
+        self.np = 0
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.kah_last_error(self.h).decode()
+            raise KahError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kah_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_topology(self, topo):
+        s, keep = make_topology_struct(topo)
+        self._check(self.lib.kah_set_topology(self.h, C.byref(s)), "kah_set_topology")
+        del keep
+        self.nc = int(topo["n_cells"])
+        off = np.asarray(topo["vs_offset"])
+        self.np = int(off[-1])
+
+    def set_state(self, w, prim, df):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        df = np.ascontiguousarray(df, dtype=np.float64)
+        self._check(self.lib.kah_set_state(self.h, _ptr(w), _ptr(prim), _ptr(df)), "kah_set_state")
+
+    def get_state(self, w=True, prim=True, qf=True, sw=False, df=True, sdf=False):
+        nw = self.dim + 2
+        out = {}
+        arrs = {
+            "w": np.empty(self.nc * nw) if w else None,
+            "prim": np.empty(self.nc * nw) if prim else None,
+            "qf": np.empty(self.nc * self.dim) if qf else None,
+            "sw": np.empty(self.nc * nw * self.dim) if sw else None,
+            "df": np.empty(self.np * self.ndf) if df else None,
+            "sdf": np.empty(self.np * self.ndf * self.dim) if sdf else None,
+        }
+        self._check(self.lib.kah_get_state(self.h, *[_ptr(arrs[k]) for k in
+                                                     ["w", "prim", "qf", "sw", "df", "sdf"]]),
+                    "kah_get_state")
+        for k, v in arrs.items():
+            if v is not None:
+                out[k] = v
+        return out
+
+    def get_flux(self):
+        flux = np.empty(self.nc * (self.dim + 2))
+        vflux = np.empty(self.np * self.ndf)
+        self._check(self.lib.kah_get_flux(self.h, _ptr(flux), _ptr(vflux)), "kah_get_flux")
+        return flux, vflux
+
+    def slope(self, dt):
+        self._check(self.lib.kah_slope(self.h, float(dt)), "kah_slope")
+
+    def flux(self, dt):
+        self._check(self.lib.kah_flux(self.h, float(dt)), "kah_flux")
+
+    def iterate(self, dt, step):
+        nw = self.dim + 2
+        r, a = np.zeros(nw), np.zeros(nw)
+        self._check(self.lib.kah_iterate(self.h, float(dt), int(step), _ptr(r), _ptr(a)),
+                    "kah_iterate")
+        return r, a
+
+    def step(self, dt, step, residual=True):
+        nw = self.dim + 2
+        r, a = (np.zeros(nw), np.zeros(nw)) if residual else (None, None)
+        self._check(self.lib.kah_step(self.h, float(dt), int(step), _ptr(r), _ptr(a)), "kah_step")
+        return r, a
+
+    def sync(self):
+        self._check(self.lib.kah_sync(self.h), "kah_sync")
+
+    def phase_cells(self):
+        return int(self.lib.kah_phase_cells(self.h))
+
+    def device_bytes(self):
+        return int(self.lib.kah_device_bytes(self.h))
+
+    def kernel_times(self, reset=False):
+        ms = np.zeros(2)
+        calls = np.zeros(2, dtype=np.int64)
+        self._check(self.lib.kah_kernel_times(self.h, int(reset), _ptr(ms),
+                                              calls.ctypes.data_as(C.POINTER(C.c_int64))),
+                    "kah_kernel_times")
+        return ms, calls
+
+
+def library_exports(path=None):
+    """Names of the kah_* entry points the shared object exports (no device call)."""
+    p = Path(path) if path else LIB_PATH
+    lib = C.CDLL(str(p))
+    return [n for n in EXPORTS if hasattr(lib, n)]
+
+
+os.environ.setdefault("HSA_ENABLE_IPC_MODE_LEGACY", "0")

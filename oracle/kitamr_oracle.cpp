@@ -1,0 +1,1128 @@
+// kitamr_oracle.cpp -- CPU FP64 restatement of KitAMR.jl's per-step phase-space update.
+//
+// TEST INFRASTRUCTURE ONLY (the checker of the HIP path; never linked by the product).
+//
+// Every function below follows the reference function named in its comment, in the
+// reference's own loop order: slope!  -> src/Flux/Slope.jl:20-1070
+//                               flux!   -> src/Flux/Flux.jl:10-502, CAIDVM.jl, DVM.jl
+//                               iterate!-> src/Theory/Iterate.jl:96-162
+//                               KitCore -> lib/KitCore/{2D,3D,2D2F,3D1F}.jl
+// Moment sums use Neumaier-compensated accumulation, so the oracle's own rounding error
+// is far below the 1e-12 parity budget (the reference uses Julia's pairwise `sum`).
+// Compiled with -ffp-contract=off: no fused multiply-adds, every operation is rounded
+// as written, in the reference's association order.
+#include "kitamr_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double EPS = 1e-12;                            // src/Abstract/Types.jl:3
+constexpr double MACH_EPS = 2.220446049250313e-16;       // Julia eps(Float64)
+constexpr double PI = 3.141592653589793;                 // Julia π as Float64
+
+inline double jl_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+inline double jl_min(double a, double b) {
+  if (a != a || b != b) return std::numeric_limits<double>::quiet_NaN();
+  return b < a ? b : a;
+}
+// minmod, src/Flux/Slope.jl:20-24
+inline double minmod(double sL, double sR) {
+  double SL = std::fabs(sL), SR = std::fabs(sR);
+  return 0.5 * (jl_sign(sL) + jl_sign(sR)) * jl_min(SL, SR);
+}
+
+struct Neumaier {
+  double s = 0.0, c = 0.0;
+  void add(double x) {
+    double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+  }
+  double get() const { return s + c; }
+};
+
+}  // namespace
+
+struct orc_ctx {
+  int dim = 2, ndf = 2, flux_kind = 0, marching = 0, st_check = 100;
+  double K = 1.0, gamma = 5.0 / 3.0, Pr = 2.0 / 3.0, mu = 0.0, omega = 0.5;
+  int64_t nc = 0, ni = 0, np = 0;
+  std::vector<int64_t> image_src;
+  std::vector<double> mid, ds;
+  std::vector<int32_t> benc, lvl;
+  std::vector<int64_t> voff;
+  std::vector<int8_t> vlev;
+  std::vector<double> vmid, vw;
+  bool vs_uniform = false;
+  std::vector<int32_t> nbs;
+  std::vector<int64_t> nboff, nbc;
+  std::vector<int64_t> ph, pt;
+  std::vector<int32_t> pd, pbc;
+  std::vector<double> prot, parea, pmid;
+  std::vector<int32_t> bck;
+  std::vector<double> bcp;
+  // state
+  std::vector<double> w, prim, qf, sw, flux;
+  std::vector<double> df, sdf, vflux;
+  std::string err;
+
+  int NW() const { return dim + 2; }
+  int64_t src(int64_t rec) const { return rec < nc ? rec : image_src[rec - nc]; }
+  int64_t off(int64_t c) const { return voff[c]; }
+  int64_t goff(int64_t c) const { return vs_uniform ? 0 : voff[c]; }  // grid arrays
+  int64_t nv(int64_t c) const { return voff[c + 1] - voff[c]; }
+  const double* rmid(int64_t rec) const { return &mid[rec * dim]; }
+  const double* rds(int64_t rec) const { return &ds[rec * dim]; }
+  int32_t rbenc(int64_t rec) const { return benc[src(rec)]; }
+  double& F(int64_t c, int64_t i, int k) { return df[off(c) * ndf + k * nv(c) + i]; }
+  double& S(int64_t c, int64_t i, int k, int d) {
+    return sdf[off(c) * ndf * dim + (int64_t(d) * ndf + k) * nv(c) + i];
+  }
+  double& VF(int64_t c, int64_t i, int k) { return vflux[off(c) * ndf + k * nv(c) + i]; }
+  double U(int64_t c, int64_t i, int d) const { return vmid[goff(c) * dim + d * nv(c) + i]; }
+  double Wt(int64_t c, int64_t i) const { return vw[goff(c) + i]; }
+  int L(int64_t c, int64_t i) const { return vlev[goff(c) + i]; }
+  int32_t state(int64_t c, int f) const { return nbs[c * 2 * dim + f]; }
+  int64_t nb_n(int64_t c, int f) const { return nboff[c * 2 * dim + f + 1] - nboff[c * 2 * dim + f]; }
+  int64_t nb(int64_t c, int f, int64_t j) const { return nbc[nboff[c * 2 * dim + f] + j]; }
+};
+
+namespace {
+
+template <class T>
+void copy_in(std::vector<T>& v, const T* p, int64_t n) {
+  v.assign(p ? p : nullptr, p ? p + n : nullptr);
+  if (!p) v.assign(size_t(n), T());
+}
+
+// ---------------------------------------------------------------- KitCore (point forms)
+// get_prim_2D / get_prim_3D, lib/KitCore/2D.jl:9-16, 3D.jl:12-20
+void get_prim(int dim, const double* w, double g, double* p) {
+  if (dim == 2) {
+    p[0] = w[0];
+    p[1] = w[1] / w[0];
+    p[2] = w[2] / w[0];
+    p[3] = 0.5 * w[0] / (g - 1.0) / (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+  } else {
+    p[0] = w[0];
+    p[1] = w[1] / w[0];
+    p[2] = w[2] / w[0];
+    p[3] = w[3] / w[0];
+    p[4] = 0.5 * w[0] / (g - 1.0) /
+           (w[4] - 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) / w[0]);
+  }
+}
+// get_conserved_2D / 3D, lib/KitCore/2D.jl:1-8, 3D.jl:1-11
+void get_conserved(int dim, const double* p, double g, double* w) {
+  if (dim == 2) {
+    w[0] = p[0];
+    w[1] = p[0] * p[1];
+    w[2] = p[0] * p[2];
+    w[3] = 0.5 * p[0] / p[3] / (g - 1.0) + 0.5 * p[0] * (p[1] * p[1] + p[2] * p[2]);
+  } else {
+    w[0] = p[0];
+    w[1] = p[0] * p[1];
+    w[2] = p[0] * p[2];
+    w[3] = p[0] * p[3];
+    w[4] = 0.5 * p[0] / p[4] / (g - 1.0) +
+           0.5 * p[0] * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+  }
+}
+// get_τ, src/Gas/Model.jl:14
+double get_tau(int dim, const double* prim, double mu, double omega) {
+  return mu * 2.0 * std::pow(prim[dim + 1], 1 - omega) / prim[0];
+}
+// discrete_maxwell_2D2F (lib/KitCore/2D2F.jl:15-21), discrete_maxwell_3D1F (3D1F.jl:14-23)
+inline void maxwell(int dim, const double* u, const double* p, double K, double* out) {
+  if (dim == 2) {
+    double c2 = (u[0] - p[1]) * (u[0] - p[1]) + (u[1] - p[2]) * (u[1] - p[2]);
+    double h = p[0] * (p[3] / PI) * std::exp(-p[3] * c2);
+    out[0] = h;
+    out[1] = h * K / (2.0 * p[3]);
+  } else {
+    double c2 = (u[0] - p[1]) * (u[0] - p[1]) + (u[1] - p[2]) * (u[1] - p[2]) +
+                (u[2] - p[3]) * (u[2] - p[3]);
+    out[0] = p[0] * std::pow(p[4] / PI, 1.5) * std::exp(-p[4] * c2);
+  }
+}
+// shakhov_part_2D2F (2D2F.jl:22-44), shakhov_part_3D1F (3D1F.jl:24-40); F = Maxwellian
+inline void shakhov(int dim, const double* u, const double* F, const double* p,
+                    const double* q, double Pr, double K, double* out) {
+  if (dim == 2) {
+    double cu = u[0] - p[1], cv = u[1] - p[2];
+    double c2 = cu * cu + cv * cv;
+    double cq = cu * q[0] + cv * q[1];
+    out[0] = 0.8 * (1 - Pr) * (p[3] * p[3]) / p[0] * cq * (2 * p[3] * c2 + K - 5) * F[0];
+    out[1] = 0.8 * (1 - Pr) * (p[3] * p[3]) / p[0] * cq * (2 * p[3] * c2 + K - 3) * F[1];
+  } else {
+    double cu = u[0] - p[1], cv = u[1] - p[2], cw = u[2] - p[3];
+    double c2 = cu * cu + cv * cv + cw * cw;
+    double cq = cu * q[0] + cv * q[1] + cw * q[2];
+    out[0] = 0.8 * (1 - Pr) * (p[4] * p[4]) / p[0] * cq * (2 * p[4] * c2 - 5) * F[0];
+  }
+}
+
+// micro_to_macro_2D2F (2D2F.jl:119-126) / micro_to_macro_3D1F (3D1F.jl:109-123) over an
+// index subset: values h[j], b[j] with velocities of rows rows[j] of cell c.
+struct Moments {
+  Neumaier m[5];
+};
+
+// ---------------------------------------------------------------- slope!
+// diff_vs!, src/Flux/Slope.jl:29-64
+void diff_vs(orc_ctx& o, int64_t c, int64_t n, double dsL, std::vector<double>& sL) {
+  const int DIM = o.dim, NDF = o.ndf;
+  int64_t nc_ = o.nv(c), nn = o.nv(n);
+  int64_t index = 0;
+  double flag = 0.0;
+  for (int64_t i = 0; i < nc_; ++i) {
+    int li = o.L(c, i);
+    int ln = o.L(n, index);
+    if (li == ln) {
+      for (int j = 0; j < NDF; ++j) sL[j * nc_ + i] += (o.F(c, i, j) - o.F(n, index, j)) / dsL;
+      index += 1;
+    } else if (li < ln) {
+      while (flag != 1.0) {
+        int lnn = o.L(n, index);
+        double den = std::ldexp(1.0, DIM * (lnn - li));
+        for (int j = 0; j < NDF; ++j)
+          sL[j * nc_ + i] += (o.F(c, i, j) - o.F(n, index, j)) / den / dsL;
+        flag += 1 / den;
+        index += 1;
+      }
+      flag = 0.0;
+    } else {
+      for (int j = 0; j < NDF; ++j) sL[j * nc_ + i] += (o.F(c, i, j) - o.F(n, index, j)) / dsL;
+      flag += 1 / std::ldexp(1.0, DIM * (li - ln));
+      if (flag == 1.0) {
+        index += 1;
+        flag = 0.0;
+      }
+    }
+    (void)nn;
+  }
+}
+// diff_vs_transverse!, src/Flux/Slope.jl:278-332
+void diff_vs_transverse(orc_ctx& o, int64_t c, int64_t n, double dsL, const double* dm,
+                        std::vector<double>& sL) {
+  const int DIM = o.dim, NDF = o.ndf;
+  int64_t nc_ = o.nv(c);
+  int64_t index = 0;
+  double flag = 0.0;
+  auto proj = [&](int64_t idx, int j) {
+    double p = o.F(n, idx, j);
+    for (int t = 0; t < DIM; ++t) p += dm[t] * o.S(n, idx, j, t);
+    return p;
+  };
+  for (int64_t i = 0; i < nc_; ++i) {
+    int li = o.L(c, i);
+    int ln = o.L(n, index);
+    if (li == ln) {
+      for (int j = 0; j < NDF; ++j) sL[j * nc_ + i] += (o.F(c, i, j) - proj(index, j)) / dsL;
+      index += 1;
+    } else if (li < ln) {
+      while (flag != 1.0) {
+        int lnn = o.L(n, index);
+        double den = std::ldexp(1.0, DIM * (lnn - li));
+        for (int j = 0; j < NDF; ++j)
+          sL[j * nc_ + i] += (o.F(c, i, j) - proj(index, j)) / den / dsL;
+        flag += 1 / den;
+        index += 1;
+      }
+      flag = 0.0;
+    } else {
+      for (int j = 0; j < NDF; ++j) sL[j * nc_ + i] += (o.F(c, i, j) - proj(index, j)) / dsL;
+      flag += 1 / std::ldexp(1.0, DIM * (li - ln));
+      if (flag == 1.0) {
+        index += 1;
+        flag = 0.0;
+      }
+    }
+  }
+}
+
+struct SlopeWs {
+  std::vector<double> sL, sR;
+};
+
+// update_slope_bound_vs!, src/Flux/Slope.jl:68-86
+void slope_bound(orc_ctx& o, int64_t c, int face, double ds, int dir, SlopeWs& ws) {
+  int64_t n = o.nv(c);
+  ws.sL.assign(size_t(n * o.ndf), 0.0);
+  int64_t N = o.nb_n(c, face);
+  for (int64_t j = 0; j < N; ++j) diff_vs(o, c, o.nb(c, face, j), ds, ws.sL);
+  for (int k = 0; k < o.ndf; ++k)
+    for (int64_t i = 0; i < n; ++i) o.S(c, i, k, dir) = ws.sL[k * n + i] / double(N);
+}
+// update_slope_inner_vs!, src/Flux/Slope.jl:90-116
+void slope_inner(orc_ctx& o, int64_t c, int dir, double dsL, double dsR, SlopeWs& ws) {
+  int64_t n = o.nv(c);
+  int fL = 2 * dir, fR = 2 * dir + 1;
+  ws.sL.assign(size_t(n * o.ndf), 0.0);
+  ws.sR.assign(size_t(n * o.ndf), 0.0);
+  int64_t nL = o.nb_n(c, fL), nR = o.nb_n(c, fR);
+  for (int64_t j = 0; j < nL; ++j) diff_vs(o, c, o.nb(c, fL, j), dsL, ws.sL);
+  for (int64_t j = 0; j < nR; ++j) diff_vs(o, c, o.nb(c, fR, j), dsR, ws.sR);
+  for (int k = 0; k < o.ndf; ++k)
+    for (int64_t i = 0; i < n; ++i)
+      o.S(c, i, k, dir) = minmod(ws.sL[k * n + i] / double(nL), ws.sR[k * n + i] / double(nR));
+}
+// _ps_has_transverse_offset, src/Flux/Slope.jl:177-199
+bool has_transverse_offset(const orc_ctx& o, int64_t c, int face, int dir) {
+  if (o.dim == 1) return false;
+  int64_t n = o.nb_n(c, face);
+  if (n == 0) return false;
+  for (int t = 0; t < o.dim; ++t) {
+    if (t == dir) continue;
+    double avg = 0.0;
+    for (int64_t j = 0; j < n; ++j) avg += o.rmid(o.nb(c, face, j))[t];
+    avg /= double(n);
+    if (avg != o.rmid(c)[t]) return true;
+  }
+  return false;
+}
+// update_slope_inner_vs_transverse!, src/Flux/Slope.jl:339-383
+void slope_inner_transverse(orc_ctx& o, int64_t c, int dir, double dsL, double dsR,
+                            SlopeWs& ws) {
+  int64_t n = o.nv(c);
+  int fL = 2 * dir, fR = 2 * dir + 1;
+  ws.sL.assign(size_t(n * o.ndf), 0.0);
+  ws.sR.assign(size_t(n * o.ndf), 0.0);
+  int64_t nL = o.nb_n(c, fL), nR = o.nb_n(c, fR);
+  bool projL = has_transverse_offset(o, c, fL, dir);
+  bool projR = has_transverse_offset(o, c, fR, dir);
+  double dm[3];
+  for (int64_t j = 0; j < nL; ++j) {
+    int64_t nb = o.nb(c, fL, j);
+    if (projL) {
+      for (int t = 0; t < o.dim; ++t) dm[t] = (t == dir) ? 0.0 : (o.rmid(c)[t] - o.rmid(nb)[t]);
+      diff_vs_transverse(o, c, nb, dsL, dm, ws.sL);
+    } else {
+      diff_vs(o, c, nb, dsL, ws.sL);
+    }
+  }
+  for (int64_t j = 0; j < nR; ++j) {
+    int64_t nb = o.nb(c, fR, j);
+    if (projR) {
+      for (int t = 0; t < o.dim; ++t) dm[t] = (t == dir) ? 0.0 : (o.rmid(c)[t] - o.rmid(nb)[t]);
+      diff_vs_transverse(o, c, nb, dsR, dm, ws.sR);
+    } else {
+      diff_vs(o, c, nb, dsR, ws.sR);
+    }
+  }
+  for (int k = 0; k < o.ndf; ++k)
+    for (int64_t i = 0; i < n; ++i)
+      o.S(c, i, k, dir) = minmod(ws.sL[k * n + i] / double(nL), ws.sR[k * n + i] / double(nR));
+}
+// update_slope_bound_vs_transverse!, src/Flux/Slope.jl:428-451
+void slope_bound_transverse(orc_ctx& o, int64_t c, int face, double ds, int dir,
+                            SlopeWs& ws) {
+  int64_t n = o.nv(c);
+  ws.sL.assign(size_t(n * o.ndf), 0.0);
+  int64_t N = o.nb_n(c, face);
+  double dm[3];
+  for (int64_t j = 0; j < N; ++j) {
+    int64_t nb = o.nb(c, face, j);
+    for (int t = 0; t < o.dim; ++t) dm[t] = (t == dir) ? 0.0 : (o.rmid(c)[t] - o.rmid(nb)[t]);
+    diff_vs_transverse(o, c, nb, ds, dm, ws.sL);
+  }
+  for (int k = 0; k < o.ndf; ++k)
+    for (int64_t i = 0; i < n; ++i) o.S(c, i, k, dir) = ws.sL[k * n + i] / double(N);
+}
+
+inline bool is_fine(const orc_ctx& o, int s) { return s == (1 << (o.dim - 1)); }
+
+// update_slope!(Val(sL),Val(sR),...) dispatch, src/Flux/Slope.jl:458-771
+int slope_dispatch(orc_ctx& o, int64_t c, int dir, SlopeWs& ws) {
+  int fL = 2 * dir, fR = 2 * dir + 1;
+  int sL = o.state(c, fL), sR = o.state(c, fR);
+  double ds = o.rds(c)[dir];
+  const double xm = o.rmid(c)[dir];
+  bool FL = is_fine(o, sL), FR = is_fine(o, sR);
+  if (sL == 1 && sR == 1) {
+    bool Ls = o.rbenc(o.nb(c, fL, 0)) < 0, Rs = o.rbenc(o.nb(c, fR, 0)) < 0;
+    if (Ls && Rs) {
+      for (int k = 0; k < o.ndf; ++k)
+        for (int64_t i = 0; i < o.nv(c); ++i) o.S(c, i, k, dir) = 0.0;
+    } else if (Ls) {
+      slope_bound(o, c, fR, xm - o.rmid(o.nb(c, fR, 0))[dir], dir, ws);
+    } else if (Rs) {
+      slope_bound(o, c, fL, xm - o.rmid(o.nb(c, fL, 0))[dir], dir, ws);
+    } else {
+      slope_inner(o, c, dir, ds, -ds, ws);
+    }
+  } else if (sL == 1 && FR) {
+    slope_inner(o, c, dir, ds, -0.75 * ds, ws);
+  } else if (FL && sR == 1) {
+    slope_inner(o, c, dir, 0.75 * ds, -ds, ws);
+  } else if (FL && FR) {
+    slope_inner(o, c, dir, 0.75 * ds, -0.75 * ds, ws);
+  } else if (sL == -1 && sR == -1) {
+    slope_inner(o, c, dir, 1.5 * ds, -1.5 * ds, ws);
+  } else if (sL == 1 && sR == -1) {
+    slope_inner(o, c, dir, ds, -1.5 * ds, ws);
+  } else if (sL == -1 && sR == 1) {
+    slope_inner(o, c, dir, 1.5 * ds, -ds, ws);
+  } else if (FL && sR == -1) {
+    slope_inner(o, c, dir, 0.75 * ds, -1.5 * ds, ws);
+  } else if (sL == -1 && FR) {
+    slope_inner(o, c, dir, 1.5 * ds, -0.75 * ds, ws);
+  } else if (sL == 0 && sR != 0) {
+    slope_bound(o, c, fR, xm - o.rmid(o.nb(c, fR, 0))[dir], dir, ws);
+  } else if (sR == 0 && sL != 0) {
+    slope_bound(o, c, fL, xm - o.rmid(o.nb(c, fL, 0))[dir], dir, ws);
+  } else {
+    o.err = "update_slope!: no method for neighbour states (" + std::to_string(sL) + "," +
+            std::to_string(sR) + ")";
+    return KAH_EINVAL;
+  }
+  return 0;
+}
+
+// update_slope_transverse_level! per cell, src/Flux/Slope.jl:849-945
+int slope_transverse_cell(orc_ctx& o, int64_t c, SlopeWs& ws) {
+  for (int dir = 0; dir < o.dim; ++dir) {
+    int sL = o.state(c, 2 * dir), sR = o.state(c, 2 * dir + 1);
+    if (sL == -1 || sR == -1) continue;
+    int rc = slope_dispatch(o, c, dir, ws);
+    if (rc) return rc;
+  }
+  if (o.dim == 1) return 0;
+  for (int dir = 0; dir < o.dim; ++dir) {
+    int fL = 2 * dir, fR = 2 * dir + 1;
+    int sL = o.state(c, fL), sR = o.state(c, fR);
+    const double xm = o.rmid(c)[dir];
+    if (sL != 0 && sR != 0) {
+      if (!(sL == -1 || sR == -1)) continue;
+      int64_t L0 = o.nb(c, fL, 0), R0 = o.nb(c, fR, 0);
+      double dsL = xm - o.rmid(L0)[dir];
+      double dsR = xm - o.rmid(R0)[dir];
+      slope_inner_transverse(o, c, dir, dsL, dsR, ws);
+      if (o.rbenc(L0) < 0)
+        slope_bound_transverse(o, c, fR, dsR, dir, ws);
+      else if (o.rbenc(R0) < 0)
+        slope_bound_transverse(o, c, fL, dsL, dir, ws);
+    } else if (sR == 0 && sL == -1) {
+      int64_t L0 = o.nb(c, fL, 0);
+      if (o.rbenc(L0) < 0) continue;
+      slope_bound_transverse(o, c, fL, xm - o.rmid(L0)[dir], dir, ws);
+    } else if (sL == 0 && sR == -1) {
+      int64_t R0 = o.nb(c, fR, 0);
+      if (o.rbenc(R0) < 0) continue;
+      slope_bound_transverse(o, c, fR, xm - o.rmid(R0)[dir], dir, ws);
+    }
+  }
+  return 0;
+}
+
+// calc_w0 over all cells of a slope/df layout: micro_to_macro_* over a full grid.
+void moments_full(const orc_ctx& o, int64_t c, const double* a, int64_t stride_k,
+                  double* wout) {
+  // a[k*stride_k + i] holds the k-th distribution of velocity cell i
+  int64_t n = o.nv(c);
+  Neumaier m[5];
+  for (int64_t i = 0; i < n; ++i) {
+    double wt = o.Wt(c, i);
+    if (o.dim == 2) {
+      double u = o.U(c, i, 0), v = o.U(c, i, 1);
+      double h = a[i], b = a[stride_k + i];
+      m[0].add(wt * h);
+      m[1].add(wt * u * h);
+      m[2].add(wt * v * h);
+      m[3].add(wt * ((u * u + v * v) * h + b));
+    } else {
+      double u = o.U(c, i, 0), v = o.U(c, i, 1), ww = o.U(c, i, 2);
+      double f = a[i];
+      m[0].add(wt * f);
+      m[1].add(wt * u * f);
+      m[2].add(wt * v * f);
+      m[3].add(wt * ww * f);
+      m[4].add(wt * (u * u + v * v + ww * ww) * f);
+    }
+  }
+  if (o.dim == 2) {
+    wout[0] = m[0].get(); wout[1] = m[1].get(); wout[2] = m[2].get();
+    wout[3] = 0.5 * m[3].get();
+  } else {
+    wout[0] = m[0].get(); wout[1] = m[1].get(); wout[2] = m[2].get(); wout[3] = m[3].get();
+    wout[4] = 0.5 * m[4].get();
+  }
+}
+
+// heat_flux_2D2F (2D2F.jl:68-88) / heat_flux_3D1F (3D1F.jl:41-72) over a full grid
+void heat_flux_full(const orc_ctx& o, int64_t c, const double* a, int64_t stride_k,
+                    const double* p, double* q) {
+  int64_t n = o.nv(c);
+  if (o.dim == 2) {
+    Neumaier s1, s2, s3, s4;
+    for (int64_t i = 0; i < n; ++i) {
+      double wt = o.Wt(c, i);
+      double cu = o.U(c, i, 0) - p[1], cv = o.U(c, i, 1) - p[2];
+      double c2 = cu * cu + cv * cv;
+      double h = a[i], b = a[stride_k + i];
+      s1.add(wt * cu * c2 * h);
+      s2.add(wt * cu * b);
+      s3.add(wt * cv * c2 * h);
+      s4.add(wt * cv * b);
+    }
+    q[0] = 0.5 * (s1.get() + s2.get());
+    q[1] = 0.5 * (s3.get() + s4.get());
+  } else {
+    Neumaier s[3];
+    for (int64_t i = 0; i < n; ++i) {
+      double wt = o.Wt(c, i);
+      double cu = o.U(c, i, 0) - p[1], cv = o.U(c, i, 1) - p[2], cw = o.U(c, i, 2) - p[3];
+      double c2 = cu * cu + cv * cv + cw * cw;
+      double f = a[i];
+      s[0].add(wt * cu * c2 * f);
+      s[1].add(wt * cv * c2 * f);
+      s[2].add(wt * cw * c2 * f);
+    }
+    for (int d = 0; d < 3; ++d) q[d] = 0.5 * s[d].get();
+  }
+}
+
+// ---------------------------------------------------------------- flux!
+struct SideMicro {   // one side of a face: the upwind subset of a velocity grid
+  int64_t cell = -1;             // data cell
+  std::vector<int64_t> rows;     // velocity-cell indices in the subset, in grid order
+  std::vector<double> micro;     // [k*rows.size() + j]
+};
+
+void side_moments(const orc_ctx& o, const SideMicro& s, double* fw) {
+  // micro_to_macro(micro, mid[subset], weight[subset]) -- 2D2F.jl:119-126, 3D1F.jl:109-123
+  int64_t m = int64_t(s.rows.size());
+  Neumaier a[5];
+  for (int64_t j = 0; j < m; ++j) {
+    int64_t i = s.rows[j];
+    double wt = o.Wt(s.cell, i);
+    if (o.dim == 2) {
+      double u = o.U(s.cell, i, 0), v = o.U(s.cell, i, 1);
+      double h = s.micro[j], b = s.micro[m + j];
+      a[0].add(wt * h);
+      a[1].add(wt * u * h);
+      a[2].add(wt * v * h);
+      a[3].add(wt * ((u * u + v * v) * h + b));
+    } else {
+      double u = o.U(s.cell, i, 0), v = o.U(s.cell, i, 1), ww = o.U(s.cell, i, 2);
+      double f = s.micro[j];
+      a[0].add(wt * f);
+      a[1].add(wt * u * f);
+      a[2].add(wt * v * f);
+      a[3].add(wt * ww * f);
+      a[4].add(wt * (u * u + v * v + ww * ww) * f);
+    }
+  }
+  for (int q = 0; q < o.dim + 1; ++q) fw[q] = a[q].get();
+  fw[o.dim + 1] = 0.5 * a[o.dim + 1].get();
+}
+
+// positivity_preserving_reconstruct, src/Flux/CAIDVM.jl:127-141 (limited), or the plain
+// df + dot(dx, sdf) reconstruction (unlimited); multiplied by the normal velocity.
+void reconstruct(orc_ctx& o, SideMicro& s, const double* xc, const double* dsc,
+                 const double* fmid, int dir, double dt, bool limited) {
+  int64_t m = int64_t(s.rows.size());
+  s.micro.assign(size_t(m * o.ndf), 0.0);
+  for (int64_t j = 0; j < m; ++j) {
+    int64_t i = s.rows[j];
+    double dx[3];
+    for (int k = 0; k < o.dim; ++k) dx[k] = fmid[k] - o.U(s.cell, i, k) * dt - xc[k];
+    double vn = o.U(s.cell, i, dir);
+    for (int k = 0; k < o.ndf; ++k) {
+      double f = o.F(s.cell, i, k);
+      double s_dx = 0.0;
+      for (int t = 0; t < o.dim; ++t) s_dx = (t == 0) ? dx[t] * o.S(s.cell, i, k, t)
+                                                      : s_dx + dx[t] * o.S(s.cell, i, k, t);
+      if (limited) {
+        double s_abs = 0.0;
+        for (int t = 0; t < o.dim; ++t)
+          s_abs = (t == 0) ? dsc[t] * std::fabs(o.S(s.cell, i, k, t))
+                           : s_abs + dsc[t] * std::fabs(o.S(s.cell, i, k, t));
+        double lim = jl_min(std::fabs((f - MACH_EPS) / (0.5 * s_abs + EPS)), 1.0);
+        s.micro[k * m + j] = (f + lim * s_dx) * vn;
+      } else {
+        s.micro[k * m + j] = (f + s_dx) * vn;
+      }
+    }
+  }
+}
+void plain_micro(orc_ctx& o, SideMicro& s, int dir) {   // df .* vn
+  int64_t m = int64_t(s.rows.size());
+  s.micro.assign(size_t(m * o.ndf), 0.0);
+  for (int64_t j = 0; j < m; ++j) {
+    int64_t i = s.rows[j];
+    double vn = o.U(s.cell, i, dir);
+    for (int k = 0; k < o.ndf; ++k) s.micro[k * m + j] = o.F(s.cell, i, k) * vn;
+  }
+}
+
+// update_micro_flux!, src/Flux/Flux.jl:151-281 (there local PsData, both branches)
+void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, double area,
+                       int64_t h, int64_t t, bool there_solid, const std::vector<char>& heavi) {
+  const int DIM = o.dim, NDF = o.ndf;
+  int64_t mh = int64_t(hs.rows.size()), mt = int64_t(ts.rows.size());
+  auto HM = [&](int64_t idx, int k) { return hs.micro[k * mh + idx] * area; };
+  auto TM = [&](int64_t idx, int k) { return ts.micro[k * mt + idx] * area; };
+  int64_t index = 0, j = 0, index_n = 0;
+  double flag = 0.0;
+  int64_t n = o.nv(h);
+  for (int64_t i = 0; i < n; ++i) {
+    int li = o.L(h, i);
+    if (heavi[i]) {
+      for (int k = 0; k < NDF; ++k) o.VF(h, i, k) += HM(index, k);
+      int lj = o.L(t, j);
+      if (li == lj) {
+        if (!there_solid)
+          for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= HM(index, k);
+        j += 1;
+      } else if (li < lj) {
+        while (flag != 1.0) {
+          int ljj = o.L(t, j);
+          if (!there_solid)
+            for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= HM(index, k);
+          flag += 1 / std::ldexp(1.0, DIM * (ljj - li));
+          j += 1;
+        }
+        flag = 0.0;
+      } else {
+        double den = std::ldexp(1.0, DIM * (li - lj));
+        if (!there_solid)
+          for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= HM(index, k) / den;
+        flag += 1 / den;
+        if (flag == 1.0) {
+          j += 1;
+          flag = 0.0;
+        }
+      }
+      index += 1;
+    } else {
+      int lj = o.L(t, j);
+      if (li == lj) {
+        for (int k = 0; k < NDF; ++k) {
+          if (!there_solid) o.VF(t, j, k) -= TM(index_n, k);
+          o.VF(h, i, k) += TM(index_n, k);
+        }
+        j += 1;
+        index_n += 1;
+      } else if (li < lj) {
+        while (flag != 1.0) {
+          int ljj = o.L(t, j);
+          double den = std::ldexp(1.0, DIM * (ljj - li));
+          for (int k = 0; k < NDF; ++k) {
+            if (!there_solid) o.VF(t, j, k) -= TM(index_n, k);
+            o.VF(h, i, k) += TM(index_n, k) / den;
+          }
+          flag += 1 / den;
+          j += 1;
+          index_n += 1;
+        }
+        flag = 0.0;
+      } else {
+        for (int k = 0; k < NDF; ++k) o.VF(h, i, k) += TM(index_n, k);
+        flag += 1 / std::ldexp(1.0, DIM * (li - lj));
+        if (flag == 1.0) {
+          if (!there_solid)
+            for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= TM(index_n, k);
+          j += 1;
+          index_n += 1;
+          flag = 0.0;
+        }
+      }
+    }
+  }
+}
+
+// calc_ρw, src/Theory/Math.jl:251-282 (wall density for zero net mass flux)
+double calc_rho_w(const orc_ctx& o, const SideMicro& here_rec, const SideMicro& there,
+                  const double* bc, int dir) {
+  int64_t mh = int64_t(here_rec.rows.size());
+  Neumaier SF, SG;
+  for (int64_t j = 0; j < mh; ++j) {
+    int64_t i = here_rec.rows[j];
+    SF.add(o.Wt(here_rec.cell, i) * o.U(here_rec.cell, i, dir) * here_rec.micro[j]);
+  }
+  for (int64_t i : there.rows) {
+    int64_t c = there.cell;
+    double c2 = 0.0;
+    for (int d = 0; d < o.dim; ++d) {
+      double cu = o.U(c, i, d) - bc[1 + d];
+      c2 = (d == 0) ? cu * cu : c2 + cu * cu;
+    }
+    SG.add(o.Wt(c, i) * o.U(c, i, dir) * std::exp(-bc[o.dim + 1] * c2));
+  }
+  double pref = (o.dim == 2) ? bc[3] / PI : std::pow(bc[4] / PI, 1.5);
+  return -SF.get() / (pref * SG.get());
+}
+
+// flux!(F, face, ka) for one DomainFace, src/Flux/Flux.jl:19-24 with CAIDVM.jl:4-94 /
+// DVM.jl:1-77, then update_domain_flux!, Flux.jl:64-83
+int domain_face(orc_ctx& o, int64_t p, double dt) {
+  const int DIM = o.dim, NDF = o.ndf, NW = o.NW();
+  int64_t h = o.ph[p];
+  int dir = o.pd[p];
+  double rot = o.prot[p], area = o.parea[p];
+  const double* fm = &o.pmid[p * DIM];
+  int kind = o.bck[o.pbc[p]];
+  std::vector<double> bc(&o.bcp[o.pbc[p] * NW], &o.bcp[o.pbc[p] * NW] + NW);
+  int64_t n = o.nv(h);
+  std::vector<char> heavi(static_cast<size_t>(n));
+  SideMicro hs, ts;
+  hs.cell = ts.cell = h;
+  for (int64_t i = 0; i < n; ++i) {
+    heavi[i] = (rot * o.U(h, i, dir) <= 0.0);
+    if (heavi[i]) hs.rows.push_back(i); else ts.rows.push_back(i);
+  }
+  const double* xh = o.rmid(h);
+  if (kind == KAH_BC_UNIFORM_OUTFLOW) {
+    plain_micro(o, hs, dir);
+    plain_micro(o, ts, dir);
+  } else if (kind == KAH_BC_MAXWELLIAN || kind == KAH_BC_SUPERSONIC_INFLOW) {
+    if (kind == KAH_BC_MAXWELLIAN && o.flux_kind == KAH_FLUX_DVM) {
+      // src/Flux/DVM.jl:12 reads here_weight/there_weight that are never defined
+      o.err = "UndefVarError: `here_weight` not defined (DVM Maxwellian domain flux)";
+      return KAH_EUNSUPPORTED;
+    }
+    reconstruct(o, hs, xh, o.rds(h), fm, dir, dt, false);  // (df + dot(dx,sdf)) * vn
+    if (kind == KAH_BC_MAXWELLIAN) {
+      // calc_ρw receives the reconstructed df (before * vn): divide the vn back out by
+      // recomputing it directly
+      SideMicro rec = hs;
+      int64_t mh = int64_t(hs.rows.size());
+      for (int64_t j = 0; j < mh; ++j) {
+        int64_t i = hs.rows[j];
+        double dx[3];
+        for (int k = 0; k < DIM; ++k) dx[k] = fm[k] - o.U(h, i, k) * dt - xh[k];
+        for (int k = 0; k < NDF; ++k) {
+          double s_dx = 0.0;
+          for (int t = 0; t < DIM; ++t) s_dx = (t == 0) ? dx[t] * o.S(h, i, k, t)
+                                                        : s_dx + dx[t] * o.S(h, i, k, t);
+          rec.micro[k * mh + j] = o.F(h, i, k) + s_dx;
+        }
+      }
+      bc[0] = calc_rho_w(o, rec, ts, bc.data(), dir);
+    }
+    int64_t mt = int64_t(ts.rows.size());
+    ts.micro.assign(size_t(mt * NDF), 0.0);
+    for (int64_t j = 0; j < mt; ++j) {
+      int64_t i = ts.rows[j];
+      double u[3] = {0, 0, 0}, M[2];
+      for (int d = 0; d < DIM; ++d) u[d] = o.U(h, i, d);
+      maxwell(DIM, u, bc.data(), o.K, M);
+      for (int k = 0; k < NDF; ++k) ts.micro[k * mt + j] = M[k] * u[dir];
+    }
+  } else if (kind == KAH_BC_INTERPOLATED_OUTFLOW) {
+    if (DIM != 2) {
+      o.err = "InterpolatedOutflow is defined for DIM=2 only (src/Flux/CAIDVM.jl:71)";
+      return KAH_EUNSUPPORTED;
+    }
+    double xt[3];
+    for (int d = 0; d < DIM; ++d) xt[d] = 2.0 * fm[d] - xh[d];
+    reconstruct(o, hs, xh, o.rds(h), fm, dir, dt, false);
+    int64_t mt = int64_t(ts.rows.size());
+    ts.micro.assign(size_t(mt * NDF), 0.0);
+    for (int64_t j = 0; j < mt; ++j) {
+      int64_t i = ts.rows[j];
+      double ndx[3];
+      for (int k = 0; k < DIM; ++k) ndx[k] = fm[k] - o.U(h, i, k) * dt - xt[k];
+      for (int k = 0; k < NDF; ++k) {
+        double tdf = o.F(h, i, k) + (xt[dir] - xh[dir]) * o.S(h, i, k, dir);
+        double s_dx = 0.0;
+        for (int t = 0; t < DIM; ++t) s_dx = (t == 0) ? ndx[t] * o.S(h, i, k, t)
+                                                      : s_dx + ndx[t] * o.S(h, i, k, t);
+        ts.micro[k * mt + j] = (tdf + s_dx) * o.U(h, i, dir);
+      }
+    }
+  } else {
+    o.err = "unknown domain boundary condition";
+    return KAH_EINVAL;
+  }
+  if (o.flux_kind == KAH_FLUX_CAIDVM) {
+    double a[5], b[5];
+    side_moments(o, hs, a);
+    side_moments(o, ts, b);
+    for (int q = 0; q < NW; ++q) o.flux[h * NW + q] += area * (a[q] + b[q]);
+  }
+  int64_t mh = int64_t(hs.rows.size()), mt = int64_t(ts.rows.size());
+  for (int64_t j = 0; j < mh; ++j)
+    for (int k = 0; k < NDF; ++k) o.VF(h, hs.rows[j], k) += area * hs.micro[k * mh + j];
+  for (int64_t j = 0; j < mt; ++j)
+    for (int k = 0; k < NDF; ++k) o.VF(h, ts.rows[j], k) += area * ts.micro[k * mt + j];
+  return 0;
+}
+
+// flux!(F, face::FullFace / FluxData, ka): make_face_vs (Flux.jl:362-424), calc_flux
+// (CAIDVM.jl:99-120 / DVM.jl:79-99), update_flux! (Flux.jl:94-111)
+int inner_face(orc_ctx& o, int64_t p, double dt) {
+  const int DIM = o.dim, NW = o.NW();
+  int64_t h = o.ph[p], trec = o.pt[p];
+  int64_t t = o.src(trec);
+  int dir = o.pd[p];
+  double rot = o.prot[p], area = o.parea[p];
+  const double* fm = &o.pmid[p * DIM];
+  bool there_solid = o.benc[t] < 0;
+  int64_t nh = o.nv(h), nt = o.nv(t);
+  std::vector<char> heavi(static_cast<size_t>(nh));
+  SideMicro hs, ts;
+  hs.cell = h;
+  ts.cell = t;
+  for (int64_t i = 0; i < nh; ++i) {
+    heavi[i] = (rot * o.U(h, i, dir) <= 0.0);
+    if (heavi[i]) hs.rows.push_back(i);
+  }
+  for (int64_t j = 0; j < nt; ++j)
+    if (rot * o.U(t, j, dir) > 0.0) ts.rows.push_back(j);
+  if (there_solid) {
+    reconstruct(o, hs, o.rmid(h), o.rds(h), fm, dir, dt, false);
+    if (o.flux_kind == KAH_FLUX_CAIDVM)
+      plain_micro(o, ts, dir);
+    else
+      reconstruct(o, ts, o.rmid(trec), o.rds(trec), fm, dir, dt, false);
+  } else {
+    reconstruct(o, hs, o.rmid(h), o.rds(h), fm, dir, dt, true);
+    reconstruct(o, ts, o.rmid(trec), o.rds(trec), fm, dir, dt, true);
+  }
+  if (o.flux_kind == KAH_FLUX_CAIDVM) {
+    double a[5], b[5], fw[5];
+    side_moments(o, hs, a);
+    side_moments(o, ts, b);
+    for (int q = 0; q < NW; ++q) fw[q] = (a[q] + b[q]) * area;
+    for (int q = 0; q < NW; ++q) o.flux[h * NW + q] += fw[q];
+    if (!there_solid)
+      for (int q = 0; q < NW; ++q) o.flux[t * NW + q] -= fw[q];
+  }
+  update_micro_flux(o, hs, ts, area, h, t, there_solid, heavi);
+  return 0;
+}
+
+}  // namespace
+
+// ===================================================================== C API
+extern "C" {
+
+orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
+  if (!P || !T) return nullptr;
+  orc_ctx* o = new orc_ctx();
+  o->dim = P->dim;
+  o->ndf = P->ndf;
+  o->flux_kind = P->flux_kind;
+  o->marching = P->marching;
+  o->st_check = P->st_check_interval > 0 ? P->st_check_interval : 100;
+  o->K = P->K;
+  o->gamma = P->gamma;
+  o->Pr = P->Pr;
+  o->mu = P->mu_ref;
+  o->omega = P->omega;
+  const int D = o->dim;
+  o->nc = T->n_cells;
+  o->ni = T->n_images;
+  int64_t nr = o->nc + o->ni;
+  copy_in(o->image_src, T->image_src, o->ni);
+  copy_in(o->mid, T->cell_mid, nr * D);
+  copy_in(o->ds, T->cell_ds, nr * D);
+  copy_in(o->benc, T->bound_enc, o->nc);
+  copy_in(o->lvl, T->ps_level, o->nc);
+  copy_in(o->voff, T->vs_offset, o->nc + 1);
+  o->np = o->voff[o->nc];
+  o->vs_uniform = T->vs_uniform != 0;
+  int64_t ng = o->vs_uniform ? (o->nc > 0 ? o->voff[1] : 0) : o->np;
+  if (o->vs_uniform)
+    for (int64_t c = 0; c <= o->nc; ++c)
+      if (o->voff[c] != c * ng) { delete o; return nullptr; }
+  copy_in(o->vlev, T->vs_level, ng);
+  copy_in(o->vmid, T->vs_mid, ng * D);
+  copy_in(o->vw, T->vs_weight, ng);
+  copy_in(o->nbs, T->nb_state, o->nc * 2 * D);
+  copy_in(o->nboff, T->nb_offset, o->nc * 2 * D + 1);
+  copy_in(o->nbc, T->nb_cell, o->nboff[o->nc * 2 * D]);
+  int64_t npair = T->n_pairs;
+  copy_in(o->ph, T->pair_here, npair);
+  copy_in(o->pt, T->pair_there, npair);
+  copy_in(o->pd, T->pair_dir, npair);
+  copy_in(o->prot, T->pair_rot, npair);
+  copy_in(o->parea, T->pair_area, npair);
+  copy_in(o->pmid, T->pair_mid, npair * D);
+  copy_in(o->pbc, T->pair_bc, npair);
+  copy_in(o->bck, T->bc_kind, T->n_bc);
+  copy_in(o->bcp, T->bc_prim, int64_t(T->n_bc) * (D + 2));
+  int NW = D + 2;
+  o->w.assign(size_t(o->nc * NW), 0.0);
+  o->prim.assign(size_t(o->nc * NW), 0.0);
+  o->qf.assign(size_t(o->nc * D), 0.0);
+  o->sw.assign(size_t(o->nc * NW * D), 0.0);
+  o->flux.assign(size_t(o->nc * NW), 0.0);
+  o->df.assign(size_t(o->np * o->ndf), 0.0);
+  o->sdf.assign(size_t(o->np * o->ndf * D), 0.0);
+  o->vflux.assign(size_t(o->np * o->ndf), 0.0);
+  return o;
+}
+
+void orc_destroy(orc_ctx* o) { delete o; }
+const char* orc_last_error(const orc_ctx* o) { return o ? o->err.c_str() : "null context"; }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int orc_set_state(orc_ctx* o, const double* w, const double* prim, const double* df) {
+  int NW = o->NW();
+  if (w) std::memcpy(o->w.data(), w, sizeof(double) * o->nc * NW);
+  if (prim) std::memcpy(o->prim.data(), prim, sizeof(double) * o->nc * NW);
+  if (df) std::memcpy(o->df.data(), df, sizeof(double) * o->np * o->ndf);
+  std::fill(o->flux.begin(), o->flux.end(), 0.0);
+  std::fill(o->vflux.begin(), o->vflux.end(), 0.0);
+  return 0;
+}
+
+int orc_get_state(orc_ctx* o, double* w, double* prim, double* qf, double* sw, double* df,
+                  double* sdf) {
+  int NW = o->NW();
+  if (w) std::memcpy(w, o->w.data(), sizeof(double) * o->nc * NW);
+  if (prim) std::memcpy(prim, o->prim.data(), sizeof(double) * o->nc * NW);
+  if (qf) std::memcpy(qf, o->qf.data(), sizeof(double) * o->nc * o->dim);
+  if (sw) std::memcpy(sw, o->sw.data(), sizeof(double) * o->nc * NW * o->dim);
+  if (df) std::memcpy(df, o->df.data(), sizeof(double) * o->np * o->ndf);
+  if (sdf) std::memcpy(sdf, o->sdf.data(), sizeof(double) * o->np * o->ndf * o->dim);
+  return 0;
+}
+
+int orc_get_flux(orc_ctx* o, double* flux, double* vs_flux) {
+  if (flux) std::memcpy(flux, o->flux.data(), sizeof(double) * o->nc * o->NW());
+  if (vs_flux) std::memcpy(vs_flux, o->vflux.data(), sizeof(double) * o->np * o->ndf);
+  return 0;
+}
+
+// slope!(p4est,ka), src/Flux/Slope.jl:1047-1070
+int orc_slope(orc_ctx* o) {
+  int Lmin = 1 << 30, Lmax = -1;
+  for (int64_t c = 0; c < o->nc; ++c) {
+    if (o->benc[c] < 0) continue;
+    Lmin = std::min(Lmin, int(o->lvl[c]));
+    Lmax = std::max(Lmax, int(o->lvl[c]));
+  }
+  if (Lmax < 0) return 0;
+  int rc_all = 0;
+  for (int L = Lmin; L <= Lmax; ++L) {
+#pragma omp parallel
+    {
+      SlopeWs ws;
+#pragma omp for schedule(dynamic, 16)
+      for (int64_t c = 0; c < o->nc; ++c) {
+        if (o->benc[c] < 0 || o->lvl[c] != L) continue;
+        int rc = 0;
+        if (L == Lmin) {   // update_slope_level!, Slope.jl:977-1019
+          for (int dir = 0; dir < o->dim && !rc; ++dir) rc = slope_dispatch(*o, c, dir, ws);
+        } else {           // update_slope_transverse_level!, Slope.jl:849-945
+          rc = slope_transverse_cell(*o, c, ws);
+        }
+        if (rc) {
+#pragma omp critical
+          rc_all = rc;
+        }
+      }
+    }
+    if (rc_all) return rc_all;
+  }
+  // update_macro_slope!, Slope.jl:1022-1036
+  int NW = o->NW();
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t c = 0; c < o->nc; ++c) {
+    if (o->benc[c] < 0) continue;
+    int64_t n = o->nv(c);
+    for (int d = 0; d < o->dim; ++d) {
+      double wv[5];
+      moments_full(*o, c, &o->sdf[o->off(c) * o->ndf * o->dim + int64_t(d) * o->ndf * n], n, wv);
+      for (int q = 0; q < NW; ++q) o->sw[(c * NW + q) * o->dim + d] = wv[q];
+    }
+  }
+  return 0;
+}
+
+// flux!(p4est,ka), src/Flux/Flux.jl:458-495 (no immersed boundaries): faces in order
+int orc_flux(orc_ctx* o, double dt) {
+  int64_t npair = int64_t(o->ph.size());
+  for (int64_t p = 0; p < npair; ++p) {
+    int rc = (o->pt[p] < 0) ? domain_face(*o, p, dt) : inner_face(*o, p, dt);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// iterate!(p4est,ka) with CAIDVM_Marching (src/Theory/Iterate.jl:96-130) or Euler
+// (:131-162); residual_check! (src/Solver/Finalize.jl:5-11)
+int orc_iterate(orc_ctx* o, double dt, int64_t step, double* sum_res, double* sum_avg) {
+  const int D = o->dim, NDF = o->ndf, NW = o->NW();
+  bool check = (step % o->st_check) == 0;
+  std::vector<double> res(size_t(o->nc * NW), 0.0), avg(size_t(o->nc * NW), 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> fc, Fm;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t c = 0; c < o->nc; ++c) {
+      if (o->benc[c] < 0) continue;
+      int64_t n = o->nv(c);
+      double* w = &o->w[c * NW];
+      double* fl = &o->flux[c * NW];
+      const double* dsc = o->rds(c);
+      double area = dsc[0];
+      for (int d = 1; d < D; ++d) area = area * dsc[d];
+      double* f = &o->df[o->off(c) * NDF];
+      double* vfl = &o->vflux[o->off(c) * NDF];
+      double prim_new[5];
+      if (o->marching == KAH_MARCH_CAIDVM) {
+        for (int q = 0; q < NW; ++q) w[q] += fl[q] * dt / area;
+        double prim_c[5];
+        get_prim(D, w, o->gamma, prim_c);
+        for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / area * vfl[x];
+        double wf[5], prim[5];
+        moments_full(*o, c, f, n, wf);
+        get_prim(D, wf, o->gamma, prim);
+        double tau = get_tau(D, prim_c, o->mu, o->omega);
+        fc.assign(size_t(n * NDF), 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+          double u[3] = {0, 0, 0}, Mc[2], M[2];
+          for (int d = 0; d < D; ++d) u[d] = o->U(c, i, d);
+          maxwell(D, u, prim_c, o->K, Mc);
+          maxwell(D, u, prim, o->K, M);
+          for (int k = 0; k < NDF; ++k) {
+            fc[k * n + i] = Mc[k];
+            f[k * n + i] += Mc[k] - M[k];
+          }
+        }
+        double q[3];
+        heat_flux_full(*o, c, f, n, prim_c, q);
+        for (int d = 0; d < D; ++d) o->qf[c * D + d] = q[d];
+        for (int64_t i = 0; i < n; ++i) {
+          double u[3] = {0, 0, 0}, Fc[2], Sp[2];
+          for (int d = 0; d < D; ++d) u[d] = o->U(c, i, d);
+          for (int k = 0; k < NDF; ++k) Fc[k] = fc[k * n + i];
+          shakhov(D, u, Fc, prim_c, q, o->Pr, o->K, Sp);
+          for (int k = 0; k < NDF; ++k) fc[k * n + i] = Fc[k] + Sp[k];
+        }
+        for (int64_t x = 0; x < n * NDF; ++x) f[x] *= tau / (tau + dt);
+        for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / (tau + dt) * fc[x];
+        for (int q2 = 0; q2 < NW; ++q2) prim_new[q2] = prim_c[q2];
+      } else {
+        // Euler: `isa(flux_type, MicroFlux)` is always false, so w always takes the
+        // macroscopic flux (Iterate.jl:144-149)
+        for (int q2 = 0; q2 < NW; ++q2) w[q2] += fl[q2] * dt / area;
+        for (int q2 = 0; q2 < NW; ++q2) fl[q2] = 0.0;
+        double prim[5];
+        get_prim(D, w, o->gamma, prim);
+        double tau = get_tau(D, prim, o->mu, o->omega);
+        double q[3];
+        heat_flux_full(*o, c, f, n, prim, q);
+        for (int d = 0; d < D; ++d) o->qf[c * D + d] = q[d];
+        for (int64_t i = 0; i < n; ++i) {
+          double u[3] = {0, 0, 0}, M[2], Sp[2];
+          for (int d = 0; d < D; ++d) u[d] = o->U(c, i, d);
+          maxwell(D, u, prim, o->K, M);
+          shakhov(D, u, M, prim, q, o->Pr, o->K, Sp);
+          for (int k = 0; k < NDF; ++k) {
+            double Fk = M[k] + Sp[k];
+            double& fx = f[k * n + i];
+            fx = (fx + dt / area * vfl[k * n + i]) * tau / (tau + dt) + dt / (tau + dt) * Fk;
+          }
+        }
+        for (int q2 = 0; q2 < NW; ++q2) prim_new[q2] = prim[q2];
+      }
+      if (check) {
+        for (int q2 = 0; q2 < NW; ++q2) {
+          double dlt = prim_new[q2] - o->prim[c * NW + q2];
+          res[c * NW + q2] = dlt * dlt;
+          avg[c * NW + q2] = std::fabs(prim_new[q2]);
+        }
+      }
+      for (int q2 = 0; q2 < NW; ++q2) o->prim[c * NW + q2] = prim_new[q2];
+      for (int q2 = 0; q2 < NW; ++q2) fl[q2] = 0.0;
+      for (int64_t x = 0; x < n * NDF; ++x) vfl[x] = 0.0;
+    }
+  }
+  // residual sums in cell order (Finalize.jl:5-11)
+  for (int q2 = 0; q2 < NW; ++q2) {
+    double sr = 0.0, sa = 0.0;
+    if (check)
+      for (int64_t c = 0; c < o->nc; ++c) {
+        if (o->benc[c] < 0) continue;
+        sr += res[c * NW + q2];
+        sa += avg[c * NW + q2];
+      }
+    if (sum_res) sum_res[q2] = sr;
+    if (sum_avg) sum_avg[q2] = sa;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- point kernels (tests)
+void orc_discrete_maxwell(int dim, int ndf, int64_t n, const double* vs_mid,
+                          const double* prim, double K, double* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double u[3] = {0, 0, 0}, M[2];
+    for (int d = 0; d < dim; ++d) u[d] = vs_mid[d * n + i];
+    maxwell(dim, u, prim, K, M);
+    for (int k = 0; k < ndf; ++k) out[k * n + i] = M[k];
+  }
+}
+void orc_micro_to_macro(int dim, int ndf, int64_t n, const double* vs_mid, const double* df,
+                        const double* weight, double* w_out) {
+  orc_ctx o;
+  o.dim = dim;
+  o.ndf = ndf;
+  o.voff = {0, n};
+  o.vmid.assign(vs_mid, vs_mid + n * dim);
+  o.vw.assign(weight, weight + n);
+  moments_full(o, 0, df, n, w_out);
+}
+void orc_heat_flux(int dim, int ndf, int64_t n, const double* vs_mid, const double* df,
+                   const double* weight, const double* prim, double* q_out) {
+  orc_ctx o;
+  o.dim = dim;
+  o.ndf = ndf;
+  o.voff = {0, n};
+  o.vmid.assign(vs_mid, vs_mid + n * dim);
+  o.vw.assign(weight, weight + n);
+  heat_flux_full(o, 0, df, n, prim, q_out);
+}
+void orc_shakhov_part(int dim, int ndf, int64_t n, const double* vs_mid, const double* F,
+                      const double* prim, const double* qf, double Pr, double K,
+                      double* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double u[3] = {0, 0, 0}, Fi[2], S[2];
+    for (int d = 0; d < dim; ++d) u[d] = vs_mid[d * n + i];
+    for (int k = 0; k < ndf; ++k) Fi[k] = F[k * n + i];
+    shakhov(dim, u, Fi, prim, qf, Pr, K, S);
+    for (int k = 0; k < ndf; ++k) out[k * n + i] = S[k];
+  }
+}
+void orc_get_prim(int dim, const double* w, double gamma, double* prim) {
+  get_prim(dim, w, gamma, prim);
+}
+void orc_get_conserved(int dim, const double* prim, double gamma, double* w) {
+  get_conserved(dim, prim, gamma, w);
+}
+double orc_get_tau(int dim, const double* prim, double mu, double omega) {
+  return get_tau(dim, prim, mu, omega);
+}
+// ref_vhs_vis, src/Gas/Model.jl:5-9
+double orc_ref_vhs_vis(double Kn, double omega, double alpha, double T_ref) {
+  double mu0 = 5.0 * (alpha + 1.0) * (alpha + 2.0) * std::sqrt(PI) /
+               (4.0 * alpha * (5.0 - 2.0 * omega) * (7.0 - 2.0 * omega)) * Kn;
+  return mu0 * std::pow(T_ref, 0.5 - omega);
+}
+
+}  // extern "C"
