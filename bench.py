@@ -1,0 +1,180 @@
+"""Benchmark of the per-step phase-space update on one MI355X.
+
+Workload: 2D four-quadrant Riemann problem on [0,1]^2, 64x64 base quadtree cells with
+static refinement to 3 levels toward the initial density interfaces (49,312 leaves after
+FULL 2:1 balance), uniform 48x48 velocity grid on [-5,5]^2, FP64, CAIDVM flux +
+CAIDVM_Marching, mesh frozen during timing. One step = slope + flux + relaxation over all
+phase-space cells (113,614,848 updates).
+
+Prints one JSON line (rank 0). The CPU baseline is the oracle (a C++ restatement of the
+reference algorithm, OpenMP over cells and faces) timed on this host on whole steps of the
+same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "kitamr.jl_amd"))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per phase-space update of the reference's 3-phase dataflow, FP64 2D2F,
+# shared velocity grid: slope R df + W sdf; flux R df, R sdf, W flux; iterate R df, R flux, W df
+B_SLOPE = 2 * 8 * (1 + 2)          # 48
+B_FLUX_ITER = 2 * 8 * (1 + 2 + 1) + 2 * 8 * 3   # 64 + 48 = 112
+B_STEP = B_SLOPE + B_FLUX_ITER     # 160
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def torch_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(case, max_steps=5, min_seconds=10.0):
+    import numpy as np
+    import oracle as O
+    from kitamr_hip.solver import kah_params
+    threads = O.set_threads(0)
+    orc = O.Oracle(kah_params(case.config), case.topo)
+    orc.set_state(case.w, case.prim, case.df)
+    times = []
+    t_all = time.perf_counter()
+    for s in range(max_steps):
+        t0 = time.perf_counter()
+        orc.step(case.dt_xi, s)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_all >= min_seconds:
+            break
+    orc.close()
+    med = float(np.median(times))
+    return {"value": case.phase_cells / med, "unit": "updates/s", "cores": int(threads),
+            "kind": "port",
+            "sample": f"{len(times)} full steps of the same workload on the oracle "
+                      f"(C++ FP64 restatement, OpenMP {threads} threads), median step "
+                      f"{med:.3f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--small", action="store_true", help="reduced mesh (profiling/debug only)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by physical-space domain decomposition, not run here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from kitamr_hip import abi, cases
+    from kitamr_hip.solver import kah_params
+
+    t0 = time.perf_counter()
+    if args.small:
+        case = cases.riemann_2d(n=32, nv=48, ps_levels=2, halo=(4.0, 2.0))
+    else:
+        case = cases.riemann_2d(n=64, nv=48, ps_levels=3)
+    P = case.phase_cells
+    log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
+        f"pairs={case.topo['n_pairs']} P={P}")
+    dev = abi.HipContext(kah_params(case.config))
+    dev.set_topology(case.topo)
+    dev.set_state(case.w, case.prim, case.df)
+    dt = case.dt_xi
+    step = 0
+    for _ in range(args.warmup):
+        dev.step(dt, step, residual=False)
+        step += 1
+    dev.sync()
+    log(f"[bench] warmup {args.warmup} steps done; device bytes {dev.device_bytes() / 1e9:.2f} GB")
+
+    dev.kernel_times(reset=True)       # start HIP-event timing of the kernels
+    torch_sync()
+    dev.sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        dev.step(dt, step, residual=False)
+        step += 1
+    dev.sync()
+    torch_sync()
+    elapsed = time.perf_counter() - t_start
+    ms, calls = dev.kernel_times()
+    log(f"[bench] timed {args.steps} steps: {elapsed:.3f}s; kernel ms slope={ms[0]:.3f} "
+        f"flux+relax={ms[1]:.3f}")
+
+    st = dev.get_state(df=False, qf=False)
+    w = st["w"].reshape(-1, 4)
+    finite = bool(np.isfinite(w).all())
+    dev.close()
+
+    ms_step = 1e3 * elapsed / args.steps
+    value = P * args.steps / elapsed
+    kr_ms = ms[1]
+    achieved = B_FLUX_ITER * P / (kr_ms * 1e-3) / 1e9
+    slope_gbs = B_SLOPE * P / (ms[0] * 1e-3) / 1e9 if ms[0] > 0 else None
+    step_gbs = B_STEP * value / 1e9
+    out = {
+        "metric": "phase-space updates/s + % HBM roofline (1 GPU)",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
+                         "uniform 48x48 velocity grid, FP64, CAIDVM + CAIDVM_Marching"
+                         if not args.small else "reduced 32x32 + 2 levels (debug)"),
+            "leaves": case.n_cells,
+            "phase_cells_per_step": P,
+            "parallelism": "single GPU",
+        },
+        "hbm_roofline_pct": 100.0 * step_gbs / HBM_PEAK_GBS,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_flux_relax (fused flux + relaxation)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "alg_bytes_per_update": B_FLUX_ITER,
+            "avg_launch_ms": kr_ms,
+        },
+        "slope_kernel": {"avg_ms_per_step": ms[0], "alg_bytes_per_update": B_SLOPE,
+                         "achieved_GBs": slope_gbs},
+        "step_alg_bytes_per_update": B_STEP,
+        "state_finite": finite,
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(case, max_steps=args.cpu_steps)
+        log(f"[bench] cpu baseline {out['cpu_baseline']['value'] / 1e6:.1f} M updates/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

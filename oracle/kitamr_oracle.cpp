@@ -71,6 +71,7 @@ struct orc_ctx {
   std::vector<double> prot, parea, pmid;
   std::vector<int32_t> bck;
   std::vector<double> bcp;
+  std::vector<std::vector<int64_t>> cell_pairs;   // per cell: pair*2 + side, in pair order
   // state
   std::vector<double> w, prim, qf, sw, flux;
   std::vector<double> df, sdf, vflux;
@@ -568,8 +569,11 @@ void plain_micro(orc_ctx& o, SideMicro& s, int dir) {   // df .* vn
 
 // update_micro_flux!, src/Flux/Flux.jl:151-281 (there local PsData, both branches)
 void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, double area,
-                       int64_t h, int64_t t, bool there_solid, const std::vector<char>& heavi) {
+                       int64_t h, int64_t t, bool there_solid, const std::vector<char>& heavi,
+                       bool do_h, bool do_t) {
+  // do_h / do_t select which side's accumulators this call updates; the walk is the same.
   const int DIM = o.dim, NDF = o.ndf;
+  if (there_solid) do_t = false;
   int64_t mh = int64_t(hs.rows.size()), mt = int64_t(ts.rows.size());
   auto HM = [&](int64_t idx, int k) { return hs.micro[k * mh + idx] * area; };
   auto TM = [&](int64_t idx, int k) { return ts.micro[k * mt + idx] * area; };
@@ -579,16 +583,16 @@ void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, dou
   for (int64_t i = 0; i < n; ++i) {
     int li = o.L(h, i);
     if (heavi[i]) {
-      for (int k = 0; k < NDF; ++k) o.VF(h, i, k) += HM(index, k);
+      for (int k = 0; k < NDF; ++k) if (do_h) o.VF(h, i, k) += HM(index, k);
       int lj = o.L(t, j);
       if (li == lj) {
-        if (!there_solid)
+        if (do_t)
           for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= HM(index, k);
         j += 1;
       } else if (li < lj) {
         while (flag != 1.0) {
           int ljj = o.L(t, j);
-          if (!there_solid)
+          if (do_t)
             for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= HM(index, k);
           flag += 1 / std::ldexp(1.0, DIM * (ljj - li));
           j += 1;
@@ -596,7 +600,7 @@ void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, dou
         flag = 0.0;
       } else {
         double den = std::ldexp(1.0, DIM * (li - lj));
-        if (!there_solid)
+        if (do_t)
           for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= HM(index, k) / den;
         flag += 1 / den;
         if (flag == 1.0) {
@@ -609,8 +613,8 @@ void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, dou
       int lj = o.L(t, j);
       if (li == lj) {
         for (int k = 0; k < NDF; ++k) {
-          if (!there_solid) o.VF(t, j, k) -= TM(index_n, k);
-          o.VF(h, i, k) += TM(index_n, k);
+          if (do_t) o.VF(t, j, k) -= TM(index_n, k);
+          if (do_h) o.VF(h, i, k) += TM(index_n, k);
         }
         j += 1;
         index_n += 1;
@@ -619,8 +623,8 @@ void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, dou
           int ljj = o.L(t, j);
           double den = std::ldexp(1.0, DIM * (ljj - li));
           for (int k = 0; k < NDF; ++k) {
-            if (!there_solid) o.VF(t, j, k) -= TM(index_n, k);
-            o.VF(h, i, k) += TM(index_n, k) / den;
+            if (do_t) o.VF(t, j, k) -= TM(index_n, k);
+            if (do_h) o.VF(h, i, k) += TM(index_n, k) / den;
           }
           flag += 1 / den;
           j += 1;
@@ -628,10 +632,10 @@ void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, dou
         }
         flag = 0.0;
       } else {
-        for (int k = 0; k < NDF; ++k) o.VF(h, i, k) += TM(index_n, k);
+        for (int k = 0; k < NDF; ++k) if (do_h) o.VF(h, i, k) += TM(index_n, k);
         flag += 1 / std::ldexp(1.0, DIM * (li - lj));
         if (flag == 1.0) {
-          if (!there_solid)
+          if (do_t)
             for (int k = 0; k < NDF; ++k) o.VF(t, j, k) -= TM(index_n, k);
           j += 1;
           index_n += 1;
@@ -664,9 +668,21 @@ double calc_rho_w(const orc_ctx& o, const SideMicro& here_rec, const SideMicro& 
   return -SF.get() / (pref * SG.get());
 }
 
+// The work of one face: the two upwind micro-flux sets and the macroscopic increment,
+// computed independently of every other face and then applied to each side's
+// accumulators (update_flux!/update_domain_flux!) in face-list order.
+struct FaceWork {
+  SideMicro hs, ts;
+  std::vector<char> heavi;
+  double fw[5] = {0, 0, 0, 0, 0};
+  bool has_fw = false, domain = false, there_solid = false;
+  int64_t h = -1, t = -1;
+  double area = 0.0;
+};
+
 // flux!(F, face, ka) for one DomainFace, src/Flux/Flux.jl:19-24 with CAIDVM.jl:4-94 /
-// DVM.jl:1-77, then update_domain_flux!, Flux.jl:64-83
-int domain_face(orc_ctx& o, int64_t p, double dt) {
+// DVM.jl:1-77 (the update_domain_flux! part, Flux.jl:64-83, is apply_face)
+int domain_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
   const int DIM = o.dim, NDF = o.ndf, NW = o.NW();
   int64_t h = o.ph[p];
   int dir = o.pd[p];
@@ -675,8 +691,13 @@ int domain_face(orc_ctx& o, int64_t p, double dt) {
   int kind = o.bck[o.pbc[p]];
   std::vector<double> bc(&o.bcp[o.pbc[p] * NW], &o.bcp[o.pbc[p] * NW] + NW);
   int64_t n = o.nv(h);
-  std::vector<char> heavi(static_cast<size_t>(n));
-  SideMicro hs, ts;
+  W.domain = true;
+  W.h = h;
+  W.area = area;
+  std::vector<char>& heavi = W.heavi;
+  heavi.assign(static_cast<size_t>(n), 0);
+  SideMicro& hs = W.hs;
+  SideMicro& ts = W.ts;
   hs.cell = ts.cell = h;
   for (int64_t i = 0; i < n; ++i) {
     heavi[i] = (rot * o.U(h, i, dir) <= 0.0);
@@ -750,19 +771,15 @@ int domain_face(orc_ctx& o, int64_t p, double dt) {
     double a[5], b[5];
     side_moments(o, hs, a);
     side_moments(o, ts, b);
-    for (int q = 0; q < NW; ++q) o.flux[h * NW + q] += area * (a[q] + b[q]);
+    for (int q = 0; q < NW; ++q) W.fw[q] = area * (a[q] + b[q]);
+    W.has_fw = true;
   }
-  int64_t mh = int64_t(hs.rows.size()), mt = int64_t(ts.rows.size());
-  for (int64_t j = 0; j < mh; ++j)
-    for (int k = 0; k < NDF; ++k) o.VF(h, hs.rows[j], k) += area * hs.micro[k * mh + j];
-  for (int64_t j = 0; j < mt; ++j)
-    for (int k = 0; k < NDF; ++k) o.VF(h, ts.rows[j], k) += area * ts.micro[k * mt + j];
   return 0;
 }
 
 // flux!(F, face::FullFace / FluxData, ka): make_face_vs (Flux.jl:362-424), calc_flux
 // (CAIDVM.jl:99-120 / DVM.jl:79-99), update_flux! (Flux.jl:94-111)
-int inner_face(orc_ctx& o, int64_t p, double dt) {
+int inner_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
   const int DIM = o.dim, NW = o.NW();
   int64_t h = o.ph[p], trec = o.pt[p];
   int64_t t = o.src(trec);
@@ -771,8 +788,14 @@ int inner_face(orc_ctx& o, int64_t p, double dt) {
   const double* fm = &o.pmid[p * DIM];
   bool there_solid = o.benc[t] < 0;
   int64_t nh = o.nv(h), nt = o.nv(t);
-  std::vector<char> heavi(static_cast<size_t>(nh));
-  SideMicro hs, ts;
+  W.h = h;
+  W.t = t;
+  W.area = area;
+  W.there_solid = there_solid;
+  std::vector<char>& heavi = W.heavi;
+  heavi.assign(static_cast<size_t>(nh), 0);
+  SideMicro& hs = W.hs;
+  SideMicro& ts = W.ts;
   hs.cell = h;
   ts.cell = t;
   for (int64_t i = 0; i < nh; ++i) {
@@ -792,16 +815,37 @@ int inner_face(orc_ctx& o, int64_t p, double dt) {
     reconstruct(o, ts, o.rmid(trec), o.rds(trec), fm, dir, dt, true);
   }
   if (o.flux_kind == KAH_FLUX_CAIDVM) {
-    double a[5], b[5], fw[5];
+    double a[5], b[5];
     side_moments(o, hs, a);
     side_moments(o, ts, b);
-    for (int q = 0; q < NW; ++q) fw[q] = (a[q] + b[q]) * area;
-    for (int q = 0; q < NW; ++q) o.flux[h * NW + q] += fw[q];
-    if (!there_solid)
-      for (int q = 0; q < NW; ++q) o.flux[t * NW + q] -= fw[q];
+    for (int q = 0; q < NW; ++q) W.fw[q] = (a[q] + b[q]) * area;
+    W.has_fw = true;
   }
-  update_micro_flux(o, hs, ts, area, h, t, there_solid, heavi);
   return 0;
+}
+
+// update_flux! / update_domain_flux! (src/Flux/Flux.jl:64-139) restricted to one side of the
+// face: do_h updates the here cell, do_t the there cell.
+void apply_face(orc_ctx& o, const FaceWork& W, bool do_h, bool do_t) {
+  const int NW = o.NW(), NDF = o.ndf;
+  if (W.domain) {
+    if (!do_h) return;
+    if (W.has_fw)
+      for (int q = 0; q < NW; ++q) o.flux[W.h * NW + q] += W.fw[q];
+    int64_t mh = int64_t(W.hs.rows.size()), mt = int64_t(W.ts.rows.size());
+    for (int64_t j = 0; j < mh; ++j)
+      for (int k = 0; k < NDF; ++k) o.VF(W.h, W.hs.rows[j], k) += W.area * W.hs.micro[k * mh + j];
+    for (int64_t j = 0; j < mt; ++j)
+      for (int k = 0; k < NDF; ++k) o.VF(W.h, W.ts.rows[j], k) += W.area * W.ts.micro[k * mt + j];
+    return;
+  }
+  if (W.has_fw) {
+    if (do_h)
+      for (int q = 0; q < NW; ++q) o.flux[W.h * NW + q] += W.fw[q];
+    if (do_t && !W.there_solid)
+      for (int q = 0; q < NW; ++q) o.flux[W.t * NW + q] -= W.fw[q];
+  }
+  update_micro_flux(o, W.hs, W.ts, W.area, W.h, W.t, W.there_solid, W.heavi, do_h, do_t);
 }
 
 }  // namespace
@@ -953,12 +997,48 @@ int orc_slope(orc_ctx* o) {
   return 0;
 }
 
-// flux!(p4est,ka), src/Flux/Flux.jl:458-495 (no immersed boundaries): faces in order
+// flux!(p4est,ka), src/Flux/Flux.jl:458-495 (no immersed boundaries). Faces are computed in
+// parallel; each cell then applies its side of its faces in face-list order, which gives
+// every accumulator the same sequence of additions as the reference's serial face loop.
 int orc_flux(orc_ctx* o, double dt) {
-  int64_t npair = int64_t(o->ph.size());
-  for (int64_t p = 0; p < npair; ++p) {
-    int rc = (o->pt[p] < 0) ? domain_face(*o, p, dt) : inner_face(*o, p, dt);
-    if (rc) return rc;
+  const int64_t npair = int64_t(o->ph.size());
+  if (o->cell_pairs.empty() && o->nc > 0) {
+    o->cell_pairs.assign(size_t(o->nc), {});
+    for (int64_t p = 0; p < npair; ++p) {
+      o->cell_pairs[o->ph[p]].push_back(p * 2);
+      if (o->pt[p] >= 0) {
+        int64_t t = o->src(o->pt[p]);
+        if (o->benc[t] >= 0 && t != o->ph[p]) o->cell_pairs[t].push_back(p * 2 + 1);
+      }
+    }
+  }
+  std::vector<size_t> cursor(size_t(o->nc), 0);
+  const int64_t CH = 4096;
+  int rc_all = 0;
+  for (int64_t p0 = 0; p0 < npair; p0 += CH) {
+    const int64_t p1 = std::min(npair, p0 + CH);
+    std::vector<FaceWork> work(size_t(p1 - p0));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t p = p0; p < p1; ++p) {
+      int rc = (o->pt[p] < 0) ? domain_face(*o, p, dt, work[p - p0])
+                              : inner_face(*o, p, dt, work[p - p0]);
+      if (rc) {
+#pragma omp critical
+        rc_all = rc;
+      }
+    }
+    if (rc_all) return rc_all;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t c = 0; c < o->nc; ++c) {
+      const auto& lst = o->cell_pairs[c];
+      size_t& k = cursor[c];
+      while (k < lst.size() && (lst[k] >> 1) < p1) {
+        int64_t p = lst[k] >> 1;
+        bool here = (lst[k] & 1) == 0;
+        apply_face(*o, work[p - p0], here, !here);
+        ++k;
+      }
+    }
   }
   return 0;
 }
