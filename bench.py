@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--small", action="store_true", help="reduced mesh (profiling/debug only)")
+    ap.add_argument("--flux-waves", type=int, default=0,
+                    help="waves per workgroup of the flux/relaxation kernel (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -96,6 +98,8 @@ def main():
     log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
         f"pairs={case.topo['n_pairs']} P={P}")
     dev = abi.HipContext(kah_params(case.config))
+    if args.flux_waves:
+        dev.set_option("flux_waves", args.flux_waves)
     dev.set_topology(case.topo)
     dev.set_state(case.w, case.prim, case.df)
     dt = case.dt_xi
@@ -123,6 +127,8 @@ def main():
     st = dev.get_state(df=False, qf=False)
     w = st["w"].reshape(-1, 4)
     finite = bool(np.isfinite(w).all())
+    copy_gbs, copy_ms = dev.copy_bandwidth(1 << 30, 10)
+    log(f"[bench] plain copy of 1 GiB: {copy_ms:.3f} ms -> {copy_gbs:.0f} GB/s")
     dev.close()
 
     ms_step = 1e3 * elapsed / args.steps
@@ -167,6 +173,7 @@ def main():
         "slope_kernel": {"avg_ms_per_step": ms[0], "alg_bytes_per_update": B_SLOPE,
                          "achieved_GBs": slope_gbs},
         "step_alg_bytes_per_update": B_STEP,
+        "copy_bandwidth_GBs": copy_gbs,
         "state_finite": finite,
     }
     if not args.no_cpu:

@@ -184,6 +184,14 @@ int kah_sync(kah_ctx* ctx);
 int64_t kah_phase_cells(const kah_ctx* ctx);
 /* Device bytes held by the context (state + topology). */
 int64_t kah_device_bytes(const kah_ctx* ctx);
+/* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
+ * the flux/relaxation kernel (4, 8, 12 or 16). No counterpart in the reference. */
+enum { KAH_OPT_FLUX_WAVES = 1 };
+int kah_set_option(kah_ctx* ctx, int option, int64_t value);
+/* Times reps device-to-device copies of `bytes` bytes with a plain streaming copy kernel
+ * (attainable HBM bandwidth = 2*bytes / ms_per_copy); the reference point for the
+ * hardware byte counters. No counterpart in the reference (measurement only). */
+int kah_copy_bandwidth(kah_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy);
 /* Average duration in ms of each kernel class over the calls since the last reset,
  * measured with HIP events on the library's stream: out[0] slope, out[1] flux+iterate. */
 int kah_kernel_times(kah_ctx* ctx, int reset, double* out_ms, int64_t* out_calls);

@@ -75,6 +75,7 @@ struct kah_ctx {
   int cur = 0;
   int64_t dev_bytes = 0;
   bool timing = false;
+  int fr_waves = 8;   // waves per workgroup of the flux/relaxation kernel
   std::vector<EvPair> events;
   double t_sum[2] = {0, 0};
   int64_t t_cnt[2] = {0, 0};
@@ -681,7 +682,7 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   if (C->timing) KAH_TRY(ev_begin(C, e, 1));
   int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER : MARCH_CAIDVM;
   KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
-                                   mode, march, C->lds_bytes, C->stream),
+                                   mode, march, C->lds_bytes, C->fr_waves, C->stream),
                  "flux/relaxation kernel launch"));
   if (C->timing) KAH_TRY(ev_end(C, e));
   return 0;
@@ -833,8 +834,47 @@ int kah_sync(kah_ctx* C) {
   return C->hip(hipStreamSynchronize(C->stream), "sync");
 }
 
+int kah_copy_bandwidth(kah_ctx* C, int64_t bytes, int reps, double* ms_per_copy) {
+  if (!C || bytes <= 0 || reps <= 0) return KAH_EINVAL;
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  bytes = (bytes / 16) * 16;
+  DBuf a, b;
+  KAH_TRY(C->alloc(a, size_t(bytes), "copy src"));
+  KAH_TRY(C->alloc(b, size_t(bytes), "copy dst"));
+  KAH_TRY(C->zero(a));
+  KAH_TRY(C->hip(launch_copy(a.p, b.p, bytes, C->stream), "copy launch"));
+  hipEvent_t e0, e1;
+  KAH_TRY(C->hip(hipEventCreate(&e0), "hipEventCreate"));
+  KAH_TRY(C->hip(hipEventCreate(&e1), "hipEventCreate"));
+  KAH_TRY(C->hip(hipEventRecord(e0, C->stream), "hipEventRecord"));
+  for (int r = 0; r < reps; ++r)
+    KAH_TRY(C->hip(launch_copy(a.p, b.p, bytes, C->stream), "copy launch"));
+  KAH_TRY(C->hip(hipEventRecord(e1, C->stream), "hipEventRecord"));
+  KAH_TRY(C->hip(hipEventSynchronize(e1), "hipEventSynchronize"));
+  float ms = 0.f;
+  KAH_TRY(C->hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  C->dev_bytes -= int64_t(a.bytes + b.bytes);
+  if (ms_per_copy) *ms_per_copy = double(ms) / reps;
+  return KAH_OK;
+}
+
 int64_t kah_phase_cells(const kah_ctx* C) { return C ? C->phase_cells : 0; }
 int64_t kah_device_bytes(const kah_ctx* C) { return C ? C->dev_bytes : 0; }
+
+int kah_set_option(kah_ctx* C, int option, int64_t value) {
+  if (!C) return KAH_EINVAL;
+  switch (option) {
+    case KAH_OPT_FLUX_WAVES:
+      if (value != 4 && value != 8 && value != 12 && value != 16)
+        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4, 8, 12 or 16");
+      C->fr_waves = int(value);
+      return KAH_OK;
+    default:
+      return C->fail(KAH_EINVAL, "unknown option " + std::to_string(option));
+  }
+}
 
 int kah_kernel_times(kah_ctx* C, int reset, double* out_ms, int64_t* out_calls) {
   if (!C) return KAH_EINVAL;

@@ -51,7 +51,9 @@ struct DevArgs {
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                         int maxh, hipStream_t st);
 hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                             int mode, int march, size_t lds, hipStream_t st);
+                             int mode, int march, size_t lds, int waves_per_block,
+                             hipStream_t st);
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                               double* sw, hipStream_t st);
 

@@ -7,16 +7,17 @@
 // Threads of a workgroup stride over the cell's velocity cells (velocity index fastest in
 // memory, so every load and store of f, sdf and the flux is coalesced); per-cell moment
 // sums are reduced deterministically (fixed wave shuffle tree, then a fixed-order sum of
-// the four wave partials), with no atomics anywhere.
+// the wave partials), with no atomics anywhere. Private arrays are only ever indexed with
+// compile-time indices (direction-dependent picks are selects), so nothing spills to
+// scratch memory.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kah_internal.h"
 #include "kah_kernels.h"
 
 namespace kah {
-
-constexpr int BS = 256;       // threads per workgroup
-constexpr int NWAVE = BS / 64;
 
 // ------------------------------------------------------------------ reductions
 __device__ inline double wave_sum(double v) {
@@ -25,10 +26,11 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Sums cnt values held per thread across the workgroup; every thread receives the totals
-// in out[]. red must hold NWAVE*cnt doubles of LDS.
-template <int CNT>
-__device__ inline void block_sum(const double* vals, int cnt, double* red, double* out) {
+// Workgroup sum of the first cnt of CNT per-thread partials. out[q] (LDS) receives the
+// total of value q; red needs NWV*CNT doubles of LDS. Two barriers, fixed order.
+template <int CNT, int NWV>
+__device__ inline void block_reduce(const double (&vals)[CNT], int cnt, double* red,
+                                    double* out) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < CNT; ++q) {
@@ -38,14 +40,11 @@ __device__ inline void block_sum(const double* vals, int cnt, double* red, doubl
     }
   }
   __syncthreads();
+  if (int(threadIdx.x) < cnt) {
+    double s = red[threadIdx.x];
 #pragma unroll
-  for (int q = 0; q < CNT; ++q) {
-    if (q < cnt) {
-      double s = red[q];
-#pragma unroll
-      for (int w = 1; w < NWAVE; ++w) s += red[w * CNT + q];
-      out[q] = s;
-    }
+    for (int w = 1; w < NWV; ++w) s += red[w * CNT + threadIdx.x];
+    out[threadIdx.x] = s;
   }
   __syncthreads();
 }
@@ -56,6 +55,15 @@ __device__ inline int xcd_remap(int i, int n) {
   const int x = i & 7, j = i >> 3;
   const int q = n >> 3, r = n & 7;
   return x * q + (x < r ? x : r) + j;
+}
+
+// a[d] for a runtime direction d, as selects over compile-time indices
+template <int D>
+__device__ inline double pick(const double (&a)[D], int d) {
+  double r = a[0];
+#pragma unroll
+  for (int t = 1; t < D; ++t) r = (d == t) ? a[t] : r;
+  return r;
 }
 
 // ------------------------------------------------------------------ grid access
@@ -118,11 +126,11 @@ __device__ inline void for_overlap(const Grid<D, NDF, UNI>& G, int i, int li, ui
 // plain reconstruction used next to solid cells and at domain walls/inflow (CAIDVM.jl:13-14,
 // :110-112); PLAIN f*vn (UniformOutflow, CAIDVM.jl:53-67).
 template <int D, int NDF>
-__device__ inline void micro_recon(int mode, const double* f, const double (*s)[NDF],
-                                   const double* u, double dt, const double* xrec,
+__device__ inline void micro_recon(int mode, const double (&f)[NDF], const double (&s)[D][NDF],
+                                   const double (&u)[D], double dt, const double* xrec,
                                    const double* dsrec, const double* fmid, int dir,
-                                   double* out) {
-  const double vn = u[dir];
+                                   double (&out)[NDF]) {
+  const double vn = pick<D>(u, dir);
   if (mode == MI_PLAIN || mode == MI_SELF_PLAIN) {
 #pragma unroll
     for (int k = 0; k < NDF; ++k) out[k] = f[k] * vn;
@@ -151,60 +159,183 @@ __device__ inline void micro_recon(int mode, const double* f, const double (*s)[
 // InterpolatedOutflow incoming micro flux (CAIDVM.jl:71-94): the ghost state mirrored
 // through the face, reconstructed at the face from the cell's own slopes.
 template <int D, int NDF>
-__device__ inline void micro_interp(const double* f, const double (*s)[NDF], const double* u,
-                                    double dt, const double* xh, const double* fmid, int dir,
-                                    double* out) {
-  double xt[D];
+__device__ inline void micro_interp(const double (&f)[NDF], const double (&s)[D][NDF],
+                                    const double (&u)[D], double dt, const double* xh,
+                                    const double* fmid, int dir, double (&out)[NDF]) {
+  double xt[D], xhd[D];
 #pragma unroll
-  for (int t = 0; t < D; ++t) xt[t] = 2.0 * fmid[t] - xh[t];
+  for (int t = 0; t < D; ++t) {
+    xt[t] = 2.0 * fmid[t] - xh[t];
+    xhd[t] = xh[t];
+  }
   double ndx[D];
 #pragma unroll
   for (int t = 0; t < D; ++t) ndx[t] = fmid[t] - u[t] * dt - xt[t];
+  const double shift = pick<D>(xt, dir) - pick<D>(xhd, dir);
+  const double vn = pick<D>(u, dir);
 #pragma unroll
   for (int k = 0; k < NDF; ++k) {
-    double tdf = f[k] + (xt[dir] - xh[dir]) * s[dir][k];
+    double sk[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) sk[t] = s[t][k];
+    double tdf = f[k] + shift * pick<D>(sk, dir);
     double s_dx = ndx[0] * s[0][k];
 #pragma unroll
     for (int t = 1; t < D; ++t) s_dx = s_dx + ndx[t] * s[t][k];
-    out[k] = (tdf + s_dx) * u[dir];
+    out[k] = (tdf + s_dx) * vn;
   }
 }
 
 // Maxwellian inflow state (bc prim, density rho for walls) times vn.
 template <int D, int NDF>
-__device__ inline void micro_maxwell(const double* bc, double rho, bool wall, const double* u,
-                                     double K, int dir, double* out) {
+__device__ inline void micro_maxwell(const double* bc, double rho, bool wall,
+                                     const double (&u)[D], double K, int dir,
+                                     double (&out)[NDF]) {
   double p[D + 2];
 #pragma unroll
   for (int m = 0; m < D + 2; ++m) p[m] = bc[m];
   if (wall) p[0] = rho;
   double M[2];
   maxwell<D>(u, p, K, M);
+  const double vn = pick<D>(u, dir);
 #pragma unroll
-  for (int k = 0; k < NDF; ++k) out[k] = M[k] * u[dir];
+  for (int k = 0; k < NDF; ++k) out[k] = M[k] * vn;
 }
 
 // ------------------------------------------------------------------ K1: slopes + half fluxes
-// update_slope!/update_slope_transverse_level! per cell (src/Flux/Slope.jl:458-945) with the
-// diff_vs!/diff_vs_transverse! sums (:29-64, :278-332) as key-interval overlaps; then the
-// moments of the cell's own upwind micro flux through each face it owns a half of.
-template <int D, int NDF, bool UNI, int MAXH>
-__global__ __launch_bounds__(BS) void k_slope(DevArgs A, const int32_t* __restrict__ cells,
-                                              int ncells) {
+// Velocity-space slope of one direction (update_slope!/update_slope_transverse_level!,
+// src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
+// taken as key-interval overlaps.
+template <int D, int NDF, bool UNI>
+__device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, int i, int li,
+                                uint32_t ki, const double (&f)[NDF], double (&a)[NDF]) {
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) a[k] = 0.0;
+  for (int e = 0; e < cnt; ++e) {
+    const SlopeNb nb = A.slnb[beg + e];
+    const Grid<D, NDF, UNI> N(A, nb.cell);
+    for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        double pj = N.f(j, k);
+        if (nb.proj) {
+#pragma unroll
+          for (int t = 0; t < D; ++t) pj += nb.dm[t] * N.s(j, k, t);
+        }
+        double term = f[k] - pj;
+        if (den != 1.0) term = term / den;
+        a[k] += term / dsx;
+      }
+    });
+  }
+}
+
+template <int D, int NDF, bool UNI>
+__device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c, int d,
+                                 int i, int li, uint32_t ki, const double (&f)[NDF],
+                                 double (&s)[NDF]) {
+  const SlopeDesc sd = A.sl[c * D + d];
+  if (sd.mode == SL_SKIP) {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) s[k] = G.s(i, k, d);
+    return;
+  }
+  if (sd.mode == SL_ZERO) {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      s[k] = 0.0;
+      *G.sp(i, k, d) = 0.0;
+    }
+    return;
+  }
+  double aL[NDF];
+  side_sum<D, NDF, UNI>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], i, li, ki, f, aL);
+  if (sd.mode == SL_INNER) {
+    double aR[NDF];
+    side_sum<D, NDF, UNI>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], i, li, ki, f, aR);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k)
+      s[k] = minmod(aL[k] / double(sd.nb_cnt[0]), aR[k] / double(sd.nb_cnt[1]));
+  } else {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) s[k] = aL[k] / double(sd.nb_cnt[0]);
+  }
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) *G.sp(i, k, d) = s[k];
+}
+
+// Moments of one velocity cell's contribution to a half flux (the cell's own upwind micro
+// flux through one face, micro_to_macro of src/Flux/CAIDVM.jl:119), or the two sums of the
+// wall density (calc_ρw, src/Theory/Math.jl:251-282) in the first sweep of a wall.
+template <int D, int NDF, bool UNI>
+__device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c,
+                                  const HalfDesc& hd, int i, const double (&u)[D],
+                                  const double (&f)[NDF], const double (&s)[D][NDF], double wt,
+                                  double* acc) {
   constexpr int NW = D + 2;
+  const double ud = pick<D>(u, hd.dir);
+  const double ru = hd.rot * ud;
+  const bool in = hd.sel ? (ru > 0.0) : (ru <= 0.0);
+  if (hd.mode == MI_WALL) {
+    const double* bc = &A.bc_prim[hd.bc * NW];
+    if (in) {
+      double c2 = 0.0;
+#pragma unroll
+      for (int t = 0; t < D; ++t) {
+        double cu = u[t] - bc[1 + t];
+        c2 = (t == 0) ? cu * cu : c2 + cu * cu;
+      }
+      acc[1] += wt * ud * exp(-bc[D + 1] * c2);
+    } else {
+      double dx[D];
+#pragma unroll
+      for (int t = 0; t < D; ++t) dx[t] = hd.fmid[t] - u[t] * A.dt - hd.xrec[t];
+      double hrec = dx[0] * s[0][0];
+#pragma unroll
+      for (int t = 1; t < D; ++t) hrec = hrec + dx[t] * s[t][0];
+      hrec = f[0] + hrec;
+      acc[0] += wt * ud * hrec;
+    }
+    return;
+  }
+  if (!in) return;
+  double mic[NDF];
+  if (hd.mode == MI_MAXWELL) {
+    micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], 0.0, false, u, A.K, hd.dir, mic);
+  } else if (hd.mode == MI_INTERP) {
+    micro_interp<D, NDF>(f, s, u, A.dt, hd.xrec, hd.fmid, hd.dir, mic);
+  } else if (hd.data_cell == c) {
+    micro_recon<D, NDF>(hd.mode, f, s, u, A.dt, hd.xrec, hd.dsrec, hd.fmid, hd.dir, mic);
+  } else {
+    // the there side of a face whose other cell is solid: f of that cell * vn (shared
+    // velocity grids only; checked on the host)
+    const Grid<D, NDF, UNI> S(A, hd.data_cell);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) mic[k] = S.f(i, k) * ud;
+  }
+  double m[NW];
+  moment_terms<D, NDF>(wt, u, mic, m);
+#pragma unroll
+  for (int q = 0; q < NW; ++q) acc[q] += m[q];
+}
+
+template <int D, int NDF, bool UNI, int MAXH, int NWV>
+__global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A, const int32_t* __restrict__ cells,
+                                                    int ncells) {
+  constexpr int NW = D + 2;
+  constexpr int BS = NWV * 64;
+  constexpr int CNT = MAXH * NW;
   const int c = cells[xcd_remap(blockIdx.x, ncells)];
   const Grid<D, NDF, UNI> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
-  const double dt = A.dt;
-  __shared__ double red[NWAVE * MAXH * NW];
+  __shared__ double red[NWV * CNT];
+  __shared__ double tot[CNT];
   __shared__ double rho_sh[MAXH];
 
-  double acc[MAXH][NW];
+  double acc[CNT];
 #pragma unroll
-  for (int h = 0; h < MAXH; ++h)
-#pragma unroll
-    for (int m = 0; m < NW; ++m) acc[h][m] = 0.0;
+  for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
 
   for (int i = threadIdx.x; i < G.n; i += BS) {
     double u[D], f[NDF], s[D][NDF];
@@ -215,140 +346,37 @@ __global__ __launch_bounds__(BS) void k_slope(DevArgs A, const int32_t* __restri
     const int li = G.lev(i);
     const uint32_t ki = G.key(i);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const SlopeDesc sd = A.sl[c * D + d];
-      if (sd.mode == SL_SKIP) {
-#pragma unroll
-        for (int k = 0; k < NDF; ++k) s[d][k] = G.s(i, k, d);
-        continue;
-      }
-      if (sd.mode == SL_ZERO) {
-#pragma unroll
-        for (int k = 0; k < NDF; ++k) {
-          s[d][k] = 0.0;
-          *G.sp(i, k, d) = 0.0;
-        }
-        continue;
-      }
-      double a[2][NDF];
-      const int nsides = (sd.mode == SL_INNER) ? 2 : 1;
-      for (int side = 0; side < nsides; ++side) {
-#pragma unroll
-        for (int k = 0; k < NDF; ++k) a[side][k] = 0.0;
-        const double dsx = sd.ds[side];
-        for (int e = 0; e < sd.nb_cnt[side]; ++e) {
-          const SlopeNb nb = A.slnb[sd.nb_beg[side] + e];
-          const Grid<D, NDF, UNI> N(A, nb.cell);
-          for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
-#pragma unroll
-            for (int k = 0; k < NDF; ++k) {
-              double pj = N.f(j, k);
-              if (nb.proj) {
-#pragma unroll
-                for (int t = 0; t < D; ++t) pj += nb.dm[t] * N.s(j, k, t);
-              }
-              double term = f[k] - pj;
-              if (den != 1.0) term = term / den;
-              a[side][k] += term / dsx;
-            }
-          });
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < NDF; ++k) {
-        double v = (sd.mode == SL_INNER)
-                       ? minmod(a[0][k] / double(sd.nb_cnt[0]), a[1][k] / double(sd.nb_cnt[1]))
-                       : a[0][k] / double(sd.nb_cnt[0]);
-        s[d][k] = v;
-        *G.sp(i, k, d) = v;
-      }
-    }
-    // half fluxes owned by this cell
+    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
     const double wt = G.wt(i);
 #pragma unroll
-    for (int h = 0; h < MAXH; ++h) {
-      if (h < nh) {
-        const HalfDesc& hd = A.halves[hb + h];
-        const double ru = hd.rot * u[hd.dir];
-        const bool in = hd.sel ? (ru > 0.0) : (ru <= 0.0);
-        double mic[NDF];
-        double m[NW];
-        if (hd.mode == MI_WALL) {
-          // first sweep of a wall: SF over the cell's upwind set, SG over the incoming set
-          // (calc_ρw, src/Theory/Math.jl:251-282)
-          const double* bc = &A.bc_prim[hd.bc * NW];
-          if (in) {
-            double c2 = 0.0;
-#pragma unroll
-            for (int t = 0; t < D; ++t) {
-              double cu = u[t] - bc[1 + t];
-              c2 = (t == 0) ? cu * cu : c2 + cu * cu;
-            }
-            acc[h][1] += wt * u[hd.dir] * exp(-bc[D + 1] * c2);
-          } else {
-            double dx[D];
-#pragma unroll
-            for (int t = 0; t < D; ++t) dx[t] = hd.fmid[t] - u[t] * dt - hd.xrec[t];
-            double hrec = dx[0] * s[0][0];
-#pragma unroll
-            for (int t = 1; t < D; ++t) hrec = hrec + dx[t] * s[t][0];
-            hrec = f[0] + hrec;
-            acc[h][0] += wt * u[hd.dir] * hrec;
-          }
-          continue;
-        }
-        if (!in) continue;
-        if (hd.mode == MI_MAXWELL) {
-          micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], 0.0, false, u, A.K, hd.dir, mic);
-        } else if (hd.mode == MI_INTERP) {
-          micro_interp<D, NDF>(f, s, u, dt, hd.xrec, hd.fmid, hd.dir, mic);
-        } else if (hd.data_cell == c) {
-          micro_recon<D, NDF>(hd.mode, f, s, u, dt, hd.xrec, hd.dsrec, hd.fmid, hd.dir, mic);
-        } else {
-          // the there side of a face whose other cell is solid: f of that cell * vn
-          // (uniform grids only; checked on the host)
-          const Grid<D, NDF, UNI> S(A, hd.data_cell);
-#pragma unroll
-          for (int k = 0; k < NDF; ++k) mic[k] = S.f(i, k) * u[hd.dir];
-        }
-        moment_terms<D, NDF>(wt, u, mic, m);
-#pragma unroll
-        for (int q = 0; q < NW; ++q) acc[h][q] += m[q];
-      }
-    }
+    for (int h = 0; h < MAXH; ++h)
+      if (h < nh) half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, &acc[h * NW]);
   }
 
-  // reduce and publish the half fluxes
-  double tot[MAXH * NW];
-  block_sum<MAXH * NW>(&acc[0][0], nh * NW, red, tot);
+  block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
   bool any_wall = false;
-  for (int h = 0; h < nh; ++h) {
+  for (int h = 0; h < nh; ++h) any_wall |= (A.halves[hb + h].mode == MI_WALL);
+  if (int(threadIdx.x) < nh * NW) {
+    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
     const HalfDesc& hd = A.halves[hb + h];
-    if (hd.mode == MI_WALL) {
-      any_wall = true;
-      const double* bc = &A.bc_prim[hd.bc * NW];
-      double pref = (D == 2) ? bc[3] / PI : pow(bc[4] / PI, 1.5);
-      double rho = -tot[h * NW + 0] / (pref * tot[h * NW + 1]);
-      if (threadIdx.x == 0) {
-        rho_sh[h] = rho;
-        A.rho_w[hd.slot >> 1] = rho;
-      }
-      continue;
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int q = 0; q < NW; ++q)
-        A.half_fw[int64_t(hd.slot) * NW + q] = (q == NW - 1) ? 0.5 * tot[h * NW + q]
-                                                             : tot[h * NW + q];
-    }
+    if (hd.mode != MI_WALL)
+      A.half_fw[int64_t(hd.slot) * NW + q] = (q == NW - 1) ? 0.5 * tot[threadIdx.x] : tot[threadIdx.x];
   }
   if (!any_wall) return;
+  // wall density rho_w = -SF/SG, then the moments of the wall Maxwellian (second sweep)
+  if (int(threadIdx.x) < nh) {
+    const HalfDesc& hd = A.halves[hb + threadIdx.x];
+    if (hd.mode == MI_WALL) {
+      const double* bc = &A.bc_prim[hd.bc * NW];
+      const double pref = (D == 2) ? bc[3] / PI : pow(bc[4] / PI, 1.5);
+      const double rho = -tot[threadIdx.x * NW + 0] / (pref * tot[threadIdx.x * NW + 1]);
+      rho_sh[threadIdx.x] = rho;
+      A.rho_w[hd.slot >> 1] = rho;
+    }
+  }
   __syncthreads();
-  // second sweep for walls: moments of the Maxwellian with the wall density
 #pragma unroll
-  for (int h = 0; h < MAXH; ++h)
-#pragma unroll
-    for (int m = 0; m < NW; ++m) acc[h][m] = 0.0;
+  for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
   for (int i = threadIdx.x; i < G.n; i += BS) {
     double u[D];
 #pragma unroll
@@ -359,25 +387,21 @@ __global__ __launch_bounds__(BS) void k_slope(DevArgs A, const int32_t* __restri
       if (h < nh) {
         const HalfDesc& hd = A.halves[hb + h];
         if (hd.mode != MI_WALL) continue;
-        const double ru = hd.rot * u[hd.dir];
-        if (!(ru > 0.0)) continue;
+        if (!(hd.rot * pick<D>(u, hd.dir) > 0.0)) continue;
         double mic[NDF], m[NW];
         micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], rho_sh[h], true, u, A.K, hd.dir, mic);
         moment_terms<D, NDF>(wt, u, mic, m);
 #pragma unroll
-        for (int q = 0; q < NW; ++q) acc[h][q] += m[q];
+        for (int q = 0; q < NW; ++q) acc[h * NW + q] += m[q];
       }
     }
   }
-  block_sum<MAXH * NW>(&acc[0][0], nh * NW, red, tot);
-  if (threadIdx.x == 0) {
-    for (int h = 0; h < nh; ++h) {
-      const HalfDesc& hd = A.halves[hb + h];
-      if (hd.mode != MI_WALL) continue;
-      for (int q = 0; q < NW; ++q)
-        A.half_fw[int64_t(hd.slot) * NW + q] = (q == NW - 1) ? 0.5 * tot[h * NW + q]
-                                                             : tot[h * NW + q];
-    }
+  block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
+  if (int(threadIdx.x) < nh * NW) {
+    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
+    const HalfDesc& hd = A.halves[hb + h];
+    if (hd.mode == MI_WALL)
+      A.half_fw[int64_t(hd.slot) * NW + q] = (q == NW - 1) ? 0.5 * tot[threadIdx.x] : tot[threadIdx.x];
   }
 }
 
@@ -385,15 +409,16 @@ __global__ __launch_bounds__(BS) void k_slope(DevArgs A, const int32_t* __restri
 // Micro flux into velocity cell i of cell c through all its faces, in face-list order
 // (update_flux!/update_domain_flux!/update_micro_flux!, src/Flux/Flux.jl:64-281).
 template <int D, int NDF, bool UNI>
-__device__ inline void gather_flux(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c,
-                                   int eb, int ne, int i, const double* u, const double* f,
-                                   const double (*s)[NDF], int li, uint32_t ki,
-                                   double* fl) {
+__device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
+                                   const double (&u)[D], const double (&f)[NDF],
+                                   const double (&s)[D][NDF], int li, uint32_t ki,
+                                   double (&fl)[NDF]) {
   constexpr int NW = D + 2;
   const double dt = A.dt;
   for (int e = 0; e < ne; ++e) {
     const EntryDesc& ed = A.ents[eb + e];
-    const double ru = ed.rot * u[ed.dir];
+    const double ud = pick<D>(u, ed.dir);
+    const double ru = ed.rot * ud;
     const bool own = ed.side ? (ru > 0.0) : (ru <= 0.0);
     const double area = ed.area;
     double mic[NDF];
@@ -417,7 +442,7 @@ __device__ inline void gather_flux(const DevArgs& A, const Grid<D, NDF, UNI>& G,
       }
       case MI_SELF_PLAIN: {
 #pragma unroll
-        for (int k = 0; k < NDF; ++k) fl[k] += area * (f[k] * u[ed.dir]);
+        for (int k = 0; k < NDF; ++k) fl[k] += area * (f[k] * ud);
         break;
       }
       case MI_INTERP: {
@@ -453,50 +478,102 @@ __device__ inline void gather_flux(const DevArgs& A, const Grid<D, NDF, UNI>& G,
   }
 }
 
+// heat_flux_2D2F / heat_flux_3D1F partial sums of one velocity cell (lib/KitCore/2D2F.jl:68-88,
+// 3D1F.jl:41-72): the sums are kept apart as in the reference and combined after reduction.
+template <int D, int NDF, int NQ>
+__device__ inline void heat_terms(double wt, const double (&u)[D], const double (&f)[NDF],
+                                  const double* p, double (&qa)[NQ]) {
+  if constexpr (D == 2) {
+    double cu = u[0] - p[1], cv = u[1] - p[2];
+    double c2 = cu * cu + cv * cv;
+    qa[0] += wt * cu * c2 * f[0];
+    qa[1] += wt * cu * f[1];
+    qa[2] += wt * cv * c2 * f[0];
+    qa[3] += wt * cv * f[1];
+  } else {
+    double cu = u[0] - p[1], cv = u[1] - p[2], cw = u[2] - p[3];
+    double c2 = cu * cu + cv * cv + cw * cw;
+    qa[0] += wt * cu * c2 * f[0];
+    qa[1] += wt * cv * c2 * f[0];
+    qa[2] += wt * cw * c2 * f[0];
+  }
+}
+template <int D>
+__device__ inline void heat_finish(const double* qs, double* q) {
+  if constexpr (D == 2) {
+    q[0] = 0.5 * (qs[0] + qs[1]);
+    q[1] = 0.5 * (qs[2] + qs[3]);
+  } else {
+    q[0] = 0.5 * qs[0];
+    q[1] = 0.5 * qs[1];
+    q[2] = 0.5 * qs[2];
+  }
+}
+
 // iterate!(CAIDVM_Marching) (src/Theory/Iterate.jl:96-130) or iterate!(Euler) (:131-162)
 // for one cell, fed by the gathered flux (MODE_FUSED), by the flux accumulators
 // (MODE_ITER), or only accumulating the flux (MODE_FLUX).
-template <int D, int NDF, bool UNI, int MODE, int MARCH, bool LDS>
-__global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __restrict__ cells,
-                                                   int ncells) {
+template <int D, int NDF, bool UNI, int MODE, int MARCH, bool LDS, int NWV>
+__global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A,
+                                                         const int32_t* __restrict__ cells,
+                                                         int ncells) {
   constexpr int NW = D + 2;
   constexpr int NQ = (D == 2) ? 4 : 3;   // heat-flux partial sums
+  constexpr int BS = NWV * 64;
   const int c = cells[xcd_remap(blockIdx.x, ncells)];
   const Grid<D, NDF, UNI> G(A, c);
   const int n = G.n;
   const int eb = A.ent_off[c];
   const int ne = A.ent_off[c + 1] - eb;
   const double dt = A.dt;
-  __shared__ double red[NWAVE * 8];
+  __shared__ double red[NWV * 8];
+  __shared__ double res[8];
+  __shared__ double cell_sh[4 * NW];   // wnew | prim_c | prim | q
   extern __shared__ double lds[];
   double* sc = LDS ? lds : &A.scratch[G.off * (NDF + 1)];
 
-  const double* dsc = &A.rec_ds[int64_t(c) * D];
-  double V = dsc[0];
+  if (threadIdx.x == 0) {
+    // macroscopic face flux, in face order: flux += fw*area / flux -= fw*area, then
+    // w += flux*dt/V (both marchings: src/Theory/Iterate.jl:108, :144-149)
+    const double* dsc = &A.rec_ds[int64_t(c) * D];
+    double V = dsc[0];
 #pragma unroll
-  for (int d = 1; d < D; ++d) V = V * dsc[d];
-
-  // macroscopic face flux, in face order: flux += fw*area / flux -= fw*area
-  double psf[NW];
+    for (int d = 1; d < D; ++d) V = V * dsc[d];
+    double psf[NW];
 #pragma unroll
-  for (int q = 0; q < NW; ++q) psf[q] = A.psflux[int64_t(c) * NW + q];
-  if (MODE != MODE_ITER && A.flux_kind == 0) {
-    for (int e = 0; e < ne; ++e) {
-      const EntryDesc& ed = A.ents[eb + e];
-      const double* h0 = &A.half_fw[int64_t(ed.pair) * 2 * NW];
-      const double* h1 = h0 + NW;
+    for (int q = 0; q < NW; ++q) psf[q] = A.psflux[int64_t(c) * NW + q];
+    if (MODE != MODE_ITER && A.flux_kind == 0) {
+      for (int e = 0; e < ne; ++e) {
+        const EntryDesc& ed = A.ents[eb + e];
+        const double* h0 = &A.half_fw[int64_t(ed.pair) * 2 * NW];
+        const double* h1 = h0 + NW;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          double t;
+          if (ed.oth_cell < 0) t = ed.area * (h0[q] + h1[q]);    // update_domain_flux!
+          else t = (h0[q] + h1[q]) * ed.area;                     // update_macro_flux!
+          psf[q] = ed.side ? psf[q] - t : psf[q] + t;
+        }
+      }
+    }
+    if (MODE == MODE_FLUX) {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) A.psflux[int64_t(c) * NW + q] = psf[q];
+    } else {
+      const double* wold = &A.w[int64_t(c) * NW];
+      double wn[NW], pc[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) wn[q] = wold[q] + psf[q] * dt / V;
+      get_prim<D>(wn, A.gamma, pc);
 #pragma unroll
       for (int q = 0; q < NW; ++q) {
-        double t;
-        if (ed.oth_cell < 0) t = ed.area * (h0[q] + h1[q]);    // update_domain_flux!
-        else t = (h0[q] + h1[q]) * ed.area;                     // update_macro_flux!
-        psf[q] = ed.side ? psf[q] - t : psf[q] + t;
+        cell_sh[q] = wn[q];
+        cell_sh[NW + q] = pc[q];
       }
+      res[0] = dt / V;
     }
   }
   if (MODE == MODE_FLUX) {
-    if (threadIdx.x == 0)
-      for (int q = 0; q < NW; ++q) A.psflux[int64_t(c) * NW + q] = psf[q];
     for (int i = threadIdx.x; i < n; i += BS) {
       double u[D], f[NDF], s[D][NDF], fl[NDF];
 #pragma unroll
@@ -508,26 +585,27 @@ __global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __r
         for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
         fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
       }
-      gather_flux<D, NDF, UNI>(A, G, c, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+      gather_flux<D, NDF, UNI>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) A.vflux[G.off * NDF + int64_t(k) * n + i] = fl[k];
     }
     return;
   }
-
-  double wnew[NW], prim_c[NW];
-  const double* wold = &A.w[int64_t(c) * NW];
-#pragma unroll
-  for (int q = 0; q < NW; ++q) wnew[q] = wold[q] + psf[q] * dt / V;
-  get_prim<D>(wnew, A.gamma, prim_c);
-  const double dtV = dt / V;
+  __syncthreads();
+  const double dtV = res[0];
   double* out = A.df_out;
 
-  if (MARCH == MARCH_CAIDVM) {
-    // pass 1: convection, f* = f + dt/V*flux, and the moments of f*
-    double mw[NW];
+  // pass 1: flux of every velocity cell (gathered or from the accumulators)
+  double mw[NW];
+  double qa[NQ];
 #pragma unroll
-    for (int q = 0; q < NW; ++q) mw[q] = 0.0;
+  for (int q = 0; q < NW; ++q) mw[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) qa[q] = 0.0;
+  {
+    double pc[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) pc[q] = cell_sh[NW + q];
     for (int i = threadIdx.x; i < n; i += BS) {
       double u[D], f[NDF], fl[NDF];
 #pragma unroll
@@ -542,7 +620,7 @@ __global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __r
 #pragma unroll
           for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
         }
-        gather_flux<D, NDF, UNI>(A, G, c, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+        gather_flux<D, NDF, UNI>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -550,66 +628,76 @@ __global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __r
           A.vflux[G.off * NDF + int64_t(k) * n + i] = 0.0;
         }
       }
-      double fs[NDF], m[NW];
-#pragma unroll
-      for (int k = 0; k < NDF; ++k) {
-        fs[k] = f[k] + dtV * fl[k];
-        sc[k * n + i] = fs[k];
-      }
-      moment_terms<D, NDF>(G.wt(i), u, fs, m);
-#pragma unroll
-      for (int q = 0; q < NW; ++q) mw[q] += m[q];
-    }
-    double wf[NW], prim[NW];
-    block_sum<NW>(mw, NW, red, wf);
-    wf[NW - 1] = 0.5 * wf[NW - 1];
-    get_prim<D>(wf, A.gamma, prim);
-    const double tau = get_tau<D>(prim_c, A.mu, A.omega);
-
-    // pass 2: conservation correction f += M(prim_c) - M(prim); heat flux of the result
-    double qa[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) qa[q] = 0.0;
-    for (int i = threadIdx.x; i < n; i += BS) {
-      double u[D], Mc[2], M[2];
-#pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-      maxwell<D>(u, prim_c, A.K, Mc);
-      maxwell<D>(u, prim, A.K, M);
-      double fs[NDF];
-#pragma unroll
-      for (int k = 0; k < NDF; ++k) {
-        fs[k] = sc[k * n + i] + (Mc[k] - M[k]);
-        sc[k * n + i] = fs[k];
-      }
-      sc[NDF * n + i] = Mc[0];
       const double wt = G.wt(i);
-      if constexpr (D == 2) {
-        double cu = u[0] - prim_c[1], cv = u[1] - prim_c[2];
-        double c2 = cu * cu + cv * cv;
-        qa[0] += wt * cu * c2 * fs[0];
-        qa[1] += wt * cu * fs[1];
-        qa[2] += wt * cv * c2 * fs[0];
-        qa[3] += wt * cv * fs[1];
+      if (MARCH == MARCH_CAIDVM) {
+        // convection first: f* = f + dt/V*flux, and the moments of f*
+        double fs[NDF], m[NW];
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fs[k] = f[k] + dtV * fl[k];
+          sc[k * n + i] = fs[k];
+        }
+        moment_terms<D, NDF>(wt, u, fs, m);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) mw[q] += m[q];
       } else {
-        double cu = u[0] - prim_c[1], cv = u[1] - prim_c[2], cw = u[2] - prim_c[3];
-        double c2 = cu * cu + cv * cv + cw * cw;
-        qa[0] += wt * cu * c2 * fs[0];
-        qa[1] += wt * cv * c2 * fs[0];
-        qa[2] += wt * cw * c2 * fs[0];
+        // Euler: heat flux of f^n with the updated prim
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) sc[k * n + i] = fl[k];
+        heat_terms<D, NDF, NQ>(wt, u, f, pc, qa);
       }
     }
-    double qs[NQ], q[D];
-    block_sum<NQ>(qa, NQ, red, qs);
-    if constexpr (D == 2) {
-      q[0] = 0.5 * (qs[0] + qs[1]);
-      q[1] = 0.5 * (qs[2] + qs[3]);
-    } else {
-      q[0] = 0.5 * qs[0];
-      q[1] = 0.5 * qs[1];
-      q[2] = 0.5 * qs[2];
-    }
+  }
 
+  if (MARCH == MARCH_CAIDVM) {
+    block_reduce<NW, NWV>(mw, NW, red, &cell_sh[2 * NW]);
+    if (threadIdx.x == 0) {
+      double wf[NW], pr[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) wf[q] = cell_sh[2 * NW + q];
+      wf[NW - 1] = 0.5 * wf[NW - 1];
+      get_prim<D>(wf, A.gamma, pr);
+#pragma unroll
+      for (int q = 0; q < NW; ++q) cell_sh[2 * NW + q] = pr[q];
+    }
+    __syncthreads();
+    // pass 2: conservation correction f += M(prim_c) - M(prim); heat flux of the result
+    {
+      double pc[NW], pr[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        pc[q] = cell_sh[NW + q];
+        pr[q] = cell_sh[2 * NW + q];
+      }
+      for (int i = threadIdx.x; i < n; i += BS) {
+        double u[D], Mc[2], M[2];
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+        maxwell<D>(u, pc, A.K, Mc);
+        maxwell<D>(u, pr, A.K, M);
+        double fs[NDF];
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fs[k] = sc[k * n + i] + (Mc[k] - M[k]);
+          sc[k * n + i] = fs[k];
+        }
+        sc[NDF * n + i] = Mc[0];
+        heat_terms<D, NDF, NQ>(G.wt(i), u, fs, pc, qa);
+      }
+    }
+  }
+  double qs_red[NQ];
+  block_reduce<NQ, NWV>(qa, NQ, red, &cell_sh[3 * NW]);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) qs_red[q] = cell_sh[3 * NW + q];
+  double q[D];
+  heat_finish<D>(qs_red, q);
+  double pc[NW];
+#pragma unroll
+  for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
+  const double tau = get_tau<D>(pc, A.mu, A.omega);
+
+  if (MARCH == MARCH_CAIDVM) {
     // pass 3: g+ = M(prim_c) + Shakhov part; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
     const double a1 = tau / (tau + dt), a2 = dt / (tau + dt);
     for (int i = threadIdx.x; i < n; i += BS) {
@@ -617,8 +705,8 @@ __global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __r
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
       Fc[0] = sc[NDF * n + i];
-      if constexpr (NDF == 2) Fc[1] = Fc[0] * A.K / (2.0 * prim_c[D + 1]);
-      shakhov<D>(u, Fc, prim_c, q, A.Pr, A.K, Sp);
+      if constexpr (NDF == 2) Fc[1] = Fc[0] * A.K / (2.0 * pc[D + 1]);
+      shakhov<D>(u, Fc, pc, q, A.Pr, A.K, Sp);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         double g = Fc[k] + Sp[k];
@@ -627,84 +715,15 @@ __global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __r
         out[G.off * NDF + int64_t(k) * n + i] = fn;
       }
     }
-    if (threadIdx.x == 0) {
-      double* pr = &A.prim[int64_t(c) * NW];
-      double* rs = &A.res[int64_t(c) * 2 * NW];
-#pragma unroll
-      for (int m = 0; m < NW; ++m) {
-        double dl = prim_c[m] - pr[m];
-        rs[m] = dl * dl;
-        rs[NW + m] = fabs(prim_c[m]);
-        pr[m] = prim_c[m];
-        A.w[int64_t(c) * NW + m] = wnew[m];
-        A.psflux[int64_t(c) * NW + m] = 0.0;
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) A.qf[int64_t(c) * D + d] = q[d];
-    }
   } else {
-    // Euler: prim from the updated w, heat flux of f^n, then the implicit update
-    const double tau = get_tau<D>(prim_c, A.mu, A.omega);
-    double qa[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) qa[q] = 0.0;
-    for (int i = threadIdx.x; i < n; i += BS) {
-      double u[D], f[NDF], fl[NDF];
-#pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-#pragma unroll
-      for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
-      if (MODE == MODE_FUSED) {
-        double s[D][NDF];
-#pragma unroll
-        for (int k = 0; k < NDF; ++k) {
-          fl[k] = 0.0;
-#pragma unroll
-          for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
-        }
-        gather_flux<D, NDF, UNI>(A, G, c, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
-      } else {
-#pragma unroll
-        for (int k = 0; k < NDF; ++k) {
-          fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
-          A.vflux[G.off * NDF + int64_t(k) * n + i] = 0.0;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < NDF; ++k) sc[k * n + i] = fl[k];
-      const double wt = G.wt(i);
-      if constexpr (D == 2) {
-        double cu = u[0] - prim_c[1], cv = u[1] - prim_c[2];
-        double c2 = cu * cu + cv * cv;
-        qa[0] += wt * cu * c2 * f[0];
-        qa[1] += wt * cu * f[1];
-        qa[2] += wt * cv * c2 * f[0];
-        qa[3] += wt * cv * f[1];
-      } else {
-        double cu = u[0] - prim_c[1], cv = u[1] - prim_c[2], cw = u[2] - prim_c[3];
-        double c2 = cu * cu + cv * cv + cw * cw;
-        qa[0] += wt * cu * c2 * f[0];
-        qa[1] += wt * cv * c2 * f[0];
-        qa[2] += wt * cw * c2 * f[0];
-      }
-    }
-    double qs[NQ], q[D];
-    block_sum<NQ>(qa, NQ, red, qs);
-    if constexpr (D == 2) {
-      q[0] = 0.5 * (qs[0] + qs[1]);
-      q[1] = 0.5 * (qs[2] + qs[3]);
-    } else {
-      q[0] = 0.5 * qs[0];
-      q[1] = 0.5 * qs[1];
-      q[2] = 0.5 * qs[2];
-    }
+    // Euler: f = (f + dt/V*flux)*tau/(tau+dt) + dt/(tau+dt)*F with F = M + Shakhov(M)
     const double tpd = tau + dt, a2 = dt / (tau + dt);
     for (int i = threadIdx.x; i < n; i += BS) {
       double u[D], M[2], Sp[2];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-      maxwell<D>(u, prim_c, A.K, M);
-      shakhov<D>(u, M, prim_c, q, A.Pr, A.K, Sp);
+      maxwell<D>(u, pc, A.K, M);
+      shakhov<D>(u, M, pc, q, A.Pr, A.K, Sp);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         double F = M[k] + Sp[k];
@@ -712,37 +731,38 @@ __global__ __launch_bounds__(BS) void k_flux_relax(DevArgs A, const int32_t* __r
         out[G.off * NDF + int64_t(k) * n + i] = fx;
       }
     }
-    if (threadIdx.x == 0) {
-      double* pr = &A.prim[int64_t(c) * NW];
-      double* rs = &A.res[int64_t(c) * 2 * NW];
+  }
+  if (threadIdx.x == 0) {
+    double* pr = &A.prim[int64_t(c) * NW];
+    double* rs = &A.res[int64_t(c) * 2 * NW];
 #pragma unroll
-      for (int m = 0; m < NW; ++m) {
-        double dl = prim_c[m] - pr[m];
-        rs[m] = dl * dl;
-        rs[NW + m] = fabs(prim_c[m]);
-        pr[m] = prim_c[m];
-        A.w[int64_t(c) * NW + m] = wnew[m];
-        A.psflux[int64_t(c) * NW + m] = 0.0;
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) A.qf[int64_t(c) * D + d] = q[d];
+    for (int m = 0; m < NW; ++m) {
+      double dl = pc[m] - pr[m];
+      rs[m] = dl * dl;
+      rs[NW + m] = fabs(pc[m]);
+      pr[m] = pc[m];
+      A.w[int64_t(c) * NW + m] = cell_sh[m];
+      A.psflux[int64_t(c) * NW + m] = 0.0;
     }
+#pragma unroll
+    for (int d = 0; d < D; ++d) A.qf[int64_t(c) * D + d] = q[d];
   }
 }
 
 // ------------------------------------------------------------------ K3: macro slope
 template <int D, int NDF, bool UNI>
-__global__ __launch_bounds__(BS) void k_macro_slope(DevArgs A, const int32_t* __restrict__ cells,
-                                                    int ncells, double* __restrict__ sw) {
+__global__ __launch_bounds__(256) void k_macro_slope(DevArgs A, const int32_t* __restrict__ cells,
+                                                     int ncells, double* __restrict__ sw) {
   constexpr int NW = D + 2;
   const int c = cells[blockIdx.x];
   const Grid<D, NDF, UNI> G(A, c);
-  __shared__ double red[NWAVE * NW];
+  __shared__ double red[4 * NW];
+  __shared__ double tot[NW];
   for (int d = 0; d < D; ++d) {
     double acc[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) acc[q] = 0.0;
-    for (int i = threadIdx.x; i < G.n; i += BS) {
+    for (int i = threadIdx.x; i < G.n; i += 256) {
       double u[D], s[NDF], m[NW];
 #pragma unroll
       for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
@@ -752,12 +772,29 @@ __global__ __launch_bounds__(BS) void k_macro_slope(DevArgs A, const int32_t* __
 #pragma unroll
       for (int q = 0; q < NW; ++q) acc[q] += m[q];
     }
-    double tot[NW];
-    block_sum<NW>(acc, NW, red, tot);
-    if (threadIdx.x == 0)
-      for (int q = 0; q < NW; ++q)
-        sw[(int64_t(c) * NW + q) * D + d] = (q == NW - 1) ? 0.5 * tot[q] : tot[q];
+    block_reduce<NW, 4>(acc, NW, red, tot);
+    if (int(threadIdx.x) < NW)
+      sw[(int64_t(c) * NW + threadIdx.x) * D + d] =
+          (int(threadIdx.x) == NW - 1) ? 0.5 * tot[threadIdx.x] : tot[threadIdx.x];
   }
+}
+
+// ------------------------------------------------------------------ copy reference
+// A plain streaming copy (16-byte vector loads/stores) used to measure attainable HBM
+// bandwidth and to calibrate hardware byte counters against a kernel of known traffic.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
+                                              double2* __restrict__ dst, int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * 256;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
+  const int64_t n = bytes / 16;
+  int blocks = int(std::min<int64_t>((n + 255) / 256, 256 * 64));
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, static_cast<const double2*>(src),
+                     static_cast<double2*>(dst), n);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ launchers
@@ -765,40 +802,54 @@ template <int D, int NDF, bool UNI>
 static hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, int maxh,
                                  hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  if (maxh <= 8)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 8>), dim3(n), dim3(BS), 0, st, A, cells, n);
+  if (maxh <= 4)
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 4, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
+  else if (maxh <= 8)
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 8, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
   else if (maxh <= 16)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 16>), dim3(n), dim3(BS), 0, st, A, cells, n);
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 16, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
   else
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24>), dim3(n), dim3(BS), 0, st, A, cells, n);
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
   return hipGetLastError();
 }
 
-template <int D, int NDF, bool UNI, int MODE, int MARCH>
+template <int D, int NDF, bool UNI, int MODE, int MARCH, int NWV>
 static hipError_t launch_fr_t(const DevArgs& A, const int32_t* cells, int n, size_t lds,
                               hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (lds > 0)
-    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, true>), dim3(n), dim3(BS), lds,
-                       st, A, cells, n);
+    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, true, NWV>), dim3(n),
+                       dim3(NWV * 64), lds, st, A, cells, n);
   else
-    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, false>), dim3(n), dim3(BS), 0,
-                       st, A, cells, n);
+    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, false, NWV>), dim3(n),
+                       dim3(NWV * 64), 0, st, A, cells, n);
   return hipGetLastError();
+}
+
+template <int D, int NDF, bool UNI, int NWV>
+static hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int mode,
+                              int march, size_t lds, hipStream_t st) {
+  if (mode == MODE_FLUX)
+    return launch_fr_t<D, NDF, UNI, MODE_FLUX, MARCH_CAIDVM, NWV>(A, cells, n, 0, st);
+  if (mode == MODE_ITER) {
+    if (march == MARCH_CAIDVM)
+      return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+    return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_EULER, NWV>(A, cells, n, lds, st);
+  }
+  if (march == MARCH_CAIDVM)
+    return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+  return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_EULER, NWV>(A, cells, n, lds, st);
 }
 
 template <int D, int NDF, bool UNI>
 static hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int mode,
-                              int march, size_t lds, hipStream_t st) {
-  if (mode == MODE_FLUX) return launch_fr_t<D, NDF, UNI, MODE_FLUX, MARCH_CAIDVM>(A, cells, n, 0, st);
-  if (mode == MODE_ITER) {
-    if (march == MARCH_CAIDVM)
-      return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_CAIDVM>(A, cells, n, lds, st);
-    return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_EULER>(A, cells, n, lds, st);
+                              int march, size_t lds, int nwv, hipStream_t st) {
+  switch (nwv) {
+    case 4: return launch_fr_w<D, NDF, UNI, 4>(A, cells, n, mode, march, lds, st);
+    case 8: return launch_fr_w<D, NDF, UNI, 8>(A, cells, n, mode, march, lds, st);
+    case 12: return launch_fr_w<D, NDF, UNI, 12>(A, cells, n, mode, march, lds, st);
+    default: return launch_fr_w<D, NDF, UNI, 16>(A, cells, n, mode, march, lds, st);
   }
-  if (march == MARCH_CAIDVM)
-    return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_CAIDVM>(A, cells, n, lds, st);
-  return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_EULER>(A, cells, n, lds, st);
 }
 
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
@@ -812,24 +863,24 @@ hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cell
 }
 
 hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                             int mode, int march, size_t lds, hipStream_t st) {
+                             int mode, int march, size_t lds, int nwv, hipStream_t st) {
   if (dim == 2) {
-    return uni ? launch_fr_d<2, 2, true>(A, cells, n, mode, march, lds, st)
-               : launch_fr_d<2, 2, false>(A, cells, n, mode, march, lds, st);
+    return uni ? launch_fr_d<2, 2, true>(A, cells, n, mode, march, lds, nwv, st)
+               : launch_fr_d<2, 2, false>(A, cells, n, mode, march, lds, nwv, st);
   }
-  return uni ? launch_fr_d<3, 1, true>(A, cells, n, mode, march, lds, st)
-             : launch_fr_d<3, 1, false>(A, cells, n, mode, march, lds, st);
+  return uni ? launch_fr_d<3, 1, true>(A, cells, n, mode, march, lds, nwv, st)
+             : launch_fr_d<3, 1, false>(A, cells, n, mode, march, lds, nwv, st);
 }
 
 hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                               double* sw, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (dim == 2) {
-    if (uni) hipLaunchKernelGGL((k_macro_slope<2, 2, true>), dim3(n), dim3(BS), 0, st, A, cells, n, sw);
-    else hipLaunchKernelGGL((k_macro_slope<2, 2, false>), dim3(n), dim3(BS), 0, st, A, cells, n, sw);
+    if (uni) hipLaunchKernelGGL((k_macro_slope<2, 2, true>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
+    else hipLaunchKernelGGL((k_macro_slope<2, 2, false>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
   } else {
-    if (uni) hipLaunchKernelGGL((k_macro_slope<3, 1, true>), dim3(n), dim3(BS), 0, st, A, cells, n, sw);
-    else hipLaunchKernelGGL((k_macro_slope<3, 1, false>), dim3(n), dim3(BS), 0, st, A, cells, n, sw);
+    if (uni) hipLaunchKernelGGL((k_macro_slope<3, 1, true>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
+    else hipLaunchKernelGGL((k_macro_slope<3, 1, false>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
   }
   return hipGetLastError();
 }

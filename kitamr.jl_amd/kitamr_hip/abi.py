@@ -24,7 +24,9 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "devi
 
 EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "kah_set_state",
            "kah_get_state", "kah_get_flux", "kah_slope", "kah_flux", "kah_iterate", "kah_step",
-           "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times"]
+           "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
+           "kah_copy_bandwidth", "kah_set_option"]
+OPT = {"flux_waves": 1}
 
 
 class KahParams(C.Structure):
@@ -138,6 +140,10 @@ def load_library(path=None):
     lib.kah_device_bytes.argtypes = [C.c_void_p]
     lib.kah_device_bytes.restype = i64
     lib.kah_kernel_times.argtypes = [C.c_void_p, C.c_int, dp, C.POINTER(i64)]
+    lib.kah_copy_bandwidth.argtypes = [C.c_void_p, i64, C.c_int, dp]
+    lib.kah_copy_bandwidth.restype = C.c_int
+    lib.kah_set_option.argtypes = [C.c_void_p, C.c_int, i64]
+    lib.kah_set_option.restype = C.c_int
     for name in ["kah_set_topology", "kah_set_state", "kah_get_state", "kah_get_flux",
                  "kah_slope", "kah_flux", "kah_iterate", "kah_step", "kah_sync",
                  "kah_kernel_times"]:
@@ -255,6 +261,16 @@ class HipContext:
 
     def device_bytes(self):
         return int(self.lib.kah_device_bytes(self.h))
+
+    def set_option(self, name, value):
+        self._check(self.lib.kah_set_option(self.h, OPT[name], int(value)), "kah_set_option")
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        """Attainable HBM bandwidth (GB/s) of a plain streaming copy of nbytes."""
+        ms = np.zeros(1)
+        self._check(self.lib.kah_copy_bandwidth(self.h, int(nbytes), int(reps), _ptr(ms)),
+                    "kah_copy_bandwidth")
+        return 2.0 * nbytes / (ms[0] * 1e-3) / 1e9, float(ms[0])
 
     def kernel_times(self, reset=False):
         ms = np.zeros(2)
