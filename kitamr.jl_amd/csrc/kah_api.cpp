@@ -199,20 +199,29 @@ struct Builder {
     }
     return false;
   }
+  static double inv_count(int32_t n) { return n > 0 ? 1.0 / double(n) : 0.0; }
   void inner(SlopeDesc& s, int64_t c, int dir, double dsL, double dsR, bool pL, bool pR) {
     s.mode = SL_INNER;
     s.ds[0] = dsL;
     s.ds[1] = dsR;
+    s.rds[0] = 1.0 / dsL;
+    s.rds[1] = 1.0 / dsR;
     add_side(s, 0, c, 2 * dir, dir, pL);
     add_side(s, 1, c, 2 * dir + 1, dir, pR);
+    s.inv_cnt[0] = inv_count(s.nb_cnt[0]);
+    s.inv_cnt[1] = inv_count(s.nb_cnt[1]);
   }
   void bound(SlopeDesc& s, int64_t c, int f, int dir, double ds, bool proj) {
     s.mode = SL_BOUND;
     s.ds[0] = ds;
     s.ds[1] = 0.0;
+    s.rds[0] = 1.0 / ds;
+    s.rds[1] = 0.0;
     add_side(s, 0, c, f, dir, proj);
     s.nb_beg[1] = 0;
     s.nb_cnt[1] = 0;
+    s.inv_cnt[0] = inv_count(s.nb_cnt[0]);
+    s.inv_cnt[1] = 0.0;
   }
 
   // update_slope!(Val(sL), Val(sR), ...), src/Flux/Slope.jl:458-771
@@ -638,6 +647,9 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.ents = C->d_ents.as<EntryDesc>();
   A.rec_ds = C->d_recds.as<double>();
   A.bc_prim = C->d_bc.as<double>();
+  A.sdf_r = A.sdf;
+  A.half_fw_r = A.half_fw;
+  A.rho_w_r = A.rho_w;
   A.K = C->P.K;
   A.gamma = C->P.gamma;
   A.Pr = C->P.Pr;

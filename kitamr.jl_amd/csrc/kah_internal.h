@@ -42,22 +42,24 @@ struct SlopeNb {        // one neighbour in a slope stencil
   double dm[3];         // transverse offsets (dm[dir] = 0)
 };
 
+// Descriptor fields are 32-bit or wider: the kernels read them with scalar loads, which
+// have no byte form on gfx9.
 struct SlopeDesc {      // per (fluid cell, direction)
-  int8_t mode;
-  int8_t pad[3];
+  int32_t mode;
+  int32_t pad;
   int32_t nb_beg[2];    // side 0 = L (or the one side of a BOUND), side 1 = R
   int32_t nb_cnt[2];
-  int32_t pad2;
   double ds[2];         // dsL, dsR
+  double rds[2];        // 1/dsL, 1/dsR (correctly rounded)
+  double inv_cnt[2];    // 1/nb_cnt (exact: the counts are powers of two)
 };
 
 struct HalfDesc {       // one half flux computed by its owner cell in K1
   int32_t slot;         // output row in half_fw (pair*2 + side)
   int32_t data_cell;    // cell whose f/sdf feed the micro flux
-  int8_t mode;          // MI_*
-  int8_t sel;           // 0: rot*u[dir] <= 0 ; 1: rot*u[dir] > 0
-  int8_t dir;
-  int8_t pad;
+  int32_t mode;         // MI_*
+  int32_t sel;          // 0: rot*u[dir] <= 0 ; 1: rot*u[dir] > 0
+  int32_t dir;
   int32_t bc;           // boundary condition (domain halves) or -1
   double rot;
   double xrec[3];       // cell-record midpoint used for the reconstruction
@@ -68,17 +70,27 @@ struct HalfDesc {       // one half flux computed by its owner cell in K1
 struct EntryDesc {      // one face side of a fluid cell, gathered in K2
   int32_t pair;         // half fluxes at half_fw[pair*2 + 0/1]
   int32_t oth_cell;     // data cell of the other side (-1 for domain faces)
-  int8_t side;          // 0: the cell is "here" (adds), 1: "there" (subtracts)
-  int8_t own_mode;      // MI_* of the cell's own upwind set
-  int8_t oth_mode;      // MI_* of the incoming set
-  int8_t dir;
+  int32_t side;         // 0: the cell is "here" (adds), 1: "there" (subtracts)
+  int32_t own_mode;     // MI_* of the cell's own upwind set
+  int32_t oth_mode;     // MI_* of the incoming set
+  int32_t dir;
   int32_t bc;
+  int32_t pad;
   double rot;
   double area;          // signed face area, rot*A (or rot*A/2^(DIM-1))
   double fmid[3];
   double xown[3], dsown[3];
   double xoth[3], dsoth[3];
 };
+
+// a / b given rb = RN(1/b): q = RN(a*rb) corrected by one FMA step on the exact residual,
+// which is the correctly rounded quotient (Markstein's theorem; no overflow or underflow
+// in the quotient). Used where b is uniform over a workgroup.
+__host__ __device__ inline double div_by(double a, double b, double rb) {
+  double q = a * rb;
+  double e = fma(-q, b, a);
+  return fma(e, rb, q);
+}
 
 // ---------------------------------------------------------------- point math (FP64)
 // Every expression keeps the reference's association order; the library is compiled
@@ -148,6 +160,63 @@ __host__ __device__ inline void shakhov(const double* u, const double* F, const 
     double c2 = cu * cu + cv * cv + cw * cw;
     double cq = cu * q[0] + cv * q[1] + cw * q[2];
     out[0] = 0.8 * (1 - Pr) * (p[4] * p[4]) / p[0] * cq * (2 * p[4] * c2 - 5) * F[0];
+  }
+}
+
+// The same two functions with the factors that depend only on prim hoisted out of the
+// per-velocity-cell work (identical roundings: the hoisted subexpressions are the leading
+// factors of the reference's left-to-right products).
+template <int D>
+struct MaxPre {
+  double p[D + 2];
+  double a0;      // rho*(lambda/pi) (2D) or rho*(lambda/pi)^(3/2) (3D)
+  double two_l;   // 2*lambda
+  double r2l;     // RN(1/(2*lambda))
+  double s0;      // 0.8*(1-Pr)*lambda^2/rho
+};
+template <int D>
+__host__ __device__ inline MaxPre<D> max_pre(const double* p, double Pr) {
+  MaxPre<D> m;
+  for (int q = 0; q < D + 2; ++q) m.p[q] = p[q];
+  const double lam = p[D + 1];
+  if constexpr (D == 2) m.a0 = p[0] * (lam / PI);
+  else m.a0 = p[0] * pow(lam / PI, 1.5);
+  m.two_l = 2.0 * lam;
+  m.r2l = 1.0 / m.two_l;
+  m.s0 = 0.8 * (1 - Pr) * (lam * lam) / p[0];
+  return m;
+}
+template <int D>
+__host__ __device__ inline void maxwell_pre(const double* u, const MaxPre<D>& m, double K,
+                                            double* out) {
+  if constexpr (D == 2) {
+    double a = u[0] - m.p[1], b = u[1] - m.p[2];
+    double c2 = a * a + b * b;
+    double h = m.a0 * exp(-m.p[3] * c2);
+    out[0] = h;
+    out[1] = div_by(h * K, m.two_l, m.r2l);
+  } else {
+    double a = u[0] - m.p[1], b = u[1] - m.p[2], c = u[2] - m.p[3];
+    double c2 = a * a + b * b + c * c;
+    out[0] = m.a0 * exp(-m.p[4] * c2);
+  }
+}
+template <int D>
+__host__ __device__ inline void shakhov_pre(const double* u, const double* F,
+                                            const MaxPre<D>& m, const double* q, double K,
+                                            double* out) {
+  const double* p = m.p;
+  if constexpr (D == 2) {
+    double cu = u[0] - p[1], cv = u[1] - p[2];
+    double c2 = cu * cu + cv * cv;
+    double cq = cu * q[0] + cv * q[1];
+    out[0] = m.s0 * cq * (2 * p[3] * c2 + K - 5) * F[0];
+    out[1] = m.s0 * cq * (2 * p[3] * c2 + K - 3) * F[1];
+  } else {
+    double cu = u[0] - p[1], cv = u[1] - p[2], cw = u[2] - p[3];
+    double c2 = cu * cu + cv * cv + cw * cw;
+    double cq = cu * q[0] + cv * q[1] + cw * q[2];
+    out[0] = m.s0 * cq * (2 * p[4] * c2 - 5) * F[0];
   }
 }
 

@@ -42,11 +42,36 @@ struct DevArgs {
   const EntryDesc* ents;
   const double* rec_ds;     // [nc*D]
   const double* bc_prim;
+  // read views of arrays a kernel does not write (bound to __restrict__ kernel arguments
+  // so that uniform loads from them can be scalar loads)
+  const double* sdf_r;
+  const double* half_fw_r;
+  const double* rho_w_r;
   // parameters
   double K, gamma, Pr, mu, omega, dt;
   int32_t flux_kind;
   int32_t pad;
 };
+
+// Read-only arrays passed to the kernels as separate __restrict__ arguments.
+#define KAH_RO_PARAMS                                                                        \
+  const double* __restrict__ ro_df, const double* __restrict__ ro_gu,                        \
+      const double* __restrict__ ro_gw, const int8_t* __restrict__ ro_glev,                  \
+      const uint32_t* __restrict__ ro_gkey, const int64_t* __restrict__ ro_voff,             \
+      const SlopeDesc* __restrict__ ro_sl, const SlopeNb* __restrict__ ro_slnb,              \
+      const int32_t* __restrict__ ro_hoff, const HalfDesc* __restrict__ ro_halves,           \
+      const int32_t* __restrict__ ro_eoff, const EntryDesc* __restrict__ ro_ents,            \
+      const double* __restrict__ ro_bc, const double* __restrict__ ro_recds
+#define KAH_RO_ARGS(A)                                                                       \
+  (A).df, (A).gu, (A).gw, (A).glev, (A).gkey, (A).voff, (A).sl, (A).slnb, (A).half_off,      \
+      (A).halves, (A).ent_off, (A).ents, (A).bc_prim, (A).rec_ds
+#define KAH_RO_BIND(B)                                                                       \
+  do {                                                                                       \
+    (B).df = ro_df; (B).gu = ro_gu; (B).gw = ro_gw; (B).glev = ro_glev; (B).gkey = ro_gkey;  \
+    (B).voff = ro_voff; (B).sl = ro_sl; (B).slnb = ro_slnb; (B).half_off = ro_hoff;          \
+    (B).halves = ro_halves; (B).ent_off = ro_eoff; (B).ents = ro_ents; (B).bc_prim = ro_bc;  \
+    (B).rec_ds = ro_recds;                                                                   \
+  } while (0)
 
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                         int maxh, hipStream_t st);

@@ -88,7 +88,7 @@ struct Grid {
   __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
   __device__ double f(int i, int k) const { return A.df[off * NDF + int64_t(k) * n + i]; }
   __device__ double s(int i, int k, int d) const {
-    return A.sdf[off * NDF * D + int64_t(d * NDF + k) * n + i];
+    return A.sdf_r[off * NDF * D + int64_t(d * NDF + k) * n + i];
   }
   __device__ double* sp(int i, int k, int d) const {
     return &A.sdf[off * NDF * D + int64_t(d * NDF + k) * n + i];
@@ -207,14 +207,16 @@ __device__ inline void micro_maxwell(const double* bc, double rho, bool wall,
 // src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
 // taken as key-interval overlaps.
 template <int D, int NDF, bool UNI>
-__device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, int i, int li,
-                                uint32_t ki, const double (&f)[NDF], double (&a)[NDF]) {
+__device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, double rdsx,
+                                int i, int li, uint32_t ki, const double (&f)[NDF],
+                                double (&a)[NDF]) {
 #pragma unroll
   for (int k = 0; k < NDF; ++k) a[k] = 0.0;
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
     const Grid<D, NDF, UNI> N(A, nb.cell);
     for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
+      const double rden = 1.0 / den;   // den is a power of two: exact
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         double pj = N.f(j, k);
@@ -223,8 +225,8 @@ __device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, 
           for (int t = 0; t < D; ++t) pj += nb.dm[t] * N.s(j, k, t);
         }
         double term = f[k] - pj;
-        if (den != 1.0) term = term / den;
-        a[k] += term / dsx;
+        if (den != 1.0) term = term * rden;
+        a[k] += div_by(term, dsx, rdsx);
       }
     });
   }
@@ -249,16 +251,16 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI>& G, i
     return;
   }
   double aL[NDF];
-  side_sum<D, NDF, UNI>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], i, li, ki, f, aL);
+  side_sum<D, NDF, UNI>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li, ki, f, aL);
   if (sd.mode == SL_INNER) {
     double aR[NDF];
-    side_sum<D, NDF, UNI>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], i, li, ki, f, aR);
+    side_sum<D, NDF, UNI>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li, ki, f,
+                          aR);
 #pragma unroll
-    for (int k = 0; k < NDF; ++k)
-      s[k] = minmod(aL[k] / double(sd.nb_cnt[0]), aR[k] / double(sd.nb_cnt[1]));
+    for (int k = 0; k < NDF; ++k) s[k] = minmod(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
   } else {
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) s[k] = aL[k] / double(sd.nb_cnt[0]);
+    for (int k = 0; k < NDF; ++k) s[k] = aL[k] * sd.inv_cnt[0];
   }
 #pragma unroll
   for (int k = 0; k < NDF; ++k) *G.sp(i, k, d) = s[k];
@@ -271,8 +273,10 @@ template <int D, int NDF, bool UNI>
 __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c,
                                   const HalfDesc& hd, int i, const double (&u)[D],
                                   const double (&f)[NDF], const double (&s)[D][NDF], double wt,
-                                  double* acc) {
+                                  double (&m)[D + 2]) {
   constexpr int NW = D + 2;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) m[q] = 0.0;
   const double ud = pick<D>(u, hd.dir);
   const double ru = hd.rot * ud;
   const bool in = hd.sel ? (ru > 0.0) : (ru <= 0.0);
@@ -285,7 +289,7 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, 
         double cu = u[t] - bc[1 + t];
         c2 = (t == 0) ? cu * cu : c2 + cu * cu;
       }
-      acc[1] += wt * ud * exp(-bc[D + 1] * c2);
+      m[1] = wt * ud * exp(-bc[D + 1] * c2);
     } else {
       double dx[D];
 #pragma unroll
@@ -294,7 +298,7 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, 
 #pragma unroll
       for (int t = 1; t < D; ++t) hrec = hrec + dx[t] * s[t][0];
       hrec = f[0] + hrec;
-      acc[0] += wt * ud * hrec;
+      m[0] = wt * ud * hrec;
     }
     return;
   }
@@ -313,18 +317,17 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, 
 #pragma unroll
     for (int k = 0; k < NDF; ++k) mic[k] = S.f(i, k) * ud;
   }
-  double m[NW];
   moment_terms<D, NDF>(wt, u, mic, m);
-#pragma unroll
-  for (int q = 0; q < NW; ++q) acc[q] += m[q];
 }
 
 template <int D, int NDF, bool UNI, int MAXH, int NWV>
-__global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A, const int32_t* __restrict__ cells,
-                                                    int ncells) {
+__global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* __restrict__ cells,
+                                                    int ncells, KAH_RO_PARAMS) {
   constexpr int NW = D + 2;
   constexpr int BS = NWV * 64;
   constexpr int CNT = MAXH * NW;
+  DevArgs A = A0;
+  KAH_RO_BIND(A);
   const int c = cells[xcd_remap(blockIdx.x, ncells)];
   const Grid<D, NDF, UNI> G(A, c);
   const int hb = A.half_off[c];
@@ -349,8 +352,17 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A, const int32_t* __
     for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
     const double wt = G.wt(i);
 #pragma unroll
-    for (int h = 0; h < MAXH; ++h)
-      if (h < nh) half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, &acc[h * NW]);
+    for (int h = 0; h < MAXH; ++h) {
+      double m[NW];
+      if (h < nh) {
+        half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) m[q] = 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < NW; ++q) acc[h * NW + q] += m[q];
+    }
   }
 
   block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
@@ -434,7 +446,7 @@ __device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
     switch (ed.oth_mode) {
       case MI_MAXWELL:
       case MI_WALL: {
-        micro_maxwell<D, NDF>(&A.bc_prim[ed.bc * NW], A.rho_w[ed.pair],
+        micro_maxwell<D, NDF>(&A.bc_prim[ed.bc * NW], A.rho_w_r[ed.pair],
                               ed.oth_mode == MI_WALL, u, A.K, ed.dir, mic);
 #pragma unroll
         for (int k = 0; k < NDF; ++k) fl[k] += area * mic[k];
@@ -514,12 +526,20 @@ __device__ inline void heat_finish(const double* qs, double* q) {
 // for one cell, fed by the gathered flux (MODE_FUSED), by the flux accumulators
 // (MODE_ITER), or only accumulating the flux (MODE_FLUX).
 template <int D, int NDF, bool UNI, int MODE, int MARCH, bool LDS, int NWV>
-__global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A,
+__global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
                                                          const int32_t* __restrict__ cells,
-                                                         int ncells) {
+                                                         int ncells, KAH_RO_PARAMS,
+                                                         const double* __restrict__ ro_sdf,
+                                                         const double* __restrict__ ro_half,
+                                                         const double* __restrict__ ro_rho) {
   constexpr int NW = D + 2;
   constexpr int NQ = (D == 2) ? 4 : 3;   // heat-flux partial sums
   constexpr int BS = NWV * 64;
+  DevArgs A = A0;
+  KAH_RO_BIND(A);
+  A.sdf_r = ro_sdf;
+  A.half_fw_r = ro_half;
+  A.rho_w_r = ro_rho;
   const int c = cells[xcd_remap(blockIdx.x, ncells)];
   const Grid<D, NDF, UNI> G(A, c);
   const int n = G.n;
@@ -545,7 +565,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A,
     if (MODE != MODE_ITER && A.flux_kind == 0) {
       for (int e = 0; e < ne; ++e) {
         const EntryDesc& ed = A.ents[eb + e];
-        const double* h0 = &A.half_fw[int64_t(ed.pair) * 2 * NW];
+        const double* h0 = &A.half_fw_r[int64_t(ed.pair) * 2 * NW];
         const double* h1 = h0 + NW;
 #pragma unroll
         for (int q = 0; q < NW; ++q) {
@@ -669,12 +689,13 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A,
         pc[q] = cell_sh[NW + q];
         pr[q] = cell_sh[2 * NW + q];
       }
+      const MaxPre<D> mpc = max_pre<D>(pc, A.Pr), mpr = max_pre<D>(pr, A.Pr);
       for (int i = threadIdx.x; i < n; i += BS) {
         double u[D], Mc[2], M[2];
 #pragma unroll
         for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-        maxwell<D>(u, pc, A.K, Mc);
-        maxwell<D>(u, pr, A.K, M);
+        maxwell_pre<D>(u, mpc, A.K, Mc);
+        maxwell_pre<D>(u, mpr, A.K, M);
         double fs[NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -700,13 +721,14 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A,
   if (MARCH == MARCH_CAIDVM) {
     // pass 3: g+ = M(prim_c) + Shakhov part; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
     const double a1 = tau / (tau + dt), a2 = dt / (tau + dt);
+    const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
     for (int i = threadIdx.x; i < n; i += BS) {
       double u[D], Fc[2], Sp[2];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
       Fc[0] = sc[NDF * n + i];
-      if constexpr (NDF == 2) Fc[1] = Fc[0] * A.K / (2.0 * pc[D + 1]);
-      shakhov<D>(u, Fc, pc, q, A.Pr, A.K, Sp);
+      if constexpr (NDF == 2) Fc[1] = div_by(Fc[0] * A.K, mpc.two_l, mpc.r2l);
+      shakhov_pre<D>(u, Fc, mpc, q, A.K, Sp);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         double g = Fc[k] + Sp[k];
@@ -718,12 +740,13 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A,
   } else {
     // Euler: f = (f + dt/V*flux)*tau/(tau+dt) + dt/(tau+dt)*F with F = M + Shakhov(M)
     const double tpd = tau + dt, a2 = dt / (tau + dt);
+    const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
     for (int i = threadIdx.x; i < n; i += BS) {
       double u[D], M[2], Sp[2];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-      maxwell<D>(u, pc, A.K, M);
-      shakhov<D>(u, M, pc, q, A.Pr, A.K, Sp);
+      maxwell_pre<D>(u, mpc, A.K, M);
+      shakhov_pre<D>(u, M, mpc, q, A.K, Sp);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         double F = M[k] + Sp[k];
@@ -803,13 +826,17 @@ static hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, 
                                  hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (maxh <= 4)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 4, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 4, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
   else if (maxh <= 8)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 8, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 8, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
   else if (maxh <= 16)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 16, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 16, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
   else
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n);
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
   return hipGetLastError();
 }
 
@@ -819,10 +846,12 @@ static hipError_t launch_fr_t(const DevArgs& A, const int32_t* cells, int n, siz
   if (n <= 0) return hipSuccess;
   if (lds > 0)
     hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, true, NWV>), dim3(n),
-                       dim3(NWV * 64), lds, st, A, cells, n);
+                       dim3(NWV * 64), lds, st, A, cells, n, KAH_RO_ARGS(A), A.sdf_r,
+                       A.half_fw_r, A.rho_w_r);
   else
     hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, false, NWV>), dim3(n),
-                       dim3(NWV * 64), 0, st, A, cells, n);
+                       dim3(NWV * 64), 0, st, A, cells, n, KAH_RO_ARGS(A), A.sdf_r,
+                       A.half_fw_r, A.rho_w_r);
   return hipGetLastError();
 }
 

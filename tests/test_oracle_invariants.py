@@ -1,0 +1,196 @@
+"""Invariants that pin the oracle's slope!/flux!/iterate! restatement (src/Flux/Slope.jl,
+src/Flux/Flux.jl, src/Theory/Iterate.jl) without the reference's own outputs.
+
+- slope: for f linear in x (per velocity cell) the limited slopes are exact on uniform and
+  refined meshes: this fixes the neighbour distances (1, 0.75, 1.5 ds), their signs, the
+  one-sided boundary forms, the level order of the sweep and the transverse projection on
+  coarse-neighbour faces (without it the coarse neighbour's offset leaks the y-gradient
+  into the x-slope);
+- flux: with periodic sides every face is counted twice with opposite signs, so the sum of
+  the micro fluxes over all cells vanishes per velocity cell and mass, momentum and energy
+  are conserved to round-off over several steps;
+- iterate: a uniform state on a periodic box stays uniform.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from kitamr_hip import cases
+from kitamr_hip.solver import Configure, Domain, Gas, PCoordFn, Solver, Uniform, build_case, kah_params
+
+
+def linear_state(case, seed=0):
+    """df[c, k, v] = a[k, v] + b[k, v]*x_c + e[k, v]*y_c (+ z term in 3D)."""
+    rng = np.random.default_rng(seed)
+    topo = case.topo
+    n, dim, ndf = case.n_cells, case.config.solver.DIM, case.config.solver.NDF
+    off = topo["vs_offset"]
+    nv = int(off[1] - off[0])
+    mid = topo["cell_mid"][:n].reshape(n, dim)
+    a = rng.uniform(1.0, 2.0, (ndf, nv))
+    g = rng.uniform(-1.0, 1.0, (dim, ndf, nv))
+    df = a[None] + np.einsum("cd,dkv->ckv", mid, g)
+    return df.reshape(-1), g
+
+
+def slope_of(case, df):
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, df)
+    o.slope()
+    st = o.get_state()
+    o.close()
+    return st["sdf"]
+
+
+@pytest.mark.parametrize("name,maker", [
+    ("uniform", lambda: cases.riemann_2d(n=8, nv=6)),
+    ("amr2", lambda: cases.riemann_2d(n=8, nv=4, ps_levels=2, halo=(2.0, 1.0))),
+    ("amr3", lambda: cases.riemann_2d(n=8, nv=4, ps_levels=3, halo=(2.0, 1.5, 1.0))),
+    ("box3d_amr", lambda: cases.box_3d(n=4, nv=4, ps_levels=1)),
+])
+def test_slope_exact_for_linear_fields(name, maker):
+    case = maker()
+    assert case.topo["vs_uniform"] == 1
+    df, g = linear_state(case)
+    sdf = slope_of(case, df)
+    n, dim, ndf = case.n_cells, case.config.solver.DIM, case.config.solver.NDF
+    nv = g.shape[2]
+    sdf = sdf.reshape(n, dim, ndf, nv)
+    expect = np.broadcast_to(g[None], sdf.shape)
+    err = np.max(np.abs(sdf - expect))
+    assert err < 1e-9, f"{name}: max slope error {err}"
+    levels = set(case.topo["ps_level"].tolist())
+    if name.startswith("amr"):
+        assert len(levels) > 2
+
+
+def test_slope_without_transverse_projection_would_fail():
+    """Control for the previous test: on the refined mesh a y-gradient alone produces
+    x-slopes only where a coarse neighbour sits beside a fine cell; the oracle removes it.
+    Here we check the size of the effect the projection removes is far above the tolerance."""
+    case = cases.riemann_2d(n=8, nv=4, ps_levels=2, halo=(2.0, 1.0))
+    n = case.n_cells
+    mid = case.topo["cell_mid"][:n].reshape(n, 2)
+    st = case.topo["nb_state"].reshape(n, 4)
+    # a coarse x-neighbour is offset by ds/2 in y from a fine cell: that would leak
+    # 0.5*ds*|dfdy|/(1.5*ds) = |dfdy|/3 into dfdx
+    has_coarse_x = np.any(st[:, :2] == -1, axis=1)
+    assert has_coarse_x.sum() > 10
+    df = np.tile(mid[:, 1][:, None, None] * np.ones((1, 2, 16)), 1).reshape(-1)
+    sdf = slope_of(case, df).reshape(n, 2, 2, 16)
+    assert np.max(np.abs(sdf[:, 0])) < 1e-9          # no leak into d/dx
+    assert np.max(np.abs(sdf[:, 1] - 1.0)) < 1e-9    # exact d/dy
+
+
+def test_minmod_clips_extrema():
+    """A V-shaped field (|x - x0|) has opposite one-sided slopes at its kink: minmod gives 0."""
+    case = cases.riemann_2d(n=8, nv=4)
+    n = case.n_cells
+    mid = case.topo["cell_mid"][:n].reshape(n, 2)
+    x0 = mid[:, 0][np.argsort(mid[:, 0])][n // 2]       # a cell-centre column
+    val = np.abs(mid[:, 0] - x0)
+    df = np.repeat(val, 2 * 16)
+    sdf = slope_of(case, df).reshape(n, 2, 2, 16)
+    kink = np.isclose(mid[:, 0], x0)
+    assert kink.sum() == 8
+    assert np.all(sdf[kink, 0] == 0.0)
+    away = np.abs(mid[:, 0] - x0) > 0.2
+    assert np.allclose(np.abs(sdf[away, 0]), 1.0, atol=1e-12)
+
+
+def periodic_box(n=8, nv=8, uniform=False, ps_levels=0):
+    solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=0, CFL=0.8)
+    gas = Gas(K=1.0, Kn=0.05, omega=0.5)
+    dom = [Domain("Period", i + 1) for i in range(4)]
+    ic = Uniform([1.0, 0.2, -0.1, 1.0]) if uniform else PCoordFn(cases.riemann_prim)
+    cfg = Configure(solver, gas, [0.0, 1.0, 0.0, 1.0], [n, n], [-5.0, 5.0, -5.0, 5.0],
+                    [nv, nv], ic, dom)
+    refine = None
+    if ps_levels:
+        def refine(mid, ds, level):
+            return (np.abs(mid[:, 0] - 0.5) < 0.2) & (np.abs(mid[:, 1] - 0.5) < 0.3)
+    return build_case(cfg, ps_refine=refine)
+
+
+@pytest.mark.parametrize("ps_levels", [0, 1])
+def test_periodic_conservation(ps_levels):
+    case = periodic_box(ps_levels=ps_levels)
+    n = case.n_cells
+    V = np.prod(case.topo["cell_ds"][:n].reshape(n, 2), axis=1)
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    tot0 = (case.w.reshape(n, 4) * V[:, None]).sum(0)
+    for s in range(5):
+        o.slope()
+        o.flux(case.dt_xi)
+        f, vf = o.get_flux()
+        # every face counted twice with opposite signs
+        vf = vf.reshape(n, 2, -1)
+        assert np.max(np.abs(vf.sum(0))) < 1e-13 * max(1.0, np.max(np.abs(vf)))
+        assert np.max(np.abs(f.reshape(n, 4).sum(0))) < 1e-13 * max(1.0, np.max(np.abs(f)))
+        o.iterate(case.dt_xi, s)
+    tot = (o.get_state()["w"].reshape(n, 4) * V[:, None]).sum(0)
+    assert np.allclose(tot, tot0, rtol=1e-13, atol=1e-14), tot - tot0
+    o.close()
+
+
+def test_uniform_state_stays_uniform():
+    case = periodic_box(uniform=True)
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    for s in range(3):
+        o.step(case.dt_xi, s)
+    st = o.get_state()
+    n = case.n_cells
+    w = st["w"].reshape(n, 4)
+    df = st["df"].reshape(n, -1)
+    assert np.max(np.abs(w - w[0])) < 1e-14
+    assert np.max(np.abs(df - df[0])) < 1e-14 * np.max(np.abs(df))
+    assert np.allclose(w[0], case.w[:4], rtol=1e-14, atol=1e-15)
+    o.close()
+
+
+def test_oracle_is_thread_count_invariant():
+    case = cases.riemann_2d(n=8, nv=8, ps_levels=2, halo=(2.0, 1.0))
+    res = []
+    for th in (1, 4):
+        O.set_threads(th)
+        o = O.Oracle(kah_params(case.config), case.topo)
+        o.set_state(case.w, case.prim, case.df)
+        for s in range(2):
+            o.step(case.dt_xi, s)
+        res.append(o.get_state())
+        o.close()
+    O.set_threads(0)
+    for k in ("df", "w", "sdf", "qf"):
+        assert np.array_equal(res[0][k], res[1][k]), k
+
+
+def test_residual_sums_match_definition():
+    """residual_check! (src/Solver/Finalize.jl:5-11): sum of (prim_new - prim_old)^2 and of
+    |prim_new| over fluid cells, sampled only on check steps."""
+    case = cases.riemann_2d(n=8, nv=8)
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    o.slope()
+    o.flux(case.dt_xi)
+    r, a = o.iterate(case.dt_xi, 0)
+    p1 = o.get_state()["prim"].reshape(-1, 4)
+    p0 = case.prim.reshape(-1, 4)
+    assert np.allclose(r, ((p1 - p0) ** 2).sum(0), rtol=1e-12)
+    assert np.allclose(a, np.abs(p1).sum(0), rtol=1e-14)
+    o.slope()
+    o.flux(case.dt_xi)
+    r, a = o.iterate(case.dt_xi, 1)     # not a check step
+    assert np.all(r == 0) and np.all(a == 0)
+    o.close()
+
+
+def test_dvm_maxwellian_wall_raises_like_the_reference():
+    """src/Flux/DVM.jl:12 reads undefined variables: DVM with a Maxwellian wall errors."""
+    case = cases.riemann_2d(n=4, nv=4, flux="DVM", bc="Maxwellian")
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    o.slope()
+    with pytest.raises(RuntimeError, match="here_weight"):
+        o.flux(case.dt_xi)
