@@ -42,8 +42,9 @@ struct DBuf {
 struct LevelBin {
   int level;
   int maxh;
-  int n;
-  DBuf cells;
+  int n;          // work items: cells (per-cell kernel) or (cell, chunk) pairs
+  bool chunked;
+  DBuf items;
 };
 
 struct EvPair {
@@ -71,7 +72,8 @@ struct kah_ctx {
   std::vector<LevelBin*> bins;
   DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey;
   DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
-  DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw;
+  DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
+  int64_t part_rows = 0;
   int cur = 0;
   int64_t dev_bytes = 0;
   bool timing = false;
@@ -563,26 +565,64 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     Lmin = std::min(Lmin, int(T->ps_level[c]));
     Lmax = std::max(Lmax, int(T->ps_level[c]));
   }
+  // Per level and half-flux count: cells owning a wall half need the whole-cell wall density
+  // and run the per-cell slope kernel (one partial row per half flux); all others run the
+  // chunked kernel (one partial row per SLOPE_CHUNK velocity cells).
+  std::vector<char> walled(size_t(nc), 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int h = hoff[c]; h < hoff[c + 1]; ++h) walled[c] |= (halves[h].mode == MI_WALL);
+  for (int32_t c : C->fluid)
+    if (eoff[c + 1] - eoff[c] > MAX_ENTRIES)
+      return C->fail(KAH_EUNSUPPORTED, "cell " + std::to_string(c) + " has more than " +
+                                           std::to_string(MAX_ENTRIES) + " face sides");
+  auto nchunks = [&](int64_t c) {
+    return int((T->vs_offset[c + 1] - T->vs_offset[c] + SLOPE_CHUNK - 1) / SLOPE_CHUNK);
+  };
   for (int L = Lmin; L <= Lmax; ++L) {
-    const int caps[3] = {8, 16, 24};
-    std::vector<int32_t> lists[3];
+    const int caps[4] = {4, 8, 16, 24};
+    std::vector<int32_t> lists[4];
+    std::vector<int2> works[4];
     for (int32_t c : C->fluid) {
       if (T->ps_level[c] != L) continue;
       int nh = hoff[c + 1] - hoff[c];
-      int b = nh <= 8 ? 0 : (nh <= 16 ? 1 : 2);
-      lists[b].push_back(c);
+      int b = nh <= 4 ? 0 : (nh <= 8 ? 1 : (nh <= 16 ? 2 : 3));
+      if (walled[c]) {
+        lists[b].push_back(c);
+      } else {
+        for (int ch = 0; ch < nchunks(c); ++ch) works[b].push_back(make_int2(c, ch));
+      }
     }
-    for (int b = 0; b < 3; ++b) {
-      if (lists[b].empty()) continue;
-      LevelBin* lb = new LevelBin();
-      lb->level = L;
-      lb->maxh = caps[b];
-      lb->n = int(lists[b].size());
-      C->bins.push_back(lb);
-      KAH_TRY(C->upload(lb->cells, lists[b].data(), lists[b].size(), "level cells"));
+    for (int b = 0; b < 4; ++b) {
+      for (int kind = 0; kind < 2; ++kind) {
+        size_t cnt = kind ? works[b].size() : lists[b].size();
+        if (cnt == 0) continue;
+        LevelBin* lb = new LevelBin();
+        lb->level = L;
+        lb->maxh = caps[b];
+        lb->n = int(cnt);
+        lb->chunked = kind == 1;
+        C->bins.push_back(lb);
+        if (kind) KAH_TRY(C->upload(lb->items, works[b].data(), cnt, "slope work items"));
+        else KAH_TRY(C->upload(lb->items, lists[b].data(), cnt, "level cells"));
+      }
     }
   }
   KAH_TRY(C->upload(C->d_fluid, C->fluid.data(), C->fluid.size(), "fluid cells"));
+
+  // partial rows of every half flux
+  std::vector<int64_t> poff(size_t(std::max<int64_t>(C->npairs, 1) * 2), 0);
+  std::vector<int32_t> pcnt(poff.size(), 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int h = hoff[c]; h < hoff[c + 1]; ++h)
+      pcnt[halves[h].slot] = walled[c] ? 1 : nchunks(c);
+  int64_t rows = 0;
+  for (size_t s = 0; s < poff.size(); ++s) {
+    poff[s] = rows;
+    rows += pcnt[s];
+  }
+  C->part_rows = rows;
+  KAH_TRY(C->upload(C->d_poff, poff.data(), poff.size(), "partial offsets"));
+  KAH_TRY(C->upload(C->d_pcnt, pcnt.data(), pcnt.size(), "partial counts"));
 
   // state buffers
   const int64_t NDF = C->NDF, NW = C->NW;
@@ -593,7 +633,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   KAH_TRY(C->alloc(C->d_w, sizeof(double) * nc * NW, "w"));
   KAH_TRY(C->alloc(C->d_prim, sizeof(double) * nc * NW, "prim"));
   KAH_TRY(C->alloc(C->d_qf, sizeof(double) * nc * D, "qf"));
-  KAH_TRY(C->alloc(C->d_half, sizeof(double) * std::max<int64_t>(C->npairs, 1) * 2 * NW, "half fluxes"));
+  KAH_TRY(C->alloc(C->d_half, sizeof(double) * std::max<int64_t>(C->part_rows, 1) * NW, "half fluxes"));
   KAH_TRY(C->alloc(C->d_rho, sizeof(double) * std::max<int64_t>(C->npairs, 1), "wall density"));
   KAH_TRY(C->alloc(C->d_res, sizeof(double) * nc * 2 * NW, "residual"));
   C->d_vflux.release();
@@ -636,6 +676,8 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.prim = C->d_prim.as<double>();
   A.qf = C->d_qf.as<double>();
   A.half_fw = C->d_half.as<double>();
+  A.part_off = C->d_poff.as<int64_t>();
+  A.part_cnt = C->d_pcnt.as<int32_t>();
   A.rho_w = C->d_rho.as<double>();
   A.res = C->d_res.as<double>();
   A.scratch = C->d_scratch.as<double>();
@@ -682,9 +724,14 @@ int ev_end(kah_ctx* C, EvPair& e) {
 int run_slope(kah_ctx* C, const DevArgs& A) {
   EvPair e{};
   if (C->timing) KAH_TRY(ev_begin(C, e, 0));
-  for (LevelBin* b : C->bins)
-    KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, b->cells.as<int32_t>(), b->n, b->maxh, C->stream),
-                   "slope kernel launch"));
+  for (LevelBin* b : C->bins) {
+    if (b->chunked)
+      KAH_TRY(C->hip(launch_slope_chunk(A, C->D, C->uni, b->items.as<int2>(), b->n, b->maxh,
+                                        C->stream), "slope kernel launch"));
+    else
+      KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, b->items.as<int32_t>(), b->n, b->maxh,
+                                  C->stream), "slope kernel launch"));
+  }
   if (C->timing) KAH_TRY(ev_end(C, e));
   return 0;
 }

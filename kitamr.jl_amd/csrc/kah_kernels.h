@@ -29,7 +29,9 @@ struct DevArgs {
   double* w;
   double* prim;
   double* qf;
-  double* half_fw;          // [npairs*2*(D+2)]
+  double* half_fw;          // partial half-flux sums: row part_off[slot]+chunk, D+2 each
+  const int64_t* part_off;  // [npairs*2] first partial row of each half flux
+  const int32_t* part_cnt;  // [npairs*2] number of partial rows (velocity chunks)
   double* rho_w;            // [npairs]
   double* res;              // [nc*2*(D+2)] residual terms of the last iterate
   double* scratch;          // global scratch when the per-cell state exceeds LDS
@@ -73,8 +75,14 @@ struct DevArgs {
     (B).rec_ds = ro_recds;                                                                   \
   } while (0)
 
+constexpr int SLOPE_CHUNK = 256;   // velocity cells per workgroup of the chunked slope kernel
+constexpr int MAX_ENTRIES = 24;    // face sides per cell (2*DIM sides, up to 2^(DIM-1) each)
+
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                         int maxh, hipStream_t st);
+// work = (cell, chunk) pairs, chunk = SLOPE_CHUNK velocity cells
+hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, const int2* work, int n,
+                              int maxh, hipStream_t st);
 hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                              int mode, int march, size_t lds, int waves_per_block,
                              hipStream_t st);

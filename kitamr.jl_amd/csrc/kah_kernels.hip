@@ -371,8 +371,7 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   if (int(threadIdx.x) < nh * NW) {
     const int h = threadIdx.x / NW, q = threadIdx.x % NW;
     const HalfDesc& hd = A.halves[hb + h];
-    if (hd.mode != MI_WALL)
-      A.half_fw[int64_t(hd.slot) * NW + q] = (q == NW - 1) ? 0.5 * tot[threadIdx.x] : tot[threadIdx.x];
+    if (hd.mode != MI_WALL) A.half_fw[A.part_off[hd.slot] * NW + q] = tot[threadIdx.x];
   }
   if (!any_wall) return;
   // wall density rho_w = -SF/SG, then the moments of the wall Maxwellian (second sweep)
@@ -412,8 +411,60 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   if (int(threadIdx.x) < nh * NW) {
     const int h = threadIdx.x / NW, q = threadIdx.x % NW;
     const HalfDesc& hd = A.halves[hb + h];
-    if (hd.mode == MI_WALL)
-      A.half_fw[int64_t(hd.slot) * NW + q] = (q == NW - 1) ? 0.5 * tot[threadIdx.x] : tot[threadIdx.x];
+    if (hd.mode == MI_WALL) A.half_fw[A.part_off[hd.slot] * NW + q] = tot[threadIdx.x];
+  }
+}
+
+// One velocity cell per thread, SLOPE_CHUNK velocity cells per workgroup: the same per-cell
+// work as k_slope without the loop over the grid, for cells without wall faces. Each
+// workgroup writes its partial sums of the cell's half fluxes to row part_off[slot]+chunk;
+// k_flux_relax adds the rows in chunk order.
+template <int D, int NDF, bool UNI, int MAXH>
+__global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
+                                                             const int2* __restrict__ work,
+                                                             int nwork, KAH_RO_PARAMS) {
+  constexpr int NW = D + 2;
+  constexpr int NWV = SLOPE_CHUNK / 64;
+  constexpr int CNT = MAXH * NW;
+  DevArgs A = A0;
+  KAH_RO_BIND(A);
+  const int2 wk = work[xcd_remap(blockIdx.x, nwork)];
+  const int c = wk.x, ch = wk.y;
+  const Grid<D, NDF, UNI> G(A, c);
+  const int hb = A.half_off[c];
+  const int nh = A.half_off[c + 1] - hb;
+  __shared__ double red[NWV * CNT];
+  __shared__ double tot[CNT];
+
+  double acc[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
+  const int i = ch * SLOPE_CHUNK + int(threadIdx.x);
+  if (i < G.n) {
+    double u[D], f[NDF], s[D][NDF];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+    const int li = G.lev(i);
+    const uint32_t ki = G.key(i);
+#pragma unroll
+    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
+    const double wt = G.wt(i);
+#pragma unroll
+    for (int h = 0; h < MAXH; ++h) {
+      if (h < nh) {
+        double m[NW];
+        half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) acc[h * NW + q] = m[q];
+      }
+    }
+  }
+  block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
+  if (int(threadIdx.x) < nh * NW) {
+    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
+    A.half_fw[(A.part_off[A.halves[hb + h].slot] + ch) * NW + q] = tot[threadIdx.x];
   }
 }
 
@@ -549,8 +600,22 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   __shared__ double red[NWV * 8];
   __shared__ double res[8];
   __shared__ double cell_sh[4 * NW];   // wnew | prim_c | prim | q
+  __shared__ double hsum[MAX_ENTRIES * 2 * NW];   // the half fluxes of the cell's faces
   extern __shared__ double lds[];
   double* sc = LDS ? lds : &A.scratch[G.off * (NDF + 1)];
+
+  // half flux = sum of its partial rows in chunk order (micro_to_macro's 1/2 on the energy
+  // row applied after the sum), one thread per (entry, side, component)
+  if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
+    const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
+    const int slot = A.ents[eb + e].pair * 2 + s;
+    const int64_t off = A.part_off[slot];
+    const int cnt = A.part_cnt[slot];
+    double sum = 0.0;
+    for (int k = 0; k < cnt; ++k) sum += A.half_fw_r[(off + k) * NW + q];
+    hsum[threadIdx.x] = (q == NW - 1) ? 0.5 * sum : sum;
+  }
+  __syncthreads();
 
   if (threadIdx.x == 0) {
     // macroscopic face flux, in face order: flux += fw*area / flux -= fw*area, then
@@ -565,7 +630,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     if (MODE != MODE_ITER && A.flux_kind == 0) {
       for (int e = 0; e < ne; ++e) {
         const EntryDesc& ed = A.ents[eb + e];
-        const double* h0 = &A.half_fw_r[int64_t(ed.pair) * 2 * NW];
+        const double* h0 = &hsum[e * 2 * NW];
         const double* h1 = h0 + NW;
 #pragma unroll
         for (int q = 0; q < NW; ++q) {
@@ -838,6 +903,35 @@ static hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, 
     hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
                        KAH_RO_ARGS(A));
   return hipGetLastError();
+}
+
+template <int D, int NDF, bool UNI>
+static hipError_t launch_slope_chunk_t(const DevArgs& A, const int2* work, int n, int maxh,
+                                       hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (maxh <= 4)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 4>), dim3(n), dim3(SLOPE_CHUNK), 0, st, A,
+                       work, n, KAH_RO_ARGS(A));
+  else if (maxh <= 8)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 8>), dim3(n), dim3(SLOPE_CHUNK), 0, st, A,
+                       work, n, KAH_RO_ARGS(A));
+  else if (maxh <= 16)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 16>), dim3(n), dim3(SLOPE_CHUNK), 0, st, A,
+                       work, n, KAH_RO_ARGS(A));
+  else
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 24>), dim3(n), dim3(SLOPE_CHUNK), 0, st, A,
+                       work, n, KAH_RO_ARGS(A));
+  return hipGetLastError();
+}
+
+hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, const int2* work, int n,
+                              int maxh, hipStream_t st) {
+  if (dim == 2) {
+    return uni ? launch_slope_chunk_t<2, 2, true>(A, work, n, maxh, st)
+               : launch_slope_chunk_t<2, 2, false>(A, work, n, maxh, st);
+  }
+  return uni ? launch_slope_chunk_t<3, 1, true>(A, work, n, maxh, st)
+             : launch_slope_chunk_t<3, 1, false>(A, work, n, maxh, st);
 }
 
 template <int D, int NDF, bool UNI, int MODE, int MARCH, int NWV>
