@@ -49,6 +49,53 @@ __device__ inline void block_reduce(const double (&vals)[CNT], int cnt, double* 
   __syncthreads();
 }
 
+// Workgroup sum (SLOPE_CHUNK = 256 threads) of the first cnt of CNT per-thread values,
+// through LDS, in batches of CHUNK_QB values: the batch is written transposed, 32 threads
+// per value each add a strided set of 8 lanes, 4 threads per value add 8 of those
+// partials, and one thread adds the last 4; fixed order throughout. buf needs
+// CHUNK_QB*SLOPE_CHUNK doubles, part CHUNK_QB*36, out cnt.
+constexpr int CHUNK_QB = 8;
+template <int CNT, int B0 = 0>
+__device__ inline void chunk_sum(const double (&vals)[CNT], int cnt, double* buf, double* part,
+                                 double* out) {
+  static_assert(SLOPE_CHUNK == 256, "chunk_sum assumes 256 threads");
+  constexpr int NT = SLOPE_CHUNK, QB = CHUNK_QB, TPV = NT / QB, RPT = NT / TPV;
+  double* part2 = part + QB * TPV;
+  const int t = threadIdx.x;
+  if (B0 >= cnt) return;   // cnt is uniform over the workgroup
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    if constexpr (B0 + QB <= CNT) {
+      if (B0 + q < cnt) buf[q * NT + t] = vals[B0 + q];
+    } else {
+      if (B0 + q < cnt) buf[q * NT + t] = vals[B0 + q < CNT ? B0 + q : CNT - 1];
+    }
+  }
+  __syncthreads();
+  {
+    const int q = t / TPV, j = t % TPV;
+    if (B0 + q < cnt) {
+      double s = buf[q * NT + j];
+#pragma unroll
+      for (int r = 1; r < RPT; ++r) s += buf[q * NT + r * TPV + j];
+      part[q * TPV + j] = s;
+    }
+  }
+  __syncthreads();
+  if (t < QB * 4 && B0 + t / 4 < cnt) {
+    const double* p = &part[(t / 4) * TPV + (t % 4) * (TPV / 4)];
+    double s = p[0];
+#pragma unroll
+    for (int j = 1; j < TPV / 4; ++j) s += p[j];
+    part2[t] = s;
+  }
+  __syncthreads();
+  if (t < QB && B0 + t < cnt)
+    out[B0 + t] = ((part2[4 * t] + part2[4 * t + 1]) + part2[4 * t + 2]) + part2[4 * t + 3];
+  __syncthreads();
+  if constexpr (B0 + QB < CNT) chunk_sum<CNT, B0 + QB>(vals, cnt, buf, part, out);
+}
+
 // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs; give every XCD
 // a contiguous run of cells so that face neighbours (close in Morton order) share its L2.
 __device__ inline int xcd_remap(int i, int n) {
@@ -57,12 +104,19 @@ __device__ inline int xcd_remap(int i, int n) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// a[d] for a runtime direction d, as selects over compile-time indices
+// a[d] for a runtime direction d, as selects over compile-time indices. With three
+// candidates the optimizer would otherwise fold the select chain back into an indexed load
+// from a stack copy of a (private memory); the empty asm makes each candidate an opaque
+// register value.
 template <int D>
 __device__ inline double pick(const double (&a)[D], int d) {
   double r = a[0];
 #pragma unroll
-  for (int t = 1; t < D; ++t) r = (d == t) ? a[t] : r;
+  for (int t = 1; t < D; ++t) {
+    double x = a[t];
+    if constexpr (D > 2) __asm__("" : "+v"(x));
+    r = (d == t) ? x : r;
+  }
   return r;
 }
 
@@ -148,7 +202,11 @@ __device__ inline void micro_recon(int mode, const double (&f)[NDF], const doubl
       double s_abs = dsrec[0] * fabs(s[0][k]);
 #pragma unroll
       for (int t = 1; t < D; ++t) s_abs = s_abs + dsrec[t] * fabs(s[t][k]);
-      double lim = jl_min(fabs((f[k] - MACH_EPS) / (0.5 * s_abs + EPS)), 1.0);
+      const double a = f[k] - MACH_EPS, b = 0.5 * s_abs + EPS;
+      // b >= EPS > 0, so |a| >= b with b finite means |a/b| >= 1 and the limiter is 1
+      // exactly; the quotient is only formed where it can be below 1 (or NaN)
+      double lim = 1.0;
+      if (!(fabs(a) >= b && b < HUGE_VAL)) lim = jl_min(fabs(a / b), 1.0);
       out[k] = (f[k] + lim * s_dx) * vn;
     } else {
       out[k] = (f[k] + s_dx) * vn;
@@ -424,7 +482,6 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
                                                              const int2* __restrict__ work,
                                                              int nwork, KAH_RO_PARAMS) {
   constexpr int NW = D + 2;
-  constexpr int NWV = SLOPE_CHUNK / 64;
   constexpr int CNT = MAXH * NW;
   DevArgs A = A0;
   KAH_RO_BIND(A);
@@ -433,7 +490,8 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
   const Grid<D, NDF, UNI> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
-  __shared__ double red[NWV * CNT];
+  __shared__ double buf[CHUNK_QB * SLOPE_CHUNK];
+  __shared__ double part[CHUNK_QB * 36];
   __shared__ double tot[CNT];
 
   double acc[CNT];
@@ -461,7 +519,7 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
       }
     }
   }
-  block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
+  chunk_sum<CNT>(acc, nh * NW, buf, part, tot);
   if (int(threadIdx.x) < nh * NW) {
     const int h = threadIdx.x / NW, q = threadIdx.x % NW;
     A.half_fw[(A.part_off[A.halves[hb + h].slot] + ch) * NW + q] = tot[threadIdx.x];

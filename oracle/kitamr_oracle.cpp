@@ -77,6 +77,15 @@ struct orc_ctx {
   std::vector<double> df, sdf, vflux;
   std::string err;
 
+  // Error message of a failing call; called from inside parallel regions, so the write is
+  // serialized and the first message is kept.
+  int fail(int code, const std::string& m) {
+#pragma omp critical(orc_err)
+    {
+      if (err.empty()) err = m;
+    }
+    return code;
+  }
   int NW() const { return dim + 2; }
   int64_t src(int64_t rec) const { return rec < nc ? rec : image_src[rec - nc]; }
   int64_t off(int64_t c) const { return voff[c]; }
@@ -383,9 +392,8 @@ int slope_dispatch(orc_ctx& o, int64_t c, int dir, SlopeWs& ws) {
   } else if (sR == 0 && sL != 0) {
     slope_bound(o, c, fL, xm - o.rmid(o.nb(c, fL, 0))[dir], dir, ws);
   } else {
-    o.err = "update_slope!: no method for neighbour states (" + std::to_string(sL) + "," +
-            std::to_string(sR) + ")";
-    return KAH_EINVAL;
+    return o.fail(KAH_EINVAL, "update_slope!: no method for neighbour states (" +
+                                  std::to_string(sL) + "," + std::to_string(sR) + ")");
   }
   return 0;
 }
@@ -710,8 +718,8 @@ int domain_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
   } else if (kind == KAH_BC_MAXWELLIAN || kind == KAH_BC_SUPERSONIC_INFLOW) {
     if (kind == KAH_BC_MAXWELLIAN && o.flux_kind == KAH_FLUX_DVM) {
       // src/Flux/DVM.jl:12 reads here_weight/there_weight that are never defined
-      o.err = "UndefVarError: `here_weight` not defined (DVM Maxwellian domain flux)";
-      return KAH_EUNSUPPORTED;
+      return o.fail(KAH_EUNSUPPORTED,
+                    "UndefVarError: `here_weight` not defined (DVM Maxwellian domain flux)");
     }
     reconstruct(o, hs, xh, o.rds(h), fm, dir, dt, false);  // (df + dot(dx,sdf)) * vn
     if (kind == KAH_BC_MAXWELLIAN) {
@@ -743,8 +751,8 @@ int domain_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
     }
   } else if (kind == KAH_BC_INTERPOLATED_OUTFLOW) {
     if (DIM != 2) {
-      o.err = "InterpolatedOutflow is defined for DIM=2 only (src/Flux/CAIDVM.jl:71)";
-      return KAH_EUNSUPPORTED;
+      return o.fail(KAH_EUNSUPPORTED,
+                    "InterpolatedOutflow is defined for DIM=2 only (src/Flux/CAIDVM.jl:71)");
     }
     double xt[3];
     for (int d = 0; d < DIM; ++d) xt[d] = 2.0 * fm[d] - xh[d];
@@ -764,8 +772,7 @@ int domain_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
       }
     }
   } else {
-    o.err = "unknown domain boundary condition";
-    return KAH_EINVAL;
+    return o.fail(KAH_EINVAL, "unknown domain boundary condition");
   }
   if (o.flux_kind == KAH_FLUX_CAIDVM) {
     double a[5], b[5];
@@ -953,6 +960,7 @@ int orc_get_flux(orc_ctx* o, double* flux, double* vs_flux) {
 
 // slope!(p4est,ka), src/Flux/Slope.jl:1047-1070
 int orc_slope(orc_ctx* o) {
+  o->err.clear();
   int Lmin = 1 << 30, Lmax = -1;
   for (int64_t c = 0; c < o->nc; ++c) {
     if (o->benc[c] < 0) continue;
@@ -1001,6 +1009,7 @@ int orc_slope(orc_ctx* o) {
 // parallel; each cell then applies its side of its faces in face-list order, which gives
 // every accumulator the same sequence of additions as the reference's serial face loop.
 int orc_flux(orc_ctx* o, double dt) {
+  o->err.clear();
   const int64_t npair = int64_t(o->ph.size());
   if (o->cell_pairs.empty() && o->nc > 0) {
     o->cell_pairs.assign(size_t(o->nc), {});
