@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="reduced mesh (profiling/debug only)")
     ap.add_argument("--flux-waves", type=int, default=0,
                     help="waves per workgroup of the flux/relaxation kernel (0: library default)")
+    ap.add_argument("--slope-vpt", type=int, default=0,
+                    help="velocity cells per thread of the slope kernel (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -100,6 +102,8 @@ def main():
     dev = abi.HipContext(kah_params(case.config))
     if args.flux_waves:
         dev.set_option("flux_waves", args.flux_waves)
+    if args.slope_vpt:
+        dev.set_option("slope_vpt", args.slope_vpt)
     dev.set_topology(case.topo)
     dev.set_state(case.w, case.prim, case.df)
     dt = case.dt_xi

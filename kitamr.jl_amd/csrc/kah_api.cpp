@@ -78,6 +78,7 @@ struct kah_ctx {
   int64_t dev_bytes = 0;
   bool timing = false;
   int fr_waves = 8;   // waves per workgroup of the flux/relaxation kernel
+  int slope_vpt = 3;  // velocity cells per thread of the chunked slope kernel
   std::vector<EvPair> events;
   double t_sum[2] = {0, 0};
   int64_t t_cnt[2] = {0, 0};
@@ -567,7 +568,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   }
   // Per level and half-flux count: cells owning a wall half need the whole-cell wall density
   // and run the per-cell slope kernel (one partial row per half flux); all others run the
-  // chunked kernel (one partial row per SLOPE_CHUNK velocity cells).
+  // chunked kernel (one partial row per slope_vpt*SLOPE_CHUNK velocity cells).
   std::vector<char> walled(size_t(nc), 0);
   for (int64_t c = 0; c < nc; ++c)
     for (int h = hoff[c]; h < hoff[c + 1]; ++h) walled[c] |= (halves[h].mode == MI_WALL);
@@ -576,7 +577,8 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       return C->fail(KAH_EUNSUPPORTED, "cell " + std::to_string(c) + " has more than " +
                                            std::to_string(MAX_ENTRIES) + " face sides");
   auto nchunks = [&](int64_t c) {
-    return int((T->vs_offset[c + 1] - T->vs_offset[c] + SLOPE_CHUNK - 1) / SLOPE_CHUNK);
+    const int64_t cv = int64_t(C->slope_vpt) * SLOPE_CHUNK;
+    return int((T->vs_offset[c + 1] - T->vs_offset[c] + cv - 1) / cv);
   };
   for (int L = Lmin; L <= Lmax; ++L) {
     const int caps[4] = {4, 8, 16, 24};
@@ -727,7 +729,7 @@ int run_slope(kah_ctx* C, const DevArgs& A) {
   for (LevelBin* b : C->bins) {
     if (b->chunked)
       KAH_TRY(C->hip(launch_slope_chunk(A, C->D, C->uni, b->items.as<int2>(), b->n, b->maxh,
-                                        C->stream), "slope kernel launch"));
+                                        C->slope_vpt, C->stream), "slope kernel launch"));
     else
       KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, b->items.as<int32_t>(), b->n, b->maxh,
                                   C->stream), "slope kernel launch"));
@@ -926,9 +928,16 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
   if (!C) return KAH_EINVAL;
   switch (option) {
     case KAH_OPT_FLUX_WAVES:
-      if (value != 4 && value != 8 && value != 12 && value != 16)
-        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4, 8, 12 or 16");
+      if (value != 4 && value != 8)
+        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4 or 8");
       C->fr_waves = int(value);
+      return KAH_OK;
+    case KAH_OPT_SLOPE_VPT:
+      if (value < 1 || value > SLOPE_MAX_VPT)
+        return C->fail(KAH_EINVAL, "slope velocity cells per thread must be 1.." +
+                                       std::to_string(SLOPE_MAX_VPT));
+      C->slope_vpt = int(value);
+      C->have_topo = false;   // chunking changes: set the topology again
       return KAH_OK;
     default:
       return C->fail(KAH_EINVAL, "unknown option " + std::to_string(option));

@@ -1,0 +1,1032 @@
+// kah_device.h -- gfx950 device code of the per-step phase-space update (included by the
+// per-dimension kernel translation units kah_slope_{2d,3d}.hip and kah_flux_{2d,3d}.hip).
+//
+// K1 k_slope_chunk one workgroup per (cell, chunk of velocity cells) of one physical level:
+//                  velocity slopes and per-chunk partial half fluxes
+//    k_slope       one workgroup per cell, for cells with a Maxwellian wall face (the wall
+//                  density needs the whole cell's sums before the half flux)
+// K2 k_flux_relax  one workgroup per fluid cell (MODE_FLUX / MODE_ITER / MODE_FUSED)
+// K3 k_macro_slope on demand: PsData.sw = moments of sdf (src/Flux/Slope.jl:1022-1036)
+//
+// Velocity index fastest in memory, so every load and store of f, sdf and the flux is
+// coalesced; per-cell moment sums are reduced deterministically (fixed-order trees through
+// wave shuffles or LDS, then fixed-order sums of the partials), with no atomics anywhere. Private arrays are only ever indexed with
+// compile-time indices (direction-dependent picks are selects), so nothing spills to
+// scratch memory.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kah_internal.h"
+#include "kah_kernels.h"
+
+namespace kah {
+
+// ------------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Workgroup sum of the first cnt of CNT per-thread partials. out[q] (LDS) receives the
+// total of value q; red needs NWV*CNT doubles of LDS. Two barriers, fixed order.
+template <int CNT, int NWV>
+__device__ inline void block_reduce(const double (&vals)[CNT], int cnt, double* red,
+                                    double* out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < CNT; ++q) {
+    if (q < cnt) {
+      double s = wave_sum(vals[q]);
+      if (lane == 0) red[wv * CNT + q] = s;
+    }
+  }
+  __syncthreads();
+  if (int(threadIdx.x) < cnt) {
+    double s = red[threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < NWV; ++w) s += red[w * CNT + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Workgroup sum (SLOPE_CHUNK = 256 threads) of the first cnt of CNT per-thread values,
+// through LDS, in batches of CHUNK_QB values: the batch is written transposed, 32 threads
+// per value each add a strided set of 8 lanes, 4 threads per value add 8 of those
+// partials, and one thread adds the last 4; fixed order throughout. buf needs
+// CHUNK_QB*SLOPE_CHUNK doubles, part CHUNK_QB*36, out cnt.
+constexpr int CHUNK_QB = 8;
+template <int CNT, int B0 = 0>
+__device__ inline void chunk_sum(const double (&vals)[CNT], int cnt, double* buf, double* part,
+                                 double* out) {
+  static_assert(SLOPE_CHUNK == 256, "chunk_sum assumes 256 threads");
+  constexpr int NT = SLOPE_CHUNK, QB = CHUNK_QB, TPV = NT / QB, RPT = NT / TPV;
+  double* part2 = part + QB * TPV;
+  const int t = threadIdx.x;
+  if (B0 >= cnt) return;   // cnt is uniform over the workgroup
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    if constexpr (B0 + QB <= CNT) {
+      if (B0 + q < cnt) buf[q * NT + t] = vals[B0 + q];
+    } else {
+      if (B0 + q < cnt) buf[q * NT + t] = vals[B0 + q < CNT ? B0 + q : CNT - 1];
+    }
+  }
+  __syncthreads();
+  {
+    const int q = t / TPV, j = t % TPV;
+    if (B0 + q < cnt) {
+      double s = buf[q * NT + j];
+#pragma unroll
+      for (int r = 1; r < RPT; ++r) s += buf[q * NT + r * TPV + j];
+      part[q * TPV + j] = s;
+    }
+  }
+  __syncthreads();
+  if (t < QB * 4 && B0 + t / 4 < cnt) {
+    const double* p = &part[(t / 4) * TPV + (t % 4) * (TPV / 4)];
+    double s = p[0];
+#pragma unroll
+    for (int j = 1; j < TPV / 4; ++j) s += p[j];
+    part2[t] = s;
+  }
+  __syncthreads();
+  if (t < QB && B0 + t < cnt)
+    out[B0 + t] = ((part2[4 * t] + part2[4 * t + 1]) + part2[4 * t + 2]) + part2[4 * t + 3];
+  __syncthreads();
+  if constexpr (B0 + QB < CNT) chunk_sum<CNT, B0 + QB>(vals, cnt, buf, part, out);
+}
+
+// XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs; give every XCD
+// a contiguous run of cells so that face neighbours (close in Morton order) share its L2.
+__device__ inline int xcd_remap(int i, int n) {
+  const int x = i & 7, j = i >> 3;
+  const int q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + j;
+}
+
+// a[d] for a runtime direction d, as selects over compile-time indices. With three
+// candidates the optimizer would otherwise fold the select chain back into an indexed load
+// from a stack copy of a (private memory); the empty asm makes each candidate an opaque
+// register value.
+template <int D>
+__device__ inline double pick(const double (&a)[D], int d) {
+  double r = a[0];
+#pragma unroll
+  for (int t = 1; t < D; ++t) {
+    double x = a[t];
+    if constexpr (D > 2) __asm__("" : "+v"(x));
+    r = (d == t) ? x : r;
+  }
+  return r;
+}
+
+// ------------------------------------------------------------------ grid access
+template <int D, int NDF, bool UNI>
+struct Grid {
+  const DevArgs& A;
+  int64_t off;   // first velocity cell of the cell
+  int n;
+  __device__ Grid(const DevArgs& a, int c) : A(a) {
+    if constexpr (UNI) {
+      n = a.uni_n;
+      off = int64_t(c) * n;
+    } else {
+      off = a.voff[c];
+      n = int(a.voff[c + 1] - off);
+    }
+  }
+  __device__ int64_t goff() const { return UNI ? 0 : off; }
+  __device__ double u(int i, int d) const { return A.gu[goff() * D + int64_t(d) * n + i]; }
+  __device__ double wt(int i) const { return A.gw[goff() + i]; }
+  __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[off + i]); }
+  __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
+  __device__ double f(int i, int k) const { return A.df[off * NDF + int64_t(k) * n + i]; }
+  __device__ double s(int i, int k, int d) const {
+    return A.sdf_r[off * NDF * D + int64_t(d * NDF + k) * n + i];
+  }
+  __device__ double* sp(int i, int k, int d) const {
+    return &A.sdf[off * NDF * D + int64_t(d * NDF + k) * n + i];
+  }
+};
+
+// Calls fn(j, den) for each velocity cell j of grid G that overlaps cell i (level li, key
+// ki) of another grid on the same root lattice: den = 1 when j is coarser or as fine as
+// i, else 2^(D*(lj-li)). This is the pairing of the two-pointer walks of diff_vs! and
+// update_micro_flux! (src/Flux/Slope.jl:29-64, src/Flux/Flux.jl:151-344), in the same order.
+template <int D, int NDF, bool UNI, class Fn>
+__device__ inline void for_overlap(const Grid<D, NDF, UNI>& G, int i, int li, uint32_t ki,
+                                   int lmax, Fn&& fn) {
+  if constexpr (UNI) {
+    fn(i, 1.0);
+  } else {
+    int lo = 0, hi = G.n - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (G.key(mid) <= ki) lo = mid; else hi = mid - 1;
+    }
+    int j = lo;
+    int lj = G.lev(j);
+    if (lj <= li) {
+      fn(j, 1.0);
+      return;
+    }
+    const uint32_t end = ki + (1u << (D * (lmax - li)));
+    for (; j < G.n && G.key(j) < end; ++j) fn(j, ldexp(1.0, D * (G.lev(j) - li)));
+  }
+}
+
+// Micro flux of one velocity cell: value times the normal velocity.
+// LIMITED is positivity_preserving_reconstruct (src/Flux/CAIDVM.jl:127-141); UNLIMITED the
+// plain reconstruction used next to solid cells and at domain walls/inflow (CAIDVM.jl:13-14,
+// :110-112); PLAIN f*vn (UniformOutflow, CAIDVM.jl:53-67).
+template <int D, int NDF>
+__device__ inline void micro_recon(int mode, const double (&f)[NDF], const double (&s)[D][NDF],
+                                   const double (&u)[D], double dt, const double* xrec,
+                                   const double* dsrec, const double* fmid, int dir,
+                                   double (&out)[NDF]) {
+  const double vn = pick<D>(u, dir);
+  if (mode == MI_PLAIN || mode == MI_SELF_PLAIN) {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) out[k] = f[k] * vn;
+    return;
+  }
+  double dx[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) dx[t] = fmid[t] - u[t] * dt - xrec[t];
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) {
+    double s_dx = dx[0] * s[0][k];
+#pragma unroll
+    for (int t = 1; t < D; ++t) s_dx = s_dx + dx[t] * s[t][k];
+    if (mode == MI_LIMITED) {
+      double s_abs = dsrec[0] * fabs(s[0][k]);
+#pragma unroll
+      for (int t = 1; t < D; ++t) s_abs = s_abs + dsrec[t] * fabs(s[t][k]);
+      const double a = f[k] - MACH_EPS, b = 0.5 * s_abs + EPS;
+      // b >= EPS > 0, so |a| >= b with b finite means |a/b| >= 1 and the limiter is 1
+      // exactly; the quotient is only formed where it can be below 1 (or NaN)
+      double lim = 1.0;
+      if (!(fabs(a) >= b && b < HUGE_VAL)) lim = jl_min(fabs(a / b), 1.0);
+      out[k] = (f[k] + lim * s_dx) * vn;
+    } else {
+      out[k] = (f[k] + s_dx) * vn;
+    }
+  }
+}
+
+// InterpolatedOutflow incoming micro flux (CAIDVM.jl:71-94): the ghost state mirrored
+// through the face, reconstructed at the face from the cell's own slopes.
+template <int D, int NDF>
+__device__ inline void micro_interp(const double (&f)[NDF], const double (&s)[D][NDF],
+                                    const double (&u)[D], double dt, const double* xh,
+                                    const double* fmid, int dir, double (&out)[NDF]) {
+  double xt[D], xhd[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) {
+    xt[t] = 2.0 * fmid[t] - xh[t];
+    xhd[t] = xh[t];
+  }
+  double ndx[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) ndx[t] = fmid[t] - u[t] * dt - xt[t];
+  const double shift = pick<D>(xt, dir) - pick<D>(xhd, dir);
+  const double vn = pick<D>(u, dir);
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) {
+    double sk[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) sk[t] = s[t][k];
+    double tdf = f[k] + shift * pick<D>(sk, dir);
+    double s_dx = ndx[0] * s[0][k];
+#pragma unroll
+    for (int t = 1; t < D; ++t) s_dx = s_dx + ndx[t] * s[t][k];
+    out[k] = (tdf + s_dx) * vn;
+  }
+}
+
+// Maxwellian inflow state (bc prim, density rho for walls) times vn.
+template <int D, int NDF>
+__device__ inline void micro_maxwell(const double* bc, double rho, bool wall,
+                                     const double (&u)[D], double K, int dir,
+                                     double (&out)[NDF]) {
+  double p[D + 2];
+#pragma unroll
+  for (int m = 0; m < D + 2; ++m) p[m] = bc[m];
+  if (wall) p[0] = rho;
+  double M[2];
+  maxwell<D>(u, p, K, M);
+  const double vn = pick<D>(u, dir);
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) out[k] = M[k] * vn;
+}
+
+// ------------------------------------------------------------------ K1: slopes + half fluxes
+// Velocity-space slope of one direction (update_slope!/update_slope_transverse_level!,
+// src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
+// taken as key-interval overlaps.
+template <int D, int NDF, bool UNI>
+__device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, double rdsx,
+                                int i, int li, uint32_t ki, const double (&f)[NDF],
+                                double (&a)[NDF]) {
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) a[k] = 0.0;
+  for (int e = 0; e < cnt; ++e) {
+    const SlopeNb nb = A.slnb[beg + e];
+    const Grid<D, NDF, UNI> N(A, nb.cell);
+    for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
+      const double rden = 1.0 / den;   // den is a power of two: exact
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        double pj = N.f(j, k);
+        if (nb.proj) {
+#pragma unroll
+          for (int t = 0; t < D; ++t) pj += nb.dm[t] * N.s(j, k, t);
+        }
+        double term = f[k] - pj;
+        if (den != 1.0) term = term * rden;
+        a[k] += div_by(term, dsx, rdsx);
+      }
+    });
+  }
+}
+
+template <int D, int NDF, bool UNI>
+__device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c, int d,
+                                 int i, int li, uint32_t ki, const double (&f)[NDF],
+                                 double (&s)[NDF]) {
+  const SlopeDesc sd = A.sl[c * D + d];
+  if (sd.mode == SL_SKIP) {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) s[k] = G.s(i, k, d);
+    return;
+  }
+  if (sd.mode == SL_ZERO) {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      s[k] = 0.0;
+      *G.sp(i, k, d) = 0.0;
+    }
+    return;
+  }
+  double aL[NDF];
+  side_sum<D, NDF, UNI>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li, ki, f, aL);
+  if (sd.mode == SL_INNER) {
+    double aR[NDF];
+    side_sum<D, NDF, UNI>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li, ki, f,
+                          aR);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) s[k] = minmod(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) s[k] = aL[k] * sd.inv_cnt[0];
+  }
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) *G.sp(i, k, d) = s[k];
+}
+
+// Moments of one velocity cell's contribution to a half flux (the cell's own upwind micro
+// flux through one face, micro_to_macro of src/Flux/CAIDVM.jl:119), or the two sums of the
+// wall density (calc_ρw, src/Theory/Math.jl:251-282) in the first sweep of a wall.
+template <int D, int NDF, bool UNI>
+__device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c,
+                                  const HalfDesc& hd, int i, const double (&u)[D],
+                                  const double (&f)[NDF], const double (&s)[D][NDF], double wt,
+                                  double (&m)[D + 2]) {
+  constexpr int NW = D + 2;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) m[q] = 0.0;
+  const double ud = pick<D>(u, hd.dir);
+  const double ru = hd.rot * ud;
+  const bool in = hd.sel ? (ru > 0.0) : (ru <= 0.0);
+  if (hd.mode == MI_WALL) {
+    const double* bc = &A.bc_prim[hd.bc * NW];
+    if (in) {
+      double c2 = 0.0;
+#pragma unroll
+      for (int t = 0; t < D; ++t) {
+        double cu = u[t] - bc[1 + t];
+        c2 = (t == 0) ? cu * cu : c2 + cu * cu;
+      }
+      m[1] = wt * ud * exp(-bc[D + 1] * c2);
+    } else {
+      double dx[D];
+#pragma unroll
+      for (int t = 0; t < D; ++t) dx[t] = hd.fmid[t] - u[t] * A.dt - hd.xrec[t];
+      double hrec = dx[0] * s[0][0];
+#pragma unroll
+      for (int t = 1; t < D; ++t) hrec = hrec + dx[t] * s[t][0];
+      hrec = f[0] + hrec;
+      m[0] = wt * ud * hrec;
+    }
+    return;
+  }
+  if (!in) return;
+  double mic[NDF];
+  if (hd.mode == MI_MAXWELL) {
+    micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], 0.0, false, u, A.K, hd.dir, mic);
+  } else if (hd.mode == MI_INTERP) {
+    micro_interp<D, NDF>(f, s, u, A.dt, hd.xrec, hd.fmid, hd.dir, mic);
+  } else if (hd.data_cell == c) {
+    micro_recon<D, NDF>(hd.mode, f, s, u, A.dt, hd.xrec, hd.dsrec, hd.fmid, hd.dir, mic);
+  } else {
+    // the there side of a face whose other cell is solid: f of that cell * vn (shared
+    // velocity grids only; checked on the host)
+    const Grid<D, NDF, UNI> S(A, hd.data_cell);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) mic[k] = S.f(i, k) * ud;
+  }
+  moment_terms<D, NDF>(wt, u, mic, m);
+}
+
+template <int D, int NDF, bool UNI, int MAXH, int NWV>
+__global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* __restrict__ cells,
+                                                    int ncells, KAH_RO_PARAMS) {
+  constexpr int NW = D + 2;
+  constexpr int BS = NWV * 64;
+  constexpr int CNT = MAXH * NW;
+  DevArgs A = A0;
+  KAH_RO_BIND(A);
+  const int c = cells[xcd_remap(blockIdx.x, ncells)];
+  const Grid<D, NDF, UNI> G(A, c);
+  const int hb = A.half_off[c];
+  const int nh = A.half_off[c + 1] - hb;
+  __shared__ double red[NWV * CNT];
+  __shared__ double tot[CNT];
+  __shared__ double rho_sh[MAXH];
+
+  double acc[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
+
+  for (int i = threadIdx.x; i < G.n; i += BS) {
+    double u[D], f[NDF], s[D][NDF];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+    const int li = G.lev(i);
+    const uint32_t ki = G.key(i);
+#pragma unroll
+    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
+    const double wt = G.wt(i);
+#pragma unroll
+    for (int h = 0; h < MAXH; ++h) {
+      double m[NW];
+      if (h < nh) {
+        half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) m[q] = 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < NW; ++q) acc[h * NW + q] += m[q];
+    }
+  }
+
+  block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
+  bool any_wall = false;
+  for (int h = 0; h < nh; ++h) any_wall |= (A.halves[hb + h].mode == MI_WALL);
+  if (int(threadIdx.x) < nh * NW) {
+    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
+    const HalfDesc& hd = A.halves[hb + h];
+    if (hd.mode != MI_WALL) A.half_fw[A.part_off[hd.slot] * NW + q] = tot[threadIdx.x];
+  }
+  if (!any_wall) return;
+  // wall density rho_w = -SF/SG, then the moments of the wall Maxwellian (second sweep)
+  if (int(threadIdx.x) < nh) {
+    const HalfDesc& hd = A.halves[hb + threadIdx.x];
+    if (hd.mode == MI_WALL) {
+      const double* bc = &A.bc_prim[hd.bc * NW];
+      const double pref = (D == 2) ? bc[3] / PI : pow(bc[4] / PI, 1.5);
+      const double rho = -tot[threadIdx.x * NW + 0] / (pref * tot[threadIdx.x * NW + 1]);
+      rho_sh[threadIdx.x] = rho;
+      A.rho_w[hd.slot >> 1] = rho;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
+  for (int i = threadIdx.x; i < G.n; i += BS) {
+    double u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+    const double wt = G.wt(i);
+#pragma unroll
+    for (int h = 0; h < MAXH; ++h) {
+      if (h < nh) {
+        const HalfDesc& hd = A.halves[hb + h];
+        if (hd.mode != MI_WALL) continue;
+        if (!(hd.rot * pick<D>(u, hd.dir) > 0.0)) continue;
+        double mic[NDF], m[NW];
+        micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], rho_sh[h], true, u, A.K, hd.dir, mic);
+        moment_terms<D, NDF>(wt, u, mic, m);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) acc[h * NW + q] += m[q];
+      }
+    }
+  }
+  block_reduce<CNT, NWV>(acc, nh * NW, red, tot);
+  if (int(threadIdx.x) < nh * NW) {
+    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
+    const HalfDesc& hd = A.halves[hb + h];
+    if (hd.mode == MI_WALL) A.half_fw[A.part_off[hd.slot] * NW + q] = tot[threadIdx.x];
+  }
+}
+
+// One velocity cell per thread, SLOPE_CHUNK velocity cells per workgroup: the same per-cell
+// work as k_slope without the loop over the grid, for cells without wall faces. Each
+// workgroup writes its partial sums of the cell's half fluxes to row part_off[slot]+chunk;
+// k_flux_relax adds the rows in chunk order.
+template <int D, int NDF, bool UNI, int MAXH, int VPT>
+__global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
+                                                             const int2* __restrict__ work,
+                                                             int nwork, KAH_RO_PARAMS) {
+  constexpr int NW = D + 2;
+  constexpr int CNT = MAXH * NW;
+  DevArgs A = A0;
+  KAH_RO_BIND(A);
+  const int2 wk = work[xcd_remap(blockIdx.x, nwork)];
+  const int c = wk.x, ch = wk.y;
+  const Grid<D, NDF, UNI> G(A, c);
+  const int hb = A.half_off[c];
+  const int nh = A.half_off[c + 1] - hb;
+  __shared__ double buf[CHUNK_QB * SLOPE_CHUNK];
+  __shared__ double part[CHUNK_QB * 36];
+  __shared__ double tot[CNT];
+
+  double acc[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
+  // velocity cells ch*VPT*SLOPE_CHUNK + r*SLOPE_CHUNK + threadIdx.x, r < VPT
+#pragma unroll
+  for (int r = 0; r < VPT; ++r) {
+    const int i = (ch * VPT + r) * SLOPE_CHUNK + int(threadIdx.x);
+    if (i < G.n) {
+      double u[D], f[NDF], s[D][NDF];
+#pragma unroll
+      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+      const int li = G.lev(i);
+      const uint32_t ki = G.key(i);
+#pragma unroll
+      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
+      const double wt = G.wt(i);
+#pragma unroll
+      for (int h = 0; h < MAXH; ++h) {
+        if (h < nh) {
+          double m[NW];
+          half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+#pragma unroll
+          for (int q = 0; q < NW; ++q) acc[h * NW + q] = (r == 0) ? m[q] : acc[h * NW + q] + m[q];
+        }
+      }
+    }
+  }
+  chunk_sum<CNT>(acc, nh * NW, buf, part, tot);
+  if (int(threadIdx.x) < nh * NW) {
+    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
+    A.half_fw[(A.part_off[A.halves[hb + h].slot] + ch) * NW + q] = tot[threadIdx.x];
+  }
+}
+
+// ------------------------------------------------------------------ K2: flux + relaxation
+// Micro flux into velocity cell i of cell c through all its faces, in face-list order
+// (update_flux!/update_domain_flux!/update_micro_flux!, src/Flux/Flux.jl:64-281).
+template <int D, int NDF, bool UNI>
+__device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
+                                   const double (&u)[D], const double (&f)[NDF],
+                                   const double (&s)[D][NDF], int li, uint32_t ki,
+                                   double (&fl)[NDF]) {
+  constexpr int NW = D + 2;
+  const double dt = A.dt;
+  for (int e = 0; e < ne; ++e) {
+    const EntryDesc& ed = A.ents[eb + e];
+    const double ud = pick<D>(u, ed.dir);
+    const double ru = ed.rot * ud;
+    const bool own = ed.side ? (ru > 0.0) : (ru <= 0.0);
+    const double area = ed.area;
+    double mic[NDF];
+    if (own) {
+      micro_recon<D, NDF>(ed.own_mode, f, s, u, dt, ed.xown, ed.dsown, ed.fmid, ed.dir, mic);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        double t = area * mic[k];
+        fl[k] += ed.side ? -t : t;
+      }
+      continue;
+    }
+    switch (ed.oth_mode) {
+      case MI_MAXWELL:
+      case MI_WALL: {
+        micro_maxwell<D, NDF>(&A.bc_prim[ed.bc * NW], A.rho_w_r[ed.pair],
+                              ed.oth_mode == MI_WALL, u, A.K, ed.dir, mic);
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) fl[k] += area * mic[k];
+        break;
+      }
+      case MI_SELF_PLAIN: {
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) fl[k] += area * (f[k] * ud);
+        break;
+      }
+      case MI_INTERP: {
+        micro_interp<D, NDF>(f, s, u, dt, ed.xown, ed.fmid, ed.dir, mic);
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) fl[k] += area * mic[k];
+        break;
+      }
+      default: {
+        const Grid<D, NDF, UNI> N(A, ed.oth_cell);
+        for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
+          double uj[D], fj[NDF], sj[D][NDF];
+#pragma unroll
+          for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : N.u(j, t);
+#pragma unroll
+          for (int k = 0; k < NDF; ++k) {
+            fj[k] = N.f(j, k);
+#pragma unroll
+            for (int t = 0; t < D; ++t) sj[t][k] = (ed.oth_mode == MI_PLAIN) ? 0.0 : N.s(j, k, t);
+          }
+          double mj[NDF];
+          micro_recon<D, NDF>(ed.oth_mode, fj, sj, uj, dt, ed.xoth, ed.dsoth, ed.fmid, ed.dir,
+                              mj);
+#pragma unroll
+          for (int k = 0; k < NDF; ++k) {
+            double t = area * mj[k];
+            if (den != 1.0) t = t / den;
+            fl[k] += ed.side ? -t : t;
+          }
+        });
+      }
+    }
+  }
+}
+
+// heat_flux_2D2F / heat_flux_3D1F partial sums of one velocity cell (lib/KitCore/2D2F.jl:68-88,
+// 3D1F.jl:41-72): the sums are kept apart as in the reference and combined after reduction.
+template <int D, int NDF, int NQ>
+__device__ inline void heat_terms(double wt, const double (&u)[D], const double (&f)[NDF],
+                                  const double* p, double (&qa)[NQ]) {
+  if constexpr (D == 2) {
+    double cu = u[0] - p[1], cv = u[1] - p[2];
+    double c2 = cu * cu + cv * cv;
+    qa[0] += wt * cu * c2 * f[0];
+    qa[1] += wt * cu * f[1];
+    qa[2] += wt * cv * c2 * f[0];
+    qa[3] += wt * cv * f[1];
+  } else {
+    double cu = u[0] - p[1], cv = u[1] - p[2], cw = u[2] - p[3];
+    double c2 = cu * cu + cv * cv + cw * cw;
+    qa[0] += wt * cu * c2 * f[0];
+    qa[1] += wt * cv * c2 * f[0];
+    qa[2] += wt * cw * c2 * f[0];
+  }
+}
+template <int D>
+__device__ inline void heat_finish(const double* qs, double* q) {
+  if constexpr (D == 2) {
+    q[0] = 0.5 * (qs[0] + qs[1]);
+    q[1] = 0.5 * (qs[2] + qs[3]);
+  } else {
+    q[0] = 0.5 * qs[0];
+    q[1] = 0.5 * qs[1];
+    q[2] = 0.5 * qs[2];
+  }
+}
+
+// iterate!(CAIDVM_Marching) (src/Theory/Iterate.jl:96-130) or iterate!(Euler) (:131-162)
+// for one cell, fed by the gathered flux (MODE_FUSED), by the flux accumulators
+// (MODE_ITER), or only accumulating the flux (MODE_FLUX).
+template <int D, int NDF, bool UNI, int MODE, int MARCH, bool LDS, int NWV>
+__global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
+                                                         const int32_t* __restrict__ cells,
+                                                         int ncells, KAH_RO_PARAMS,
+                                                         const double* __restrict__ ro_sdf,
+                                                         const double* __restrict__ ro_half,
+                                                         const double* __restrict__ ro_rho) {
+  constexpr int NW = D + 2;
+  constexpr int NQ = (D == 2) ? 4 : 3;   // heat-flux partial sums
+  constexpr int BS = NWV * 64;
+  DevArgs A = A0;
+  KAH_RO_BIND(A);
+  A.sdf_r = ro_sdf;
+  A.half_fw_r = ro_half;
+  A.rho_w_r = ro_rho;
+  const int c = cells[xcd_remap(blockIdx.x, ncells)];
+  const Grid<D, NDF, UNI> G(A, c);
+  const int n = G.n;
+  const int eb = A.ent_off[c];
+  const int ne = A.ent_off[c + 1] - eb;
+  const double dt = A.dt;
+  __shared__ double red[NWV * 8];
+  __shared__ double res[8];
+  __shared__ double cell_sh[4 * NW];   // wnew | prim_c | prim | q
+  __shared__ double hsum[MAX_ENTRIES * 2 * NW];   // the half fluxes of the cell's faces
+  extern __shared__ double lds[];
+  double* sc = LDS ? lds : &A.scratch[G.off * (NDF + 1)];
+
+  // half flux = sum of its partial rows in chunk order (micro_to_macro's 1/2 on the energy
+  // row applied after the sum), one thread per (entry, side, component)
+  if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
+    const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
+    const int slot = A.ents[eb + e].pair * 2 + s;
+    const int64_t off = A.part_off[slot];
+    const int cnt = A.part_cnt[slot];
+    double sum = 0.0;
+    for (int k = 0; k < cnt; ++k) sum += A.half_fw_r[(off + k) * NW + q];
+    hsum[threadIdx.x] = (q == NW - 1) ? 0.5 * sum : sum;
+  }
+  __syncthreads();
+
+  if (threadIdx.x == 0) {
+    // macroscopic face flux, in face order: flux += fw*area / flux -= fw*area, then
+    // w += flux*dt/V (both marchings: src/Theory/Iterate.jl:108, :144-149)
+    const double* dsc = &A.rec_ds[int64_t(c) * D];
+    double V = dsc[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) V = V * dsc[d];
+    double psf[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) psf[q] = A.psflux[int64_t(c) * NW + q];
+    if (MODE != MODE_ITER && A.flux_kind == 0) {
+      for (int e = 0; e < ne; ++e) {
+        const EntryDesc& ed = A.ents[eb + e];
+        const double* h0 = &hsum[e * 2 * NW];
+        const double* h1 = h0 + NW;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          double t;
+          if (ed.oth_cell < 0) t = ed.area * (h0[q] + h1[q]);    // update_domain_flux!
+          else t = (h0[q] + h1[q]) * ed.area;                     // update_macro_flux!
+          psf[q] = ed.side ? psf[q] - t : psf[q] + t;
+        }
+      }
+    }
+    if (MODE == MODE_FLUX) {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) A.psflux[int64_t(c) * NW + q] = psf[q];
+    } else {
+      const double* wold = &A.w[int64_t(c) * NW];
+      double wn[NW], pc[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) wn[q] = wold[q] + psf[q] * dt / V;
+      get_prim<D>(wn, A.gamma, pc);
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        cell_sh[q] = wn[q];
+        cell_sh[NW + q] = pc[q];
+      }
+      res[0] = dt / V;
+    }
+  }
+  if (MODE == MODE_FLUX) {
+    for (int i = threadIdx.x; i < n; i += BS) {
+      double u[D], f[NDF], s[D][NDF], fl[NDF];
+#pragma unroll
+      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        f[k] = G.f(i, k);
+#pragma unroll
+        for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
+        fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
+      }
+      gather_flux<D, NDF, UNI>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) A.vflux[G.off * NDF + int64_t(k) * n + i] = fl[k];
+    }
+    return;
+  }
+  __syncthreads();
+  const double dtV = res[0];
+  double* out = A.df_out;
+
+  // pass 1: flux of every velocity cell (gathered or from the accumulators)
+  double mw[NW];
+  double qa[NQ];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) mw[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) qa[q] = 0.0;
+  {
+    double pc[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) pc[q] = cell_sh[NW + q];
+    for (int i = threadIdx.x; i < n; i += BS) {
+      double u[D], f[NDF], fl[NDF];
+#pragma unroll
+      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+      if (MODE == MODE_FUSED) {
+        double s[D][NDF];
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fl[k] = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
+        }
+        gather_flux<D, NDF, UNI>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
+          A.vflux[G.off * NDF + int64_t(k) * n + i] = 0.0;
+        }
+      }
+      const double wt = G.wt(i);
+      if (MARCH == MARCH_CAIDVM) {
+        // convection first: f* = f + dt/V*flux, and the moments of f*
+        double fs[NDF], m[NW];
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fs[k] = f[k] + dtV * fl[k];
+          sc[k * n + i] = fs[k];
+        }
+        moment_terms<D, NDF>(wt, u, fs, m);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) mw[q] += m[q];
+      } else {
+        // Euler: heat flux of f^n with the updated prim
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) sc[k * n + i] = fl[k];
+        heat_terms<D, NDF, NQ>(wt, u, f, pc, qa);
+      }
+    }
+  }
+
+  if (MARCH == MARCH_CAIDVM) {
+    block_reduce<NW, NWV>(mw, NW, red, &cell_sh[2 * NW]);
+    if (threadIdx.x == 0) {
+      double wf[NW], pr[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) wf[q] = cell_sh[2 * NW + q];
+      wf[NW - 1] = 0.5 * wf[NW - 1];
+      get_prim<D>(wf, A.gamma, pr);
+#pragma unroll
+      for (int q = 0; q < NW; ++q) cell_sh[2 * NW + q] = pr[q];
+    }
+    __syncthreads();
+    // pass 2: conservation correction f += M(prim_c) - M(prim); heat flux of the result
+    {
+      double pc[NW], pr[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        pc[q] = cell_sh[NW + q];
+        pr[q] = cell_sh[2 * NW + q];
+      }
+      const MaxPre<D> mpc = max_pre<D>(pc, A.Pr), mpr = max_pre<D>(pr, A.Pr);
+      for (int i = threadIdx.x; i < n; i += BS) {
+        double u[D], Mc[2], M[2];
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+        maxwell_pre<D>(u, mpc, A.K, Mc);
+        maxwell_pre<D>(u, mpr, A.K, M);
+        double fs[NDF];
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fs[k] = sc[k * n + i] + (Mc[k] - M[k]);
+          sc[k * n + i] = fs[k];
+        }
+        sc[NDF * n + i] = Mc[0];
+        heat_terms<D, NDF, NQ>(G.wt(i), u, fs, pc, qa);
+      }
+    }
+  }
+  double qs_red[NQ];
+  block_reduce<NQ, NWV>(qa, NQ, red, &cell_sh[3 * NW]);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) qs_red[q] = cell_sh[3 * NW + q];
+  double q[D];
+  heat_finish<D>(qs_red, q);
+  double pc[NW];
+#pragma unroll
+  for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
+  const double tau = get_tau<D>(pc, A.mu, A.omega);
+
+  if (MARCH == MARCH_CAIDVM) {
+    // pass 3: g+ = M(prim_c) + Shakhov part; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
+    const double a1 = tau / (tau + dt), a2 = dt / (tau + dt);
+    const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
+    for (int i = threadIdx.x; i < n; i += BS) {
+      double u[D], Fc[2], Sp[2];
+#pragma unroll
+      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      Fc[0] = sc[NDF * n + i];
+      if constexpr (NDF == 2) Fc[1] = div_by(Fc[0] * A.K, mpc.two_l, mpc.r2l);
+      shakhov_pre<D>(u, Fc, mpc, q, A.K, Sp);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        double g = Fc[k] + Sp[k];
+        double fn = sc[k * n + i] * a1;
+        fn = fn + a2 * g;
+        out[G.off * NDF + int64_t(k) * n + i] = fn;
+      }
+    }
+  } else {
+    // Euler: f = (f + dt/V*flux)*tau/(tau+dt) + dt/(tau+dt)*F with F = M + Shakhov(M)
+    const double tpd = tau + dt, a2 = dt / (tau + dt);
+    const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
+    for (int i = threadIdx.x; i < n; i += BS) {
+      double u[D], M[2], Sp[2];
+#pragma unroll
+      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      maxwell_pre<D>(u, mpc, A.K, M);
+      shakhov_pre<D>(u, M, mpc, q, A.K, Sp);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        double F = M[k] + Sp[k];
+        double fx = (G.f(i, k) + dtV * sc[k * n + i]) * tau / tpd + a2 * F;
+        out[G.off * NDF + int64_t(k) * n + i] = fx;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    double* pr = &A.prim[int64_t(c) * NW];
+    double* rs = &A.res[int64_t(c) * 2 * NW];
+#pragma unroll
+    for (int m = 0; m < NW; ++m) {
+      double dl = pc[m] - pr[m];
+      rs[m] = dl * dl;
+      rs[NW + m] = fabs(pc[m]);
+      pr[m] = pc[m];
+      A.w[int64_t(c) * NW + m] = cell_sh[m];
+      A.psflux[int64_t(c) * NW + m] = 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) A.qf[int64_t(c) * D + d] = q[d];
+  }
+}
+
+// ------------------------------------------------------------------ K3: macro slope
+template <int D, int NDF, bool UNI>
+__global__ __launch_bounds__(256) void k_macro_slope(DevArgs A, const int32_t* __restrict__ cells,
+                                                     int ncells, double* __restrict__ sw) {
+  constexpr int NW = D + 2;
+  const int c = cells[blockIdx.x];
+  const Grid<D, NDF, UNI> G(A, c);
+  __shared__ double red[4 * NW];
+  __shared__ double tot[NW];
+  for (int d = 0; d < D; ++d) {
+    double acc[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) acc[q] = 0.0;
+    for (int i = threadIdx.x; i < G.n; i += 256) {
+      double u[D], s[NDF], m[NW];
+#pragma unroll
+      for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) s[k] = G.s(i, k, d);
+      moment_terms<D, NDF>(G.wt(i), u, s, m);
+#pragma unroll
+      for (int q = 0; q < NW; ++q) acc[q] += m[q];
+    }
+    block_reduce<NW, 4>(acc, NW, red, tot);
+    if (int(threadIdx.x) < NW)
+      sw[(int64_t(c) * NW + threadIdx.x) * D + d] =
+          (int(threadIdx.x) == NW - 1) ? 0.5 * tot[threadIdx.x] : tot[threadIdx.x];
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <int D, int NDF, bool UNI>
+inline hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, int maxh,
+                                 hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (maxh <= 4)
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 4, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
+  else if (maxh <= 8)
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 8, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
+  else if (maxh <= 16)
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 16, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
+  else
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+                       KAH_RO_ARGS(A));
+  return hipGetLastError();
+}
+
+template <int D, int NDF, bool UNI, int VPT>
+inline hipError_t launch_slope_chunk_v(const DevArgs& A, const int2* work, int n, int maxh,
+                                       hipStream_t st) {
+  if (maxh <= 4)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 4, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+                       A, work, n, KAH_RO_ARGS(A));
+  else if (maxh <= 8)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 8, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+                       A, work, n, KAH_RO_ARGS(A));
+  else if (maxh <= 16)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 16, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+                       A, work, n, KAH_RO_ARGS(A));
+  else
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 24, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+                       A, work, n, KAH_RO_ARGS(A));
+  return hipGetLastError();
+}
+
+template <int D, int NDF, bool UNI>
+inline hipError_t launch_slope_chunk_t(const DevArgs& A, const int2* work, int n, int maxh,
+                                       int vpt, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (vpt) {
+    case 1: return launch_slope_chunk_v<D, NDF, UNI, 1>(A, work, n, maxh, st);
+    case 2: return launch_slope_chunk_v<D, NDF, UNI, 2>(A, work, n, maxh, st);
+    case 3: return launch_slope_chunk_v<D, NDF, UNI, 3>(A, work, n, maxh, st);
+    case 4: return launch_slope_chunk_v<D, NDF, UNI, 4>(A, work, n, maxh, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int D, int NDF, bool UNI, int MODE, int MARCH, int NWV>
+inline hipError_t launch_fr_t(const DevArgs& A, const int32_t* cells, int n, size_t lds,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (lds > 0)
+    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, true, NWV>), dim3(n),
+                       dim3(NWV * 64), lds, st, A, cells, n, KAH_RO_ARGS(A), A.sdf_r,
+                       A.half_fw_r, A.rho_w_r);
+  else
+    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, false, NWV>), dim3(n),
+                       dim3(NWV * 64), 0, st, A, cells, n, KAH_RO_ARGS(A), A.sdf_r,
+                       A.half_fw_r, A.rho_w_r);
+  return hipGetLastError();
+}
+
+template <int D, int NDF, bool UNI, int NWV>
+inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int mode,
+                              int march, size_t lds, hipStream_t st) {
+  if (mode == MODE_FLUX)
+    return launch_fr_t<D, NDF, UNI, MODE_FLUX, MARCH_CAIDVM, NWV>(A, cells, n, 0, st);
+  if (mode == MODE_ITER) {
+    if (march == MARCH_CAIDVM)
+      return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+    return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_EULER, NWV>(A, cells, n, lds, st);
+  }
+  if (march == MARCH_CAIDVM)
+    return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+  return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_EULER, NWV>(A, cells, n, lds, st);
+}
+
+template <int D, int NDF, bool UNI>
+inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int mode,
+                              int march, size_t lds, int nwv, hipStream_t st) {
+  if (nwv == 4) return launch_fr_w<D, NDF, UNI, 4>(A, cells, n, mode, march, lds, st);
+  return launch_fr_w<D, NDF, UNI, 8>(A, cells, n, mode, march, lds, st);
+}
+
+template <int D, int NDF, bool UNI>
+inline hipError_t launch_macro_slope_t(const DevArgs& A, const int32_t* cells, int n, double* sw,
+                                       hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_macro_slope<D, NDF, UNI>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
+  return hipGetLastError();
+}
+
+}  // namespace kah

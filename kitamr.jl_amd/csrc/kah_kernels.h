@@ -75,14 +75,15 @@ struct DevArgs {
     (B).rec_ds = ro_recds;                                                                   \
   } while (0)
 
-constexpr int SLOPE_CHUNK = 256;   // velocity cells per workgroup of the chunked slope kernel
+constexpr int SLOPE_CHUNK = 256;   // threads per workgroup of the chunked slope kernel
+constexpr int SLOPE_MAX_VPT = 4;   // velocity cells per thread of the chunked slope kernel: 1..4
 constexpr int MAX_ENTRIES = 24;    // face sides per cell (2*DIM sides, up to 2^(DIM-1) each)
 
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                         int maxh, hipStream_t st);
-// work = (cell, chunk) pairs, chunk = SLOPE_CHUNK velocity cells
+// work = (cell, chunk) pairs, chunk = vpt*SLOPE_CHUNK velocity cells
 hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, const int2* work, int n,
-                              int maxh, hipStream_t st);
+                              int maxh, int vpt, hipStream_t st);
 hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
                              int mode, int march, size_t lds, int waves_per_block,
                              hipStream_t st);

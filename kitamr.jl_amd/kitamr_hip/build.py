@@ -9,7 +9,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "lib" / "libkitamr_hip.so"
-SOURCES = ["kah_api.cpp", "kah_kernels.hip"]
+SOURCES = ["kah_api.cpp", "kah_kernels.hip", "kah_slope_2d.hip", "kah_slope_3d.hip",
+           "kah_flux_2d.hip", "kah_flux_3d.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
