@@ -44,6 +44,21 @@ def torch_sync():
         pass
 
 
+PROFILE = ROOT / "profiles" / "k_flux_relax_riemann2d_49k_48x48.md"
+
+
+def profiled_traffic():
+    """HBM bytes per k_flux_relax launch on this workload, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (counters scaled by the copy-kernel calibration)."""
+    try:
+        for line in PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch k_flux_relax = "):
+                return float(line.rsplit("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(case, max_steps=5, min_seconds=10.0):
     import numpy as np
     import oracle as O
@@ -170,7 +185,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic() if not args.small else None,
             "alg_bytes_per_update": B_FLUX_ITER,
             "avg_launch_ms": kr_ms,
         },
