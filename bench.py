@@ -24,11 +24,15 @@ sys.path.insert(0, str(ROOT / "kitamr.jl_amd"))
 sys.path.insert(0, str(ROOT / "oracle"))
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per phase-space update of the reference's 3-phase dataflow, FP64 2D2F,
-# shared velocity grid: slope R df + W sdf; flux R df, R sdf, W flux; iterate R df, R flux, W df
-B_SLOPE = 2 * 8 * (1 + 2)          # 48
-B_FLUX_ITER = 2 * 8 * (1 + 2 + 1) + 2 * 8 * 3   # 64 + 48 = 112
-B_STEP = B_SLOPE + B_FLUX_ITER     # 160
+
+
+def alg_bytes(s_f):
+    """Algorithmic bytes per phase-space update of the reference's 3-phase dataflow, 2D2F on a
+    shared velocity grid, with s_f bytes per stored f value (8: FP64, 4: FP32-f): slope
+    R df + W sdf; flux R df, R sdf, W flux; iterate R df, R flux, W df."""
+    b_slope = 2 * s_f * (1 + 2)
+    b_flux_iter = 2 * s_f * (1 + 2 + 1) + 2 * s_f * 3
+    return b_slope, b_flux_iter, b_slope + b_flux_iter   # FP64: 48, 112, 160
 
 
 def log(msg):
@@ -95,6 +99,10 @@ def main():
                     help="waves per workgroup of the flux/relaxation kernel (0: library default)")
     ap.add_argument("--slope-vpt", type=int, default=0,
                     help="velocity cells per thread of the slope kernel (0: library default)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32f"],
+                    help="fp64 (parity form) or fp32f (f stored in FP32, FP64 moments)")
+    ap.add_argument("--vel-tiles", type=int, default=-1,
+                    help="1/0: wave tiles of the velocity grid on/off (-1: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -108,9 +116,11 @@ def main():
 
     t0 = time.perf_counter()
     if args.small:
-        case = cases.riemann_2d(n=32, nv=48, ps_levels=2, halo=(4.0, 2.0))
+        case = cases.riemann_2d(n=32, nv=48, ps_levels=2, halo=(4.0, 2.0),
+                                precision=args.precision)
     else:
-        case = cases.riemann_2d(n=64, nv=48, ps_levels=3)
+        case = cases.riemann_2d(n=64, nv=48, ps_levels=3, precision=args.precision)
+    B_SLOPE, B_FLUX_ITER, B_STEP = alg_bytes(8 if args.precision == "fp64" else 4)
     P = case.phase_cells
     log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
         f"pairs={case.topo['n_pairs']} P={P}")
@@ -119,6 +129,8 @@ def main():
         dev.set_option("flux_waves", args.flux_waves)
     if args.slope_vpt:
         dev.set_option("slope_vpt", args.slope_vpt)
+    if args.vel_tiles >= 0:
+        dev.set_option("vel_tiles", args.vel_tiles)
     dev.set_topology(case.topo)
     dev.set_state(case.w, case.prim, case.df)
     dt = case.dt_xi
@@ -167,12 +179,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 storage / f64 arithmetic",
         "data": "synthetic",
         "config": {
             "workload": ("2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
-                         "uniform 48x48 velocity grid, FP64, CAIDVM + CAIDVM_Marching"
+                         "uniform 48x48 velocity grid, CAIDVM + CAIDVM_Marching"
                          if not args.small else "reduced 32x32 + 2 levels (debug)"),
+            "precision": args.precision,
             "leaves": case.n_cells,
             "phase_cells_per_step": P,
             "parallelism": "single GPU",
@@ -185,7 +198,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic() if not args.small else None,
+            "traffic": profiled_traffic() if not args.small and args.precision == "fp64" else None,
             "alg_bytes_per_update": B_FLUX_ITER,
             "avg_launch_ms": kr_ms,
         },

@@ -186,9 +186,10 @@ int64_t kah_phase_cells(const kah_ctx* ctx);
 int64_t kah_device_bytes(const kah_ctx* ctx);
 /* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
  * the flux/relaxation kernel (4 or 8). KAH_OPT_SLOPE_VPT: velocity cells per thread
- * of the slope kernel (1..4; takes effect at the next kah_set_topology). No counterpart in
- * the reference. */
-enum { KAH_OPT_FLUX_WAVES = 1, KAH_OPT_SLOPE_VPT = 2 };
+ * of the slope kernel (1..4). KAH_OPT_VEL_TILES: 1 (default) processes a shared
+ * tensor-product velocity grid in 8x8 / 4x4x4 tiles per wave, 0 in storage order. The last
+ * two take effect at the next kah_set_topology. No counterpart in the reference. */
+enum { KAH_OPT_FLUX_WAVES = 1, KAH_OPT_SLOPE_VPT = 2, KAH_OPT_VEL_TILES = 3 };
 int kah_set_option(kah_ctx* ctx, int option, int64_t value);
 /* Times reps device-to-device copies of `bytes` bytes with a plain streaming copy kernel
  * (attainable HBM bandwidth = 2*bytes / ms_per_copy); the reference point for the

@@ -62,6 +62,7 @@ struct kah_ctx {
   int D = 2, NDF = 2, NW = 4;
   int64_t nc = 0, np = 0, npairs = 0;
   bool uni = true;
+  bool f32 = false;   // FP32-f form: f, sdf stored as float
   int n0 = 0;
   int lmax = 0;
   int max_n = 0;
@@ -73,12 +74,15 @@ struct kah_ctx {
   DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey;
   DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
   DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
+  DBuf d_vtb;
   int64_t part_rows = 0;
   int cur = 0;
   int64_t dev_bytes = 0;
   bool timing = false;
   int fr_waves = 8;   // waves per workgroup of the flux/relaxation kernel
   int slope_vpt = 3;  // velocity cells per thread of the chunked slope kernel
+  bool vt_enable = true;   // wave-tile order of shared tensor-product velocity grids
+  int vt_mode = 0, vt_n0 = 0, vt_n1 = 0;
   std::vector<EvPair> events;
   double t_sum[2] = {0, 0};
   int64_t t_cnt[2] = {0, 0};
@@ -120,6 +124,32 @@ struct kah_ctx {
     if (rc) return rc;
     if (count == 0) return 0;
     return hip(hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice), what);
+  }
+  size_t fsize() const { return f32 ? sizeof(float) : sizeof(double); }
+  // host double array -> device storage type (float in the FP32-f form: rounded to nearest)
+  int upload_f(DBuf& b, const double* src, size_t count, const char* what) {
+    if (!f32)
+      return hip(hipMemcpyAsync(b.p, src, count * sizeof(double), hipMemcpyHostToDevice, stream),
+                 what);
+    std::vector<float> tmp(count);
+    for (size_t x = 0; x < count; ++x) tmp[x] = float(src[x]);
+    int rc = hip(hipMemcpyAsync(b.p, tmp.data(), count * sizeof(float), hipMemcpyHostToDevice,
+                                stream), what);
+    if (rc) return rc;
+    return hip(hipStreamSynchronize(stream), what);
+  }
+  int download_f(double* dst, const DBuf& b, size_t count, const char* what) {
+    if (!dst) return 0;
+    if (!f32)
+      return hip(hipMemcpyAsync(dst, b.p, count * sizeof(double), hipMemcpyDeviceToHost, stream),
+                 what);
+    std::vector<float> tmp(count);
+    int rc = hip(hipMemcpyAsync(tmp.data(), b.p, count * sizeof(float), hipMemcpyDeviceToHost,
+                                stream), what);
+    if (!rc) rc = hip(hipStreamSynchronize(stream), what);
+    if (rc) return rc;
+    for (size_t x = 0; x < count; ++x) dst[x] = double(tmp[x]);
+    return 0;
   }
   int zero(DBuf& b) {
     if (!b.p) return 0;
@@ -411,13 +441,15 @@ int kah_create(const kah_params* params, kah_ctx** out) {
     return KAH_EINVAL;
   if (params->marching != KAH_MARCH_CAIDVM && params->marching != KAH_MARCH_EULER)
     return KAH_EINVAL;
-  if (params->precision != KAH_PREC_FP64) return KAH_EUNSUPPORTED;
+  if (params->precision != KAH_PREC_FP64 && params->precision != KAH_PREC_FP32F)
+    return KAH_EUNSUPPORTED;
   kah_ctx* C = new kah_ctx();
   C->P = *params;
   if (C->P.st_check_interval <= 0) C->P.st_check_interval = 100;
   C->D = params->dim;
   C->NDF = params->ndf;
   C->NW = params->dim + 2;
+  C->f32 = params->precision == KAH_PREC_FP32F;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     delete C;
@@ -484,6 +516,37 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     if (T->nb_cell[x] < 0 || T->nb_cell[x] >= nc)
       return C->fail(KAH_EINVAL, "neighbour id out of range");
   C->benc.assign(T->bound_enc, T->bound_enc + nc);
+
+  // Wave tiles of a shared tensor-product velocity grid (vs_trees_num cells per direction,
+  // first direction fastest): 8x8 (2D) or 4x4x4 (3D) blocks, so that the 64 lanes of a wave
+  // share the sign of every velocity component when the zero planes fall on tile
+  // boundaries, and upwind branches do not split waves. Any tile order is a bijection of
+  // the grid, so results do not depend on it.
+  C->vt_mode = 0;
+  C->d_vtb.release();
+  if (C->uni && C->vt_enable) {
+    const int tw = D == 2 ? 8 : 4;
+    int64_t prod = 1;
+    bool ok = C->n0 % 64 == 0;
+    for (int d = 0; d < D; ++d) {
+      ok = ok && T->vs_trees_num[d] > 0 && T->vs_trees_num[d] % tw == 0;
+      prod *= std::max(T->vs_trees_num[d], 1);
+    }
+    if (ok && prod == C->n0) {
+      const int n0 = T->vs_trees_num[0], n1 = T->vs_trees_num[1];
+      const int tx = n0 / tw, ty = n1 / tw, tz = D == 3 ? T->vs_trees_num[2] / tw : 1;
+      std::vector<int32_t> base;
+      base.reserve(size_t(tx) * ty * tz);
+      for (int z = 0; z < tz; ++z)
+        for (int y = 0; y < ty; ++y)
+          for (int x = 0; x < tx; ++x)
+            base.push_back(int32_t((int64_t(z) * tw * n1 + int64_t(y) * tw) * n0 + x * tw));
+      KAH_TRY(C->upload(C->d_vtb, base.data(), base.size(), "velocity tiles"));
+      C->vt_mode = D;
+      C->vt_n0 = n0;
+      C->vt_n1 = n1;
+    }
+  }
 
   // velocity grids: upload, and keys for adapted grids
   const int64_t ng = C->uni ? C->n0 : C->np;
@@ -628,9 +691,10 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
 
   // state buffers
   const int64_t NDF = C->NDF, NW = C->NW;
-  KAH_TRY(C->alloc(C->d_df[0], sizeof(double) * C->np * NDF, "df"));
-  KAH_TRY(C->alloc(C->d_df[1], sizeof(double) * C->np * NDF, "df"));
-  KAH_TRY(C->alloc(C->d_sdf, sizeof(double) * C->np * NDF * D, "sdf"));
+  const size_t fs = C->fsize();
+  KAH_TRY(C->alloc(C->d_df[0], fs * C->np * NDF, "df"));
+  KAH_TRY(C->alloc(C->d_df[1], fs * C->np * NDF, "df"));
+  KAH_TRY(C->alloc(C->d_sdf, fs * C->np * NDF * D, "sdf"));
   KAH_TRY(C->alloc(C->d_psflux, sizeof(double) * nc * NW, "flux"));
   KAH_TRY(C->alloc(C->d_w, sizeof(double) * nc * NW, "w"));
   KAH_TRY(C->alloc(C->d_prim, sizeof(double) * nc * NW, "prim"));
@@ -664,6 +728,10 @@ DevArgs make_args(kah_ctx* C, double dt) {
   DevArgs A{};
   A.uni_n = C->n0;
   A.lmax = C->lmax;
+  A.vt_base = C->d_vtb.as<int32_t>();
+  A.vt_mode = C->vt_mode;
+  A.vt_n0 = C->vt_n0;
+  A.vt_n1 = C->vt_n1;
   A.voff = C->d_voff.as<int64_t>();
   A.gu = C->d_gu.as<double>();
   A.gw = C->d_gw.as<double>();
@@ -728,10 +796,10 @@ int run_slope(kah_ctx* C, const DevArgs& A) {
   if (C->timing) KAH_TRY(ev_begin(C, e, 0));
   for (LevelBin* b : C->bins) {
     if (b->chunked)
-      KAH_TRY(C->hip(launch_slope_chunk(A, C->D, C->uni, b->items.as<int2>(), b->n, b->maxh,
+      KAH_TRY(C->hip(launch_slope_chunk(A, C->D, C->uni, C->f32, b->items.as<int2>(), b->n, b->maxh,
                                         C->slope_vpt, C->stream), "slope kernel launch"));
     else
-      KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, b->items.as<int32_t>(), b->n, b->maxh,
+      KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, C->f32, b->items.as<int32_t>(), b->n, b->maxh,
                                   C->stream), "slope kernel launch"));
   }
   if (C->timing) KAH_TRY(ev_end(C, e));
@@ -742,7 +810,7 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   EvPair e{};
   if (C->timing) KAH_TRY(ev_begin(C, e, 1));
   int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER : MARCH_CAIDVM;
-  KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
+  KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
                                    mode, march, C->lds_bytes, C->fr_waves, C->stream),
                  "flux/relaxation kernel launch"));
   if (C->timing) KAH_TRY(ev_end(C, e));
@@ -790,8 +858,7 @@ int kah_set_state(kah_ctx* C, const double* w, const double* prim, const double*
                                 C->stream), "w upload"));
   KAH_TRY(C->hip(hipMemcpyAsync(C->d_prim.p, prim, sizeof(double) * C->nc * NW,
                                 hipMemcpyHostToDevice, C->stream), "prim upload"));
-  KAH_TRY(C->hip(hipMemcpyAsync(C->d_df[C->cur].p, df, sizeof(double) * C->np * C->NDF,
-                                hipMemcpyHostToDevice, C->stream), "df upload"));
+  KAH_TRY(C->upload_f(C->d_df[C->cur], df, size_t(C->np * C->NDF), "df upload"));
   KAH_TRY(C->zero(C->d_psflux));
   KAH_TRY(C->zero(C->d_vflux));
   KAH_TRY(C->zero(C->d_qf));
@@ -810,7 +877,7 @@ int kah_get_state(kah_ctx* C, double* w, double* prim, double* qf, double* sw, d
     if (!C->d_sw.p) KAH_TRY(C->alloc(C->d_sw, sizeof(double) * C->nc * NW * D, "sw"));
     KAH_TRY(C->zero(C->d_sw));
     DevArgs A = make_args(C, 0.0);
-    KAH_TRY(C->hip(launch_macro_slope(A, C->D, C->uni, C->d_fluid.as<int32_t>(),
+    KAH_TRY(C->hip(launch_macro_slope(A, C->D, C->uni, C->f32, C->d_fluid.as<int32_t>(),
                                       int(C->fluid.size()), C->d_sw.as<double>(), C->stream),
                    "macro slope launch"));
     KAH_TRY(C->hip(hipMemcpyAsync(sw, C->d_sw.p, sizeof(double) * C->nc * NW * D,
@@ -823,8 +890,8 @@ int kah_get_state(kah_ctx* C, double* w, double* prim, double* qf, double* sw, d
   KAH_TRY(dl(w, C->d_w, size_t(C->nc * NW), "w download"));
   KAH_TRY(dl(prim, C->d_prim, size_t(C->nc * NW), "prim download"));
   KAH_TRY(dl(qf, C->d_qf, size_t(C->nc * D), "qf download"));
-  KAH_TRY(dl(df, C->d_df[C->cur], size_t(C->np * C->NDF), "df download"));
-  KAH_TRY(dl(sdf, C->d_sdf, size_t(C->np * C->NDF * D), "sdf download"));
+  KAH_TRY(C->download_f(df, C->d_df[C->cur], size_t(C->np * C->NDF), "df download"));
+  KAH_TRY(C->download_f(sdf, C->d_sdf, size_t(C->np * C->NDF * D), "sdf download"));
   return C->hip(hipStreamSynchronize(C->stream), "state download");
 }
 
@@ -931,6 +998,11 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
       if (value != 4 && value != 8)
         return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4 or 8");
       C->fr_waves = int(value);
+      return KAH_OK;
+    case KAH_OPT_VEL_TILES:
+      if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "velocity tiles must be 0 or 1");
+      C->vt_enable = value != 0;
+      C->have_topo = false;   // the tile table is built by kah_set_topology
       return KAH_OK;
     case KAH_OPT_SLOPE_VPT:
       if (value < 1 || value > SLOPE_MAX_VPT)

@@ -125,7 +125,7 @@ __device__ inline double pick(const double (&a)[D], int d) {
 }
 
 // ------------------------------------------------------------------ grid access
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 struct Grid {
   const DevArgs& A;
   int64_t off;   // first velocity cell of the cell
@@ -140,25 +140,49 @@ struct Grid {
     }
   }
   __device__ int64_t goff() const { return UNI ? 0 : off; }
+  // velocity cell handled at thread position p: the identity, or the wave-tile order of a
+  // shared tensor-product grid (the 64 positions of a wave form one tile; any bijection of
+  // [0, n) gives the same per-cell results, the order only groups velocity signs per wave)
+  __device__ int vi(int p) const {
+    if constexpr (UNI) {
+      if (A.vt_mode != 0) {
+        const int T = __builtin_amdgcn_readfirstlane(p >> 6);
+        const int l = p & 63;
+        const int o = (A.vt_mode == 2) ? (l >> 3) * A.vt_n0 + (l & 7)
+                                       : ((l >> 4) * A.vt_n1 + ((l >> 2) & 3)) * A.vt_n0 + (l & 3);
+        return A.vt_base[T] + o;
+      }
+    }
+    return p;
+  }
   __device__ double u(int i, int d) const { return A.gu[goff() * D + int64_t(d) * n + i]; }
   __device__ double wt(int i) const { return A.gw[goff() + i]; }
   __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[off + i]); }
   __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
-  __device__ double f(int i, int k) const { return A.df[off * NDF + int64_t(k) * n + i]; }
+  // f, sdf and the next f are stored as FT (double, or float in the FP32-f form); all
+  // arithmetic is FP64
+  __device__ double f(int i, int k) const {
+    return double(reinterpret_cast<const FT*>(A.df)[off * NDF + int64_t(k) * n + i]);
+  }
   __device__ double s(int i, int k, int d) const {
-    return A.sdf_r[off * NDF * D + int64_t(d * NDF + k) * n + i];
+    return double(reinterpret_cast<const FT*>(A.sdf_r)[off * NDF * D + int64_t(d * NDF + k) * n + i]);
   }
-  __device__ double* sp(int i, int k, int d) const {
-    return &A.sdf[off * NDF * D + int64_t(d * NDF + k) * n + i];
+  __device__ void set_s(int i, int k, int d, double v) const {
+    reinterpret_cast<FT*>(A.sdf)[off * NDF * D + int64_t(d * NDF + k) * n + i] = FT(v);
   }
+  __device__ void set_out(int i, int k, double v) const {
+    reinterpret_cast<FT*>(A.df_out)[off * NDF + int64_t(k) * n + i] = FT(v);
+  }
+  // a value as the storage holds it
+  static __device__ double stored(double v) { return double(FT(v)); }
 };
 
 // Calls fn(j, den) for each velocity cell j of grid G that overlaps cell i (level li, key
 // ki) of another grid on the same root lattice: den = 1 when j is coarser or as fine as
 // i, else 2^(D*(lj-li)). This is the pairing of the two-pointer walks of diff_vs! and
 // update_micro_flux! (src/Flux/Slope.jl:29-64, src/Flux/Flux.jl:151-344), in the same order.
-template <int D, int NDF, bool UNI, class Fn>
-__device__ inline void for_overlap(const Grid<D, NDF, UNI>& G, int i, int li, uint32_t ki,
+template <int D, int NDF, bool UNI, class FT, class Fn>
+__device__ inline void for_overlap(const Grid<D, NDF, UNI, FT>& G, int i, int li, uint32_t ki,
                                    int lmax, Fn&& fn) {
   if constexpr (UNI) {
     fn(i, 1.0);
@@ -268,7 +292,7 @@ __device__ inline void micro_maxwell(const double* bc, double rho, bool wall,
 // Velocity-space slope of one direction (update_slope!/update_slope_transverse_level!,
 // src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
 // taken as key-interval overlaps.
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 __device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, double rdsx,
                                 int i, int li, uint32_t ki, const double (&f)[NDF],
                                 double (&a)[NDF]) {
@@ -276,8 +300,8 @@ __device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, 
   for (int k = 0; k < NDF; ++k) a[k] = 0.0;
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
-    const Grid<D, NDF, UNI> N(A, nb.cell);
-    for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
+    const Grid<D, NDF, UNI, FT> N(A, nb.cell);
+    for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, [&](int j, double den) {
       const double rden = 1.0 / den;   // den is a power of two: exact
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
@@ -294,8 +318,8 @@ __device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, 
   }
 }
 
-template <int D, int NDF, bool UNI>
-__device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c, int d,
+template <int D, int NDF, bool UNI, class FT>
+__device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c, int d,
                                  int i, int li, uint32_t ki, const double (&f)[NDF],
                                  double (&s)[NDF]) {
   const SlopeDesc sd = A.sl[c * D + d];
@@ -308,15 +332,15 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI>& G, i
 #pragma unroll
     for (int k = 0; k < NDF; ++k) {
       s[k] = 0.0;
-      *G.sp(i, k, d) = 0.0;
+      G.set_s(i, k, d, 0.0);
     }
     return;
   }
   double aL[NDF];
-  side_sum<D, NDF, UNI>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li, ki, f, aL);
+  side_sum<D, NDF, UNI, FT>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li, ki, f, aL);
   if (sd.mode == SL_INNER) {
     double aR[NDF];
-    side_sum<D, NDF, UNI>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li, ki, f,
+    side_sum<D, NDF, UNI, FT>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li, ki, f,
                           aR);
 #pragma unroll
     for (int k = 0; k < NDF; ++k) s[k] = minmod(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
@@ -325,14 +349,17 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI>& G, i
     for (int k = 0; k < NDF; ++k) s[k] = aL[k] * sd.inv_cnt[0];
   }
 #pragma unroll
-  for (int k = 0; k < NDF; ++k) *G.sp(i, k, d) = s[k];
+  for (int k = 0; k < NDF; ++k) {
+    s[k] = G.stored(s[k]);   // the flux uses the slope as stored
+    G.set_s(i, k, d, s[k]);
+  }
 }
 
 // Moments of one velocity cell's contribution to a half flux (the cell's own upwind micro
 // flux through one face, micro_to_macro of src/Flux/CAIDVM.jl:119), or the two sums of the
 // wall density (calc_ρw, src/Theory/Math.jl:251-282) in the first sweep of a wall.
-template <int D, int NDF, bool UNI>
-__device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, int c,
+template <int D, int NDF, bool UNI, class FT>
+__device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c,
                                   const HalfDesc& hd, int i, const double (&u)[D],
                                   const double (&f)[NDF], const double (&s)[D][NDF], double wt,
                                   double (&m)[D + 2]) {
@@ -375,14 +402,14 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI>& G, 
   } else {
     // the there side of a face whose other cell is solid: f of that cell * vn (shared
     // velocity grids only; checked on the host)
-    const Grid<D, NDF, UNI> S(A, hd.data_cell);
+    const Grid<D, NDF, UNI, FT> S(A, hd.data_cell);
 #pragma unroll
     for (int k = 0; k < NDF; ++k) mic[k] = S.f(i, k) * ud;
   }
   moment_terms<D, NDF>(wt, u, mic, m);
 }
 
-template <int D, int NDF, bool UNI, int MAXH, int NWV>
+template <int D, int NDF, bool UNI, class FT, int MAXH, int NWV>
 __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* __restrict__ cells,
                                                     int ncells, KAH_RO_PARAMS) {
   constexpr int NW = D + 2;
@@ -391,7 +418,7 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   DevArgs A = A0;
   KAH_RO_BIND(A);
   const int c = cells[xcd_remap(blockIdx.x, ncells)];
-  const Grid<D, NDF, UNI> G(A, c);
+  const Grid<D, NDF, UNI, FT> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
   __shared__ double red[NWV * CNT];
@@ -411,13 +438,13 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
     const int li = G.lev(i);
     const uint32_t ki = G.key(i);
 #pragma unroll
-    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
+    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT>(A, G, c, d, i, li, ki, f, s[d]);
     const double wt = G.wt(i);
 #pragma unroll
     for (int h = 0; h < MAXH; ++h) {
       double m[NW];
       if (h < nh) {
-        half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+        half_terms<D, NDF, UNI, FT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
       } else {
 #pragma unroll
         for (int q = 0; q < NW; ++q) m[q] = 0.0;
@@ -481,7 +508,7 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
 // work as k_slope without the loop over the grid, for cells without wall faces. Each
 // workgroup writes its partial sums of the cell's half fluxes to row part_off[slot]+chunk;
 // k_flux_relax adds the rows in chunk order.
-template <int D, int NDF, bool UNI, int MAXH, int VPT>
+template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT>
 __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
                                                              const int2* __restrict__ work,
                                                              int nwork, KAH_RO_PARAMS) {
@@ -491,7 +518,7 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
   KAH_RO_BIND(A);
   const int2 wk = work[xcd_remap(blockIdx.x, nwork)];
   const int c = wk.x, ch = wk.y;
-  const Grid<D, NDF, UNI> G(A, c);
+  const Grid<D, NDF, UNI, FT> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
   __shared__ double buf[CHUNK_QB * SLOPE_CHUNK];
@@ -504,8 +531,9 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
   // velocity cells ch*VPT*SLOPE_CHUNK + r*SLOPE_CHUNK + threadIdx.x, r < VPT
 #pragma unroll
   for (int r = 0; r < VPT; ++r) {
-    const int i = (ch * VPT + r) * SLOPE_CHUNK + int(threadIdx.x);
-    if (i < G.n) {
+    const int p = (ch * VPT + r) * SLOPE_CHUNK + int(threadIdx.x);
+    if (p < G.n) {
+      const int i = G.vi(p);
       double u[D], f[NDF], s[D][NDF];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
@@ -514,13 +542,13 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
       const int li = G.lev(i);
       const uint32_t ki = G.key(i);
 #pragma unroll
-      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI>(A, G, c, d, i, li, ki, f, s[d]);
+      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT>(A, G, c, d, i, li, ki, f, s[d]);
       const double wt = G.wt(i);
 #pragma unroll
       for (int h = 0; h < MAXH; ++h) {
         if (h < nh) {
           double m[NW];
-          half_terms<D, NDF, UNI>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+          half_terms<D, NDF, UNI, FT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
 #pragma unroll
           for (int q = 0; q < NW; ++q) acc[h * NW + q] = (r == 0) ? m[q] : acc[h * NW + q] + m[q];
         }
@@ -537,7 +565,7 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
 // ------------------------------------------------------------------ K2: flux + relaxation
 // Micro flux into velocity cell i of cell c through all its faces, in face-list order
 // (update_flux!/update_domain_flux!/update_micro_flux!, src/Flux/Flux.jl:64-281).
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 __device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
                                    const double (&u)[D], const double (&f)[NDF],
                                    const double (&s)[D][NDF], int li, uint32_t ki,
@@ -581,8 +609,8 @@ __device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
         break;
       }
       default: {
-        const Grid<D, NDF, UNI> N(A, ed.oth_cell);
-        for_overlap<D, NDF, UNI>(N, i, li, ki, A.lmax, [&](int j, double den) {
+        const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
+        for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, [&](int j, double den) {
           double uj[D], fj[NDF], sj[D][NDF];
 #pragma unroll
           for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : N.u(j, t);
@@ -642,7 +670,7 @@ __device__ inline void heat_finish(const double* qs, double* q) {
 // iterate!(CAIDVM_Marching) (src/Theory/Iterate.jl:96-130) or iterate!(Euler) (:131-162)
 // for one cell, fed by the gathered flux (MODE_FUSED), by the flux accumulators
 // (MODE_ITER), or only accumulating the flux (MODE_FLUX).
-template <int D, int NDF, bool UNI, int MODE, int MARCH, bool LDS, int NWV>
+template <int D, int NDF, bool UNI, class FT, int MODE, int MARCH, bool LDS, int NWV>
 __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
                                                          const int32_t* __restrict__ cells,
                                                          int ncells, KAH_RO_PARAMS,
@@ -658,7 +686,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   A.half_fw_r = ro_half;
   A.rho_w_r = ro_rho;
   const int c = cells[xcd_remap(blockIdx.x, ncells)];
-  const Grid<D, NDF, UNI> G(A, c);
+  const Grid<D, NDF, UNI, FT> G(A, c);
   const int n = G.n;
   const int eb = A.ent_off[c];
   const int ne = A.ent_off[c + 1] - eb;
@@ -725,7 +753,8 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     }
   }
   if (MODE == MODE_FLUX) {
-    for (int i = threadIdx.x; i < n; i += BS) {
+    for (int p = threadIdx.x; p < n; p += BS) {
+      const int i = G.vi(p);
       double u[D], f[NDF], s[D][NDF], fl[NDF];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
@@ -736,7 +765,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
         fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
       }
-      gather_flux<D, NDF, UNI>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+      gather_flux<D, NDF, UNI, FT>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) A.vflux[G.off * NDF + int64_t(k) * n + i] = fl[k];
     }
@@ -744,7 +773,6 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   }
   __syncthreads();
   const double dtV = res[0];
-  double* out = A.df_out;
 
   // pass 1: flux of every velocity cell (gathered or from the accumulators)
   double mw[NW];
@@ -757,7 +785,8 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     double pc[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) pc[q] = cell_sh[NW + q];
-    for (int i = threadIdx.x; i < n; i += BS) {
+    for (int p = threadIdx.x; p < n; p += BS) {
+      const int i = G.vi(p);
       double u[D], f[NDF], fl[NDF];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
@@ -771,7 +800,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 #pragma unroll
           for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
         }
-        gather_flux<D, NDF, UNI>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+        gather_flux<D, NDF, UNI, FT>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -821,7 +850,8 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         pr[q] = cell_sh[2 * NW + q];
       }
       const MaxPre<D> mpc = max_pre<D>(pc, A.Pr), mpr = max_pre<D>(pr, A.Pr);
-      for (int i = threadIdx.x; i < n; i += BS) {
+      for (int p = threadIdx.x; p < n; p += BS) {
+        const int i = G.vi(p);
         double u[D], Mc[2], M[2];
 #pragma unroll
         for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
@@ -853,7 +883,8 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     // pass 3: g+ = M(prim_c) + Shakhov part; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
     const double a1 = tau / (tau + dt), a2 = dt / (tau + dt);
     const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
-    for (int i = threadIdx.x; i < n; i += BS) {
+    for (int p = threadIdx.x; p < n; p += BS) {
+      const int i = G.vi(p);
       double u[D], Fc[2], Sp[2];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
@@ -865,14 +896,15 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         double g = Fc[k] + Sp[k];
         double fn = sc[k * n + i] * a1;
         fn = fn + a2 * g;
-        out[G.off * NDF + int64_t(k) * n + i] = fn;
+        G.set_out(i, k, fn);
       }
     }
   } else {
     // Euler: f = (f + dt/V*flux)*tau/(tau+dt) + dt/(tau+dt)*F with F = M + Shakhov(M)
     const double tpd = tau + dt, a2 = dt / (tau + dt);
     const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
-    for (int i = threadIdx.x; i < n; i += BS) {
+    for (int p = threadIdx.x; p < n; p += BS) {
+      const int i = G.vi(p);
       double u[D], M[2], Sp[2];
 #pragma unroll
       for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
@@ -882,7 +914,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       for (int k = 0; k < NDF; ++k) {
         double F = M[k] + Sp[k];
         double fx = (G.f(i, k) + dtV * sc[k * n + i]) * tau / tpd + a2 * F;
-        out[G.off * NDF + int64_t(k) * n + i] = fx;
+        G.set_out(i, k, fx);
       }
     }
   }
@@ -904,12 +936,12 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 }
 
 // ------------------------------------------------------------------ K3: macro slope
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 __global__ __launch_bounds__(256) void k_macro_slope(DevArgs A, const int32_t* __restrict__ cells,
                                                      int ncells, double* __restrict__ sw) {
   constexpr int NW = D + 2;
   const int c = cells[blockIdx.x];
-  const Grid<D, NDF, UNI> G(A, c);
+  const Grid<D, NDF, UNI, FT> G(A, c);
   __shared__ double red[4 * NW];
   __shared__ double tot[NW];
   for (int d = 0; d < D; ++d) {
@@ -934,98 +966,98 @@ __global__ __launch_bounds__(256) void k_macro_slope(DevArgs A, const int32_t* _
 }
 
 // ------------------------------------------------------------------ launchers
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, int maxh,
                                  hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (maxh <= 4)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 4, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, FT, 4, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
                        KAH_RO_ARGS(A));
   else if (maxh <= 8)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 8, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, FT, 8, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
                        KAH_RO_ARGS(A));
   else if (maxh <= 16)
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 16, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, FT, 16, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
                        KAH_RO_ARGS(A));
   else
-    hipLaunchKernelGGL((k_slope<D, NDF, UNI, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
+    hipLaunchKernelGGL((k_slope<D, NDF, UNI, FT, 24, 4>), dim3(n), dim3(256), 0, st, A, cells, n,
                        KAH_RO_ARGS(A));
   return hipGetLastError();
 }
 
-template <int D, int NDF, bool UNI, int VPT>
+template <int D, int NDF, bool UNI, class FT, int VPT>
 inline hipError_t launch_slope_chunk_v(const DevArgs& A, const int2* work, int n, int maxh,
                                        hipStream_t st) {
   if (maxh <= 4)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 4, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 8)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 8, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 8, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 16)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 16, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 16, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, 24, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 24, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   return hipGetLastError();
 }
 
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_slope_chunk_t(const DevArgs& A, const int2* work, int n, int maxh,
                                        int vpt, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   switch (vpt) {
-    case 1: return launch_slope_chunk_v<D, NDF, UNI, 1>(A, work, n, maxh, st);
-    case 2: return launch_slope_chunk_v<D, NDF, UNI, 2>(A, work, n, maxh, st);
-    case 3: return launch_slope_chunk_v<D, NDF, UNI, 3>(A, work, n, maxh, st);
-    case 4: return launch_slope_chunk_v<D, NDF, UNI, 4>(A, work, n, maxh, st);
+    case 1: return launch_slope_chunk_v<D, NDF, UNI, FT, 1>(A, work, n, maxh, st);
+    case 2: return launch_slope_chunk_v<D, NDF, UNI, FT, 2>(A, work, n, maxh, st);
+    case 3: return launch_slope_chunk_v<D, NDF, UNI, FT, 3>(A, work, n, maxh, st);
+    case 4: return launch_slope_chunk_v<D, NDF, UNI, FT, 4>(A, work, n, maxh, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int D, int NDF, bool UNI, int MODE, int MARCH, int NWV>
+template <int D, int NDF, bool UNI, class FT, int MODE, int MARCH, int NWV>
 inline hipError_t launch_fr_t(const DevArgs& A, const int32_t* cells, int n, size_t lds,
                               hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (lds > 0)
-    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, true, NWV>), dim3(n),
+    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, FT, MODE, MARCH, true, NWV>), dim3(n),
                        dim3(NWV * 64), lds, st, A, cells, n, KAH_RO_ARGS(A), A.sdf_r,
                        A.half_fw_r, A.rho_w_r);
   else
-    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, MODE, MARCH, false, NWV>), dim3(n),
+    hipLaunchKernelGGL((k_flux_relax<D, NDF, UNI, FT, MODE, MARCH, false, NWV>), dim3(n),
                        dim3(NWV * 64), 0, st, A, cells, n, KAH_RO_ARGS(A), A.sdf_r,
                        A.half_fw_r, A.rho_w_r);
   return hipGetLastError();
 }
 
-template <int D, int NDF, bool UNI, int NWV>
+template <int D, int NDF, bool UNI, class FT, int NWV>
 inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int mode,
                               int march, size_t lds, hipStream_t st) {
   if (mode == MODE_FLUX)
-    return launch_fr_t<D, NDF, UNI, MODE_FLUX, MARCH_CAIDVM, NWV>(A, cells, n, 0, st);
+    return launch_fr_t<D, NDF, UNI, FT, MODE_FLUX, MARCH_CAIDVM, NWV>(A, cells, n, 0, st);
   if (mode == MODE_ITER) {
     if (march == MARCH_CAIDVM)
-      return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
-    return launch_fr_t<D, NDF, UNI, MODE_ITER, MARCH_EULER, NWV>(A, cells, n, lds, st);
+      return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+    return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_EULER, NWV>(A, cells, n, lds, st);
   }
   if (march == MARCH_CAIDVM)
-    return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
-  return launch_fr_t<D, NDF, UNI, MODE_FUSED, MARCH_EULER, NWV>(A, cells, n, lds, st);
+    return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+  return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_EULER, NWV>(A, cells, n, lds, st);
 }
 
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int mode,
                               int march, size_t lds, int nwv, hipStream_t st) {
-  if (nwv == 4) return launch_fr_w<D, NDF, UNI, 4>(A, cells, n, mode, march, lds, st);
-  return launch_fr_w<D, NDF, UNI, 8>(A, cells, n, mode, march, lds, st);
+  if (nwv == 4) return launch_fr_w<D, NDF, UNI, FT, 4>(A, cells, n, mode, march, lds, st);
+  return launch_fr_w<D, NDF, UNI, FT, 8>(A, cells, n, mode, march, lds, st);
 }
 
-template <int D, int NDF, bool UNI>
+template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_macro_slope_t(const DevArgs& A, const int32_t* cells, int n, double* sw,
                                        hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_macro_slope<D, NDF, UNI>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
+  hipLaunchKernelGGL((k_macro_slope<D, NDF, UNI, FT>), dim3(n), dim3(256), 0, st, A, cells, n, sw);
   return hipGetLastError();
 }
 

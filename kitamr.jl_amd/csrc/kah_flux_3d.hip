@@ -1,12 +1,12 @@
-// kah_flux_3d.hip -- flux + relaxation kernel (K2) of the 3D1F form.
+// kah_flux_3d.hip -- flux + relaxation kernel (K2) of the 3D1F form, FP64 storage.
 #include "kah_device.h"
 
 namespace kah {
 
 hipError_t launch_flux_relax_3d(const DevArgs& A, bool uni, const int32_t* cells, int n,
                                int mode, int march, size_t lds, int nwv, hipStream_t st) {
-  return uni ? launch_fr_d<3, 1, true>(A, cells, n, mode, march, lds, nwv, st)
-             : launch_fr_d<3, 1, false>(A, cells, n, mode, march, lds, nwv, st);
+  return uni ? launch_fr_d<3, 1, true, double>(A, cells, n, mode, march, lds, nwv, st)
+             : launch_fr_d<3, 1, false, double>(A, cells, n, mode, march, lds, nwv, st);
 }
 
 }  // namespace kah

@@ -1,0 +1,12 @@
+// kah_flux_3d_f32.hip -- flux + relaxation kernel (K2) of the 3D1F form, FP32-f storage.
+#include "kah_device.h"
+
+namespace kah {
+
+hipError_t launch_flux_relax_3d_f32(const DevArgs& A, bool uni, const int32_t* cells, int n,
+                               int mode, int march, size_t lds, int nwv, hipStream_t st) {
+  return uni ? launch_fr_d<3, 1, true, float>(A, cells, n, mode, march, lds, nwv, st)
+             : launch_fr_d<3, 1, false, float>(A, cells, n, mode, march, lds, nwv, st);
+}
+
+}  // namespace kah

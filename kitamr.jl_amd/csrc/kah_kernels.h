@@ -49,6 +49,10 @@ struct DevArgs {
   const double* sdf_r;
   const double* half_fw_r;
   const double* rho_w_r;
+  // wave tiles of a shared tensor-product velocity grid (vt_mode 2: 8x8, 3: 4x4x4 blocks of
+  // velocity cells per 64 consecutive thread positions; 0: identity order)
+  const int32_t* vt_base;   // first velocity cell of each tile
+  int32_t vt_mode, vt_n0, vt_n1, vt_pad;
   // parameters
   double K, gamma, Pr, mu, omega, dt;
   int32_t flux_kind;
@@ -79,16 +83,17 @@ constexpr int SLOPE_CHUNK = 256;   // threads per workgroup of the chunked slope
 constexpr int SLOPE_MAX_VPT = 4;   // velocity cells per thread of the chunked slope kernel: 1..4
 constexpr int MAX_ENTRIES = 24;    // face sides per cell (2*DIM sides, up to 2^(DIM-1) each)
 
-hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                        int maxh, hipStream_t st);
+// f32: f, sdf and the next f stored as float (FP32-f form), else double
+hipError_t launch_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                        int n, int maxh, hipStream_t st);
 // work = (cell, chunk) pairs, chunk = vpt*SLOPE_CHUNK velocity cells
-hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, const int2* work, int n,
-                              int maxh, int vpt, hipStream_t st);
-hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                             int mode, int march, size_t lds, int waves_per_block,
+hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, bool f32, const int2* work,
+                              int n, int maxh, int vpt, hipStream_t st);
+hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                             int n, int mode, int march, size_t lds, int waves_per_block,
                              hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
-hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                              double* sw, hipStream_t st);
+hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                              int n, double* sw, hipStream_t st);
 
 }  // namespace kah

@@ -1,6 +1,7 @@
 // kah_kernels.hip -- the plain copy kernel (bandwidth reference) and the dispatch of the
 // kernel launchers over the physical dimension. The kernels themselves are in kah_device.h,
-// instantiated per dimension in kah_slope_{2d,3d}.hip and kah_flux_{2d,3d}.hip.
+// instantiated per dimension and storage type in kah_slope_{2d,3d}[_f32].hip and
+// kah_flux_{2d,3d}[_f32].hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,15 +11,25 @@
 namespace kah {
 
 hipError_t launch_slope_2d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
-hipError_t launch_slope_3d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
 hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
-hipError_t launch_slope_chunk_3d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
 hipError_t launch_macro_slope_2d(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
-hipError_t launch_macro_slope_3d(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
 hipError_t launch_flux_relax_2d(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
-                                hipStream_t);
+                                  hipStream_t);
+hipError_t launch_slope_3d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
+hipError_t launch_slope_chunk_3d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_macro_slope_3d(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
 hipError_t launch_flux_relax_3d(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
-                                hipStream_t);
+                                  hipStream_t);
+hipError_t launch_slope_2d_f32(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
+hipError_t launch_slope_chunk_2d_f32(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_macro_slope_2d_f32(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
+hipError_t launch_flux_relax_2d_f32(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
+                                  hipStream_t);
+hipError_t launch_slope_3d_f32(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
+hipError_t launch_slope_chunk_3d_f32(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_macro_slope_3d_f32(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
+hipError_t launch_flux_relax_3d_f32(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
+                                  hipStream_t);
 
 // ------------------------------------------------------------------ copy reference
 // A plain streaming copy (16-byte vector loads/stores) used to measure attainable HBM
@@ -38,26 +49,38 @@ hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st
   return hipGetLastError();
 }
 
-hipError_t launch_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                        int maxh, hipStream_t st) {
+hipError_t launch_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                        int n, int maxh, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_slope_2d_f32(A, uni, cells, n, maxh, st)
+                    : launch_slope_3d_f32(A, uni, cells, n, maxh, st);
   return dim == 2 ? launch_slope_2d(A, uni, cells, n, maxh, st)
                   : launch_slope_3d(A, uni, cells, n, maxh, st);
 }
 
-hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, const int2* work, int n,
-                              int maxh, int vpt, hipStream_t st) {
+hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, bool f32, const int2* work,
+                              int n, int maxh, int vpt, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_slope_chunk_2d_f32(A, uni, work, n, maxh, vpt, st)
+                    : launch_slope_chunk_3d_f32(A, uni, work, n, maxh, vpt, st);
   return dim == 2 ? launch_slope_chunk_2d(A, uni, work, n, maxh, vpt, st)
                   : launch_slope_chunk_3d(A, uni, work, n, maxh, vpt, st);
 }
 
-hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                             int mode, int march, size_t lds, int nwv, hipStream_t st) {
+hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                             int n, int mode, int march, size_t lds, int nwv, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_flux_relax_2d_f32(A, uni, cells, n, mode, march, lds, nwv, st)
+                    : launch_flux_relax_3d_f32(A, uni, cells, n, mode, march, lds, nwv, st);
   return dim == 2 ? launch_flux_relax_2d(A, uni, cells, n, mode, march, lds, nwv, st)
                   : launch_flux_relax_3d(A, uni, cells, n, mode, march, lds, nwv, st);
 }
 
-hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, const int32_t* cells, int n,
-                              double* sw, hipStream_t st) {
+hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                              int n, double* sw, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_macro_slope_2d_f32(A, uni, cells, n, sw, st)
+                    : launch_macro_slope_3d_f32(A, uni, cells, n, sw, st);
   return dim == 2 ? launch_macro_slope_2d(A, uni, cells, n, sw, st)
                   : launch_macro_slope_3d(A, uni, cells, n, sw, st);
 }

@@ -1,0 +1,24 @@
+// kah_slope_2d_f32.hip -- slope kernels (K1, K3) of the 2D2F form, FP32-f storage.
+#include "kah_device.h"
+
+namespace kah {
+
+hipError_t launch_slope_2d_f32(const DevArgs& A, bool uni, const int32_t* cells, int n,
+                          int maxh, hipStream_t st) {
+  return uni ? launch_slope_t<2, 2, true, float>(A, cells, n, maxh, st)
+             : launch_slope_t<2, 2, false, float>(A, cells, n, maxh, st);
+}
+
+hipError_t launch_slope_chunk_2d_f32(const DevArgs& A, bool uni, const int2* work, int n,
+                                int maxh, int vpt, hipStream_t st) {
+  return uni ? launch_slope_chunk_t<2, 2, true, float>(A, work, n, maxh, vpt, st)
+             : launch_slope_chunk_t<2, 2, false, float>(A, work, n, maxh, vpt, st);
+}
+
+hipError_t launch_macro_slope_2d_f32(const DevArgs& A, bool uni, const int32_t* cells, int n,
+                                double* sw, hipStream_t st) {
+  return uni ? launch_macro_slope_t<2, 2, true, float>(A, cells, n, sw, st)
+             : launch_macro_slope_t<2, 2, false, float>(A, cells, n, sw, st);
+}
+
+}  // namespace kah

@@ -1,24 +1,24 @@
-// kah_slope_3d.hip -- slope kernels (K1, K3) of the 3D1F form.
+// kah_slope_3d.hip -- slope kernels (K1, K3) of the 3D1F form, FP64 storage.
 #include "kah_device.h"
 
 namespace kah {
 
-hipError_t launch_slope_3d(const DevArgs& A, bool uni, const int32_t* cells, int n, int maxh,
-                          hipStream_t st) {
-  return uni ? launch_slope_t<3, 1, true>(A, cells, n, maxh, st)
-             : launch_slope_t<3, 1, false>(A, cells, n, maxh, st);
+hipError_t launch_slope_3d(const DevArgs& A, bool uni, const int32_t* cells, int n,
+                          int maxh, hipStream_t st) {
+  return uni ? launch_slope_t<3, 1, true, double>(A, cells, n, maxh, st)
+             : launch_slope_t<3, 1, false, double>(A, cells, n, maxh, st);
 }
 
-hipError_t launch_slope_chunk_3d(const DevArgs& A, bool uni, const int2* work, int n, int maxh,
-                                int vpt, hipStream_t st) {
-  return uni ? launch_slope_chunk_t<3, 1, true>(A, work, n, maxh, vpt, st)
-             : launch_slope_chunk_t<3, 1, false>(A, work, n, maxh, vpt, st);
+hipError_t launch_slope_chunk_3d(const DevArgs& A, bool uni, const int2* work, int n,
+                                int maxh, int vpt, hipStream_t st) {
+  return uni ? launch_slope_chunk_t<3, 1, true, double>(A, work, n, maxh, vpt, st)
+             : launch_slope_chunk_t<3, 1, false, double>(A, work, n, maxh, vpt, st);
 }
 
 hipError_t launch_macro_slope_3d(const DevArgs& A, bool uni, const int32_t* cells, int n,
                                 double* sw, hipStream_t st) {
-  return uni ? launch_macro_slope_t<3, 1, true>(A, cells, n, sw, st)
-             : launch_macro_slope_t<3, 1, false>(A, cells, n, sw, st);
+  return uni ? launch_macro_slope_t<3, 1, true, double>(A, cells, n, sw, st)
+             : launch_macro_slope_t<3, 1, false, double>(A, cells, n, sw, st);
 }
 
 }  // namespace kah

@@ -9,8 +9,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "lib" / "libkitamr_hip.so"
-SOURCES = ["kah_api.cpp", "kah_kernels.hip", "kah_slope_2d.hip", "kah_slope_3d.hip",
-           "kah_flux_2d.hip", "kah_flux_3d.hip"]
+SOURCES = ["kah_api.cpp", "kah_kernels.hip"] + [
+    f"kah_{k}_{d}d{p}.hip" for k in ("slope", "flux") for d in (2, 3) for p in ("", "_f32")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -6,6 +6,8 @@ riemann_2d    2D four-quadrant Riemann problem on [0,1]^2 (UniformOutflow on all
 wall_channel  the reference's own smoke case (test/runtests.jl:1-42): Maxwellian walls in x,
               periodic in y.
 box_3d        3D1F analogue on a small octree brick.
+periodic_box_2d  the Riemann initial state (or a uniform state) with periodic boundaries on
+              all sides, where mass, momentum and energy are conserved exactly.
 """
 from __future__ import annotations
 
@@ -67,6 +69,25 @@ def riemann_2d(n=64, nv=28, vmax=5.0, ps_levels=0, halo=(6.0, 4.0, 2.5), Kn=1e-2
         def vs_fn(c, prim, base, _seed=seed):
             return refine_toward(base.copy(), prim, rounds=vs_adapt_rounds or vs_levels,
                                  rng=_seed + c)
+    return build_case(cfg, ps_refine=refine, vs_grid_fn=vs_fn)
+
+
+def periodic_box_2d(n=8, nv=8, uniform=False, ps_levels=0, vs_levels=0, precision="fp64",
+                    seed=20261202):
+    solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=vs_levels, CFL=0.8)
+    gas = Gas(K=1.0, Kn=0.05, omega=0.5)
+    dom = [Domain("Period", i + 1) for i in range(4)]
+    ic = Uniform([1.0, 0.2, -0.1, 1.0]) if uniform else PCoordFn(riemann_prim)
+    cfg = Configure(solver, gas, [0.0, 1.0, 0.0, 1.0], [n, n], [-5.0, 5.0, -5.0, 5.0],
+                    [nv, nv], ic, dom, precision=precision)
+    refine = None
+    if ps_levels:
+        def refine(mid, ds, level):
+            return (np.abs(mid[:, 0] - 0.5) < 0.2) & (np.abs(mid[:, 1] - 0.5) < 0.3)
+    vs_fn = None
+    if vs_levels > 0:
+        def vs_fn(c, prim, base, _seed=seed):
+            return refine_toward(base.copy(), prim, rounds=vs_levels, rng=_seed + c)
     return build_case(cfg, ps_refine=refine, vs_grid_fn=vs_fn)
 
 

@@ -1,0 +1,169 @@
+"""Serving loop: flow cases arrive over time, are admitted by HBM budget and slot count,
+stay resident on the device and are stepped concurrently (continuous batching).
+
+Each admitted case owns a device context with its own HIP stream (kah_create); a round of
+the loop enqueues one step of every active case without waiting, so the kernels of
+different cases overlap on the GPU. The host only blocks on one case's stream at that
+case's residual-check steps (every ``ST_CHECK_INTERVAL`` steps, as the reference's solve
+loop checks, src/Solver/Solver.jl:71-80). A case leaves when its residual is below the
+solver's TOLERANCE for more than REDUNDANT_STEPS_NUM steps (check_for_convergence,
+src/Solver/Finalize.jl:79-82) or at its step limit; its slot and memory are then free for
+the next queued case (FIFO).
+
+No counterpart in the reference (it runs one case per MPI job); this is the request loop
+of the serving pattern, built on the same per-case API (``solver.KA``, ``solver.step``).
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+
+@dataclass
+class Request:
+    rid: int
+    case: object                  # a prepared case (kitamr_hip.cases / solver.build_case)
+    arrival: float                # seconds after the start of the run
+    max_steps: int = 2000
+    tag: str = ""
+    nbytes: int = 0               # device bytes charged against the budget
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    steps: int = 0
+    converged: bool = False
+    result: Optional[dict] = None
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+def device_bytes_estimate(case):
+    """Device bytes of one case before it is created: the f ping-pong buffers and the slopes
+    (the bulk), velocity grids, per-cell state and descriptors (upper bounds)."""
+    s = case.config.solver
+    D, NDF = s.DIM, s.NDF
+    fs = 4 if getattr(case.config, "precision", "fp64") == "fp32f" else 8
+    P = int(np.asarray(case.topo["vs_offset"])[-1])
+    nc = int(case.topo["n_cells"])
+    npairs = int(case.topo["n_pairs"])
+    state = P * NDF * fs * (2 + D)
+    grids = (len(case.topo["vs_weight"]) * (D * 8 + 8 + 1 + 4))
+    per_cell = nc * (8 * (4 * (D + 2) + 2 * (D + 2) + D + 2 * D * 3) + 2 * D * 96 + 8 * 32 * 160)
+    halves = npairs * 2 * (D + 2) * 8 * max(1, P // max(nc, 1) // 256 + 1)
+    scratch = P * (NDF + 1) * 8 if P // max(nc, 1) * (NDF + 1) * 8 > 64 * 1024 else 0
+    return int(state + grids + per_cell + halves + scratch + (1 << 20))
+
+
+def _default_factory(case):
+    from .solver import KA
+    return KA(case)
+
+
+class Server:
+    """Continuous-batching request loop over device-resident cases.
+
+    budget_bytes: device memory the resident cases may hold together.
+    max_slots:    cases stepped concurrently (one stream each).
+    factory:      case -> KA-like handle with ``.ctx`` (``step``, ``sync``, ``get_state``,
+                  ``close``) and ``.status``; the default creates a device context.
+    """
+
+    def __init__(self, budget_bytes, max_slots=8, factory: Callable = None,
+                 clock: Callable[[], float] = time.perf_counter, log=None):
+        self.budget = int(budget_bytes)
+        self.max_slots = int(max_slots)
+        self.factory = factory or _default_factory
+        self.clock = clock
+        self.log = log
+        self.used = 0
+        self.peak_used = 0
+        self.peak_active = 0
+        self.admit_order = []
+
+    def _say(self, msg):
+        if self.log:
+            self.log(msg)
+
+    def run(self, requests, keep_df=True):
+        from . import solver as S
+        for r in requests:
+            if not r.nbytes:
+                r.nbytes = device_bytes_estimate(r.case)
+            if r.nbytes > self.budget:
+                raise ValueError(f"request {r.rid} needs {r.nbytes} B, over the budget "
+                                 f"{self.budget} B")
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        queue = collections.deque()
+        active = []
+        t0 = self.clock()
+        updates = 0
+        while pending or queue or active:
+            now = self.clock() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            # admission: FIFO, while a slot and the memory for the head request are free
+            while (queue and len(active) < self.max_slots
+                   and self.used + queue[0].nbytes <= self.budget):
+                r = queue.popleft()
+                r.ka = self.factory(r.case)
+                r.admitted = self.clock() - t0
+                self.used += r.nbytes
+                self.peak_used = max(self.peak_used, self.used)
+                active.append(r)
+                self.admit_order.append(r.rid)
+                self._say(f"[serve] t={r.admitted:.3f}s admit {r.rid} {r.tag} "
+                          f"({r.nbytes / 1e6:.1f} MB, {len(active)} active)")
+            self.peak_active = max(self.peak_active, len(active))
+            if not active:
+                if pending:
+                    wait = pending[0].arrival - (self.clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                continue
+            # one step of every active case, enqueued on its own stream
+            for r in active:
+                S.limit_dt(r.ka)
+                S.step(r.ka)
+                r.steps += 1
+            done = []
+            for r in active:
+                conv = S.check_for_convergence(r.ka)
+                if conv or r.steps >= r.max_steps or S.reached_max_time(r.ka):
+                    r.converged = bool(conv)
+                    done.append(r)
+            for r in done:
+                r.ka.ctx.sync()
+                st = r.ka.ctx.get_state(df=keep_df)
+                r.result = st
+                r.finished = self.clock() - t0
+                updates += int(r.case.phase_cells) * r.steps
+                r.ka.close()
+                r.ka = None
+                self.used -= r.nbytes
+                active.remove(r)
+                self._say(f"[serve] t={r.finished:.3f}s done {r.rid} {r.tag} steps={r.steps} "
+                          f"latency={r.latency:.3f}s")
+        wall = self.clock() - t0
+        lat = np.array([r.latency for r in requests])
+        return {
+            "requests": len(requests),
+            "wall_s": wall,
+            "updates": updates,
+            "updates_per_s": updates / wall if wall > 0 else 0.0,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "peak_active": self.peak_active,
+            "peak_bytes": self.peak_used,
+            "budget_bytes": self.budget,
+        }
+
+
+def poisson_arrivals(n, rate, seed):
+    """Arrival times (s) of n requests of a Poisson process of the given rate (1/s)."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

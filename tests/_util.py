@@ -24,11 +24,13 @@ def rel_err_rows(a, b, width):
     return float(np.max(np.max(np.abs(a - b), axis=1) / scale))
 
 
-def pair(case, **kw):
+def pair(case, options=None, **kw):
     params = kah_params(case.config)
     for k, v in kw.items():
         setattr(params, k, v)
     dev = abi.HipContext(params)
+    for k, v in (options or {}).items():
+        dev.set_option(k, v)
     dev.set_topology(case.topo)
     dev.set_state(case.w, case.prim, case.df)
     orc = O.Oracle(params, case.topo)

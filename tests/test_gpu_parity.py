@@ -23,6 +23,17 @@ CASES = {
                                                      vs_levels=1),
     "box3d": lambda: cases.box_3d(n=4, nv=6),
     "box3d_amr_vsamr": lambda: cases.box_3d(n=4, nv=6, ps_levels=1, vs_levels=1),
+    # grids whose size admits wave tiles (8x8 in 2D, 4x4x4 in 3D)
+    "riemann8_amr1_nv16": lambda: cases.riemann_2d(n=8, nv=16, ps_levels=1, halo=(2.0, 1.5)),
+    "box3d_nv8": lambda: cases.box_3d(n=4, nv=8),
+}
+
+# launch options that change the schedule but not the arithmetic of each update
+OPTION_SETS = {
+    "no_tiles": {"vel_tiles": 0},
+    "vpt1": {"slope_vpt": 1},
+    "vpt4": {"slope_vpt": 4},
+    "waves4": {"flux_waves": 4},
 }
 
 
@@ -71,4 +82,22 @@ def test_fused_steps(name):
     assert rel_err(g["df"], o["df"]) <= TOL, "df"
     assert rel_err_rows(g["w"], o["w"], dim + 2) <= TOL, "w"
     assert rel_err_rows(g["prim"], o["prim"], dim + 2) <= TOL, "prim"
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opt", list(OPTION_SETS))
+@pytest.mark.parametrize("name", ["riemann8_amr1_nv16", "box3d_nv8", "riemann16_amr2"])
+def test_launch_options(name, opt):
+    case = CASES[name]()
+    say(f"[options] {name} {opt}: cells={case.n_cells} P={case.phase_cells}")
+    dev, orc = pair(case, options=OPTION_SETS[opt])
+    dt = case.dt_xi
+    dim = case.config.solver.DIM
+    for s in range(3):
+        dev.step(dt, s)
+        orc.step(dt, s)
+    g, o = dev.get_state(), orc.get_state()
+    assert rel_err(g["df"], o["df"]) <= TOL, "df"
+    assert rel_err_rows(g["w"], o["w"], dim + 2) <= TOL, "w"
     dev.close()
