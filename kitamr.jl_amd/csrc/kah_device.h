@@ -130,6 +130,14 @@ struct Grid {
   const DevArgs& A;
   int64_t off;   // first velocity cell of the cell
   int n;
+  // per-cell base pointers (uniform over the workgroup): every access is base + a 32-bit
+  // offset, which the loads take as a scalar base and one vector register
+  const FT* fb;
+  const FT* sb;
+  FT* swb;
+  FT* ob;
+  const double* gub;
+  const double* gwb;
   __device__ Grid(const DevArgs& a, int c) : A(a) {
     if constexpr (UNI) {
       n = a.uni_n;
@@ -138,6 +146,12 @@ struct Grid {
       off = a.voff[c];
       n = int(a.voff[c + 1] - off);
     }
+    fb = reinterpret_cast<const FT*>(a.df) + off * NDF;
+    sb = reinterpret_cast<const FT*>(a.sdf_r) + off * NDF * D;
+    swb = reinterpret_cast<FT*>(a.sdf) + off * NDF * D;
+    ob = reinterpret_cast<FT*>(a.df_out) + off * NDF;
+    gub = a.gu + goff() * D;
+    gwb = a.gw + goff();
   }
   __device__ int64_t goff() const { return UNI ? 0 : off; }
   // velocity cell handled at thread position p: the identity, or the wave-tile order of a
@@ -155,24 +169,28 @@ struct Grid {
     }
     return p;
   }
-  __device__ double u(int i, int d) const { return A.gu[goff() * D + int64_t(d) * n + i]; }
-  __device__ double wt(int i) const { return A.gw[goff() + i]; }
+  // element x of a per-cell array as base + 32-bit byte offset
+  template <class T>
+  static __device__ T& at(T* b, int x) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(b) + uint32_t(x) * uint32_t(sizeof(T)));
+  }
+  template <class T>
+  static __device__ const T& at(const T* b, int x) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(b) +
+                                       uint32_t(x) * uint32_t(sizeof(T)));
+  }
+  __device__ double u(int i, int d) const { return at(gub, d * n + i); }
+  __device__ double wt(int i) const { return at(gwb, i); }
   __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[off + i]); }
   __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
   // f, sdf and the next f are stored as FT (double, or float in the FP32-f form); all
   // arithmetic is FP64
-  __device__ double f(int i, int k) const {
-    return double(reinterpret_cast<const FT*>(A.df)[off * NDF + int64_t(k) * n + i]);
-  }
-  __device__ double s(int i, int k, int d) const {
-    return double(reinterpret_cast<const FT*>(A.sdf_r)[off * NDF * D + int64_t(d * NDF + k) * n + i]);
-  }
+  __device__ double f(int i, int k) const { return double(at(fb, k * n + i)); }
+  __device__ double s(int i, int k, int d) const { return double(at(sb, (d * NDF + k) * n + i)); }
   __device__ void set_s(int i, int k, int d, double v) const {
-    reinterpret_cast<FT*>(A.sdf)[off * NDF * D + int64_t(d * NDF + k) * n + i] = FT(v);
+    at(swb, (d * NDF + k) * n + i) = FT(v);
   }
-  __device__ void set_out(int i, int k, double v) const {
-    reinterpret_cast<FT*>(A.df_out)[off * NDF + int64_t(k) * n + i] = FT(v);
-  }
+  __device__ void set_out(int i, int k, double v) const { at(ob, k * n + i) = FT(v); }
   // a value as the storage holds it
   static __device__ double stored(double v) { return double(FT(v)); }
 };
@@ -343,7 +361,7 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
     side_sum<D, NDF, UNI, FT>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li, ki, f,
                           aR);
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) s[k] = minmod(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
+    for (int k = 0; k < NDF; ++k) s[k] = minmod_sel(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
   } else {
 #pragma unroll
     for (int k = 0; k < NDF; ++k) s[k] = aL[k] * sd.inv_cnt[0];

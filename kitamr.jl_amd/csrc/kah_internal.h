@@ -104,10 +104,21 @@ __host__ __device__ inline double jl_min(double a, double b) {
   if (b != b) return b;
   return b < a ? b : a;
 }
-// minmod, src/Flux/Slope.jl:20-24
+// minmod, src/Flux/Slope.jl:20-24: 0.5*(sign(sL)+sign(sR))*min(|sL|,|sR|)
 __host__ __device__ inline double minmod(double sL, double sR) {
   double SL = fabs(sL), SR = fabs(sR);
   return 0.5 * (jl_sign(sL) + jl_sign(sR)) * jl_min(SL, SR);
+}
+// The same value in select form for the kernels: for same-sign nonzero slopes the factor
+// 0.5*(±1±1) is exactly ±1; otherwise it is 0 or ±1/2 with min = 0, and the result is
+// 0*min (NaN for a NaN input or for opposite infinities, as in the formula). Only the sign
+// of a zero result may differ, which no later use of a slope can observe (slopes only
+// enter products and sums with nonzero terms).
+__host__ __device__ inline double minmod_sel(double sL, double sR) {
+  const double SL = fabs(sL), SR = fabs(sR);
+  const double m = (SL != SL || SR != SR) ? SL + SR : (SR < SL ? SR : SL);
+  const bool pos = sL > 0.0 && sR > 0.0, neg = sL < 0.0 && sR < 0.0;
+  return pos ? m : (neg ? -m : 0.0 * m);
 }
 
 // get_prim_2D / get_prim_3D, lib/KitCore/2D.jl:9-16, 3D.jl:12-20
