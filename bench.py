@@ -26,13 +26,16 @@ sys.path.insert(0, str(ROOT / "oracle"))
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes(s_f):
-    """Algorithmic bytes per phase-space update of the reference's 3-phase dataflow, 2D2F on a
-    shared velocity grid, with s_f bytes per stored f value (8: FP64, 4: FP32-f): slope
-    R df + W sdf; flux R df, R sdf, W flux; iterate R df, R flux, W df."""
-    b_slope = 2 * s_f * (1 + 2)
-    b_flux_iter = 2 * s_f * (1 + 2 + 1) + 2 * s_f * 3
-    return b_slope, b_flux_iter, b_slope + b_flux_iter   # FP64: 48, 112, 160
+def alg_bytes(s_f, adaptive=False):
+    """Algorithmic bytes per phase-space update of the reference's 3-phase dataflow, 2D2F,
+    with s_f bytes per stored f value (8: FP64, 4: FP32-f): slope R df (+ R level) + W sdf;
+    flux R df, R sdf (+ R u, R weight, R level), W flux; iterate R df, R flux (+ R u,
+    R weight), W df. The geometry terms (8-byte u and weight, 1-byte level) count only for
+    per-cell adapted velocity grids; a shared grid is read once."""
+    g_slope, g_flux, g_iter = (1, 2 * 8 + 8 + 1, 2 * 8 + 8) if adaptive else (0, 0, 0)
+    b_slope = 2 * s_f * (1 + 2) + g_slope
+    b_flux_iter = 2 * s_f * (1 + 2 + 1) + g_flux + 2 * s_f * 3 + g_iter
+    return b_slope, b_flux_iter, b_slope + b_flux_iter   # FP64: 48, 112, 160 (adaptive 210)
 
 
 def log(msg):
@@ -99,6 +102,10 @@ def main():
                     help="waves per workgroup of the flux/relaxation kernel (0: library default)")
     ap.add_argument("--slope-vpt", type=int, default=0,
                     help="velocity cells per thread of the slope kernel (0: library default)")
+    ap.add_argument("--workload", default="riemann2d", choices=["riemann2d", "riemann2d-vsamr"],
+                    help="riemann2d: shared 48x48 velocity grid (the headline workload); "
+                         "riemann2d-vsamr: per-cell adapted grids (20x20 base on [-15,15]^2, "
+                         "2 levels)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32f"],
                     help="fp64 (parity form) or fp32f (f stored in FP32, FP64 moments)")
     ap.add_argument("--vel-tiles", type=int, default=-1,
@@ -118,9 +125,13 @@ def main():
     if args.small:
         case = cases.riemann_2d(n=32, nv=48, ps_levels=2, halo=(4.0, 2.0),
                                 precision=args.precision)
+    elif args.workload == "riemann2d-vsamr":
+        case = cases.riemann_2d(n=64, nv=20, vmax=15.0, ps_levels=3, vs_levels=2, vs_variants=6,
+                                precision=args.precision)
     else:
         case = cases.riemann_2d(n=64, nv=48, ps_levels=3, precision=args.precision)
-    B_SLOPE, B_FLUX_ITER, B_STEP = alg_bytes(8 if args.precision == "fp64" else 4)
+    adaptive = args.workload == "riemann2d-vsamr"
+    B_SLOPE, B_FLUX_ITER, B_STEP = alg_bytes(8 if args.precision == "fp64" else 4, adaptive)
     P = case.phase_cells
     log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
         f"pairs={case.topo['n_pairs']} P={P}")
@@ -182,9 +193,12 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32 storage / f64 arithmetic",
         "data": "synthetic",
         "config": {
-            "workload": ("2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
-                         "uniform 48x48 velocity grid, CAIDVM + CAIDVM_Marching"
-                         if not args.small else "reduced 32x32 + 2 levels (debug)"),
+            "workload": ("reduced 32x32 + 2 levels (debug)" if args.small else
+                         "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
+                         "per-cell adapted velocity grids (20x20 base on [-15,15]^2, 2 levels, "
+                         "1567-1651 cells each), CAIDVM + CAIDVM_Marching" if adaptive else
+                         "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
+                         "uniform 48x48 velocity grid, CAIDVM + CAIDVM_Marching"),
             "precision": args.precision,
             "leaves": case.n_cells,
             "phase_cells_per_step": P,
@@ -198,7 +212,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic() if not args.small and args.precision == "fp64" else None,
+            "traffic": (profiled_traffic() if not args.small and not adaptive
+                        and args.precision == "fp64" else None),
             "alg_bytes_per_update": B_FLUX_ITER,
             "avg_launch_ms": kr_ms,
         },

@@ -187,9 +187,13 @@ int64_t kah_device_bytes(const kah_ctx* ctx);
 /* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
  * the flux/relaxation kernel (4 or 8). KAH_OPT_SLOPE_VPT: velocity cells per thread
  * of the slope kernel (1..4). KAH_OPT_VEL_TILES: 1 (default) processes a shared
- * tensor-product velocity grid in 8x8 / 4x4x4 tiles per wave, 0 in storage order. The last
- * two take effect at the next kah_set_topology. No counterpart in the reference. */
-enum { KAH_OPT_FLUX_WAVES = 1, KAH_OPT_SLOPE_VPT = 2, KAH_OPT_VEL_TILES = 3 };
+ * tensor-product velocity grid in 8x8 / 4x4x4 tiles per wave, 0 in storage order.
+ * KAH_OPT_OVERLAP_MAPS: 1 (default) precomputes, per fluid cell and neighbour with another
+ * adapted velocity grid, the overlapping velocity cells (4 bytes per velocity cell and
+ * neighbour), 0 searches the neighbour's keys in every step. The last three take effect at
+ * the next kah_set_topology. No counterpart in the reference. */
+enum { KAH_OPT_FLUX_WAVES = 1, KAH_OPT_SLOPE_VPT = 2, KAH_OPT_VEL_TILES = 3,
+       KAH_OPT_OVERLAP_MAPS = 4 };
 int kah_set_option(kah_ctx* ctx, int option, int64_t value);
 /* Times reps device-to-device copies of `bytes` bytes with a plain streaming copy kernel
  * (attainable HBM bandwidth = 2*bytes / ms_per_copy); the reference point for the

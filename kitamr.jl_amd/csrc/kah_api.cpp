@@ -74,10 +74,10 @@ struct kah_ctx {
   DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey;
   DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
   DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
-  DBuf d_vtb;
+  DBuf d_vtb, d_ovmap, d_moff;
+  bool ov_enable = true;   // overlap maps for adapted grids
   int64_t part_rows = 0;
   int cur = 0;
-  int64_t dev_bytes = 0;
   bool timing = false;
   int fr_waves = 8;   // waves per workgroup of the flux/relaxation kernel
   int slope_vpt = 3;  // velocity cells per thread of the chunked slope kernel
@@ -94,6 +94,18 @@ struct kah_ctx {
       (void)hipEventDestroy(e.b);
     }
     if (stream) (void)hipStreamDestroy(stream);
+  }
+  // device memory held by the context
+  int64_t device_bytes() const {
+    const DBuf* bufs[] = {&d_fluid, &d_voff, &d_gu, &d_gw, &d_glev, &d_gkey, &d_df[0], &d_df[1],
+                          &d_sdf, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
+                          &d_res, &d_scratch, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
+                          &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
+                          &d_moff};
+    int64_t t = 0;
+    for (const DBuf* b : bufs) t += int64_t(b->bytes);
+    for (const LevelBin* b : bins) t += int64_t(b->items.bytes);
+    return t;
   }
   int fail(int code, const std::string& m) {
     err = m;
@@ -115,7 +127,6 @@ struct kah_ctx {
       return e == hipErrorOutOfMemory ? KAH_ENOMEM : KAH_EDEVICE;
     }
     b.bytes = bytes;
-    dev_bytes += int64_t(bytes);
     return 0;
   }
   template <class T>
@@ -213,6 +224,7 @@ struct Builder {
       SlopeNb e{};
       e.cell = int32_t(nb(c, f, j));
       e.proj = proj ? 1 : 0;
+      e.map = -1;
       for (int t = 0; t < 3; ++t) e.dm[t] = 0.0;
       if (proj)
         for (int t = 0; t < D; ++t) e.dm[t] = (t == dir) ? 0.0 : (mid(c)[t] - mid(e.cell)[t]);
@@ -581,7 +593,6 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       }
     }
   }
-  C->dev_bytes = 0;
   KAH_TRY(C->upload(C->d_voff, T->vs_offset, size_t(nc + 1), "vs_offset"));
   KAH_TRY(C->upload(C->d_gu, T->vs_mid, size_t(ng * D), "vs_mid"));
   KAH_TRY(C->upload(C->d_gw, T->vs_weight, size_t(ng), "vs_weight"));
@@ -592,6 +603,62 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   // descriptors
   Builder B(*C, *T);
   KAH_TRY(B.build());
+
+  // Overlap maps of adapted grids: one map slot per distinct neighbour cell that a fluid
+  // cell's slope stencil or face gather reads across grids (see OvJob).
+  std::vector<int64_t> moff(size_t(nc + 1), 0);
+  std::vector<OvJob> jobs;
+  const bool maps = !C->uni && C->ov_enable && D * C->lmax <= 7;
+  for (auto& e : B.slnb) e.map = -1;
+  for (int64_t c = 0; c < nc; ++c) {
+    for (auto& ed : B.ent[c]) ed.map = -1;
+    moff[c + 1] = moff[c];
+    if (!maps || T->bound_enc[c] < 0) continue;
+    std::vector<int32_t> lst;
+    auto slot = [&](int32_t nbc) {
+      for (size_t k = 0; k < lst.size(); ++k)
+        if (lst[k] == nbc) return int32_t(k);
+      lst.push_back(nbc);
+      return int32_t(lst.size() - 1);
+    };
+    for (int d = 0; d < D; ++d) {
+      const SlopeDesc& sd = B.sl[size_t(c * D + d)];
+      const int sides = sd.mode == SL_INNER ? 2 : (sd.mode == SL_BOUND ? 1 : 0);
+      for (int sdx = 0; sdx < sides; ++sdx)
+        for (int e = 0; e < sd.nb_cnt[sdx]; ++e) {
+          SlopeNb& nbr = B.slnb[size_t(sd.nb_beg[sdx] + e)];
+          nbr.map = slot(nbr.cell);
+        }
+    }
+    for (auto& ed : B.ent[c])
+      if (ed.oth_cell >= 0 &&
+          (ed.oth_mode == MI_LIMITED || ed.oth_mode == MI_UNLIMITED || ed.oth_mode == MI_PLAIN))
+        ed.map = slot(ed.oth_cell);
+    const int64_t nvc = T->vs_offset[c + 1] - T->vs_offset[c];
+    for (size_t k = 0; k < lst.size(); ++k)
+      jobs.push_back(OvJob{int32_t(c), lst[k], moff[c] + int64_t(k) * nvc});
+    moff[c + 1] = moff[c] + int64_t(lst.size()) * nvc;
+  }
+  C->d_ovmap.release();
+  C->d_moff.release();
+  if (maps && !jobs.empty()) {
+    KAH_TRY(C->upload(C->d_moff, moff.data(), moff.size(), "overlap map offsets"));
+    KAH_TRY(C->alloc(C->d_ovmap, sizeof(uint32_t) * size_t(moff[nc]), "overlap maps"));
+    DBuf d_jobs;
+    KAH_TRY(C->upload(d_jobs, jobs.data(), jobs.size(), "overlap jobs"));
+    DevArgs A{};
+    A.lmax = C->lmax;
+    A.voff = C->d_voff.as<int64_t>();
+    A.glev = C->d_glev.as<int8_t>();
+    A.gkey = C->d_gkey.as<uint32_t>();
+    for (size_t j0 = 0; j0 < jobs.size(); j0 += (1u << 30)) {
+      const int nj = int(std::min<size_t>(jobs.size() - j0, size_t(1) << 30));
+      KAH_TRY(C->hip(launch_overlap_map(A, D, d_jobs.as<OvJob>() + j0, nj,
+                                        C->d_ovmap.as<uint32_t>(), C->stream),
+                     "overlap map launch"));
+    }
+    KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "overlap maps"));
+  }
   std::vector<int32_t> hoff(size_t(nc + 1), 0), eoff(size_t(nc + 1), 0);
   std::vector<HalfDesc> halves;
   std::vector<EntryDesc> ents;
@@ -729,6 +796,8 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.uni_n = C->n0;
   A.lmax = C->lmax;
   A.vt_base = C->d_vtb.as<int32_t>();
+  A.ovmap = C->d_ovmap.as<uint32_t>();
+  A.map_off = C->d_moff.as<int64_t>();
   A.vt_mode = C->vt_mode;
   A.vt_n0 = C->vt_n0;
   A.vt_n1 = C->vt_n1;
@@ -983,13 +1052,12 @@ int kah_copy_bandwidth(kah_ctx* C, int64_t bytes, int reps, double* ms_per_copy)
   KAH_TRY(C->hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  C->dev_bytes -= int64_t(a.bytes + b.bytes);
   if (ms_per_copy) *ms_per_copy = double(ms) / reps;
   return KAH_OK;
 }
 
 int64_t kah_phase_cells(const kah_ctx* C) { return C ? C->phase_cells : 0; }
-int64_t kah_device_bytes(const kah_ctx* C) { return C ? C->dev_bytes : 0; }
+int64_t kah_device_bytes(const kah_ctx* C) { return C ? C->device_bytes() : 0; }
 
 int kah_set_option(kah_ctx* C, int option, int64_t value) {
   if (!C) return KAH_EINVAL;
@@ -998,6 +1066,11 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
       if (value != 4 && value != 8)
         return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4 or 8");
       C->fr_waves = int(value);
+      return KAH_OK;
+    case KAH_OPT_OVERLAP_MAPS:
+      if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "overlap maps must be 0 or 1");
+      C->ov_enable = value != 0;
+      C->have_topo = false;   // the maps are built by kah_set_topology
       return KAH_OK;
     case KAH_OPT_VEL_TILES:
       if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "velocity tiles must be 0 or 1");

@@ -221,6 +221,67 @@ __device__ inline void for_overlap(const Grid<D, NDF, UNI, FT>& G, int i, int li
   }
 }
 
+// for_overlap through the overlap map of (owner cell c with n_own velocity cells, map slot):
+// one coalesced load instead of a binary search over the neighbour's keys; the same cells
+// and factors in the same order. Without a map (slot < 0) the search is done in place.
+template <int D, int NDF, bool UNI, class FT, class Fn>
+__device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, FT>& N, int c,
+                                     int n_own, int slot, int i, int li, uint32_t ki,
+                                     Fn&& fn) {
+  if constexpr (UNI) {
+    fn(i, 1.0);
+  } else {
+    if (slot >= 0) {
+      const uint32_t code = A.ovmap[A.map_off[c] + int64_t(slot) * n_own + i];
+      const int j = int(code >> 8), cnt = int(code & 255u);
+      if (cnt == 0) {
+        fn(j, 1.0);
+        return;
+      }
+      for (int t = 0; t < cnt; ++t) fn(j + t, ldexp(1.0, D * (N.lev(j + t) - li)));
+      return;
+    }
+    for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, fn);
+  }
+}
+
+// Overlap map of one (owner cell, neighbour) job, one velocity cell of the owner per thread
+// (the search of for_overlap, done once per topology).
+template <int D, int NDF>
+__global__ __launch_bounds__(256) void k_overlap_map(DevArgs A, const OvJob* __restrict__ jobs,
+                                                     int njobs, uint32_t* __restrict__ map) {
+  const OvJob jb = jobs[blockIdx.x];
+  const int64_t go = A.voff[jb.cell], no = A.voff[jb.nb];
+  const int gn = int(A.voff[jb.cell + 1] - go), nn = int(A.voff[jb.nb + 1] - no);
+  const uint32_t* gk = A.gkey + go;
+  const uint32_t* nk = A.gkey + no;
+  const int8_t* gl = A.glev + go;
+  const int8_t* nl = A.glev + no;
+  for (int i = threadIdx.x; i < gn; i += 256) {
+    const uint32_t ki = gk[i];
+    const int li = gl[i];
+    int lo = 0, hi = nn - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (nk[mid] <= ki) lo = mid; else hi = mid - 1;
+    }
+    uint32_t cnt = 0;
+    if (nl[lo] > li) {
+      const uint32_t end = ki + (1u << (D * (A.lmax - li)));
+      for (int t = lo; t < nn && nk[t] < end; ++t) ++cnt;
+    }
+    map[jb.base + i] = (uint32_t(lo) << 8) | cnt;
+  }
+}
+
+template <int D, int NDF>
+inline hipError_t launch_overlap_map_t(const DevArgs& A, const OvJob* jobs, int njobs,
+                                       uint32_t* map, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_overlap_map<D, NDF>), dim3(njobs), dim3(256), 0, st, A, jobs, njobs, map);
+  return hipGetLastError();
+}
+
 // Micro flux of one velocity cell: value times the normal velocity.
 // LIMITED is positivity_preserving_reconstruct (src/Flux/CAIDVM.jl:127-141); UNLIMITED the
 // plain reconstruction used next to solid cells and at domain walls/inflow (CAIDVM.jl:13-14,
@@ -311,15 +372,15 @@ __device__ inline void micro_maxwell(const double* bc, double rho, bool wall,
 // src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
 // taken as key-interval overlaps.
 template <int D, int NDF, bool UNI, class FT>
-__device__ inline void side_sum(const DevArgs& A, int beg, int cnt, double dsx, double rdsx,
-                                int i, int li, uint32_t ki, const double (&f)[NDF],
-                                double (&a)[NDF]) {
+__device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int cnt,
+                                double dsx, double rdsx, int i, int li, uint32_t ki,
+                                const double (&f)[NDF], double (&a)[NDF]) {
 #pragma unroll
   for (int k = 0; k < NDF; ++k) a[k] = 0.0;
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
     const Grid<D, NDF, UNI, FT> N(A, nb.cell);
-    for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, [&](int j, double den) {
+    for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, nb.map, i, li, ki, [&](int j, double den) {
       const double rden = 1.0 / den;   // den is a power of two: exact
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
@@ -355,11 +416,12 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
     return;
   }
   double aL[NDF];
-  side_sum<D, NDF, UNI, FT>(A, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li, ki, f, aL);
+  side_sum<D, NDF, UNI, FT>(A, c, G.n, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li,
+                            ki, f, aL);
   if (sd.mode == SL_INNER) {
     double aR[NDF];
-    side_sum<D, NDF, UNI, FT>(A, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li, ki, f,
-                          aR);
+    side_sum<D, NDF, UNI, FT>(A, c, G.n, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li,
+                              ki, f, aR);
 #pragma unroll
     for (int k = 0; k < NDF; ++k) s[k] = minmod_sel(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
   } else {
@@ -584,7 +646,7 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
 // Micro flux into velocity cell i of cell c through all its faces, in face-list order
 // (update_flux!/update_domain_flux!/update_micro_flux!, src/Flux/Flux.jl:64-281).
 template <int D, int NDF, bool UNI, class FT>
-__device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
+__device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, int ne, int i,
                                    const double (&u)[D], const double (&f)[NDF],
                                    const double (&s)[D][NDF], int li, uint32_t ki,
                                    double (&fl)[NDF]) {
@@ -628,7 +690,7 @@ __device__ inline void gather_flux(const DevArgs& A, int eb, int ne, int i,
       }
       default: {
         const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
-        for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, [&](int j, double den) {
+        for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, ki, [&](int j, double den) {
           double uj[D], fj[NDF], sj[D][NDF];
 #pragma unroll
           for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : N.u(j, t);
@@ -783,7 +845,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
         fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
       }
-      gather_flux<D, NDF, UNI, FT>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+      gather_flux<D, NDF, UNI, FT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) A.vflux[G.off * NDF + int64_t(k) * n + i] = fl[k];
     }
@@ -818,7 +880,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 #pragma unroll
           for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
         }
-        gather_flux<D, NDF, UNI, FT>(A, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+        gather_flux<D, NDF, UNI, FT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {

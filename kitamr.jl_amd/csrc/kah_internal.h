@@ -39,7 +39,19 @@ enum : int8_t {
 struct SlopeNb {        // one neighbour in a slope stencil
   int32_t cell;         // data cell
   int32_t proj;         // 1: project with the neighbour's transverse slopes
+  int32_t map;          // overlap-map slot of (owner cell, this neighbour), -1: none
+  int32_t pad;
   double dm[3];         // transverse offsets (dm[dir] = 0)
+};
+
+// Overlap map of adapted velocity grids: for owner cell c, its map slot k (one per distinct
+// neighbour cell) and velocity cell i, entry map[map_off[c] + k*n_c + i] =
+// (j << 8) | cnt: j is the first overlapping velocity cell of the neighbour, cnt = 0 when
+// it is as coarse or coarser than i (one cell, factor 1), else the number of finer cells
+// j, j+1, ... inside i. One entry per (owner velocity cell, neighbour).
+struct OvJob {
+  int32_t cell, nb;
+  int64_t base;
 };
 
 // Descriptor fields are 32-bit or wider: the kernels read them with scalar loads, which
@@ -75,7 +87,7 @@ struct EntryDesc {      // one face side of a fluid cell, gathered in K2
   int32_t oth_mode;     // MI_* of the incoming set
   int32_t dir;
   int32_t bc;
-  int32_t pad;
+  int32_t map;          // overlap-map slot of the other cell (adapted grids), -1: none
   double rot;
   double area;          // signed face area, rot*A (or rot*A/2^(DIM-1))
   double fmid[3];

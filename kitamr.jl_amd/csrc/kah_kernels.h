@@ -51,6 +51,8 @@ struct DevArgs {
   const double* rho_w_r;
   // wave tiles of a shared tensor-product velocity grid (vt_mode 2: 8x8, 3: 4x4x4 blocks of
   // velocity cells per 64 consecutive thread positions; 0: identity order)
+  const uint32_t* ovmap;    // overlap maps of adapted grids (see OvJob), or null
+  const int64_t* map_off;   // [nc] first map entry of each owner cell
   const int32_t* vt_base;   // first velocity cell of each tile
   int32_t vt_mode, vt_n0, vt_n1, vt_pad;
   // parameters
@@ -93,6 +95,9 @@ hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, cons
                              int n, int mode, int march, size_t lds, int waves_per_block,
                              hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
+// fills the overlap maps of the given (cell, neighbour) jobs (adapted grids)
+hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,
+                              uint32_t* map, hipStream_t st);
 hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
                               int n, double* sw, hipStream_t st);
 

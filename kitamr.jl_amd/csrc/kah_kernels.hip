@@ -10,6 +10,8 @@
 
 namespace kah {
 
+hipError_t launch_overlap_map_2d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
+hipError_t launch_overlap_map_3d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
 hipError_t launch_slope_2d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
 hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
 hipError_t launch_macro_slope_2d(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
@@ -74,6 +76,12 @@ hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, cons
                     : launch_flux_relax_3d_f32(A, uni, cells, n, mode, march, lds, nwv, st);
   return dim == 2 ? launch_flux_relax_2d(A, uni, cells, n, mode, march, lds, nwv, st)
                   : launch_flux_relax_3d(A, uni, cells, n, mode, march, lds, nwv, st);
+}
+
+hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,
+                              uint32_t* map, hipStream_t st) {
+  return dim == 2 ? launch_overlap_map_2d(A, jobs, njobs, map, st)
+                  : launch_overlap_map_3d(A, jobs, njobs, map, st);
 }
 
 hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,

@@ -21,4 +21,9 @@ hipError_t launch_macro_slope_2d(const DevArgs& A, bool uni, const int32_t* cell
              : launch_macro_slope_t<2, 2, false, double>(A, cells, n, sw, st);
 }
 
+hipError_t launch_overlap_map_2d(const DevArgs& A, const OvJob* jobs, int njobs, uint32_t* map,
+                                hipStream_t st) {
+  return launch_overlap_map_t<2, 2>(A, jobs, njobs, map, st);
+}
+
 }  // namespace kah

@@ -21,4 +21,9 @@ hipError_t launch_macro_slope_3d(const DevArgs& A, bool uni, const int32_t* cell
              : launch_macro_slope_t<3, 1, false, double>(A, cells, n, sw, st);
 }
 
+hipError_t launch_overlap_map_3d(const DevArgs& A, const OvJob* jobs, int njobs, uint32_t* map,
+                                hipStream_t st) {
+  return launch_overlap_map_t<3, 1>(A, jobs, njobs, map, st);
+}
+
 }  // namespace kah

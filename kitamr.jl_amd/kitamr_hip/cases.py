@@ -53,7 +53,11 @@ def interface_refine(halo_cells=(6.0, 4.0, 2.5), x0=0.5, y0=0.5, base_h=1.0 / 64
 
 def riemann_2d(n=64, nv=28, vmax=5.0, ps_levels=0, halo=(6.0, 4.0, 2.5), Kn=1e-2, omega=0.5,
                CFL=0.8, flux="CAIDVM", marching="CAIDVM_Marching", vs_levels=0,
-               vs_adapt_rounds=0, seed=20261202, bc="UniformOutflow", precision="fp64"):
+               vs_adapt_rounds=0, seed=20261202, bc="UniformOutflow", precision="fp64",
+               vs_variants=0):
+    """vs_variants > 0: the adapted grid of cell c is the one of seed + (c mod vs_variants)
+    for its initial state (built once per variant and state), so large meshes build fast
+    while neighbouring cells still carry different grids."""
     solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=vs_levels,
                     flux=flux, time_marching=marching, CFL=CFL)
     gas = Gas(K=1.0, gamma=5.0 / 3.0, Pr=2.0 / 3.0, Kn=Kn, omega=omega)
@@ -66,9 +70,18 @@ def riemann_2d(n=64, nv=28, vmax=5.0, ps_levels=0, halo=(6.0, 4.0, 2.5), Kn=1e-2
     refine = interface_refine(halo, base_h=1.0 / n) if ps_levels > 0 else None
     vs_fn = None
     if vs_levels > 0:
+        cache = {}
+
         def vs_fn(c, prim, base, _seed=seed):
-            return refine_toward(base.copy(), prim, rounds=vs_adapt_rounds or vs_levels,
-                                 rng=_seed + c)
+            key = c if vs_variants <= 0 else (tuple(np.asarray(prim).tolist()), c % vs_variants)
+            if key not in cache:
+                r = c if vs_variants <= 0 else c % vs_variants
+                cache[key] = refine_toward(base.copy(), prim,
+                                           rounds=vs_adapt_rounds or vs_levels, rng=_seed + r)
+            g = cache[key]
+            if vs_variants <= 0:
+                del cache[key]
+            return g
     return build_case(cfg, ps_refine=refine, vs_grid_fn=vs_fn)
 
 

@@ -34,6 +34,7 @@ OPTION_SETS = {
     "vpt1": {"slope_vpt": 1},
     "vpt4": {"slope_vpt": 4},
     "waves4": {"flux_waves": 4},
+    "no_maps": {"overlap_maps": 0},
 }
 
 
@@ -87,7 +88,8 @@ def test_fused_steps(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("opt", list(OPTION_SETS))
-@pytest.mark.parametrize("name", ["riemann8_amr1_nv16", "box3d_nv8", "riemann16_amr2"])
+@pytest.mark.parametrize("name", ["riemann8_amr1_nv16", "box3d_nv8", "riemann16_amr2",
+                                  "riemann16_amr_vsamr", "box3d_amr_vsamr"])
 def test_launch_options(name, opt):
     case = CASES[name]()
     say(f"[options] {name} {opt}: cells={case.n_cells} P={case.phase_cells}")
@@ -101,3 +103,22 @@ def test_launch_options(name, opt):
     assert rel_err(g["df"], o["df"]) <= TOL, "df"
     assert rel_err_rows(g["w"], o["w"], dim + 2) <= TOL, "w"
     dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["riemann8_vsamr", "riemann16_amr_vsamr", "box3d_amr_vsamr"])
+def test_overlap_maps_equal_search(name):
+    """Adapted grids: the precomputed overlap maps give bit-identical steps to the in-kernel
+    key search (same overlapping cells, factors and order)."""
+    case = CASES[name]()
+    say(f"[maps] {name}: cells={case.n_cells} P={case.phase_cells}")
+    a, _ = pair(case)
+    b, _ = pair(case, options={"overlap_maps": 0})
+    for s in range(3):
+        a.step(case.dt_xi, s)
+        b.step(case.dt_xi, s)
+    ga, gb = a.get_state(sdf=True), b.get_state(sdf=True)
+    for k in ("df", "sdf", "w", "prim"):
+        assert np.array_equal(ga[k], gb[k]), k
+    a.close()
+    b.close()
