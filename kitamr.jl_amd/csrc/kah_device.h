@@ -155,8 +155,9 @@ struct Grid {
   }
   __device__ int64_t goff() const { return UNI ? 0 : off; }
   // velocity cell handled at thread position p: the identity, or the wave-tile order of a
-  // shared tensor-product grid (the 64 positions of a wave form one tile; any bijection of
-  // [0, n) gives the same per-cell results, the order only groups velocity signs per wave)
+  // shared tensor-product grid (the 64 positions of a wave form one tile). Any bijection of
+  // [0, n) gives the same per-cell results; the order only makes the upwind branches
+  // uniform per wave.
   __device__ int vi(int p) const {
     if constexpr (UNI) {
       if (A.vt_mode != 0) {
@@ -195,15 +196,15 @@ struct Grid {
   static __device__ double stored(double v) { return double(FT(v)); }
 };
 
-// Calls fn(j, den) for each velocity cell j of grid G that overlaps cell i (level li, key
-// ki) of another grid on the same root lattice: den = 1 when j is coarser or as fine as
-// i, else 2^(D*(lj-li)). This is the pairing of the two-pointer walks of diff_vs! and
+// Calls fn(j, den, 1/den) for each velocity cell j of grid G that overlaps cell i (level li,
+// key ki) of another grid on the same root lattice: den = 1 when j is coarser or as fine as
+// i, else 2^(D*(lj-li)) (a power of two: 1/den is exact). This is the pairing of the two-pointer walks of diff_vs! and
 // update_micro_flux! (src/Flux/Slope.jl:29-64, src/Flux/Flux.jl:151-344), in the same order.
 template <int D, int NDF, bool UNI, class FT, class Fn>
 __device__ inline void for_overlap(const Grid<D, NDF, UNI, FT>& G, int i, int li, uint32_t ki,
                                    int lmax, Fn&& fn) {
   if constexpr (UNI) {
-    fn(i, 1.0);
+    fn(i, 1.0, 1.0);
   } else {
     int lo = 0, hi = G.n - 1;
     while (lo < hi) {
@@ -213,11 +214,14 @@ __device__ inline void for_overlap(const Grid<D, NDF, UNI, FT>& G, int i, int li
     int j = lo;
     int lj = G.lev(j);
     if (lj <= li) {
-      fn(j, 1.0);
+      fn(j, 1.0, 1.0);
       return;
     }
     const uint32_t end = ki + (1u << (D * (lmax - li)));
-    for (; j < G.n && G.key(j) < end; ++j) fn(j, ldexp(1.0, D * (G.lev(j) - li)));
+    for (; j < G.n && G.key(j) < end; ++j) {
+      const int e = D * (G.lev(j) - li);
+      fn(j, ldexp(1.0, e), ldexp(1.0, -e));
+    }
   }
 }
 
@@ -229,16 +233,19 @@ __device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, F
                                      int n_own, int slot, int i, int li, uint32_t ki,
                                      Fn&& fn) {
   if constexpr (UNI) {
-    fn(i, 1.0);
+    fn(i, 1.0, 1.0);
   } else {
     if (slot >= 0) {
       const uint32_t code = A.ovmap[A.map_off[c] + int64_t(slot) * n_own + i];
       const int j = int(code >> 8), cnt = int(code & 255u);
       if (cnt == 0) {
-        fn(j, 1.0);
+        fn(j, 1.0, 1.0);
         return;
       }
-      for (int t = 0; t < cnt; ++t) fn(j + t, ldexp(1.0, D * (N.lev(j + t) - li)));
+      for (int t = 0; t < cnt; ++t) {
+        const int e = D * (N.lev(j + t) - li);
+        fn(j + t, ldexp(1.0, e), ldexp(1.0, -e));
+      }
       return;
     }
     for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, fn);
@@ -380,8 +387,8 @@ __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
     const Grid<D, NDF, UNI, FT> N(A, nb.cell);
-    for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, nb.map, i, li, ki, [&](int j, double den) {
-      const double rden = 1.0 / den;   // den is a power of two: exact
+    for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, nb.map, i, li, ki,
+                                   [&](int j, double den, double rden) {
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         double pj = N.f(j, k);
@@ -690,7 +697,8 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
       }
       default: {
         const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
-        for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, ki, [&](int j, double den) {
+        for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, ki,
+                                       [&](int j, double den, double rden) {
           double uj[D], fj[NDF], sj[D][NDF];
 #pragma unroll
           for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : N.u(j, t);
@@ -706,7 +714,7 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
 #pragma unroll
           for (int k = 0; k < NDF; ++k) {
             double t = area * mj[k];
-            if (den != 1.0) t = t / den;
+            if (den != 1.0) t = t * rden;   // exact: den is a power of two
             fl[k] += ed.side ? -t : t;
           }
         });

@@ -109,7 +109,7 @@ def test_launch_options(name, opt):
 @pytest.mark.parametrize("name", ["riemann8_vsamr", "riemann16_amr_vsamr", "box3d_amr_vsamr"])
 def test_overlap_maps_equal_search(name):
     """Adapted grids: the precomputed overlap maps give bit-identical steps to the in-kernel
-    key search (same overlapping cells, factors and order)."""
+    key search (same overlapping cells, factors and order; same thread order)."""
     case = CASES[name]()
     say(f"[maps] {name}: cells={case.n_cells} P={case.phase_cells}")
     a, _ = pair(case)
