@@ -35,8 +35,9 @@ extern "C" {
 /* ---- enumerations ------------------------------------------------------------- */
 /* numerical flux, src/Abstract/Types.jl:85-91 (DVM, CAIDVM) */
 enum { KAH_FLUX_CAIDVM = 0, KAH_FLUX_DVM = 1 };
-/* time marching, src/Abstract/Types.jl:96-102 (CAIDVM_Marching, Euler) */
-enum { KAH_MARCH_CAIDVM = 0, KAH_MARCH_EULER = 1 };
+/* time marching, src/Abstract/Types.jl:96-102 (CAIDVM_Marching, Euler, CIP_Marching; the last
+ * is iterate!(::Type{CIP_Marching}), src/Theory/I-projection.jl:161-192) */
+enum { KAH_MARCH_CAIDVM = 0, KAH_MARCH_EULER = 1, KAH_MARCH_CIP = 2 };
 /* storage / arithmetic form of f on the device */
 enum { KAH_PREC_FP64 = 0,    /* f, sdf, flux in FP64 (parity form)                      */
        KAH_PREC_FP32F = 1 }; /* f, sdf stored and advected in FP32, moments/w in FP64    */

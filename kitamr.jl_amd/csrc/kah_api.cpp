@@ -451,7 +451,8 @@ int kah_create(const kah_params* params, kah_ctx** out) {
     return KAH_EUNSUPPORTED;
   if (params->flux_kind != KAH_FLUX_CAIDVM && params->flux_kind != KAH_FLUX_DVM)
     return KAH_EINVAL;
-  if (params->marching != KAH_MARCH_CAIDVM && params->marching != KAH_MARCH_EULER)
+  if (params->marching != KAH_MARCH_CAIDVM && params->marching != KAH_MARCH_EULER &&
+      params->marching != KAH_MARCH_CIP)
     return KAH_EINVAL;
   if (params->precision != KAH_PREC_FP64 && params->precision != KAH_PREC_FP32F)
     return KAH_EUNSUPPORTED;
@@ -528,6 +529,12 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     if (T->nb_cell[x] < 0 || T->nb_cell[x] >= nc)
       return C->fail(KAH_EINVAL, "neighbour id out of range");
   C->benc.assign(T->bound_enc, T->bound_enc + nc);
+  if (C->P.marching == KAH_MARCH_CIP)
+    for (int64_t c = 0; c < nc; ++c)
+      if (T->bound_enc[c] > 0)
+        return C->fail(KAH_EUNSUPPORTED,
+                       "CIP_Marching next to immersed-boundary cells needs "
+                       "positivity_preserving_ib! (src/Boundary/Positivity.jl), not built");
 
   // Wave tiles of a shared tensor-product velocity grid (vs_trees_num cells per direction,
   // first direction fastest): 8x8 (2D) or 4x4x4 (3D) blocks, so that the 64 lanes of a wave
@@ -878,7 +885,8 @@ int run_slope(kah_ctx* C, const DevArgs& A) {
 int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   EvPair e{};
   if (C->timing) KAH_TRY(ev_begin(C, e, 1));
-  int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER : MARCH_CAIDVM;
+  int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER
+               : (C->P.marching == KAH_MARCH_CIP ? MARCH_CIP : MARCH_CAIDVM);
   KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
                                    mode, march, C->lds_bytes, C->fr_waves, C->stream),
                  "flux/relaxation kernel launch"));

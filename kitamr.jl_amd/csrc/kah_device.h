@@ -755,6 +755,227 @@ __device__ inline void heat_finish(const double* qs, double* q) {
   }
 }
 
+// Block minimum of two values per thread (fixed shuffle tree, then the wave partials).
+template <int NWV>
+__device__ inline void block_min2(double a, double b, double* red, double* out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64);
+    a = x < a ? x : a;
+    b = y < b ? y : b;
+  }
+  if (lane == 0) {
+    red[2 * wv] = a;
+    red[2 * wv + 1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double m = red[threadIdx.x];
+    for (int w = 1; w < NWV; ++w) m = red[2 * w + threadIdx.x] < m ? red[2 * w + threadIdx.x] : m;
+    out[threadIdx.x] = m;
+  }
+  __syncthreads();
+}
+
+// The CIP part of iterate!(CIP_Marching) (src/Theory/I-projection.jl:161-192) after the
+// convection pass, which left f* in sc and the partial sums of the heat flux (w.r.t. prim_c),
+// of e_int and the minima of h: g+ = M(prim_c) + Shakhov(q), the conserved I-projection of h
+// (conserved_I_porjection!, :144-158, with solve_I_projection, :55-141: shave, Newton on the
+// dual, Armijo backtracking evaluated without cancellation), then f = f*tau/(tau+dt) +
+// dt/(tau+dt)*g+, written to the next f. The same algorithm as the oracle's cip_projection;
+// thread 0 takes the per-cell decisions, all threads loop uniformly on them.
+template <int D, int NDF, bool UNI, class FT, int NWV>
+__device__ inline void relax_cip(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int n,
+                                 double* sc, double* red, double* cell_sh,
+                                 const double (&qa)[(D == 2) ? 4 : 3], double eint, double fmn,
+                                 double fmp, double (&q)[D], double (&pc)[D + 2]) {
+  constexpr int NW = D + 2, M = D + 2, NQ = (D == 2) ? 4 : 3, NJ = M * (M + 1) / 2;
+  constexpr int BS = NWV * 64;
+  __shared__ double st[64];   // W[5] lam[5] dl[5] | scalars from 16 on
+  double* W = st;
+  double* lam = st + 5;
+  double* dl = st + 10;
+  // scalars: 16 tol_eff, 17 g_prev, 18 stall, 19 done, 20 slope, 21 dW, 22 alpha, 23 accept,
+  // 24 f_min, 25 fp_min, 26 d, 27 1/d, 28 shave, 29.. q, 40.. sums
+  {
+    double v[NQ + 1];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) v[m] = qa[m];
+    v[NQ] = eint;
+    block_reduce<NQ + 1, NWV>(v, NQ + 1, red, &st[40]);
+    block_min2<NWV>(fmn, fmp, red, &st[46]);
+  }
+  if (threadIdx.x == 0) {
+    double qs[NQ], qq[D];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) qs[m] = st[40 + m];
+    heat_finish<D>(qs, qq);
+#pragma unroll
+    for (int d = 0; d < D; ++d) st[29 + d] = qq[d];
+    double wn = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      W[m] = cell_sh[m];
+      lam[m] = 0.0;
+    }
+    if constexpr (NDF == 2) W[M - 1] -= st[40 + NQ] / 2;   // e_int = dot(weight, b)/2
+#pragma unroll
+    for (int m = 0; m < M; ++m) wn += W[m] * W[m];
+    st[16] = 1e-10 * fmax(1.0, sqrt(wn));
+    st[17] = HUGE_VAL;
+    st[18] = 0.0;
+    st[19] = 0.0;
+    const double f_min = 1.1 * st[46];
+    st[24] = f_min;
+    st[25] = st[47];
+    st[26] = st[47] - f_min;
+    st[27] = 1.0 / st[26];
+    st[28] = f_min < 0 ? 1.0 : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = st[29 + d];
+#pragma unroll
+  for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
+  // shave negative entries of h: (h - f_min)/d*fp_min
+  if (st[28] != 0.0) {
+    const double f_min = st[24], fp_min = st[25], dd = st[26], rd = st[27];
+    for (int p = threadIdx.x; p < n; p += BS) {
+      const int i = G.vi(p);
+      const double x = sc[i];
+      if (x < 0) sc[i] = div_by(x - f_min, dd, rd) * fp_min;
+    }
+    __syncthreads();
+  }
+  double* ck = sc + NDF * n;   // c_k = weight*h*exp(lam.psi) of the current Newton step
+  for (int it = 0; it < 10; ++it) {
+    double lm[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) lm[m] = lam[m];
+    double acc[NJ + M];
+#pragma unroll
+    for (int m = 0; m < NJ + M; ++m) acc[m] = 0.0;
+    for (int p = threadIdx.x; p < n; p += BS) {
+      const int i = G.vi(p);
+      double u[D], ps[M];
+#pragma unroll
+      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      cip_psi<D>(u, ps);
+      double dot = 0.0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) dot += lm[m] * ps[m];
+      const double cc = G.wt(i) * sc[i] * exp(dot);
+      ck[i] = cc;
+      int t = M;
+#pragma unroll
+      for (int a = 0; a < M; ++a) {
+        const double ci = cc * ps[a];
+        acc[a] += ci;
+#pragma unroll
+        for (int b = a; b < M; ++b) acc[t++] += ci * ps[b];
+      }
+    }
+    block_reduce<NJ + M, NWV>(acc, NJ + M, red, &st[40]);
+    if (threadIdx.x == 0) {
+      double Gv[5], J[5][5], gn = 0.0;
+      for (int a = 0; a < M; ++a) {
+        Gv[a] = st[40 + a] - W[a];
+        gn += Gv[a] * Gv[a];
+      }
+      gn = sqrt(gn);
+      bool done = gn < st[16];
+      if (!done) {
+        if (gn > 0.9 * st[17]) {
+          st[18] += 1.0;
+          done = st[18] >= 2.0;
+        } else {
+          st[18] = 0.0;
+        }
+      }
+      st[19] = done ? 1.0 : 0.0;
+      if (!done) {
+        st[17] = gn;
+        int t = 40 + M;
+        for (int a = 0; a < M; ++a)
+          for (int b = a; b < M; ++b) J[a][b] = J[b][a] = st[t++];
+        double x[5];
+        cholesky_solve(M, J, Gv, x);
+        double slope = 0.0, dW = 0.0;
+        for (int a = 0; a < M; ++a) {
+          dl[a] = -x[a];
+          slope += Gv[a] * dl[a];
+          dW += dl[a] * W[a];
+        }
+        st[20] = slope;
+        st[21] = dW;
+        st[22] = 1.0;
+      }
+    }
+    __syncthreads();
+    if (st[19] != 0.0) break;
+    double dd[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) dd[m] = dl[m];
+    // Armijo: sum c_k*expm1(alpha*dl.psi_k) - alpha*dl.W <= 1e-4*alpha*slope
+    for (int tr = 0; tr < 10; ++tr) {
+      const double alpha = st[22];
+      double sv[1] = {0.0};
+      for (int p = threadIdx.x; p < n; p += BS) {
+        const int i = G.vi(p);
+        double u[D], ps[M];
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+        cip_psi<D>(u, ps);
+        double e = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) e += dd[m] * ps[m];
+        sv[0] += ck[i] * expm1(alpha * e);
+      }
+      block_reduce<1, NWV>(sv, 1, red, &st[40]);
+      if (threadIdx.x == 0) {
+        const bool ok = st[40] - alpha * st[21] <= 1e-4 * alpha * st[20];
+        st[23] = ok ? 1.0 : 0.0;
+        if (!ok) st[22] = alpha * 0.5;
+      }
+      __syncthreads();
+      if (st[23] != 0.0) break;
+    }
+    if (threadIdx.x == 0)
+      for (int m = 0; m < M; ++m) lam[m] += st[22] * dl[m];
+    __syncthreads();
+  }
+  // h *= exp(lam.psi); g+ = M(prim_c) + Shakhov; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
+  double lm[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) lm[m] = lam[m];
+  const double dt = A.dt;
+  const double tau = get_tau<D>(pc, A.mu, A.omega);
+  const double a1 = tau / (tau + dt), a2 = dt / (tau + dt);
+  const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
+  for (int p = threadIdx.x; p < n; p += BS) {
+    const int i = G.vi(p);
+    double u[D], ps[M], Fc[2], Sp[2];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+    cip_psi<D>(u, ps);
+    double dot = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) dot += lm[m] * ps[m];
+    maxwell_pre<D>(u, mpc, A.K, Fc);
+    shakhov_pre<D>(u, Fc, mpc, q, A.K, Sp);
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      double fk = sc[k * n + i];
+      if (k == 0) fk = fk * exp(dot);
+      const double g = Fc[k] + Sp[k];
+      double fn = fk * a1;
+      fn = fn + a2 * g;
+      G.set_out(i, k, fn);
+    }
+  }
+}
+
 // iterate!(CAIDVM_Marching) (src/Theory/Iterate.jl:96-130) or iterate!(Euler) (:131-162)
 // for one cell, fed by the gathered flux (MODE_FUSED), by the flux accumulators
 // (MODE_ITER), or only accumulating the flux (MODE_FLUX).
@@ -779,7 +1000,9 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   const int eb = A.ent_off[c];
   const int ne = A.ent_off[c + 1] - eb;
   const double dt = A.dt;
-  __shared__ double red[NWV * 8];
+  constexpr int NRED = (MARCH == MARCH_CIP) ? (D + 2) * (D + 5) / 2 : 8;   // G and J of the
+                                                                         // I-projection
+  __shared__ double red[NWV * NRED];
   __shared__ double res[8];
   __shared__ double cell_sh[4 * NW];   // wnew | prim_c | prim | q
   __shared__ double hsum[MAX_ENTRIES * 2 * NW];   // the half fluxes of the cell's faces
@@ -865,6 +1088,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   // pass 1: flux of every velocity cell (gathered or from the accumulators)
   double mw[NW];
   double qa[NQ];
+  double eint = 0.0, fmn = HUGE_VAL, fmp = HUGE_VAL;   // CIP: b's energy, min h, min h > 0
 #pragma unroll
   for (int q = 0; q < NW; ++q) mw[q] = 0.0;
 #pragma unroll
@@ -908,6 +1132,20 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         moment_terms<D, NDF>(wt, u, fs, m);
 #pragma unroll
         for (int q = 0; q < NW; ++q) mw[q] += m[q];
+      } else if (MARCH == MARCH_CIP) {
+        // convection first, then the heat flux of f* with prim_c (calc_qf after the
+        // convection, src/Theory/I-projection.jl:177-179); e_int and the minima of h for the
+        // I-projection
+        double fs[NDF];
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          fs[k] = f[k] + dtV * fl[k];
+          sc[k * n + i] = fs[k];
+        }
+        heat_terms<D, NDF, NQ>(wt, u, fs, pc, qa);
+        if constexpr (NDF == 2) eint += wt * fs[1];
+        fmn = fs[0] < fmn ? fs[0] : fmn;
+        if (fs[0] > 0.0 && fs[0] < fmp) fmp = fs[0];
       } else {
         // Euler: heat flux of f^n with the updated prim
 #pragma unroll
@@ -917,6 +1155,11 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     }
   }
 
+  double q[D];
+  double pc[NW];
+  if constexpr (MARCH == MARCH_CIP) {
+    relax_cip<D, NDF, UNI, FT, NWV>(A, G, n, sc, red, cell_sh, qa, eint, fmn, fmp, q, pc);
+  } else {
   if (MARCH == MARCH_CAIDVM) {
     block_reduce<NW, NWV>(mw, NW, red, &cell_sh[2 * NW]);
     if (threadIdx.x == 0) {
@@ -959,10 +1202,8 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   double qs_red[NQ];
   block_reduce<NQ, NWV>(qa, NQ, red, &cell_sh[3 * NW]);
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) qs_red[q] = cell_sh[3 * NW + q];
-  double q[D];
+  for (int m = 0; m < NQ; ++m) qs_red[m] = cell_sh[3 * NW + m];
   heat_finish<D>(qs_red, q);
-  double pc[NW];
 #pragma unroll
   for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
   const double tau = get_tau<D>(pc, A.mu, A.omega);
@@ -1005,6 +1246,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         G.set_out(i, k, fx);
       }
     }
+  }
   }
   if (threadIdx.x == 0) {
     double* pr = &A.prim[int64_t(c) * NW];
@@ -1127,10 +1369,14 @@ inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int
   if (mode == MODE_ITER) {
     if (march == MARCH_CAIDVM)
       return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+    if (march == MARCH_CIP)
+      return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_CIP, NWV>(A, cells, n, lds, st);
     return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_EULER, NWV>(A, cells, n, lds, st);
   }
   if (march == MARCH_CAIDVM)
     return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+  if (march == MARCH_CIP)
+    return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_CIP, NWV>(A, cells, n, lds, st);
   return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_EULER, NWV>(A, cells, n, lds, st);
 }
 

@@ -133,6 +133,47 @@ __host__ __device__ inline double minmod_sel(double sL, double sR) {
   return pos ? m : (neg ? -m : 0.0 * m);
 }
 
+// x = J^-1 g for a symmetric positive definite J (M <= 5, upper triangle read) by Cholesky
+// L L^T = J: the Newton step of the I-projection (the reference's Symmetric(J,:U) \ g,
+// src/Theory/I-projection.jl:124, which LAPACK solves by Bunch-Kaufman: equal to rounding).
+__host__ __device__ inline void cholesky_solve(int M, const double (&J)[5][5], const double* g,
+                                               double* x) {
+  double L[5][5] = {};
+  for (int j = 0; j < M; ++j) {
+    double s = J[j][j];
+    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    L[j][j] = sqrt(s);
+    for (int i = j + 1; i < M; ++i) {
+      double t = J[j][i];
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+      L[i][j] = t / L[j][j];
+    }
+  }
+  double y[5];
+  for (int i = 0; i < M; ++i) {
+    double t = g[i];
+    for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
+    y[i] = t / L[i][i];
+  }
+  for (int i = M - 1; i >= 0; --i) {
+    double t = y[i];
+    for (int k = i + 1; k < M; ++k) t -= L[k][i] * x[k];
+    x[i] = t / L[i][i];
+  }
+}
+
+// psi = (1, u, |u|^2/2) of build_psi_matrix (src/Theory/I-projection.jl:7-21)
+template <int D>
+__host__ __device__ inline void cip_psi(const double (&u)[D], double (&ps)[D + 2]) {
+  ps[0] = 1.0;
+  double s = 0.0;
+  for (int d = 0; d < D; ++d) {
+    ps[1 + d] = u[d];
+    s += u[d] * u[d];
+  }
+  ps[D + 1] = s / 2;
+}
+
 // get_prim_2D / get_prim_3D, lib/KitCore/2D.jl:9-16, 3D.jl:12-20
 template <int D>
 __host__ __device__ inline void get_prim(const double* w, double g, double* p) {

@@ -8,7 +8,7 @@
 namespace kah {
 
 enum { MODE_FLUX = 0, MODE_ITER = 1, MODE_FUSED = 2 };
-enum { MARCH_CAIDVM = 0, MARCH_EULER = 1 };
+enum { MARCH_CAIDVM = 0, MARCH_EULER = 1, MARCH_CIP = 2 };
 
 // Everything a kernel reads, passed by value (device pointers + scalars).
 struct DevArgs {

@@ -16,7 +16,7 @@ PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_ROOT / "lib" / "libkitamr_hip.so"
 
 FLUX = {"CAIDVM": 0, "DVM": 1}
-MARCH = {"CAIDVM_Marching": 0, "Euler": 1}
+MARCH = {"CAIDVM_Marching": 0, "Euler": 1, "CIP_Marching": 2}
 PREC = {"fp64": 0, "fp32f": 1}
 BC = {"Maxwellian": 0, "SuperSonicInflow": 1, "UniformOutflow": 2, "InterpolatedOutflow": 3}
 ERRORS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "device error",

@@ -501,6 +501,146 @@ void heat_flux_full(const orc_ctx& o, int64_t c, const double* a, int64_t stride
   }
 }
 
+// x = J^-1 g for a symmetric positive definite J (M <= 5) by Cholesky, L L^T = J
+void cholesky_solve(int M, const double (&J)[5][5], const double* g, double* x) {
+  double L[5][5] = {};
+  for (int j = 0; j < M; ++j) {
+    double s = J[j][j];
+    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    L[j][j] = std::sqrt(s);
+    for (int i = j + 1; i < M; ++i) {
+      double t = J[j][i];
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+      L[i][j] = t / L[j][j];
+    }
+  }
+  double y[5];
+  for (int i = 0; i < M; ++i) {
+    double t = g[i];
+    for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
+    y[i] = t / L[i][i];
+  }
+  for (int i = M - 1; i >= 0; --i) {
+    double t = y[i];
+    for (int k = i + 1; k < M; ++k) t -= L[k][i] * x[k];
+    x[i] = t / L[i][i];
+  }
+}
+
+// conserved_I_porjection! (src/Theory/I-projection.jl:144-158) of one cell: the h (2D2F) or f
+// (3D1F) row a[0..n) is multiplied by exp(lambda.psi) with lambda from solve_I_projection
+// (:55-141), so that its moments (with b's internal energy, 2D2F) equal w.
+//   psi = (1, u, |u|^2/2) (build_psi_matrix, :7-21); negative entries are first shaved
+//   (:74-84); Newton on the dual G(lambda) = <psi a exp(lambda.psi)> - W, at most 10 steps,
+//   stop at |G| < 1e-10*max(1,|W|) or after two steps that reduce |G| by less than 10 %;
+//   Newton step by a Cholesky solve of the symmetric J (the reference calls
+//   Symmetric(J,:U) \ G: LAPACK's Bunch-Kaufman, equal up to rounding); Armijo backtracking
+//   with the same sufficient-decrease inequality Phi(l + a*dl) <= Phi(l) + 1e-4*a*G.dl, its
+//   left side evaluated as the difference sum(c_k*expm1(a*dl.psi_k)) - a*dl.W (the reference
+//   subtracts two O(1) sums, whose rounding decides the test near convergence).
+void cip_projection(const orc_ctx& o, int64_t c, double* a, int64_t n, const double* b,
+                    const double* w) {
+  const int D = o.dim, M = D + 2;
+  double W[5];
+  for (int q = 0; q < M; ++q) W[q] = w[q];
+  if (D == 2) {   // e_int = dot(weight, df[:,2])/2 ; w_trans[end] -= e_int
+    Neumaier e;
+    for (int64_t i = 0; i < n; ++i) e.add(o.Wt(c, i) * b[i]);
+    W[M - 1] -= e.get() / 2;
+  }
+  auto psi = [&](int64_t i, double* ps) {
+    ps[0] = 1.0;
+    double s = 0.0;
+    for (int d = 0; d < D; ++d) {
+      const double x = o.U(c, i, d);
+      ps[1 + d] = x;
+      s += x * x;
+    }
+    ps[D + 1] = s / 2;
+  };
+  // shave negative distribution
+  double mn = a[0];
+  for (int64_t i = 1; i < n; ++i) mn = std::min(mn, a[i]);
+  const double f_min = 1.1 * mn;
+  if (f_min < 0) {
+    double fp_min = HUGE_VAL;
+    for (int64_t i = 0; i < n; ++i)
+      if (a[i] > 0) fp_min = std::min(fp_min, a[i]);
+    const double d = fp_min - f_min;
+    for (int64_t i = 0; i < n; ++i)
+      if (a[i] < 0) a[i] = (a[i] - f_min) / d * fp_min;
+  }
+  double lam[5] = {0, 0, 0, 0, 0};
+  double wn = 0.0;
+  for (int q = 0; q < M; ++q) wn += W[q] * W[q];
+  const double tol_eff = 1e-10 * std::max(1.0, std::sqrt(wn));
+  double g_prev = HUGE_VAL;
+  int stall = 0;
+  std::vector<double> ck(static_cast<size_t>(n));
+  for (int it = 0; it < 10; ++it) {
+    Neumaier Gs[5], Js[15];
+    for (int64_t i = 0; i < n; ++i) {
+      double ps[5];
+      psi(i, ps);
+      double dot = 0.0;
+      for (int q = 0; q < M; ++q) dot += lam[q] * ps[q];
+      const double cc = o.Wt(c, i) * a[i] * std::exp(dot);
+      ck[size_t(i)] = cc;
+      int t = 0;
+      for (int p = 0; p < M; ++p) {
+        const double ci = cc * ps[p];
+        Gs[p].add(ci);
+        for (int r = p; r < M; ++r) Js[t++].add(ci * ps[r]);
+      }
+    }
+    double G[5], J[5][5], gn = 0.0;
+    for (int p = 0; p < M; ++p) {
+      G[p] = Gs[p].get() - W[p];
+      gn += G[p] * G[p];
+    }
+    gn = std::sqrt(gn);
+    if (gn < tol_eff) break;
+    if (gn > 0.9 * g_prev) {
+      if (++stall >= 2) break;
+    } else {
+      stall = 0;
+    }
+    g_prev = gn;
+    int t = 0;
+    for (int p = 0; p < M; ++p)
+      for (int r = p; r < M; ++r) J[p][r] = J[r][p] = Js[t++].get();
+    double dl[5];
+    cholesky_solve(M, J, G, dl);
+    double slope = 0.0, dW = 0.0;
+    for (int p = 0; p < M; ++p) {
+      dl[p] = -dl[p];
+      slope += G[p] * dl[p];
+      dW += dl[p] * W[p];
+    }
+    double alpha = 1.0;
+    for (int tr = 0; tr < 10; ++tr) {
+      Neumaier s;
+      for (int64_t i = 0; i < n; ++i) {
+        double ps[5];
+        psi(i, ps);
+        double e = 0.0;
+        for (int q = 0; q < M; ++q) e += dl[q] * ps[q];
+        s.add(ck[size_t(i)] * std::expm1(alpha * e));
+      }
+      if (s.get() - alpha * dW <= 1e-4 * alpha * slope) break;
+      alpha *= 0.5;
+    }
+    for (int p = 0; p < M; ++p) lam[p] += alpha * dl[p];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    double ps[5];
+    psi(i, ps);
+    double dot = 0.0;
+    for (int q = 0; q < M; ++q) dot += lam[q] * ps[q];
+    a[i] *= std::exp(dot);
+  }
+}
+
 // ---------------------------------------------------------------- flux!
 struct SideMicro {   // one side of a face: the upwind subset of a velocity grid
   int64_t cell = -1;             // data cell
@@ -1106,6 +1246,29 @@ int orc_iterate(orc_ctx* o, double dt, int64_t step, double* sum_res, double* su
         for (int64_t x = 0; x < n * NDF; ++x) f[x] *= tau / (tau + dt);
         for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / (tau + dt) * fc[x];
         for (int q2 = 0; q2 < NW; ++q2) prim_new[q2] = prim_c[q2];
+      } else if (o->marching == KAH_MARCH_CIP) {
+        // iterate!(CIP_Marching), src/Theory/I-projection.jl:161-192 (positivity_preserving_ib!
+        // is a no-op for bound_enc == 0, src/Boundary/Positivity.jl:3-5)
+        for (int q2 = 0; q2 < NW; ++q2) w[q2] += fl[q2] * dt / area;
+        double prim_c[5];
+        get_prim(D, w, o->gamma, prim_c);
+        for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / area * vfl[x];
+        double q[3];
+        heat_flux_full(*o, c, f, n, prim_c, q);
+        for (int d = 0; d < D; ++d) o->qf[c * D + d] = q[d];
+        fc.assign(size_t(n * NDF), 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+          double u[3] = {0, 0, 0}, Fc[2], Sp[2];
+          for (int d = 0; d < D; ++d) u[d] = o->U(c, i, d);
+          maxwell(D, u, prim_c, o->K, Fc);
+          shakhov(D, u, Fc, prim_c, q, o->Pr, o->K, Sp);
+          for (int k = 0; k < NDF; ++k) fc[k * n + i] = Fc[k] + Sp[k];
+        }
+        cip_projection(*o, c, f, n, NDF == 2 ? f + n : nullptr, w);
+        double tau = get_tau(D, prim_c, o->mu, o->omega);
+        for (int64_t x = 0; x < n * NDF; ++x) f[x] *= tau / (tau + dt);
+        for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / (tau + dt) * fc[x];
+        for (int q2 = 0; q2 < NW; ++q2) prim_new[q2] = prim_c[q2];
       } else {
         // Euler: `isa(flux_type, MicroFlux)` is always false, so w always takes the
         // macroscopic flux (Iterate.jl:144-149)
@@ -1186,6 +1349,16 @@ void orc_heat_flux(int dim, int ndf, int64_t n, const double* vs_mid, const doub
   o.vmid.assign(vs_mid, vs_mid + n * dim);
   o.vw.assign(weight, weight + n);
   heat_flux_full(o, 0, df, n, prim, q_out);
+}
+void orc_i_projection(int dim, int ndf, int64_t n, const double* vs_mid, const double* weight,
+                      double* df, const double* w) {
+  orc_ctx o;
+  o.dim = dim;
+  o.ndf = ndf;
+  o.voff = {0, n};
+  o.vmid.assign(vs_mid, vs_mid + n * dim);
+  o.vw.assign(weight, weight + n);
+  cip_projection(o, 0, df, n, ndf == 2 ? df + n : nullptr, w);
 }
 void orc_shakhov_part(int dim, int ndf, int64_t n, const double* vs_mid, const double* F,
                       const double* prim, const double* qf, double Pr, double K,

@@ -43,6 +43,10 @@ void orc_heat_flux(int dim, int ndf, int64_t n, const double* vs_mid, const doub
 void orc_shakhov_part(int dim, int ndf, int64_t n, const double* vs_mid, const double* F,
                       const double* prim, const double* qf, double Pr, double K,
                       double* out);
+/* conserved_I_porjection! of one grid: df ([ndf, n], h row first) is modified in place so
+ * that its moments (with b's internal energy, ndf == 2) equal w. */
+void orc_i_projection(int dim, int ndf, int64_t n, const double* vs_mid, const double* weight,
+                      double* df, const double* w);
 void orc_get_prim(int dim, const double* w, double gamma, double* prim);
 void orc_get_conserved(int dim, const double* prim, double gamma, double* w);
 double orc_get_tau(int dim, const double* prim, double mu, double omega);

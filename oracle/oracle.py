@@ -50,6 +50,7 @@ def lib():
         L.orc_discrete_maxwell.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp, C.c_double, dp]
         L.orc_micro_to_macro.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp, dp, dp]
         L.orc_heat_flux.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp, dp, dp, dp]
+        L.orc_i_projection.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp, dp, dp]
         L.orc_shakhov_part.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp, dp, dp,
                                        C.c_double, C.c_double, dp]
         L.orc_get_prim.argtypes = [C.c_int, dp, C.c_double, dp]
@@ -161,6 +162,15 @@ def heat_flux(dim, ndf, mid_colmajor, df, weight, prim):
     out = np.empty(dim)
     lib().orc_heat_flux(dim, ndf, n, _p(_f(mid_colmajor)), _p(_f(df)), _p(_f(weight)),
                         _p(_f(prim)), _p(out))
+    return out
+
+
+def i_projection(dim, ndf, mid_colmajor, weight, df, w):
+    """conserved_I_porjection! of one grid: returns the projected df ([ndf, n] flattened)."""
+    n = len(weight)
+    out = np.array(df, dtype=np.float64, copy=True)
+    lib().orc_i_projection(dim, ndf, n, _p(_f(mid_colmajor)), _p(_f(weight)), _p(out),
+                           _p(_f(w)))
     return out
 
 

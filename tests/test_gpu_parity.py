@@ -23,6 +23,13 @@ CASES = {
                                                      vs_levels=1),
     "box3d": lambda: cases.box_3d(n=4, nv=6),
     "box3d_amr_vsamr": lambda: cases.box_3d(n=4, nv=6, ps_levels=1, vs_levels=1),
+    # CIP_Marching (conserved I-projection)
+    "riemann8_cip": lambda: cases.riemann_2d(n=8, nv=16, marching="CIP_Marching"),
+    "riemann16_amr2_cip": lambda: cases.riemann_2d(n=16, nv=12, ps_levels=2, halo=(3.0, 2.0),
+                                                   marching="CIP_Marching"),
+    "riemann8_vsamr_cip": lambda: cases.riemann_2d(n=8, nv=8, vs_levels=2,
+                                                   marching="CIP_Marching"),
+    "box3d_cip": lambda: cases.box_3d(n=4, nv=12, marching="CIP_Marching"),
     # grids whose size admits wave tiles (8x8 in 2D, 4x4x4 in 3D)
     "riemann8_amr1_nv16": lambda: cases.riemann_2d(n=8, nv=16, ps_levels=1, halo=(2.0, 1.5)),
     "box3d_nv8": lambda: cases.box_3d(n=4, nv=8),
