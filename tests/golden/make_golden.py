@@ -26,6 +26,7 @@ CASES = {
     "box3d": lambda: cases.box_3d(n=3, nv=6, bcs=("SuperSonicInflow", "UniformOutflow",
                                                   "Period", "Period", "Maxwellian",
                                                   "UniformOutflow")),
+    "riemann8_cip": lambda: cases.riemann_2d(n=8, nv=16, marching="CIP_Marching"),
 }
 STEPS = 3
 STRIDE = 7
@@ -48,6 +49,6 @@ def run(name):
 
 if __name__ == "__main__":
     out = Path(__file__).resolve().parent
-    for name in CASES:
+    for name in sys.argv[1:] or CASES:
         np.savez_compressed(out / f"{name}.npz", **run(name))
         print("wrote", name)
