@@ -66,6 +66,46 @@ def profiled_traffic():
     return None
 
 
+def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
+    """A short stream of the serving workload: 2D Riemann cases (outflow or Maxwellian-inflow
+    boundaries) with Kn log-uniform in [1e-3, 1], 256-8.3k physical cells (16^2-48^2 base,
+    0-2 refinement levels), 16^2-48^2 velocity grids, FP64 or FP32-f, Poisson arrivals;
+    each runs to the residual tolerance or its step budget.
+    Reports p50/p99 request latency (arrival to result) and the aggregate updates/s."""
+    import numpy as np
+    from kitamr_hip import cases, serve
+    rng = np.random.default_rng(seed)
+    arrivals = serve.poisson_arrivals(n_req, rate, seed)
+    reqs = []
+    t0 = time.perf_counter()
+    for k in range(n_req):
+        n = int(rng.choice([16, 24, 32, 48]))
+        lv = int(rng.integers(0, 3))
+        nv = int(rng.choice([16, 24, 32, 48]))
+        Kn = float(10 ** rng.uniform(-3, 0))
+        prec = "fp32f" if rng.random() < 0.25 else "fp64"
+        if rng.random() < 0.5:
+            case = cases.riemann_2d(n=n, nv=nv, ps_levels=lv, halo=(3.0, 2.0, 1.5), Kn=Kn,
+                                    precision=prec)
+            tag = f"riemann n={n} L={lv} nv={nv}"
+        else:
+            case = cases.riemann_2d(n=n, nv=nv, ps_levels=lv, halo=(3.0, 2.0, 1.5), Kn=Kn,
+                                    bc="SuperSonicInflow", precision=prec)
+            tag = f"inflow n={n} L={lv} nv={nv}"
+        steps = int(rng.integers(40, 160))
+        reqs.append(serve.Request(rid=k, case=case, arrival=float(arrivals[k]), max_steps=steps,
+                                  tag=f"{tag} {prec} Kn={Kn:.2g}"))
+    log(f"[bench] serving stream of {n_req} requests built in {time.perf_counter() - t0:.1f}s")
+    srv = serve.Server(budget_bytes=int(budget_gb * 1e9), max_slots=slots)
+    rep = srv.run(reqs, keep_df=False)
+    rep["workload"] = (f"{n_req} requests, Poisson arrivals at {rate}/s, 2D Riemann cases "
+                       "(outflow or Maxwellian-inflow boundaries), 256-8.3k cells, 16^2-48^2 "
+                       "velocity grids, FP64 or FP32-f, 40-160 steps each (step budget reduced "
+                       "from 2000 to keep the run short)")
+    rep["slots"] = slots
+    return rep
+
+
 def cpu_baseline(case, max_steps=5, min_seconds=10.0):
     import numpy as np
     import oracle as O
@@ -96,6 +136,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--small", action="store_true", help="reduced mesh (profiling/debug only)")
     ap.add_argument("--flux-waves", type=int, default=0,
@@ -223,6 +264,11 @@ def main():
         "copy_bandwidth_GBs": copy_gbs,
         "state_finite": finite,
     }
+    if not args.no_serve and not args.small:
+        rep = serving_stream()
+        log(f"[bench] serving: p50 {rep['p50_latency_s']:.3f}s p99 {rep['p99_latency_s']:.3f}s "
+            f"{rep['updates_per_s'] / 1e9:.2f} G updates/s")
+        out["serving"] = rep
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(case, max_steps=args.cpu_steps)
