@@ -84,10 +84,22 @@ struct kah_ctx {
   bool vt_enable = true;   // wave-tile order of shared tensor-product velocity grids
   int vt_mode = 0, vt_n0 = 0, vt_n1 = 0;
   std::vector<EvPair> events;
+  // graph replay of the fused step: one executable graph per ping-pong parity, captured for
+  // the step size dt it was recorded with
+  bool use_graph = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double gdt[2] = {0.0, 0.0};
+  void drop_graphs() {
+    for (int k = 0; k < 2; ++k) {
+      if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+      gexec[k] = nullptr;
+    }
+  }
   double t_sum[2] = {0, 0};
   int64_t t_cnt[2] = {0, 0};
 
   ~kah_ctx() {
+    drop_graphs();
     for (auto* b : bins) delete b;
     for (auto& e : events) {
       (void)hipEventDestroy(e.a);
@@ -491,6 +503,7 @@ const char* kah_last_error(const kah_ctx* ctx) {
 
 int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   if (!C) return KAH_EINVAL;
+  C->drop_graphs();   // captured launches refer to the buffers replaced below
   if (!T) return C->fail(KAH_EINVAL, "null topology");
   KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
   C->have_topo = C->have_state = false;
@@ -1026,8 +1039,34 @@ int kah_step(kah_ctx* C, double dt, int64_t step, double* sum_res, double* sum_a
   if (!C) return KAH_EINVAL;
   KAH_TRY(check_ready(C, dt));
   DevArgs A = make_args(C, dt);
-  KAH_TRY(run_slope(C, A));
-  KAH_TRY(run_flux_relax(C, A, MODE_FUSED));
+  if (C->use_graph && !C->timing) {
+    // the launches of one fused step, captured once per buffer parity and dt, then replayed
+    // as a single graph launch (the step records no memset or host work)
+    const int k = C->cur;
+    if (!C->gexec[k] || C->gdt[k] != dt) {
+      if (C->gexec[k]) (void)hipGraphExecDestroy(C->gexec[k]);
+      C->gexec[k] = nullptr;
+      hipGraph_t g = nullptr;
+      KAH_TRY(C->hip(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal),
+                     "hipStreamBeginCapture"));
+      int rc = run_slope(C, A);
+      if (!rc) rc = run_flux_relax(C, A, MODE_FUSED);
+      hipError_t ec = hipStreamEndCapture(C->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      KAH_TRY(C->hip(ec, "hipStreamEndCapture"));
+      hipError_t ei = hipGraphInstantiate(&C->gexec[k], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      KAH_TRY(C->hip(ei, "hipGraphInstantiate"));
+      C->gdt[k] = dt;
+    }
+    KAH_TRY(C->hip(hipGraphLaunch(C->gexec[k], C->stream), "hipGraphLaunch"));
+  } else {
+    KAH_TRY(run_slope(C, A));
+    KAH_TRY(run_flux_relax(C, A, MODE_FUSED));
+  }
   bool check = (step % C->P.st_check_interval) == 0 && (sum_res || sum_avg);
   if (check) KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "step"));
   else KAH_TRY(C->hip(hipGetLastError(), "step"));
@@ -1074,6 +1113,11 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
       if (value != 4 && value != 8)
         return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4 or 8");
       C->fr_waves = int(value);
+      return KAH_OK;
+    case KAH_OPT_GRAPHS:
+      if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "graphs must be 0 or 1");
+      C->use_graph = value != 0;
+      C->drop_graphs();
       return KAH_OK;
     case KAH_OPT_OVERLAP_MAPS:
       if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "overlap maps must be 0 or 1");

@@ -26,7 +26,7 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_get_state", "kah_get_flux", "kah_slope", "kah_flux", "kah_iterate", "kah_step",
            "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
            "kah_copy_bandwidth", "kah_set_option"]
-OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4}
+OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5}
 
 
 class KahParams(C.Structure):

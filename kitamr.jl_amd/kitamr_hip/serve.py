@@ -59,9 +59,9 @@ def device_bytes_estimate(case):
     return int(state + grids + per_cell + halves + scratch + (1 << 20))
 
 
-def _default_factory(case):
+def _default_factory(case, options=None):
     from .solver import KA
-    return KA(case)
+    return KA(case, options=options)
 
 
 class Server:
@@ -70,14 +70,17 @@ class Server:
     budget_bytes: device memory the resident cases may hold together.
     max_slots:    cases stepped concurrently (one stream each).
     factory:      case -> KA-like handle with ``.ctx`` (``step``, ``sync``, ``get_state``,
-                  ``close``) and ``.status``; the default creates a device context.
+                  ``close``) and ``.status``; the default creates a device context with
+                  ``options`` (launch options; by default each case's step is replayed
+                  from captured graphs).
     """
 
     def __init__(self, budget_bytes, max_slots=8, factory: Callable = None,
-                 clock: Callable[[], float] = time.perf_counter, log=None):
+                 clock: Callable[[], float] = time.perf_counter, log=None, options=None):
         self.budget = int(budget_bytes)
         self.max_slots = int(max_slots)
-        self.factory = factory or _default_factory
+        opts = {"graphs": 1} if options is None else options
+        self.factory = factory or (lambda case: _default_factory(case, opts))
         self.clock = clock
         self.log = log
         self.used = 0

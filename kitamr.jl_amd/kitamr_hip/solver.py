@@ -283,10 +283,12 @@ class Status:
 class KA:
     """Host handle of one case on the device (the reference's KA for this path)."""
 
-    def __init__(self, case, device=0):
+    def __init__(self, case, device=0, options=None):
         self.case = case
         self.config = case.config
         self.ctx = abi.HipContext(kah_params(case.config, device))
+        for k, v in (options or {}).items():
+            self.ctx.set_option(k, v)
         self.ctx.set_topology(case.topo)
         self.ctx.set_state(case.w, case.prim, case.df)
         nw = case.config.solver.DIM + 2

@@ -129,3 +129,26 @@ def test_overlap_maps_equal_search(name):
         assert np.array_equal(ga[k], gb[k]), k
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["riemann8_amr1_nv16", "riemann16_amr_vsamr", "riemann8_cip",
+                                  "box3d_nv8", "wall_channel"])
+def test_graph_replay_equals_launches(name):
+    """kah_step replayed from captured graphs (one per buffer parity and dt) gives the same
+    state as the launches one by one, step after step, with host reads between the replays
+    and a change of dt (recapture)."""
+    case = CASES[name]()
+    say(f"[graph] {name}: cells={case.n_cells} P={case.phase_cells}")
+    a, _ = pair(case)
+    b, _ = pair(case, options={"graphs": 1})
+    dts = [case.dt_xi] * 4 + [0.5 * case.dt_xi] * 2 + [case.dt_xi] * 2
+    for s, dt in enumerate(dts):
+        ra = a.step(dt, s)
+        rb = b.step(dt, s)
+        assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1])
+        ga, gb = a.get_state(), b.get_state()
+        for k in ("df", "w", "prim", "qf"):
+            assert np.array_equal(ga[k], gb[k]), (s, k)
+    a.close()
+    b.close()
