@@ -149,6 +149,8 @@ def main():
                          "2 levels)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32f"],
                     help="fp64 (parity form) or fp32f (f stored in FP32, FP64 moments)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch options on the workload first (coordinate descent)")
     ap.add_argument("--vel-tiles", type=int, default=-1,
                     help="1/0: wave tiles of the velocity grid on/off (-1: library default)")
     args = ap.parse_args()
@@ -176,7 +178,16 @@ def main():
     P = case.phase_cells
     log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
         f"pairs={case.topo['n_pairs']} P={P}")
+    tuned = None
+    if args.autotune:
+        from kitamr_hip import autotune
+        t1 = time.perf_counter()
+        tuned, tms, trials = autotune.autotune(case, steps=3, warmup=1, log=log)
+        log(f"[bench] autotune ({len(trials)} trials, {time.perf_counter() - t1:.1f}s): {tuned} "
+            f"{tms:.3f} ms/step")
     dev = abi.HipContext(kah_params(case.config))
+    for k, v in (tuned or {}).items():
+        dev.set_option(k, v)
     if args.flux_waves:
         dev.set_option("flux_waves", args.flux_waves)
     if args.slope_vpt:
@@ -241,6 +252,7 @@ def main():
                          "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
                          "uniform 48x48 velocity grid, CAIDVM + CAIDVM_Marching"),
             "precision": args.precision,
+            "launch_options": tuned or "library defaults",
             "leaves": case.n_cells,
             "phase_cells_per_step": P,
             "parallelism": "single GPU",

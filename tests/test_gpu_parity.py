@@ -152,3 +152,15 @@ def test_graph_replay_equals_launches(name):
             assert np.array_equal(ga[k], gb[k]), (s, k)
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+def test_autotune_on_device_picks_a_timed_setting():
+    """The autotuner times real steps on the device and returns one of the settings it
+    timed, the fastest of them."""
+    from kitamr_hip import autotune
+    case = CASES["riemann8_amr1_nv16"]()
+    say(f"[autotune] cells={case.n_cells} P={case.phase_cells}")
+    best, ms, trials = autotune.autotune(case, steps=2, warmup=1, log=say)
+    assert ms > 0 and best in [t[0] for t in trials]
+    assert ms == min(t[1] for t in trials)
