@@ -708,13 +708,12 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   C->phase_cells = 0;
   for (auto* b : C->bins) delete b;
   C->bins.clear();
-  int Lmin = 1 << 30, Lmax = -1;
+  int Lmin = 1 << 30;
   for (int64_t c = 0; c < nc; ++c) {
     if (T->bound_enc[c] < 0) continue;
     C->fluid.push_back(int32_t(c));
     C->phase_cells += T->vs_offset[c + 1] - T->vs_offset[c];
     Lmin = std::min(Lmin, int(T->ps_level[c]));
-    Lmax = std::max(Lmax, int(T->ps_level[c]));
   }
   // Per level and half-flux count: cells owning a wall half need the whole-cell wall density
   // and run the per-cell slope kernel (one partial row per half flux); all others run the
@@ -730,12 +729,44 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     const int64_t cv = int64_t(C->slope_vpt) * SLOPE_CHUNK;
     return int((T->vs_offset[c + 1] - T->vs_offset[c] + cv - 1) / cv);
   };
-  for (int L = Lmin; L <= Lmax; ++L) {
+  // Launch order of the slopes: the reference sweeps coarse to fine levels because the
+  // transverse projection (update_slope_transverse_level!, src/Flux/Slope.jl:849-945) reads
+  // the final slopes of coarser neighbours. Each cell gets the depth of that dependency (0 when
+  // its stencil projects no other cell's slopes, else 1 + the depth of those neighbours), and
+  // the launches go by depth: every cell still runs after the cells it reads, and the bulk of
+  // the mesh (depth 0) is one launch.
+  std::vector<int> depth(size_t(nc), 0);
+  int Dmax = 0;
+  {
+    std::vector<int32_t> order(C->fluid);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return T->ps_level[a] < T->ps_level[b]; });
+    bool by_level = false;
+    for (int32_t c : order) {
+      int dc = 0;
+      for (int d = 0; d < D; ++d) {
+        const SlopeDesc& sd = B.sl[size_t(c) * D + d];
+        const int sides = sd.mode == SL_INNER ? 2 : (sd.mode == SL_BOUND ? 1 : 0);
+        for (int sx = 0; sx < sides; ++sx)
+          for (int e = 0; e < sd.nb_cnt[sx]; ++e) {
+            const SlopeNb& nbr = B.slnb[size_t(sd.nb_beg[sx] + e)];
+            if (!nbr.proj) continue;
+            if (T->ps_level[nbr.cell] >= T->ps_level[c]) by_level = true;
+            dc = std::max(dc, 1 + depth[size_t(nbr.cell)]);
+          }
+      }
+      depth[size_t(c)] = dc;
+    }
+    if (by_level)   // a projection from a level that is not coarser: keep the level order
+      for (int32_t c : C->fluid) depth[size_t(c)] = T->ps_level[c] - Lmin;
+    for (int32_t c : C->fluid) Dmax = std::max(Dmax, depth[size_t(c)]);
+  }
+  for (int L = 0; L <= Dmax; ++L) {
     const int caps[4] = {4, 8, 16, 24};
     std::vector<int32_t> lists[4];
     std::vector<int2> works[4];
     for (int32_t c : C->fluid) {
-      if (T->ps_level[c] != L) continue;
+      if (depth[size_t(c)] != L) continue;
       int nh = hoff[c + 1] - hoff[c];
       int b = nh <= 4 ? 0 : (nh <= 8 ? 1 : (nh <= 16 ? 2 : 3));
       if (walled[c]) {
