@@ -158,6 +158,29 @@ int kah_get_state(kah_ctx* ctx, double* w, double* prim, double* qf, double* sw,
  * after kah_iterate / kah_step, as after the reference's iterate!. */
 int kah_get_flux(kah_ctx* ctx, double* flux, double* vs_flux);
 
+/* Ghost-exchange buffers of a list of cells, in the reference's wire format, one buffer per
+ * listed cell concatenated in list order (a cell may be listed more than once):
+ *   KAH_MIRROR_DATA : [ds(dim) midpoint(dim) w(dim+2) vs_num bound_enc df(vs_num*ndf)]
+ *                     (size_Ghost_Data; solid cells: w = Inf, vs_num = 0, no df)
+ *   KAH_MIRROR_SLOPE: [sdf(vs_num*ndf*dim) sw((dim+2)*dim)], arrays column-major
+ *                     (size_Ghost_Slope; solid cells: [EPS 0 ...])
+ * replacing get_mirror_data / get_mirror_slope and update_mirror_data! /
+ * update_mirror_slope! (src/Parallel/Ghost.jl:525-579, :757-808) on the sending side and the
+ * ghost wrap of the receiving side (src/Parallel/Ghost.jl:688-749). */
+enum { KAH_MIRROR_DATA = 0, KAH_MIRROR_SLOPE = 1 };
+/* offsets[0..n]: start of each cell's buffer in doubles; offsets[n] is the total length. */
+int kah_mirror_layout(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind,
+                      int64_t* offsets);
+/* Packs the buffers of the listed cells from the device state into out (host memory,
+ * offsets[n] doubles). SLOPE buffers carry sw recomputed from the current sdf. */
+int kah_pack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind, double* out);
+/* Writes w and df (DATA) or sdf and sw (SLOPE) of the listed fluid cells from in (host memory,
+ * the layout above); solid cells are skipped. DATA buffers are checked first: a solid buffer
+ * (w = Inf) for a fluid cell, the converse, or a vs_num other than the cell's velocity-cell
+ * count is KAH_EINVAL and nothing is written. FP32-f contexts round df and sdf to float. */
+int kah_unpack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind,
+                       const double* in);
+
 /* ---- the per-step path --------------------------------------------------------------- */
 /* slope!(p4est,ka), src/Flux/Slope.jl:1047-1070: level-ordered velocity-space slopes
  * with the transverse correction on coarse-neighbour faces. dt is status.Δt of the step

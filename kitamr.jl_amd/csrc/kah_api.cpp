@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/kitamr_hip.h"
@@ -75,6 +76,10 @@ struct kah_ctx {
   DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
   DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
   DBuf d_vtb, d_ovmap, d_moff;
+  // ghost-exchange buffers: cell midpoints and bound_enc on the device, and the staging of
+  // kah_pack_mirrors / kah_unpack_mirrors (grown on demand)
+  std::vector<int64_t> voff_h;
+  DBuf d_cmid, d_benc, d_xcells, d_xoff, d_xbuf, d_xfluid;
   bool ov_enable = true;   // overlap maps for adapted grids
   int64_t part_rows = 0;
   int cur = 0;
@@ -113,7 +118,7 @@ struct kah_ctx {
                           &d_sdf, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
                           &d_res, &d_scratch, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
                           &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
-                          &d_moff};
+                          &d_moff, &d_cmid, &d_benc, &d_xcells, &d_xoff, &d_xbuf, &d_xfluid};
     int64_t t = 0;
     for (const DBuf* b : bufs) t += int64_t(b->bytes);
     for (const LevelBin* b : bins) t += int64_t(b->items.bytes);
@@ -140,6 +145,11 @@ struct kah_ctx {
     }
     b.bytes = bytes;
     return 0;
+  }
+  // grow-only allocation (contents not kept)
+  int reserve(DBuf& b, size_t bytes, const char* what) {
+    if (b.p && b.bytes >= bytes) return 0;
+    return alloc(b, bytes, what);
   }
   template <class T>
   int upload(DBuf& b, const T* src, size_t count, const char* what) {
@@ -699,6 +709,9 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   KAH_TRY(C->upload(C->d_eoff, eoff.data(), eoff.size(), "entry offsets"));
   KAH_TRY(C->upload(C->d_ents, ents.data(), ents.size(), "entry descriptors"));
   KAH_TRY(C->upload(C->d_recds, T->cell_ds, size_t(nc * D), "cell ds"));
+  KAH_TRY(C->upload(C->d_cmid, T->cell_mid, size_t(nc * D), "cell midpoints"));
+  KAH_TRY(C->upload(C->d_benc, T->bound_enc, size_t(nc), "bound_enc"));
+  C->voff_h.assign(T->vs_offset, T->vs_offset + nc + 1);
   std::vector<double> bc(size_t(std::max(T->n_bc, 1) * C->NW), 0.0);
   if (T->n_bc > 0) std::memcpy(bc.data(), T->bc_prim, sizeof(double) * T->n_bc * C->NW);
   KAH_TRY(C->upload(C->d_bc, bc.data(), bc.size(), "bc prim"));
@@ -1014,6 +1027,122 @@ int kah_get_state(kah_ctx* C, double* w, double* prim, double* qf, double* sw, d
   KAH_TRY(C->download_f(df, C->d_df[C->cur], size_t(C->np * C->NDF), "df download"));
   KAH_TRY(C->download_f(sdf, C->d_sdf, size_t(C->np * C->NDF * D), "sdf download"));
   return C->hip(hipStreamSynchronize(C->stream), "state download");
+}
+
+int kah_mirror_layout(kah_ctx* C, const int32_t* cells, int64_t n, int kind, int64_t* offsets) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_topo) return C->fail(KAH_ESTATE, "no topology: call kah_set_topology first");
+  if (kind != KAH_MIRROR_DATA && kind != KAH_MIRROR_SLOPE)
+    return C->fail(KAH_EINVAL, "unknown mirror buffer kind " + std::to_string(kind));
+  if (n < 0 || n >= (int64_t(1) << 31)) return C->fail(KAH_EINVAL, "mirror count out of range");
+  if (!offsets || (n > 0 && !cells)) return C->fail(KAH_EINVAL, "null mirror array");
+  const int64_t D = C->D, NW = C->NW, NDF = C->NDF;
+  offsets[0] = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    const int32_t c = cells[k];
+    if (c < 0 || c >= C->nc) return C->fail(KAH_EINVAL, "mirror cell id out of range");
+    const int64_t nv = C->benc[c] < 0 ? 0 : C->voff_h[c + 1] - C->voff_h[c];
+    // size_Ghost_Data / size_Ghost_Slope (src/Parallel/Ghost.jl:1-2)
+    const int64_t len = kind == KAH_MIRROR_DATA ? 3 * D + 4 + NDF * nv
+                                                 : std::max<int64_t>(nv * NDF * D + D * NW, 1);
+    offsets[k + 1] = offsets[k] + len;
+  }
+  return KAH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// shared part of kah_pack_mirrors / kah_unpack_mirrors: offsets, cell list and staging buffer
+// on the device
+int mirror_prepare(kah_ctx* C, const int32_t* cells, int64_t n, int kind,
+                   std::vector<int64_t>& off) {
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  off.resize(size_t(n) + 1);
+  KAH_TRY(kah_mirror_layout(C, cells, n, kind, off.data()));
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  if (n == 0) return KAH_OK;
+  KAH_TRY(C->reserve(C->d_xcells, sizeof(int32_t) * size_t(n), "mirror cells"));
+  KAH_TRY(C->reserve(C->d_xoff, sizeof(int64_t) * size_t(n), "mirror offsets"));
+  KAH_TRY(C->reserve(C->d_xbuf, sizeof(double) * size_t(off[n]), "mirror buffer"));
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_xcells.p, cells, sizeof(int32_t) * size_t(n),
+                                hipMemcpyHostToDevice, C->stream), "mirror cells upload"));
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_xoff.p, off.data(), sizeof(int64_t) * size_t(n),
+                                hipMemcpyHostToDevice, C->stream), "mirror offsets upload"));
+  if (!C->d_sw.p) KAH_TRY(C->alloc(C->d_sw, sizeof(double) * C->nc * C->NW * C->D, "sw"));
+  return KAH_OK;
+}
+
+int mirror_launch(kah_ctx* C, int64_t n, int kind, int unpack) {
+  DevArgs A = make_args(C, 0.0);
+  return C->hip(launch_mirror(A, C->D, C->uni, C->f32, C->d_xcells.as<int32_t>(),
+                              C->d_xoff.as<int64_t>(), int(n), kind, unpack,
+                              C->d_xbuf.as<double>(), C->d_cmid.as<double>(),
+                              C->d_benc.as<int32_t>(), C->d_sw.as<double>(),
+                              C->d_df[C->cur].p, C->stream),
+                "mirror kernel launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int kah_pack_mirrors(kah_ctx* C, const int32_t* cells, int64_t n, int kind, double* out) {
+  if (!C) return KAH_EINVAL;
+  std::vector<int64_t> off;
+  KAH_TRY(mirror_prepare(C, cells, n, kind, off));
+  if (n == 0) return KAH_OK;
+  if (!out) return C->fail(KAH_EINVAL, "null mirror buffer");
+  if (kind == KAH_MIRROR_SLOPE) {
+    // PsData.sw of the listed fluid cells, the w-moments of sdf (src/Flux/Slope.jl:1022-1036)
+    std::vector<int32_t> fl;
+    for (int64_t k = 0; k < n; ++k)
+      if (C->benc[cells[k]] >= 0) fl.push_back(cells[k]);
+    if (!fl.empty()) {
+      KAH_TRY(C->reserve(C->d_xfluid, sizeof(int32_t) * fl.size(), "mirror fluid cells"));
+      KAH_TRY(C->hip(hipMemcpyAsync(C->d_xfluid.p, fl.data(), sizeof(int32_t) * fl.size(),
+                                    hipMemcpyHostToDevice, C->stream), "mirror cells upload"));
+      DevArgs A = make_args(C, 0.0);
+      KAH_TRY(C->hip(launch_macro_slope(A, C->D, C->uni, C->f32, C->d_xfluid.as<int32_t>(),
+                                        int(fl.size()), C->d_sw.as<double>(), C->stream),
+                     "macro slope launch"));
+    }
+  }
+  KAH_TRY(mirror_launch(C, n, kind, 0));
+  KAH_TRY(C->hip(hipMemcpyAsync(out, C->d_xbuf.p, sizeof(double) * size_t(off[n]),
+                                hipMemcpyDeviceToHost, C->stream), "mirror buffer download"));
+  return C->hip(hipStreamSynchronize(C->stream), "mirror pack");
+}
+
+int kah_unpack_mirrors(kah_ctx* C, const int32_t* cells, int64_t n, int kind, const double* in) {
+  if (!C) return KAH_EINVAL;
+  std::vector<int64_t> off;
+  KAH_TRY(mirror_prepare(C, cells, n, kind, off));
+  if (n == 0) return KAH_OK;
+  if (!in) return C->fail(KAH_EINVAL, "null mirror buffer");
+  if (kind == KAH_MIRROR_DATA) {
+    // the receiving side tells a solid ghost by w[1] == Inf (src/Parallel/Ghost.jl:725-728)
+    const int64_t D = C->D;
+    for (int64_t k = 0; k < n; ++k) {
+      const double* b = in + off[k];
+      const bool solid = C->benc[cells[k]] < 0;
+      if ((b[2 * D] == std::numeric_limits<double>::infinity()) != solid)
+        return C->fail(KAH_EINVAL, "mirror buffer " + std::to_string(k) +
+                                       (solid ? " holds fluid data for a solid cell"
+                                              : " holds solid data for a fluid cell"));
+      const int64_t nv = C->voff_h[cells[k] + 1] - C->voff_h[cells[k]];
+      if (!solid && b[3 * D + 2] != double(nv))
+        return C->fail(KAH_EINVAL, "mirror buffer " + std::to_string(k) + ": vs_num " +
+                                       std::to_string(b[3 * D + 2]) + " != " +
+                                       std::to_string(nv) + " velocity cells of cell " +
+                                       std::to_string(cells[k]));
+    }
+  }
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_xbuf.p, in, sizeof(double) * size_t(off[n]),
+                                hipMemcpyHostToDevice, C->stream), "mirror buffer upload"));
+  KAH_TRY(mirror_launch(C, n, kind, 1));
+  return C->hip(hipStreamSynchronize(C->stream), "mirror unpack");
 }
 
 int kah_get_flux(kah_ctx* C, double* flux, double* vs_flux) {

@@ -1265,6 +1265,87 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   }
 }
 
+// ------------------------------------------------------------------ mirror (ghost) buffers
+// The per-cell buffers of the ghost exchange, concatenated in list order at buf + off[k]:
+// DATA (kind 0) is [ds(D) midpoint(D) w(D+2) vs_num bound_enc df(vs_num*NDF)], SLOPE (kind 1)
+// is [sdf(vs_num*NDF*D) sw(D*(D+2))], every array column-major as the reference reshapes it
+// (src/Parallel/Ghost.jl:1-3, :42-55, :525-579). A cell inside a solid (bound_enc < 0) packs
+// as [ds midpoint Inf*(D+2) 0 bound_enc] and [EPS 0 ...]. Unpack (the ghost side,
+// src/Parallel/Ghost.jl:688-749) writes w and df, or sdf and sw, of each listed fluid cell.
+// One workgroup per listed cell.
+template <int D, int NDF, bool UNI, class FT>
+__global__ __launch_bounds__(256) void k_mirror(DevArgs A, const int32_t* __restrict__ cells,
+                                                const int64_t* __restrict__ off, int kind,
+                                                int unpack, double* __restrict__ buf,
+                                                const double* __restrict__ mid,
+                                                const int32_t* __restrict__ benc,
+                                                double* __restrict__ sw, FT* __restrict__ dfw) {
+  constexpr int NW = D + 2;
+  const int c = cells[blockIdx.x];
+  const bool solid = benc[c] < 0;
+  double* b = buf + off[blockIdx.x];
+  if (solid) {
+    if (unpack || threadIdx.x != 0) return;
+    if (kind == 0) {
+      for (int d = 0; d < D; ++d) {
+        b[d] = A.rec_ds[int64_t(c) * D + d];
+        b[D + d] = mid[int64_t(c) * D + d];
+      }
+      for (int q = 0; q < NW; ++q) b[2 * D + q] = __builtin_inf();
+      b[3 * D + 2] = 0.0;
+      b[3 * D + 3] = double(benc[c]);
+    } else {
+      b[0] = 1e-12;   // EPS, src/Abstract/Types.jl:3
+      for (int x = 1; x < D * NW; ++x) b[x] = 0.0;
+    }
+    return;
+  }
+  const Grid<D, NDF, UNI, FT> G(A, c);
+  const int n = G.n;
+  if (kind == 0) {
+    if (threadIdx.x == 0) {
+      if (!unpack) {
+        for (int d = 0; d < D; ++d) {
+          b[d] = A.rec_ds[int64_t(c) * D + d];
+          b[D + d] = mid[int64_t(c) * D + d];
+        }
+        for (int q = 0; q < NW; ++q) b[2 * D + q] = A.w[int64_t(c) * NW + q];
+        b[3 * D + 2] = double(n);
+        b[3 * D + 3] = double(benc[c]);
+      } else {
+        for (int q = 0; q < NW; ++q) A.w[int64_t(c) * NW + q] = b[2 * D + q];
+      }
+    }
+    double* fb = b + 3 * D + 4;
+    FT* fo = dfw + G.off * NDF;
+    for (int x = threadIdx.x; x < NDF * n; x += 256) {
+      if (!unpack) fb[x] = double(G.at(G.fb, x));
+      else G.at(fo, x) = FT(fb[x]);
+    }
+  } else {
+    for (int x = threadIdx.x; x < NDF * D * n; x += 256) {
+      if (!unpack) b[x] = double(G.at(G.sb, x));
+      else G.at(G.swb, x) = FT(b[x]);
+    }
+    double* sb = b + int64_t(NDF) * D * n;
+    for (int x = threadIdx.x; x < D * NW; x += 256) {
+      const int d = x / NW, q = x - d * NW;
+      if (!unpack) sb[x] = sw[(int64_t(c) * NW + q) * D + d];
+      else sw[(int64_t(c) * NW + q) * D + d] = sb[x];
+    }
+  }
+}
+
+template <int D, int NDF, bool UNI, class FT>
+inline hipError_t launch_mirror_t(const DevArgs& A, const int32_t* cells, const int64_t* off,
+                                  int n, int kind, int unpack, double* buf, const double* mid,
+                                  const int32_t* benc, double* sw, void* dfw, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_mirror<D, NDF, UNI, FT>), dim3(n), dim3(256), 0, st, A, cells, off, kind,
+                     unpack, buf, mid, benc, sw, static_cast<FT*>(dfw));
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ K3: macro slope
 template <int D, int NDF, bool UNI, class FT>
 __global__ __launch_bounds__(256) void k_macro_slope(DevArgs A, const int32_t* __restrict__ cells,

@@ -95,6 +95,12 @@ hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, cons
                              int n, int mode, int march, size_t lds, int waves_per_block,
                              hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
+// packs (unpack = 0) or unpacks the ghost-exchange buffers of the listed cells (kind 0: data,
+// 1: slope), concatenated at buf + off[k]
+hipError_t launch_mirror(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                         const int64_t* off, int n, int kind, int unpack, double* buf,
+                         const double* mid, const int32_t* benc, double* sw, void* dfw,
+                         hipStream_t st);
 // fills the overlap maps of the given (cell, neighbour) jobs (adapted grids)
 hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,
                               uint32_t* map, hipStream_t st);

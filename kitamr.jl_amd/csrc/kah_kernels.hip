@@ -11,6 +11,14 @@
 namespace kah {
 
 hipError_t launch_overlap_map_2d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
+#define KAH_MIRROR_DECL(name)                                                                 \
+  hipError_t name(const DevArgs&, bool, const int32_t*, const int64_t*, int, int, int, double*, \
+                  const double*, const int32_t*, double*, void*, hipStream_t);
+KAH_MIRROR_DECL(launch_mirror_2d)
+KAH_MIRROR_DECL(launch_mirror_3d)
+KAH_MIRROR_DECL(launch_mirror_2d_f32)
+KAH_MIRROR_DECL(launch_mirror_3d_f32)
+#undef KAH_MIRROR_DECL
 hipError_t launch_overlap_map_3d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
 hipError_t launch_slope_2d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
 hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
@@ -76,6 +84,21 @@ hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, cons
                     : launch_flux_relax_3d_f32(A, uni, cells, n, mode, march, lds, nwv, st);
   return dim == 2 ? launch_flux_relax_2d(A, uni, cells, n, mode, march, lds, nwv, st)
                   : launch_flux_relax_3d(A, uni, cells, n, mode, march, lds, nwv, st);
+}
+
+hipError_t launch_mirror(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                         const int64_t* off, int n, int kind, int unpack, double* buf,
+                         const double* mid, const int32_t* benc, double* sw, void* dfw,
+                         hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_mirror_2d_f32(A, uni, cells, off, n, kind, unpack, buf, mid, benc, sw,
+                                           dfw, st)
+                    : launch_mirror_3d_f32(A, uni, cells, off, n, kind, unpack, buf, mid, benc, sw,
+                                           dfw, st);
+  return dim == 2 ? launch_mirror_2d(A, uni, cells, off, n, kind, unpack, buf, mid, benc, sw, dfw,
+                                     st)
+                  : launch_mirror_3d(A, uni, cells, off, n, kind, unpack, buf, mid, benc, sw, dfw,
+                                     st);
 }
 
 hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,

@@ -26,4 +26,14 @@ hipError_t launch_overlap_map_3d(const DevArgs& A, const OvJob* jobs, int njobs,
   return launch_overlap_map_t<3, 1>(A, jobs, njobs, map, st);
 }
 
+hipError_t launch_mirror_3d(const DevArgs& A, bool uni, const int32_t* cells,
+                              const int64_t* off, int n, int kind, int unpack, double* buf,
+                              const double* mid, const int32_t* benc, double* sw, void* dfw,
+                              hipStream_t st) {
+  return uni ? launch_mirror_t<3, 1, true, double>(A, cells, off, n, kind, unpack, buf, mid,
+                                                     benc, sw, dfw, st)
+             : launch_mirror_t<3, 1, false, double>(A, cells, off, n, kind, unpack, buf, mid,
+                                                      benc, sw, dfw, st);
+}
+
 }  // namespace kah

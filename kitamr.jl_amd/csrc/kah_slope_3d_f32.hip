@@ -21,4 +21,14 @@ hipError_t launch_macro_slope_3d_f32(const DevArgs& A, bool uni, const int32_t* 
              : launch_macro_slope_t<3, 1, false, float>(A, cells, n, sw, st);
 }
 
+hipError_t launch_mirror_3d_f32(const DevArgs& A, bool uni, const int32_t* cells,
+                              const int64_t* off, int n, int kind, int unpack, double* buf,
+                              const double* mid, const int32_t* benc, double* sw, void* dfw,
+                              hipStream_t st) {
+  return uni ? launch_mirror_t<3, 1, true, float>(A, cells, off, n, kind, unpack, buf, mid,
+                                                     benc, sw, dfw, st)
+             : launch_mirror_t<3, 1, false, float>(A, cells, off, n, kind, unpack, buf, mid,
+                                                      benc, sw, dfw, st);
+}
+
 }  // namespace kah

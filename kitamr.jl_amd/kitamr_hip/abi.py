@@ -25,7 +25,9 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "devi
 EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "kah_set_state",
            "kah_get_state", "kah_get_flux", "kah_slope", "kah_flux", "kah_iterate", "kah_step",
            "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
-           "kah_copy_bandwidth", "kah_set_option"]
+           "kah_copy_bandwidth", "kah_set_option", "kah_mirror_layout", "kah_pack_mirrors",
+           "kah_unpack_mirrors"]
+MIRROR = {"data": 0, "slope": 1}
 OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5}
 
 
@@ -144,9 +146,14 @@ def load_library(path=None):
     lib.kah_copy_bandwidth.restype = C.c_int
     lib.kah_set_option.argtypes = [C.c_void_p, C.c_int, i64]
     lib.kah_set_option.restype = C.c_int
+    ip = C.POINTER(C.c_int32)
+    lib.kah_mirror_layout.argtypes = [C.c_void_p, ip, i64, C.c_int, C.POINTER(i64)]
+    lib.kah_pack_mirrors.argtypes = [C.c_void_p, ip, i64, C.c_int, dp]
+    lib.kah_unpack_mirrors.argtypes = [C.c_void_p, ip, i64, C.c_int, dp]
     for name in ["kah_set_topology", "kah_set_state", "kah_get_state", "kah_get_flux",
                  "kah_slope", "kah_flux", "kah_iterate", "kah_step", "kah_sync",
-                 "kah_kernel_times"]:
+                 "kah_kernel_times", "kah_mirror_layout", "kah_pack_mirrors",
+                 "kah_unpack_mirrors"]:
         getattr(lib, name).restype = C.c_int
     del i32
     if path is None:
@@ -233,6 +240,36 @@ class HipContext:
         vflux = np.empty(self.np * self.ndf)
         self._check(self.lib.kah_get_flux(self.h, _ptr(flux), _ptr(vflux)), "kah_get_flux")
         return flux, vflux
+
+    def _cells(self, cells):
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        return c, c.ctypes.data_as(C.POINTER(C.c_int32))
+
+    def mirror_layout(self, cells, kind):
+        """Offsets (n+1, in doubles) of the listed cells' ghost-exchange buffers."""
+        c, cp = self._cells(cells)
+        off = np.empty(len(c) + 1, dtype=np.int64)
+        self._check(self.lib.kah_mirror_layout(self.h, cp, len(c), MIRROR[kind],
+                                               off.ctypes.data_as(C.POINTER(C.c_int64))),
+                    "kah_mirror_layout")
+        return off
+
+    def pack_mirrors(self, cells, kind):
+        """Concatenated DATA or SLOPE buffers of the listed cells (get_mirror_data /
+        get_mirror_slope, src/Parallel/Ghost.jl:525-579) and their offsets."""
+        off = self.mirror_layout(cells, kind)
+        c, cp = self._cells(cells)
+        out = np.empty(int(off[-1]))
+        self._check(self.lib.kah_pack_mirrors(self.h, cp, len(c), MIRROR[kind], _ptr(out)),
+                    "kah_pack_mirrors")
+        return out, off
+
+    def unpack_mirrors(self, cells, kind, buf):
+        """Writes the listed cells' state from concatenated DATA or SLOPE buffers."""
+        c, cp = self._cells(cells)
+        b = np.ascontiguousarray(buf, dtype=np.float64)
+        self._check(self.lib.kah_unpack_mirrors(self.h, cp, len(c), MIRROR[kind], _ptr(b)),
+                    "kah_unpack_mirrors")
 
     def slope(self, dt):
         self._check(self.lib.kah_slope(self.h, float(dt)), "kah_slope")
