@@ -181,6 +181,21 @@ int kah_pack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind, do
 int kah_unpack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind,
                        const double* in);
 
+/* Per-rank phase-space arrays of a restart checkpoint, as save_for_restart collects them
+ * (src/IO/Restart.jl:80-117) and _restart_build_ps_list reads them (:306-321): N = n_cells,
+ * Nv = sum of vs_num over the cells outside solids; matrices column-major:
+ *   vs_nums [N] (0 inside a solid), bound_encs [N], ws [N x (dim+2)] (0 inside a solid),
+ *   vs_levels [Nv], vs_midpoints [Nv x dim], vs_df [Nv x ndf].
+ * The container (JLD2 files, the p4est forest, configuration and status) stays with the host. */
+int kah_restart_sizes(kah_ctx* ctx, int64_t* n_cells, int64_t* n_vs);
+int kah_save_restart(kah_ctx* ctx, int64_t* vs_nums, int64_t* bound_encs, double* ws,
+                     int8_t* vs_levels, double* vs_midpoints, double* vs_df);
+/* Sets the state from restart arrays of the same topology (vs_nums and bound_encs must match
+ * it, else KAH_EINVAL and nothing is written): w = ws, prim = get_prim(w), df = vs_df, slopes
+ * and flux accumulators zero. May replace kah_set_state. */
+int kah_load_restart(kah_ctx* ctx, const int64_t* vs_nums, const int64_t* bound_encs,
+                     const double* ws, const double* vs_df);
+
 /* ---- the per-step path --------------------------------------------------------------- */
 /* slope!(p4est,ka), src/Flux/Slope.jl:1047-1070: level-ordered velocity-space slopes
  * with the transverse correction on coarse-neighbour faces. dt is status.Δt of the step

@@ -1145,6 +1145,116 @@ int kah_unpack_mirrors(kah_ctx* C, const int32_t* cells, int64_t n, int kind, co
   return C->hip(hipStreamSynchronize(C->stream), "mirror unpack");
 }
 
+int kah_restart_sizes(kah_ctx* C, int64_t* n_cells, int64_t* n_vs) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_topo) return C->fail(KAH_ESTATE, "no topology: call kah_set_topology first");
+  int64_t nv = 0;
+  for (int64_t c = 0; c < C->nc; ++c)
+    if (C->benc[c] >= 0) nv += C->voff_h[c + 1] - C->voff_h[c];
+  if (n_cells) *n_cells = C->nc;
+  if (n_vs) *n_vs = nv;
+  return KAH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// first restart row of each cell (-1 inside a solid) and the row count
+int64_t restart_rows(const kah_ctx* C, std::vector<int64_t>& row) {
+  row.resize(size_t(C->nc));
+  int64_t r = 0;
+  for (int64_t c = 0; c < C->nc; ++c) {
+    if (C->benc[c] < 0) {
+      row[c] = -1;
+      continue;
+    }
+    row[c] = r;
+    r += C->voff_h[c + 1] - C->voff_h[c];
+  }
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kah_save_restart(kah_ctx* C, int64_t* vs_nums, int64_t* bound_encs, double* ws,
+                     int8_t* vs_levels, double* vs_midpoints, double* vs_df) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (!vs_nums || !bound_encs || !ws || !vs_levels || !vs_midpoints || !vs_df)
+    return C->fail(KAH_EINVAL, "null restart array");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  const int64_t N = C->nc, NW = C->NW, D = C->D, NDF = C->NDF;
+  std::vector<int64_t> row;
+  const int64_t Nv = restart_rows(C, row);
+  for (int64_t c = 0; c < N; ++c) {
+    vs_nums[c] = C->benc[c] < 0 ? 0 : C->voff_h[c + 1] - C->voff_h[c];
+    bound_encs[c] = C->benc[c];
+  }
+  DBuf d_row, d_ws, d_lev, d_mid, d_df;   // staging, freed on return
+  KAH_TRY(C->upload(d_row, row.data(), row.size(), "restart rows"));
+  KAH_TRY(C->alloc(d_ws, sizeof(double) * size_t(N * NW), "restart ws"));
+  KAH_TRY(C->alloc(d_lev, size_t(Nv), "restart vs_levels"));
+  KAH_TRY(C->alloc(d_mid, sizeof(double) * size_t(Nv * D), "restart vs_midpoints"));
+  KAH_TRY(C->alloc(d_df, sizeof(double) * size_t(Nv * NDF), "restart vs_df"));
+  KAH_TRY(C->zero(d_ws));   // cells inside a solid keep w = 0 (src/IO/Restart.jl:83)
+  DevArgs A = make_args(C, 0.0);
+  KAH_TRY(C->hip(launch_restart(A, C->D, C->uni, C->f32, d_row.as<int64_t>(), N, Nv, 0,
+                                d_ws.as<double>(), d_lev.as<int8_t>(), d_mid.as<double>(),
+                                d_df.as<double>(), C->d_df[C->cur].p, C->stream),
+                 "restart kernel launch"));
+  KAH_TRY(C->hip(hipMemcpyAsync(ws, d_ws.p, d_ws.bytes, hipMemcpyDeviceToHost, C->stream),
+                 "restart ws download"));
+  if (Nv > 0) {
+    KAH_TRY(C->hip(hipMemcpyAsync(vs_levels, d_lev.p, size_t(Nv), hipMemcpyDeviceToHost,
+                                  C->stream), "restart vs_levels download"));
+    KAH_TRY(C->hip(hipMemcpyAsync(vs_midpoints, d_mid.p, sizeof(double) * size_t(Nv * D),
+                                  hipMemcpyDeviceToHost, C->stream),
+                   "restart vs_midpoints download"));
+    KAH_TRY(C->hip(hipMemcpyAsync(vs_df, d_df.p, sizeof(double) * size_t(Nv * NDF),
+                                  hipMemcpyDeviceToHost, C->stream), "restart vs_df download"));
+  }
+  return C->hip(hipStreamSynchronize(C->stream), "restart save");
+}
+
+int kah_load_restart(kah_ctx* C, const int64_t* vs_nums, const int64_t* bound_encs,
+                     const double* ws, const double* vs_df) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_topo) return C->fail(KAH_ESTATE, "no topology: call kah_set_topology first");
+  if (!vs_nums || !bound_encs || !ws || !vs_df) return C->fail(KAH_EINVAL, "null restart array");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  const int64_t N = C->nc, NW = C->NW, NDF = C->NDF;
+  for (int64_t c = 0; c < N; ++c) {
+    const int64_t nv = C->benc[c] < 0 ? 0 : C->voff_h[c + 1] - C->voff_h[c];
+    if (vs_nums[c] != nv || bound_encs[c] != C->benc[c])
+      return C->fail(KAH_EINVAL, "restart arrays do not match the topology at cell " +
+                                     std::to_string(c) + " (vs_num " +
+                                     std::to_string(vs_nums[c]) + ", bound_enc " +
+                                     std::to_string(bound_encs[c]) + ")");
+  }
+  std::vector<int64_t> row;
+  const int64_t Nv = restart_rows(C, row);
+  DBuf d_row, d_ws, d_df;
+  KAH_TRY(C->upload(d_row, row.data(), row.size(), "restart rows"));
+  KAH_TRY(C->upload(d_ws, ws, size_t(N * NW), "restart ws"));
+  KAH_TRY(C->upload(d_df, vs_df, size_t(Nv * NDF), "restart vs_df"));
+  DevArgs A = make_args(C, 0.0);
+  KAH_TRY(C->hip(launch_restart(A, C->D, C->uni, C->f32, d_row.as<int64_t>(), N, Nv, 1,
+                                d_ws.as<double>(), nullptr, nullptr, d_df.as<double>(),
+                                C->d_df[C->cur].p, C->stream),
+                 "restart kernel launch"));
+  // the loaded state starts with zero slopes and flux accumulators (src/IO/Restart.jl:317-318)
+  KAH_TRY(C->zero(C->d_sdf));
+  KAH_TRY(C->zero(C->d_psflux));
+  KAH_TRY(C->zero(C->d_vflux));
+  KAH_TRY(C->zero(C->d_qf));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "restart load"));
+  C->have_state = true;
+  return KAH_OK;
+}
+
 int kah_get_flux(kah_ctx* C, double* flux, double* vs_flux) {
   if (!C) return KAH_EINVAL;
   if (!C->have_state) return C->fail(KAH_ESTATE, "no state");

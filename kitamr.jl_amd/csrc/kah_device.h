@@ -1336,6 +1336,61 @@ __global__ __launch_bounds__(256) void k_mirror(DevArgs A, const int32_t* __rest
   }
 }
 
+// ------------------------------------------------------------------ restart arrays
+// The per-rank phase-space arrays of a restart checkpoint (src/IO/Restart.jl:80-117, read back
+// at :306-321): ws (N x (D+2)), vs_levels (Nv), vs_midpoints (Nv x D) and vs_df (Nv x NDF),
+// every matrix column-major, the velocity cells of the fluid cells concatenated in cell order
+// (row[c] = first row of cell c, -1 inside a solid). Save gathers them from the device state;
+// load scatters ws and vs_df back and sets prim = get_prim(w) as PsData(...) does.
+template <int D, int NDF, bool UNI, class FT>
+__global__ __launch_bounds__(256) void k_restart(DevArgs A, const int64_t* __restrict__ row,
+                                                 int64_t N, int64_t Nv, int load,
+                                                 double* __restrict__ ws,
+                                                 int8_t* __restrict__ lev,
+                                                 double* __restrict__ vmid,
+                                                 double* __restrict__ vdf, FT* __restrict__ dfw) {
+  constexpr int NW = D + 2;
+  const int c = blockIdx.x;
+  const int64_t r0 = row[c];
+  if (r0 < 0) return;
+  const Grid<D, NDF, UNI, FT> G(A, c);
+  const int n = G.n;
+  if (threadIdx.x == 0) {
+    if (!load) {
+      for (int q = 0; q < NW; ++q) ws[q * N + c] = A.w[int64_t(c) * NW + q];
+    } else {
+      double w[NW], p[NW];
+      for (int q = 0; q < NW; ++q) w[q] = ws[q * N + c];
+      get_prim<D>(w, A.gamma, p);
+      for (int q = 0; q < NW; ++q) {
+        A.w[int64_t(c) * NW + q] = w[q];
+        A.prim[int64_t(c) * NW + q] = p[q];
+      }
+    }
+  }
+  FT* fo = dfw + G.off * NDF;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int64_t r = r0 + i;
+    if (!load) {
+      lev[r] = A.glev[G.goff() + i];
+      for (int d = 0; d < D; ++d) vmid[d * Nv + r] = G.u(i, d);
+      for (int k = 0; k < NDF; ++k) vdf[k * Nv + r] = G.f(i, k);
+    } else {
+      for (int k = 0; k < NDF; ++k) G.at(fo, k * n + i) = FT(vdf[k * Nv + r]);
+    }
+  }
+}
+
+template <int D, int NDF, bool UNI, class FT>
+inline hipError_t launch_restart_t(const DevArgs& A, const int64_t* row, int64_t N, int64_t Nv,
+                                   int load, double* ws, int8_t* lev, double* vmid, double* vdf,
+                                   void* dfw, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_restart<D, NDF, UNI, FT>), dim3(unsigned(N)), dim3(256), 0, st, A, row,
+                     N, Nv, load, ws, lev, vmid, vdf, static_cast<FT*>(dfw));
+  return hipGetLastError();
+}
+
 template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_mirror_t(const DevArgs& A, const int32_t* cells, const int64_t* off,
                                   int n, int kind, int unpack, double* buf, const double* mid,

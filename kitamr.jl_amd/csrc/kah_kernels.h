@@ -101,6 +101,11 @@ hipError_t launch_mirror(const DevArgs& A, int dim, bool uni, bool f32, const in
                          const int64_t* off, int n, int kind, int unpack, double* buf,
                          const double* mid, const int32_t* benc, double* sw, void* dfw,
                          hipStream_t st);
+// gathers (load = 0) or scatters the per-rank restart arrays of all cells; row[c] is the first
+// velocity row of cell c, -1 for a solid cell
+hipError_t launch_restart(const DevArgs& A, int dim, bool uni, bool f32, const int64_t* row,
+                          int64_t N, int64_t Nv, int load, double* ws, int8_t* lev, double* vmid,
+                          double* vdf, void* dfw, hipStream_t st);
 // fills the overlap maps of the given (cell, neighbour) jobs (adapted grids)
 hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,
                               uint32_t* map, hipStream_t st);

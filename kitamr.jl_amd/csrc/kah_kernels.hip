@@ -19,6 +19,14 @@ KAH_MIRROR_DECL(launch_mirror_3d)
 KAH_MIRROR_DECL(launch_mirror_2d_f32)
 KAH_MIRROR_DECL(launch_mirror_3d_f32)
 #undef KAH_MIRROR_DECL
+#define KAH_RESTART_DECL(name)                                                                \
+  hipError_t name(const DevArgs&, bool, const int64_t*, int64_t, int64_t, int, double*, int8_t*, \
+                  double*, double*, void*, hipStream_t);
+KAH_RESTART_DECL(launch_restart_2d)
+KAH_RESTART_DECL(launch_restart_3d)
+KAH_RESTART_DECL(launch_restart_2d_f32)
+KAH_RESTART_DECL(launch_restart_3d_f32)
+#undef KAH_RESTART_DECL
 hipError_t launch_overlap_map_3d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
 hipError_t launch_slope_2d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
 hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
@@ -99,6 +107,16 @@ hipError_t launch_mirror(const DevArgs& A, int dim, bool uni, bool f32, const in
                                      st)
                   : launch_mirror_3d(A, uni, cells, off, n, kind, unpack, buf, mid, benc, sw, dfw,
                                      st);
+}
+
+hipError_t launch_restart(const DevArgs& A, int dim, bool uni, bool f32, const int64_t* row,
+                          int64_t N, int64_t Nv, int load, double* ws, int8_t* lev, double* vmid,
+                          double* vdf, void* dfw, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_restart_2d_f32(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st)
+                    : launch_restart_3d_f32(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st);
+  return dim == 2 ? launch_restart_2d(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st)
+                  : launch_restart_3d(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st);
 }
 
 hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,

@@ -36,4 +36,13 @@ hipError_t launch_mirror_3d(const DevArgs& A, bool uni, const int32_t* cells,
                                                       benc, sw, dfw, st);
 }
 
+hipError_t launch_restart_3d(const DevArgs& A, bool uni, const int64_t* row, int64_t N,
+                               int64_t Nv, int load, double* ws, int8_t* lev, double* vmid,
+                               double* vdf, void* dfw, hipStream_t st) {
+  return uni ? launch_restart_t<3, 1, true, double>(A, row, N, Nv, load, ws, lev, vmid, vdf,
+                                                      dfw, st)
+             : launch_restart_t<3, 1, false, double>(A, row, N, Nv, load, ws, lev, vmid, vdf,
+                                                       dfw, st);
+}
+
 }  // namespace kah

@@ -26,7 +26,7 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_get_state", "kah_get_flux", "kah_slope", "kah_flux", "kah_iterate", "kah_step",
            "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
            "kah_copy_bandwidth", "kah_set_option", "kah_mirror_layout", "kah_pack_mirrors",
-           "kah_unpack_mirrors"]
+           "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart", "kah_load_restart"]
 MIRROR = {"data": 0, "slope": 1}
 OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5}
 
@@ -150,10 +150,15 @@ def load_library(path=None):
     lib.kah_mirror_layout.argtypes = [C.c_void_p, ip, i64, C.c_int, C.POINTER(i64)]
     lib.kah_pack_mirrors.argtypes = [C.c_void_p, ip, i64, C.c_int, dp]
     lib.kah_unpack_mirrors.argtypes = [C.c_void_p, ip, i64, C.c_int, dp]
+    lp = C.POINTER(i64)
+    lib.kah_restart_sizes.argtypes = [C.c_void_p, lp, lp]
+    lib.kah_save_restart.argtypes = [C.c_void_p, lp, lp, dp, C.POINTER(C.c_int8), dp, dp]
+    lib.kah_load_restart.argtypes = [C.c_void_p, lp, lp, dp, dp]
     for name in ["kah_set_topology", "kah_set_state", "kah_get_state", "kah_get_flux",
                  "kah_slope", "kah_flux", "kah_iterate", "kah_step", "kah_sync",
                  "kah_kernel_times", "kah_mirror_layout", "kah_pack_mirrors",
-                 "kah_unpack_mirrors"]:
+                 "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart",
+                 "kah_load_restart"]:
         getattr(lib, name).restype = C.c_int
     del i32
     if path is None:
@@ -270,6 +275,39 @@ class HipContext:
         b = np.ascontiguousarray(buf, dtype=np.float64)
         self._check(self.lib.kah_unpack_mirrors(self.h, cp, len(c), MIRROR[kind], _ptr(b)),
                     "kah_unpack_mirrors")
+
+    def save_restart(self):
+        """Per-rank restart arrays (save_for_restart, src/IO/Restart.jl:80-117) as numpy arrays
+        of the reference's logical shapes: ws (N, dim+2), vs_midpoints (Nv, dim),
+        vs_df (Nv, ndf)."""
+        n, nv = C.c_int64(), C.c_int64()
+        self._check(self.lib.kah_restart_sizes(self.h, C.byref(n), C.byref(nv)),
+                    "kah_restart_sizes")
+        N, Nv, nw = n.value, nv.value, self.dim + 2
+        a = dict(vs_nums=np.empty(N, np.int64), bound_encs=np.empty(N, np.int64),
+                 ws=np.empty(N * nw), vs_levels=np.empty(Nv, np.int8),
+                 vs_midpoints=np.empty(Nv * self.dim), vs_df=np.empty(Nv * self.ndf))
+        lp = lambda x: x.ctypes.data_as(C.POINTER(C.c_int64))
+        self._check(self.lib.kah_save_restart(
+            self.h, lp(a["vs_nums"]), lp(a["bound_encs"]), _ptr(a["ws"]),
+            a["vs_levels"].ctypes.data_as(C.POINTER(C.c_int8)), _ptr(a["vs_midpoints"]),
+            _ptr(a["vs_df"])), "kah_save_restart")
+        # column-major matrices -> numpy arrays of the same logical shape
+        a["ws"] = a["ws"].reshape(nw, N).T
+        a["vs_midpoints"] = a["vs_midpoints"].reshape(self.dim, Nv).T
+        a["vs_df"] = a["vs_df"].reshape(self.ndf, Nv).T
+        return a
+
+    def load_restart(self, arrays):
+        """Sets the state from restart arrays of this topology (logical shapes as returned by
+        save_restart)."""
+        vn = np.ascontiguousarray(arrays["vs_nums"], dtype=np.int64)
+        be = np.ascontiguousarray(arrays["bound_encs"], dtype=np.int64)
+        ws = np.ascontiguousarray(np.asarray(arrays["ws"], dtype=np.float64).T)
+        df = np.ascontiguousarray(np.asarray(arrays["vs_df"], dtype=np.float64).T)
+        lp = lambda x: x.ctypes.data_as(C.POINTER(C.c_int64))
+        self._check(self.lib.kah_load_restart(self.h, lp(vn), lp(be), _ptr(ws), _ptr(df)),
+                    "kah_load_restart")
 
     def slope(self, dt):
         self._check(self.lib.kah_slope(self.h, float(dt)), "kah_slope")
