@@ -124,6 +124,26 @@ __device__ inline double pick(const double (&a)[D], int d) {
   return r;
 }
 
+// A value every lane holds alike (read from LDS after a barrier) as a wave-uniform scalar:
+// kept in SGPRs instead of one VGPR pair per lane.
+__device__ inline double uniform(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(b));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(b >> 32));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+template <int D>
+__device__ inline MaxPre<D> uniform(const MaxPre<D>& m) {
+  MaxPre<D> r;
+#pragma unroll
+  for (int q = 0; q < D + 2; ++q) r.p[q] = uniform(m.p[q]);
+  r.a0 = uniform(m.a0);
+  r.two_l = uniform(m.two_l);
+  r.r2l = uniform(m.r2l);
+  r.s0 = uniform(m.s0);
+  return r;
+}
+
 // ------------------------------------------------------------------ grid access
 template <int D, int NDF, bool UNI, class FT>
 struct Grid {
@@ -1006,6 +1026,10 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   __shared__ double res[8];
   __shared__ double cell_sh[4 * NW];   // wnew | prim_c | prim | q
   __shared__ double hsum[MAX_ENTRIES * 2 * NW];   // the half fluxes of the cell's faces
+  // per-cell constants of the relaxation, computed once: the Maxwellian pre-factors of
+  // prim_c and of the moments of f*, tau(prim_c) and the relaxation weights
+  __shared__ MaxPre<D> pre_c, pre_r;
+  __shared__ double pre_t[3];
   extern __shared__ double lds[];
   double* sc = LDS ? lds : &A.scratch[G.off * (NDF + 1)];
 
@@ -1083,7 +1107,18 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     return;
   }
   __syncthreads();
-  const double dtV = res[0];
+  const double dtV = uniform(res[0]);
+  if (MARCH != MARCH_CIP && threadIdx.x == 0) {
+    // needed after the first reduction; wave 0 computes them before its share of pass 1
+    double pc[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) pc[q] = cell_sh[NW + q];
+    pre_c = max_pre<D>(pc, A.Pr);
+    const double tau = get_tau<D>(pc, A.mu, A.omega);
+    pre_t[0] = tau;
+    pre_t[1] = tau / (tau + dt);
+    pre_t[2] = dt / (tau + dt);
+  }
 
   // pass 1: flux of every velocity cell (gathered or from the accumulators)
   double mw[NW];
@@ -1170,17 +1205,15 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       get_prim<D>(wf, A.gamma, pr);
 #pragma unroll
       for (int q = 0; q < NW; ++q) cell_sh[2 * NW + q] = pr[q];
+      pre_r = max_pre<D>(pr, A.Pr);
     }
     __syncthreads();
     // pass 2: conservation correction f += M(prim_c) - M(prim); heat flux of the result
     {
-      double pc[NW], pr[NW];
+      double pc[NW];
+      const MaxPre<D> mpc = uniform<D>(pre_c), mpr = uniform<D>(pre_r);
 #pragma unroll
-      for (int q = 0; q < NW; ++q) {
-        pc[q] = cell_sh[NW + q];
-        pr[q] = cell_sh[2 * NW + q];
-      }
-      const MaxPre<D> mpc = max_pre<D>(pc, A.Pr), mpr = max_pre<D>(pr, A.Pr);
+      for (int q = 0; q < NW; ++q) pc[q] = mpc.p[q];
       for (int p = threadIdx.x; p < n; p += BS) {
         const int i = G.vi(p);
         double u[D], Mc[2], M[2];
@@ -1205,13 +1238,15 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   for (int m = 0; m < NQ; ++m) qs_red[m] = cell_sh[3 * NW + m];
   heat_finish<D>(qs_red, q);
 #pragma unroll
-  for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
-  const double tau = get_tau<D>(pc, A.mu, A.omega);
+  for (int m = 0; m < D; ++m) q[m] = uniform(q[m]);
+  const MaxPre<D> mpc = uniform<D>(pre_c);
+#pragma unroll
+  for (int m = 0; m < NW; ++m) pc[m] = mpc.p[m];
+  const double tau = uniform(pre_t[0]);
 
   if (MARCH == MARCH_CAIDVM) {
     // pass 3: g+ = M(prim_c) + Shakhov part; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
-    const double a1 = tau / (tau + dt), a2 = dt / (tau + dt);
-    const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
+    const double a1 = uniform(pre_t[1]), a2 = uniform(pre_t[2]);
     for (int p = threadIdx.x; p < n; p += BS) {
       const int i = G.vi(p);
       double u[D], Fc[2], Sp[2];
@@ -1230,8 +1265,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     }
   } else {
     // Euler: f = (f + dt/V*flux)*tau/(tau+dt) + dt/(tau+dt)*F with F = M + Shakhov(M)
-    const double tpd = tau + dt, a2 = dt / (tau + dt);
-    const MaxPre<D> mpc = max_pre<D>(pc, A.Pr);
+    const double tpd = tau + dt, a2 = uniform(pre_t[2]);
     for (int p = threadIdx.x; p < n; p += BS) {
       const int i = G.vi(p);
       double u[D], M[2], Sp[2];
