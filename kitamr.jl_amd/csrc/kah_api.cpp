@@ -834,7 +834,11 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   KAH_TRY(C->alloc(C->d_rho, sizeof(double) * std::max<int64_t>(C->npairs, 1), "wall density"));
   KAH_TRY(C->alloc(C->d_res, sizeof(double) * nc * 2 * NW, "residual"));
   C->d_vflux.release();
-  C->lds_bytes = sizeof(double) * size_t(NDF + 1) * size_t(C->max_n);
+  // f* and M(prim_c) of the cell in LDS, in the per-velocity arithmetic type (float in the
+  // FP32 form, except under CIP_Marching)
+  const size_t sc_elem =
+      (C->f32 && C->P.marching != KAH_MARCH_CIP) ? sizeof(float) : sizeof(double);
+  C->lds_bytes = sc_elem * size_t(NDF + 1) * size_t(C->max_n);
   if (C->lds_bytes > 64 * 1024) {
     C->lds_bytes = 0;
     KAH_TRY(C->alloc(C->d_scratch, sizeof(double) * C->np * (NDF + 1), "scratch"));

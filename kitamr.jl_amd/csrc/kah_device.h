@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kah_internal.h"
 #include "kah_kernels.h"
@@ -112,12 +113,12 @@ __device__ inline int xcd_remap(int i, int n) {
 // candidates the optimizer would otherwise fold the select chain back into an indexed load
 // from a stack copy of a (private memory); the empty asm makes each candidate an opaque
 // register value.
-template <int D>
-__device__ inline double pick(const double (&a)[D], int d) {
-  double r = a[0];
+template <int D, class T>
+__device__ inline T pick(const T (&a)[D], int d) {
+  T r = a[0];
 #pragma unroll
   for (int t = 1; t < D; ++t) {
-    double x = a[t];
+    T x = a[t];
     if constexpr (D > 2) __asm__("" : "+v"(x));
     r = (d == t) ? x : r;
   }
@@ -143,6 +144,18 @@ __device__ inline MaxPre<D> uniform(const MaxPre<D>& m) {
   r.s0 = uniform(m.s0);
   return r;
 }
+
+// Arithmetic type of the per-velocity work for a storage type: FP64 for FP64 storage, FP32
+// for the FP32 form (f stored and advected in FP32; moment sums, prim, tau, q and the
+// Maxwellian parameters in FP64).
+template <class FT>
+struct Arith {
+  using type = double;
+};
+template <>
+struct Arith<float> {
+  using type = float;
+};
 
 // ------------------------------------------------------------------ grid access
 template <int D, int NDF, bool UNI, class FT>
@@ -313,34 +326,40 @@ inline hipError_t launch_overlap_map_t(const DevArgs& A, const OvJob* jobs, int 
 // LIMITED is positivity_preserving_reconstruct (src/Flux/CAIDVM.jl:127-141); UNLIMITED the
 // plain reconstruction used next to solid cells and at domain walls/inflow (CAIDVM.jl:13-14,
 // :110-112); PLAIN f*vn (UniformOutflow, CAIDVM.jl:53-67).
-template <int D, int NDF>
-__device__ inline void micro_recon(int mode, const double (&f)[NDF], const double (&s)[D][NDF],
-                                   const double (&u)[D], double dt, const double* xrec,
+// T is the arithmetic type: double, or float in the FP32 advection form. There the
+// offset face midpoint - reconstruction point is formed in double before the rounding (it is
+// a difference of two coordinates of order one).
+template <int D, int NDF, class T>
+__device__ inline void micro_recon(int mode, const T (&f)[NDF], const T (&s)[D][NDF],
+                                   const T (&u)[D], T dt, const double* xrec,
                                    const double* dsrec, const double* fmid, int dir,
-                                   double (&out)[NDF]) {
-  const double vn = pick<D>(u, dir);
+                                   T (&out)[NDF]) {
+  const T vn = pick<D>(u, dir);
   if (mode == MI_PLAIN || mode == MI_SELF_PLAIN) {
 #pragma unroll
     for (int k = 0; k < NDF; ++k) out[k] = f[k] * vn;
     return;
   }
-  double dx[D];
+  T dx[D];
 #pragma unroll
-  for (int t = 0; t < D; ++t) dx[t] = fmid[t] - u[t] * dt - xrec[t];
+  for (int t = 0; t < D; ++t) {
+    if constexpr (sizeof(T) == 8) dx[t] = fmid[t] - u[t] * dt - xrec[t];
+    else dx[t] = T(fmid[t] - xrec[t]) - u[t] * dt;
+  }
 #pragma unroll
   for (int k = 0; k < NDF; ++k) {
-    double s_dx = dx[0] * s[0][k];
+    T s_dx = dx[0] * s[0][k];
 #pragma unroll
     for (int t = 1; t < D; ++t) s_dx = s_dx + dx[t] * s[t][k];
     if (mode == MI_LIMITED) {
-      double s_abs = dsrec[0] * fabs(s[0][k]);
+      T s_abs = T(dsrec[0]) * fabs(s[0][k]);
 #pragma unroll
-      for (int t = 1; t < D; ++t) s_abs = s_abs + dsrec[t] * fabs(s[t][k]);
-      const double a = f[k] - MACH_EPS, b = 0.5 * s_abs + EPS;
+      for (int t = 1; t < D; ++t) s_abs = s_abs + T(dsrec[t]) * fabs(s[t][k]);
+      const T a = f[k] - T(MACH_EPS), b = T(0.5) * s_abs + T(EPS);
       // b >= EPS > 0, so |a| >= b with b finite means |a/b| >= 1 and the limiter is 1
       // exactly; the quotient is only formed where it can be below 1 (or NaN)
-      double lim = 1.0;
-      if (!(fabs(a) >= b && b < HUGE_VAL)) lim = jl_min(fabs(a / b), 1.0);
+      T lim = T(1);
+      if (!(fabs(a) >= b && b < T(HUGE_VAL))) lim = jl_min(T(fabs(a / b)), T(1));
       out[k] = (f[k] + lim * s_dx) * vn;
     } else {
       out[k] = (f[k] + s_dx) * vn;
@@ -350,60 +369,70 @@ __device__ inline void micro_recon(int mode, const double (&f)[NDF], const doubl
 
 // InterpolatedOutflow incoming micro flux (CAIDVM.jl:71-94): the ghost state mirrored
 // through the face, reconstructed at the face from the cell's own slopes.
-template <int D, int NDF>
-__device__ inline void micro_interp(const double (&f)[NDF], const double (&s)[D][NDF],
-                                    const double (&u)[D], double dt, const double* xh,
-                                    const double* fmid, int dir, double (&out)[NDF]) {
+template <int D, int NDF, class T>
+__device__ inline void micro_interp(const T (&f)[NDF], const T (&s)[D][NDF], const T (&u)[D],
+                                    T dt, const double* xh, const double* fmid, int dir,
+                                    T (&out)[NDF]) {
   double xt[D], xhd[D];
 #pragma unroll
   for (int t = 0; t < D; ++t) {
     xt[t] = 2.0 * fmid[t] - xh[t];
     xhd[t] = xh[t];
   }
-  double ndx[D];
+  T ndx[D];
 #pragma unroll
-  for (int t = 0; t < D; ++t) ndx[t] = fmid[t] - u[t] * dt - xt[t];
-  const double shift = pick<D>(xt, dir) - pick<D>(xhd, dir);
-  const double vn = pick<D>(u, dir);
+  for (int t = 0; t < D; ++t) {
+    if constexpr (sizeof(T) == 8) ndx[t] = fmid[t] - u[t] * dt - xt[t];
+    else ndx[t] = T(fmid[t] - xt[t]) - u[t] * dt;
+  }
+  const T shift = T(pick<D>(xt, dir) - pick<D>(xhd, dir));
+  const T vn = pick<D>(u, dir);
 #pragma unroll
   for (int k = 0; k < NDF; ++k) {
-    double sk[D];
+    T sk[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) sk[t] = s[t][k];
-    double tdf = f[k] + shift * pick<D>(sk, dir);
-    double s_dx = ndx[0] * s[0][k];
+    T tdf = f[k] + shift * pick<D>(sk, dir);
+    T s_dx = ndx[0] * s[0][k];
 #pragma unroll
     for (int t = 1; t < D; ++t) s_dx = s_dx + ndx[t] * s[t][k];
     out[k] = (tdf + s_dx) * vn;
   }
 }
 
-// Maxwellian inflow state (bc prim, density rho for walls) times vn.
-template <int D, int NDF>
-__device__ inline void micro_maxwell(const double* bc, double rho, bool wall,
-                                     const double (&u)[D], double K, int dir,
-                                     double (&out)[NDF]) {
-  double p[D + 2];
+// Maxwellian inflow state (bc prim, density rho for walls) times vn (evaluated in double).
+template <int D, int NDF, class T>
+__device__ inline void micro_maxwell(const double* bc, double rho, bool wall, const T (&u)[D],
+                                     double K, int dir, T (&out)[NDF]) {
+  double p[D + 2], ud[D];
 #pragma unroll
   for (int m = 0; m < D + 2; ++m) p[m] = bc[m];
   if (wall) p[0] = rho;
-  double M[2];
-  maxwell<D>(u, p, K, M);
-  const double vn = pick<D>(u, dir);
 #pragma unroll
-  for (int k = 0; k < NDF; ++k) out[k] = M[k] * vn;
+  for (int t = 0; t < D; ++t) ud[t] = u[t];
+  double M[2];
+  maxwell<D>(ud, p, K, M);
+  const T vn = pick<D>(u, dir);
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) out[k] = T(M[k]) * vn;
 }
 
 // ------------------------------------------------------------------ K1: slopes + half fluxes
 // Velocity-space slope of one direction (update_slope!/update_slope_transverse_level!,
 // src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
 // taken as key-interval overlaps.
-template <int D, int NDF, bool UNI, class FT>
+template <class T>
+__device__ inline T div_ds(T a, double ds, double rds) {
+  if constexpr (sizeof(T) == 8) return div_by(a, ds, rds);
+  else return a * T(rds);
+}
+
+template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int cnt,
                                 double dsx, double rdsx, int i, int li, uint32_t ki,
-                                const double (&f)[NDF], double (&a)[NDF]) {
+                                const T (&f)[NDF], T (&a)[NDF]) {
 #pragma unroll
-  for (int k = 0; k < NDF; ++k) a[k] = 0.0;
+  for (int k = 0; k < NDF; ++k) a[k] = T(0);
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
     const Grid<D, NDF, UNI, FT> N(A, nb.cell);
@@ -411,53 +440,53 @@ __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int
                                    [&](int j, double den, double rden) {
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
-        double pj = N.f(j, k);
+        T pj = T(N.f(j, k));
         if (nb.proj) {
 #pragma unroll
-          for (int t = 0; t < D; ++t) pj += nb.dm[t] * N.s(j, k, t);
+          for (int t = 0; t < D; ++t) pj += T(nb.dm[t]) * T(N.s(j, k, t));
         }
-        double term = f[k] - pj;
-        if (den != 1.0) term = term * rden;
-        a[k] += div_by(term, dsx, rdsx);
+        T term = f[k] - pj;
+        if (den != 1.0) term = term * T(rden);
+        a[k] += div_ds(term, dsx, rdsx);
       }
     });
   }
 }
 
-template <int D, int NDF, bool UNI, class FT>
+template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c, int d,
-                                 int i, int li, uint32_t ki, const double (&f)[NDF],
-                                 double (&s)[NDF]) {
+                                 int i, int li, uint32_t ki, const T (&f)[NDF], T (&s)[NDF]) {
   const SlopeDesc sd = A.sl[c * D + d];
   if (sd.mode == SL_SKIP) {
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) s[k] = G.s(i, k, d);
+    for (int k = 0; k < NDF; ++k) s[k] = T(G.s(i, k, d));
     return;
   }
   if (sd.mode == SL_ZERO) {
 #pragma unroll
     for (int k = 0; k < NDF; ++k) {
-      s[k] = 0.0;
+      s[k] = T(0);
       G.set_s(i, k, d, 0.0);
     }
     return;
   }
-  double aL[NDF];
-  side_sum<D, NDF, UNI, FT>(A, c, G.n, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li,
-                            ki, f, aL);
+  T aL[NDF];
+  side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li,
+                               ki, f, aL);
   if (sd.mode == SL_INNER) {
-    double aR[NDF];
-    side_sum<D, NDF, UNI, FT>(A, c, G.n, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i, li,
-                              ki, f, aR);
+    T aR[NDF];
+    side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i,
+                                 li, ki, f, aR);
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) s[k] = minmod_sel(aL[k] * sd.inv_cnt[0], aR[k] * sd.inv_cnt[1]);
+    for (int k = 0; k < NDF; ++k)
+      s[k] = minmod_sel(aL[k] * T(sd.inv_cnt[0]), aR[k] * T(sd.inv_cnt[1]));
   } else {
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) s[k] = aL[k] * sd.inv_cnt[0];
+    for (int k = 0; k < NDF; ++k) s[k] = aL[k] * T(sd.inv_cnt[0]);
   }
 #pragma unroll
   for (int k = 0; k < NDF; ++k) {
-    s[k] = G.stored(s[k]);   // the flux uses the slope as stored
+    s[k] = T(G.stored(s[k]));   // the flux uses the slope as stored
     G.set_s(i, k, d, s[k]);
   }
 }
@@ -465,32 +494,34 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
 // Moments of one velocity cell's contribution to a half flux (the cell's own upwind micro
 // flux through one face, micro_to_macro of src/Flux/CAIDVM.jl:119), or the two sums of the
 // wall density (calc_ρw, src/Theory/Math.jl:251-282) in the first sweep of a wall.
-template <int D, int NDF, bool UNI, class FT>
+template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c,
-                                  const HalfDesc& hd, int i, const double (&u)[D],
-                                  const double (&f)[NDF], const double (&s)[D][NDF], double wt,
-                                  double (&m)[D + 2]) {
+                                  const HalfDesc& hd, int i, const T (&u)[D], const T (&f)[NDF],
+                                  const T (&s)[D][NDF], T wt, double (&m)[D + 2]) {
   constexpr int NW = D + 2;
 #pragma unroll
   for (int q = 0; q < NW; ++q) m[q] = 0.0;
-  const double ud = pick<D>(u, hd.dir);
-  const double ru = hd.rot * ud;
-  const bool in = hd.sel ? (ru > 0.0) : (ru <= 0.0);
+  const T ud = pick<D>(u, hd.dir);
+  const T ru = T(hd.rot) * ud;
+  const bool in = hd.sel ? (ru > T(0)) : (ru <= T(0));
   if (hd.mode == MI_WALL) {
     const double* bc = &A.bc_prim[hd.bc * NW];
     if (in) {
-      double c2 = 0.0;
+      T c2 = T(0);
 #pragma unroll
       for (int t = 0; t < D; ++t) {
-        double cu = u[t] - bc[1 + t];
+        T cu = u[t] - T(bc[1 + t]);
         c2 = (t == 0) ? cu * cu : c2 + cu * cu;
       }
-      m[1] = wt * ud * exp(-bc[D + 1] * c2);
+      m[1] = wt * ud * exp(-T(bc[D + 1]) * c2);
     } else {
-      double dx[D];
+      T dx[D];
 #pragma unroll
-      for (int t = 0; t < D; ++t) dx[t] = hd.fmid[t] - u[t] * A.dt - hd.xrec[t];
-      double hrec = dx[0] * s[0][0];
+      for (int t = 0; t < D; ++t) {
+        if constexpr (sizeof(T) == 8) dx[t] = hd.fmid[t] - u[t] * A.dt - hd.xrec[t];
+        else dx[t] = T(hd.fmid[t] - hd.xrec[t]) - u[t] * T(A.dt);
+      }
+      T hrec = dx[0] * s[0][0];
 #pragma unroll
       for (int t = 1; t < D; ++t) hrec = hrec + dx[t] * s[t][0];
       hrec = f[0] + hrec;
@@ -499,21 +530,21 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
     return;
   }
   if (!in) return;
-  double mic[NDF];
+  T mic[NDF];
   if (hd.mode == MI_MAXWELL) {
     micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], 0.0, false, u, A.K, hd.dir, mic);
   } else if (hd.mode == MI_INTERP) {
-    micro_interp<D, NDF>(f, s, u, A.dt, hd.xrec, hd.fmid, hd.dir, mic);
+    micro_interp<D, NDF>(f, s, u, T(A.dt), hd.xrec, hd.fmid, hd.dir, mic);
   } else if (hd.data_cell == c) {
-    micro_recon<D, NDF>(hd.mode, f, s, u, A.dt, hd.xrec, hd.dsrec, hd.fmid, hd.dir, mic);
+    micro_recon<D, NDF>(hd.mode, f, s, u, T(A.dt), hd.xrec, hd.dsrec, hd.fmid, hd.dir, mic);
   } else {
     // the there side of a face whose other cell is solid: f of that cell * vn (shared
     // velocity grids only; checked on the host)
     const Grid<D, NDF, UNI, FT> S(A, hd.data_cell);
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) mic[k] = S.f(i, k) * ud;
+    for (int k = 0; k < NDF; ++k) mic[k] = T(S.f(i, k)) * ud;
   }
-  moment_terms<D, NDF>(wt, u, mic, m);
+  moment_terms<D, NDF, T>(wt, u, mic, m);
 }
 
 template <int D, int NDF, bool UNI, class FT, int MAXH, int NWV>
@@ -532,26 +563,27 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   __shared__ double tot[CNT];
   __shared__ double rho_sh[MAXH];
 
+  using CT = typename Arith<FT>::type;
   double acc[CNT];
 #pragma unroll
   for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
 
   for (int i = threadIdx.x; i < G.n; i += BS) {
-    double u[D], f[NDF], s[D][NDF];
+    CT u[D], f[NDF], s[D][NDF];
 #pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+    for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+    for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
     const int li = G.lev(i);
     const uint32_t ki = G.key(i);
 #pragma unroll
-    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT>(A, G, c, d, i, li, ki, f, s[d]);
-    const double wt = G.wt(i);
+    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, ki, f, s[d]);
+    const CT wt = CT(G.wt(i));
 #pragma unroll
     for (int h = 0; h < MAXH; ++h) {
       double m[NW];
       if (h < nh) {
-        half_terms<D, NDF, UNI, FT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+        half_terms<D, NDF, UNI, FT, CT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
       } else {
 #pragma unroll
         for (int q = 0; q < NW; ++q) m[q] = 0.0;
@@ -585,19 +617,20 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
 #pragma unroll
   for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
   for (int i = threadIdx.x; i < G.n; i += BS) {
-    double u[D];
+    CT u[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-    const double wt = G.wt(i);
+    for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
+    const CT wt = CT(G.wt(i));
 #pragma unroll
     for (int h = 0; h < MAXH; ++h) {
       if (h < nh) {
         const HalfDesc& hd = A.halves[hb + h];
         if (hd.mode != MI_WALL) continue;
-        if (!(hd.rot * pick<D>(u, hd.dir) > 0.0)) continue;
-        double mic[NDF], m[NW];
+        if (!(CT(hd.rot) * pick<D>(u, hd.dir) > CT(0))) continue;
+        CT mic[NDF];
+        double m[NW];
         micro_maxwell<D, NDF>(&A.bc_prim[hd.bc * NW], rho_sh[h], true, u, A.K, hd.dir, mic);
-        moment_terms<D, NDF>(wt, u, mic, m);
+        moment_terms<D, NDF, CT>(wt, u, mic, m);
 #pragma unroll
         for (int q = 0; q < NW; ++q) acc[h * NW + q] += m[q];
       }
@@ -640,22 +673,23 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
   for (int r = 0; r < VPT; ++r) {
     const int p = (ch * VPT + r) * SLOPE_CHUNK + int(threadIdx.x);
     if (p < G.n) {
+      using CT = typename Arith<FT>::type;
       const int i = G.vi(p);
-      double u[D], f[NDF], s[D][NDF];
+      CT u[D], f[NDF], s[D][NDF];
 #pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
-      for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+      for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
       const int li = G.lev(i);
       const uint32_t ki = G.key(i);
 #pragma unroll
-      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT>(A, G, c, d, i, li, ki, f, s[d]);
-      const double wt = G.wt(i);
+      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, ki, f, s[d]);
+      const CT wt = CT(G.wt(i));
 #pragma unroll
       for (int h = 0; h < MAXH; ++h) {
         if (h < nh) {
           double m[NW];
-          half_terms<D, NDF, UNI, FT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+          half_terms<D, NDF, UNI, FT, CT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
 #pragma unroll
           for (int q = 0; q < NW; ++q) acc[h * NW + q] = (r == 0) ? m[q] : acc[h * NW + q] + m[q];
         }
@@ -672,25 +706,23 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
 // ------------------------------------------------------------------ K2: flux + relaxation
 // Micro flux into velocity cell i of cell c through all its faces, in face-list order
 // (update_flux!/update_domain_flux!/update_micro_flux!, src/Flux/Flux.jl:64-281).
-template <int D, int NDF, bool UNI, class FT>
+template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, int ne, int i,
-                                   const double (&u)[D], const double (&f)[NDF],
-                                   const double (&s)[D][NDF], int li, uint32_t ki,
-                                   double (&fl)[NDF]) {
-  constexpr int NW = D + 2;
-  const double dt = A.dt;
+                                   const T (&u)[D], const T (&f)[NDF], const T (&s)[D][NDF],
+                                   int li, uint32_t ki, T (&fl)[NDF]) {
+  const T dt = T(A.dt);
   for (int e = 0; e < ne; ++e) {
     const EntryDesc& ed = A.ents[eb + e];
-    const double ud = pick<D>(u, ed.dir);
-    const double ru = ed.rot * ud;
-    const bool own = ed.side ? (ru > 0.0) : (ru <= 0.0);
-    const double area = ed.area;
-    double mic[NDF];
+    const T ud = pick<D>(u, ed.dir);
+    const T ru = T(ed.rot) * ud;
+    const bool own = ed.side ? (ru > T(0)) : (ru <= T(0));
+    const T area = T(ed.area);
+    T mic[NDF];
     if (own) {
       micro_recon<D, NDF>(ed.own_mode, f, s, u, dt, ed.xown, ed.dsown, ed.fmid, ed.dir, mic);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
-        double t = area * mic[k];
+        T t = area * mic[k];
         fl[k] += ed.side ? -t : t;
       }
       continue;
@@ -698,7 +730,7 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
     switch (ed.oth_mode) {
       case MI_MAXWELL:
       case MI_WALL: {
-        micro_maxwell<D, NDF>(&A.bc_prim[ed.bc * NW], A.rho_w_r[ed.pair],
+        micro_maxwell<D, NDF>(&A.bc_prim[ed.bc * (D + 2)], A.rho_w_r[ed.pair],
                               ed.oth_mode == MI_WALL, u, A.K, ed.dir, mic);
 #pragma unroll
         for (int k = 0; k < NDF; ++k) fl[k] += area * mic[k];
@@ -719,22 +751,23 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
         const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
         for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, ki,
                                        [&](int j, double den, double rden) {
-          double uj[D], fj[NDF], sj[D][NDF];
+          T uj[D], fj[NDF], sj[D][NDF];
 #pragma unroll
-          for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : N.u(j, t);
+          for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : T(N.u(j, t));
 #pragma unroll
           for (int k = 0; k < NDF; ++k) {
-            fj[k] = N.f(j, k);
+            fj[k] = T(N.f(j, k));
 #pragma unroll
-            for (int t = 0; t < D; ++t) sj[t][k] = (ed.oth_mode == MI_PLAIN) ? 0.0 : N.s(j, k, t);
+            for (int t = 0; t < D; ++t)
+              sj[t][k] = (ed.oth_mode == MI_PLAIN) ? T(0) : T(N.s(j, k, t));
           }
-          double mj[NDF];
+          T mj[NDF];
           micro_recon<D, NDF>(ed.oth_mode, fj, sj, uj, dt, ed.xoth, ed.dsoth, ed.fmid, ed.dir,
                               mj);
 #pragma unroll
           for (int k = 0; k < NDF; ++k) {
-            double t = area * mj[k];
-            if (den != 1.0) t = t * rden;   // exact: den is a power of two
+            T t = area * mj[k];
+            if (den != 1.0) t = t * T(rden);   // exact: den is a power of two
             fl[k] += ed.side ? -t : t;
           }
         });
@@ -745,9 +778,14 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
 
 // heat_flux_2D2F / heat_flux_3D1F partial sums of one velocity cell (lib/KitCore/2D2F.jl:68-88,
 // 3D1F.jl:41-72): the sums are kept apart as in the reference and combined after reduction.
-template <int D, int NDF, int NQ>
-__device__ inline void heat_terms(double wt, const double (&u)[D], const double (&f)[NDF],
+// In FP64 throughout (q is a small sum of large terms of both signs): the velocities are the
+// grid's FP64 ones, f is widened from the arithmetic type.
+template <int D, int NDF, int NQ, class T>
+__device__ inline void heat_terms(double wt, const double (&u)[D], const T (&fv)[NDF],
                                   const double* p, double (&qa)[NQ]) {
+  double f[NDF];
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) f[k] = fv[k];
   if constexpr (D == 2) {
     double cu = u[0] - p[1], cv = u[1] - p[2];
     double c2 = cu * cu + cv * cv;
@@ -1031,7 +1069,11 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   __shared__ MaxPre<D> pre_c, pre_r;
   __shared__ double pre_t[3];
   extern __shared__ double lds[];
-  double* sc = LDS ? lds : &A.scratch[G.off * (NDF + 1)];
+  // per-velocity arithmetic type: FP32 in the FP32 form, except for CIP_Marching, whose
+  // Newton iteration on the projected distribution stays FP64
+  using CT = std::conditional_t<(sizeof(FT) == 4 && MARCH != MARCH_CIP), float, double>;
+  CT* sc = LDS ? reinterpret_cast<CT*>(lds)
+               : reinterpret_cast<CT*>(&A.scratch[G.off * (NDF + 1)]);
 
   // half flux = sum of its partial rows in chunk order (micro_to_macro's 1/2 on the energy
   // row applied after the sum), one thread per (entry, side, component)
@@ -1090,17 +1132,17 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   if (MODE == MODE_FLUX) {
     for (int p = threadIdx.x; p < n; p += BS) {
       const int i = G.vi(p);
-      double u[D], f[NDF], s[D][NDF], fl[NDF];
+      CT u[D], f[NDF], s[D][NDF], fl[NDF];
 #pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
-        f[k] = G.f(i, k);
+        f[k] = CT(G.f(i, k));
 #pragma unroll
-        for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
-        fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
+        for (int d = 0; d < D; ++d) s[d][k] = CT(G.s(i, k, d));
+        fl[k] = CT(A.vflux[G.off * NDF + int64_t(k) * n + i]);
       }
-      gather_flux<D, NDF, UNI, FT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+      gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) A.vflux[G.off * NDF + int64_t(k) * n + i] = fl[k];
     }
@@ -1132,52 +1174,57 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     double pc[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) pc[q] = cell_sh[NW + q];
+    const CT dtVc = CT(dtV);
     for (int p = threadIdx.x; p < n; p += BS) {
       const int i = G.vi(p);
-      double u[D], f[NDF], fl[NDF];
+      CT u[D], f[NDF], fl[NDF];
 #pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
-      for (int k = 0; k < NDF; ++k) f[k] = G.f(i, k);
+      for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
       if (MODE == MODE_FUSED) {
-        double s[D][NDF];
+        CT s[D][NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
-          fl[k] = 0.0;
+          fl[k] = CT(0);
 #pragma unroll
-          for (int d = 0; d < D; ++d) s[d][k] = G.s(i, k, d);
+          for (int d = 0; d < D; ++d) s[d][k] = CT(G.s(i, k, d));
         }
-        gather_flux<D, NDF, UNI, FT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+        gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
-          fl[k] = A.vflux[G.off * NDF + int64_t(k) * n + i];
+          fl[k] = CT(A.vflux[G.off * NDF + int64_t(k) * n + i]);
           A.vflux[G.off * NDF + int64_t(k) * n + i] = 0.0;
         }
       }
-      const double wt = G.wt(i);
+      const CT wt = CT(G.wt(i));
       if (MARCH == MARCH_CAIDVM) {
         // convection first: f* = f + dt/V*flux, and the moments of f*
-        double fs[NDF], m[NW];
+        CT fs[NDF];
+        double m[NW];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
-          fs[k] = f[k] + dtV * fl[k];
+          fs[k] = f[k] + dtVc * fl[k];
           sc[k * n + i] = fs[k];
         }
-        moment_terms<D, NDF>(wt, u, fs, m);
+        moment_terms<D, NDF, CT>(wt, u, fs, m);
 #pragma unroll
         for (int q = 0; q < NW; ++q) mw[q] += m[q];
       } else if (MARCH == MARCH_CIP) {
         // convection first, then the heat flux of f* with prim_c (calc_qf after the
         // convection, src/Theory/I-projection.jl:177-179); e_int and the minima of h for the
         // I-projection
-        double fs[NDF];
+        CT fs[NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
-          fs[k] = f[k] + dtV * fl[k];
+          fs[k] = f[k] + dtVc * fl[k];
           sc[k * n + i] = fs[k];
         }
-        heat_terms<D, NDF, NQ>(wt, u, fs, pc, qa);
+        double ud[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) ud[d] = G.u(i, d);
+        heat_terms<D, NDF, NQ, CT>(G.wt(i), ud, fs, pc, qa);
         if constexpr (NDF == 2) eint += wt * fs[1];
         fmn = fs[0] < fmn ? fs[0] : fmn;
         if (fs[0] > 0.0 && fs[0] < fmp) fmp = fs[0];
@@ -1185,7 +1232,10 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         // Euler: heat flux of f^n with the updated prim
 #pragma unroll
         for (int k = 0; k < NDF; ++k) sc[k * n + i] = fl[k];
-        heat_terms<D, NDF, NQ>(wt, u, f, pc, qa);
+        double ud[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) ud[d] = G.u(i, d);
+        heat_terms<D, NDF, NQ, CT>(G.wt(i), ud, f, pc, qa);
       }
     }
   }
@@ -1214,21 +1264,26 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       const MaxPre<D> mpc = uniform<D>(pre_c), mpr = uniform<D>(pre_r);
 #pragma unroll
       for (int q = 0; q < NW; ++q) pc[q] = mpc.p[q];
+      const CT K = CT(A.K);
       for (int p = threadIdx.x; p < n; p += BS) {
         const int i = G.vi(p);
-        double u[D], Mc[2], M[2];
+        double ud[D];
+        CT u[D], Mc[2], M[2];
 #pragma unroll
-        for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-        maxwell_pre<D>(u, mpc, A.K, Mc);
-        maxwell_pre<D>(u, mpr, A.K, M);
-        double fs[NDF];
+        for (int d = 0; d < D; ++d) {
+          ud[d] = G.u(i, d);
+          u[d] = CT(ud[d]);
+        }
+        maxwell_pre<D, CT>(ud, mpc, K, Mc);
+        maxwell_pre<D, CT>(ud, mpr, K, M);
+        CT fs[NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
           fs[k] = sc[k * n + i] + (Mc[k] - M[k]);
           sc[k * n + i] = fs[k];
         }
         sc[NDF * n + i] = Mc[0];
-        heat_terms<D, NDF, NQ>(G.wt(i), u, fs, pc, qa);
+        heat_terms<D, NDF, NQ, CT>(G.wt(i), ud, fs, pc, qa);
       }
     }
   }
@@ -1247,36 +1302,51 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   if (MARCH == MARCH_CAIDVM) {
     // pass 3: g+ = M(prim_c) + Shakhov part; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
     const double a1 = uniform(pre_t[1]), a2 = uniform(pre_t[2]);
+    const CT K = CT(A.K), a1c = CT(a1), a2c = CT(a2);
+    CT qc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) qc[d] = CT(q[d]);
     for (int p = threadIdx.x; p < n; p += BS) {
       const int i = G.vi(p);
-      double u[D], Fc[2], Sp[2];
+      CT u[D], Fc[2], Sp[2];
 #pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+      for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
       Fc[0] = sc[NDF * n + i];
-      if constexpr (NDF == 2) Fc[1] = div_by(Fc[0] * A.K, mpc.two_l, mpc.r2l);
-      shakhov_pre<D>(u, Fc, mpc, q, A.K, Sp);
+      if constexpr (NDF == 2) {
+        if constexpr (sizeof(CT) == 8) Fc[1] = div_by(Fc[0] * A.K, mpc.two_l, mpc.r2l);
+        else Fc[1] = Fc[0] * K * CT(mpc.r2l);
+      }
+      shakhov_pre<D, CT>(u, Fc, mpc, qc, K, Sp);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
-        double g = Fc[k] + Sp[k];
-        double fn = sc[k * n + i] * a1;
-        fn = fn + a2 * g;
+        CT g = Fc[k] + Sp[k];
+        CT fn = sc[k * n + i] * a1c;
+        fn = fn + a2c * g;
         G.set_out(i, k, fn);
       }
     }
   } else {
     // Euler: f = (f + dt/V*flux)*tau/(tau+dt) + dt/(tau+dt)*F with F = M + Shakhov(M)
     const double tpd = tau + dt, a2 = uniform(pre_t[2]);
+    const CT K = CT(A.K), dtVc = CT(dtV), tauc = CT(tau), tpdc = CT(tpd), a2c = CT(a2);
+    CT qc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) qc[d] = CT(q[d]);
     for (int p = threadIdx.x; p < n; p += BS) {
       const int i = G.vi(p);
-      double u[D], M[2], Sp[2];
+      double ud[D];
+      CT u[D], M[2], Sp[2];
 #pragma unroll
-      for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
-      maxwell_pre<D>(u, mpc, A.K, M);
-      shakhov_pre<D>(u, M, mpc, q, A.K, Sp);
+      for (int d = 0; d < D; ++d) {
+        ud[d] = G.u(i, d);
+        u[d] = CT(ud[d]);
+      }
+      maxwell_pre<D, CT>(ud, mpc, K, M);
+      shakhov_pre<D, CT>(u, M, mpc, qc, K, Sp);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
-        double F = M[k] + Sp[k];
-        double fx = (G.f(i, k) + dtV * sc[k * n + i]) * tau / tpd + a2 * F;
+        CT F = M[k] + Sp[k];
+        CT fx = (CT(G.f(i, k)) + dtVc * sc[k * n + i]) * tauc / tpdc + a2c * F;
         G.set_out(i, k, fx);
       }
     }

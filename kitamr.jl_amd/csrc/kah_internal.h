@@ -111,7 +111,8 @@ __host__ __device__ inline double div_by(double a, double b, double rb) {
 __host__ __device__ inline double jl_sign(double x) {
   return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x);
 }
-__host__ __device__ inline double jl_min(double a, double b) {
+template <class T>
+__host__ __device__ inline T jl_min(T a, T b) {
   if (a != a) return a;
   if (b != b) return b;
   return b < a ? b : a;
@@ -126,11 +127,12 @@ __host__ __device__ inline double minmod(double sL, double sR) {
 // 0*min (NaN for a NaN input or for opposite infinities, as in the formula). Only the sign
 // of a zero result may differ, which no later use of a slope can observe (slopes only
 // enter products and sums with nonzero terms).
-__host__ __device__ inline double minmod_sel(double sL, double sR) {
-  const double SL = fabs(sL), SR = fabs(sR);
-  const double m = (SL != SL || SR != SR) ? SL + SR : (SR < SL ? SR : SL);
-  const bool pos = sL > 0.0 && sR > 0.0, neg = sL < 0.0 && sR < 0.0;
-  return pos ? m : (neg ? -m : 0.0 * m);
+template <class T>
+__host__ __device__ inline T minmod_sel(T sL, T sR) {
+  const T SL = fabs(sL), SR = fabs(sR);
+  const T m = (SL != SL || SR != SR) ? SL + SR : (SR < SL ? SR : SL);
+  const bool pos = sL > T(0) && sR > T(0), neg = sL < T(0) && sR < T(0);
+  return pos ? m : (neg ? -m : T(0) * m);
 }
 
 // x = J^-1 g for a symmetric positive definite J (M <= 5, upper triangle read) by Cholesky
@@ -250,37 +252,42 @@ __host__ __device__ inline MaxPre<D> max_pre(const double* p, double Pr) {
   m.s0 = 0.8 * (1 - Pr) * (lam * lam) / p[0];
   return m;
 }
-template <int D>
-__host__ __device__ inline void maxwell_pre(const double* u, const MaxPre<D>& m, double K,
-                                            double* out) {
+// Discrete Maxwellian and Shakhov part of one velocity cell from the pre-factors; T is the
+// arithmetic type (double, or float in the FP32 advection form, where the FP64 parameters
+// are rounded once and exp is the float one).
+template <int D, class T = double>
+__host__ __device__ inline void maxwell_pre(const double* u, const MaxPre<D>& m, T K, T* out) {
+  // the exponent in FP64 from the grid's FP64 velocities (a float u would put its rounding,
+  // times 2*lambda*|c|, into every value); exp and the products in T
   if constexpr (D == 2) {
     double a = u[0] - m.p[1], b = u[1] - m.p[2];
     double c2 = a * a + b * b;
-    double h = m.a0 * exp(-m.p[3] * c2);
+    T h = T(m.a0) * exp(T(-m.p[3] * c2));
     out[0] = h;
-    out[1] = div_by(h * K, m.two_l, m.r2l);
+    if constexpr (sizeof(T) == 8) out[1] = div_by(h * K, m.two_l, m.r2l);
+    else out[1] = h * K * T(m.r2l);
   } else {
     double a = u[0] - m.p[1], b = u[1] - m.p[2], c = u[2] - m.p[3];
     double c2 = a * a + b * b + c * c;
-    out[0] = m.a0 * exp(-m.p[4] * c2);
+    out[0] = T(m.a0) * exp(T(-m.p[4] * c2));
   }
 }
-template <int D>
-__host__ __device__ inline void shakhov_pre(const double* u, const double* F,
-                                            const MaxPre<D>& m, const double* q, double K,
-                                            double* out) {
+template <int D, class T = double>
+__host__ __device__ inline void shakhov_pre(const T* u, const T* F, const MaxPre<D>& m,
+                                            const T* q, T K, T* out) {
   const double* p = m.p;
+  const T s0 = T(m.s0);
   if constexpr (D == 2) {
-    double cu = u[0] - p[1], cv = u[1] - p[2];
-    double c2 = cu * cu + cv * cv;
-    double cq = cu * q[0] + cv * q[1];
-    out[0] = m.s0 * cq * (2 * p[3] * c2 + K - 5) * F[0];
-    out[1] = m.s0 * cq * (2 * p[3] * c2 + K - 3) * F[1];
+    T cu = u[0] - T(p[1]), cv = u[1] - T(p[2]);
+    T c2 = cu * cu + cv * cv;
+    T cq = cu * q[0] + cv * q[1];
+    out[0] = s0 * cq * (2 * T(p[3]) * c2 + K - 5) * F[0];
+    out[1] = s0 * cq * (2 * T(p[3]) * c2 + K - 3) * F[1];
   } else {
-    double cu = u[0] - p[1], cv = u[1] - p[2], cw = u[2] - p[3];
-    double c2 = cu * cu + cv * cv + cw * cw;
-    double cq = cu * q[0] + cv * q[1] + cw * q[2];
-    out[0] = m.s0 * cq * (2 * p[4] * c2 - 5) * F[0];
+    T cu = u[0] - T(p[1]), cv = u[1] - T(p[2]), cw = u[2] - T(p[3]);
+    T c2 = cu * cu + cv * cv + cw * cw;
+    T cq = cu * q[0] + cv * q[1] + cw * q[2];
+    out[0] = s0 * cq * (2 * T(p[4]) * c2 - 5) * F[0];
   }
 }
 
@@ -292,9 +299,9 @@ __host__ __device__ inline double get_tau(const double* prim, double mu, double 
 
 // Contribution of one velocity cell to micro_to_macro (2D2F.jl:119-126, 3D1F.jl:109-123);
 // the 1/2 of the energy row is applied after the sum.
-template <int D, int NDF>
-__host__ __device__ inline void moment_terms(double wt, const double* u, const double* f,
-                                             double* m) {
+// Terms of type T (float in the FP32 advection form), returned as double for the FP64 sums.
+template <int D, int NDF, class T = double>
+__host__ __device__ inline void moment_terms(T wt, const T* u, const T* f, double* m) {
   if constexpr (D == 2) {
     m[0] = wt * f[0];
     m[1] = wt * u[0] * f[0];
