@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32-form leg of an FP64 run")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--small", action="store_true", help="reduced mesh (profiling/debug only)")
     ap.add_argument("--flux-waves", type=int, default=0,
@@ -185,45 +187,51 @@ def main():
         tuned, tms, trials = autotune.autotune(case, steps=3, warmup=1, log=log)
         log(f"[bench] autotune ({len(trials)} trials, {time.perf_counter() - t1:.1f}s): {tuned} "
             f"{tms:.3f} ms/step")
-    dev = abi.HipContext(kah_params(case.config))
-    for k, v in (tuned or {}).items():
-        dev.set_option(k, v)
+    def timed_run(case, options, steps, warmup, copy=False):
+        """Fused steps of one case on a fresh context: (elapsed s, kernel ms, finite, copy)."""
+        dev = abi.HipContext(kah_params(case.config))
+        for k, v in options.items():
+            dev.set_option(k, v)
+        dev.set_topology(case.topo)
+        dev.set_state(case.w, case.prim, case.df)
+        dt = case.dt_xi
+        step = 0
+        for _ in range(warmup):
+            dev.step(dt, step, residual=False)
+            step += 1
+        dev.sync()
+        log(f"[bench] warmup {warmup} steps done; device bytes {dev.device_bytes() / 1e9:.2f} GB")
+        dev.kernel_times(reset=True)       # start HIP-event timing of the kernels
+        torch_sync()
+        dev.sync()
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            dev.step(dt, step, residual=False)
+            step += 1
+        dev.sync()
+        torch_sync()
+        elapsed = time.perf_counter() - t_start
+        ms, _ = dev.kernel_times()
+        log(f"[bench] timed {steps} steps: {elapsed:.3f}s; kernel ms slope={ms[0]:.3f} "
+            f"flux+relax={ms[1]:.3f}")
+        st = dev.get_state(df=False, qf=False)
+        finite = bool(np.isfinite(st["w"]).all())
+        cp = None
+        if copy:
+            cp = dev.copy_bandwidth(1 << 30, 10)
+            log(f"[bench] plain copy of 1 GiB: {cp[1]:.3f} ms -> {cp[0]:.0f} GB/s")
+        dev.close()
+        return elapsed, ms, finite, cp
+
+    options = dict(tuned or {})
     if args.flux_waves:
-        dev.set_option("flux_waves", args.flux_waves)
+        options["flux_waves"] = args.flux_waves
     if args.slope_vpt:
-        dev.set_option("slope_vpt", args.slope_vpt)
+        options["slope_vpt"] = args.slope_vpt
     if args.vel_tiles >= 0:
-        dev.set_option("vel_tiles", args.vel_tiles)
-    dev.set_topology(case.topo)
-    dev.set_state(case.w, case.prim, case.df)
-    dt = case.dt_xi
-    step = 0
-    for _ in range(args.warmup):
-        dev.step(dt, step, residual=False)
-        step += 1
-    dev.sync()
-    log(f"[bench] warmup {args.warmup} steps done; device bytes {dev.device_bytes() / 1e9:.2f} GB")
-
-    dev.kernel_times(reset=True)       # start HIP-event timing of the kernels
-    torch_sync()
-    dev.sync()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        dev.step(dt, step, residual=False)
-        step += 1
-    dev.sync()
-    torch_sync()
-    elapsed = time.perf_counter() - t_start
-    ms, calls = dev.kernel_times()
-    log(f"[bench] timed {args.steps} steps: {elapsed:.3f}s; kernel ms slope={ms[0]:.3f} "
-        f"flux+relax={ms[1]:.3f}")
-
-    st = dev.get_state(df=False, qf=False)
-    w = st["w"].reshape(-1, 4)
-    finite = bool(np.isfinite(w).all())
-    copy_gbs, copy_ms = dev.copy_bandwidth(1 << 30, 10)
-    log(f"[bench] plain copy of 1 GiB: {copy_ms:.3f} ms -> {copy_gbs:.0f} GB/s")
-    dev.close()
+        options["vel_tiles"] = args.vel_tiles
+    elapsed, ms, finite, (copy_gbs, copy_ms) = timed_run(case, options, args.steps, args.warmup,
+                                                         copy=True)
 
     ms_step = 1e3 * elapsed / args.steps
     value = P * args.steps / elapsed
@@ -242,7 +250,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 storage / f64 arithmetic",
+        "dtype": "f64" if args.precision == "fp64" else "f32 (f stored and advected; f64 moments)",
         "data": "synthetic",
         "config": {
             "workload": ("reduced 32x32 + 2 levels (debug)" if args.small else
@@ -276,6 +284,29 @@ def main():
         "copy_bandwidth_GBs": copy_gbs,
         "state_finite": finite,
     }
+    if args.precision == "fp64" and not args.no_fp32 and not args.small:
+        # the FP32 form on the same workload: f stored and advected in FP32, moment sums,
+        # prim, tau, q and the Maxwellian parameters in FP64
+        c32 = (cases.riemann_2d(n=64, nv=20, vmax=15.0, ps_levels=3, vs_levels=2, vs_variants=6,
+                                precision="fp32f") if adaptive else
+               cases.riemann_2d(n=64, nv=48, ps_levels=3, precision="fp32f"))
+        b_s32, b_fi32, b_st32 = alg_bytes(4, adaptive)
+        e32, ms32, fin32, _ = timed_run(c32, {}, args.steps, args.warmup)
+        v32 = c32.phase_cells * args.steps / e32
+        a32 = b_fi32 * c32.phase_cells / (ms32[1] * 1e-3) / 1e9
+        out["fp32_form"] = {
+            "value": v32, "unit": "updates/s", "ms_per_step": 1e3 * e32 / args.steps,
+            "speedup_vs_fp64": v32 / value,
+            "hbm_roofline_pct": 100.0 * b_st32 * v32 / 1e9 / HBM_PEAK_GBS,
+            "step_alg_bytes_per_update": b_st32,
+            "k_flux_relax": {"avg_launch_ms": ms32[1], "achieved_GBs": a32,
+                             "frac": a32 / HBM_PEAK_GBS, "alg_bytes_per_update": b_fi32},
+            "slope_kernel_ms": ms32[0],
+            "state_finite": fin32,
+            "accuracy_vs_fp64_oracle": "rho, rhoU, rhoE within 1e-7 and q within 1e-5 "
+                                       "relative after 5 steps (tests/test_gpu_fp32.py)",
+        }
+        log(f"[bench] fp32 form: {v32 / 1e9:.2f} G updates/s ({v32 / value:.2f}x FP64)")
     if not args.no_serve and not args.small:
         rep = serving_stream()
         log(f"[bench] serving: p50 {rep['p50_latency_s']:.3f}s p99 {rep['p99_latency_s']:.3f}s "
