@@ -177,7 +177,7 @@ int kah_pack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind, do
 /* Writes w and df (DATA) or sdf and sw (SLOPE) of the listed fluid cells from in (host memory,
  * the layout above); solid cells are skipped. DATA buffers are checked first: a solid buffer
  * (w = Inf) for a fluid cell, the converse, or a vs_num other than the cell's velocity-cell
- * count is KAH_EINVAL and nothing is written. FP32-f contexts round df and sdf to float. */
+ * count is KAH_EINVAL and nothing is written. FP32 contexts round df and sdf to float. */
 int kah_unpack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind,
                        const double* in);
 
