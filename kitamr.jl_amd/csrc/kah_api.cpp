@@ -357,7 +357,10 @@ struct Builder {
       h.xrec[t] = mid(rec)[t];
       h.dsrec[t] = dsr(rec)[t];
       h.fmid[t] = fm[t];
+      h.dx_f[t] = float(h.fmid[t] - h.xrec[t]);
+      h.ds_f[t] = float(h.dsrec[t]);
     }
+    h.rot_f = float(rot);
     return h;
   }
   EntryDesc entry(int32_t pair, int8_t side, int8_t om, int8_t tm, int32_t oth, int dir,
@@ -379,7 +382,13 @@ struct Builder {
       e.dsown[t] = dsr(own_rec)[t];
       e.xoth[t] = oth_rec >= 0 ? mid(oth_rec)[t] : 0.0;
       e.dsoth[t] = oth_rec >= 0 ? dsr(oth_rec)[t] : 0.0;
+      e.dxo_f[t] = float(e.fmid[t] - e.xown[t]);
+      e.dso_f[t] = float(e.dsown[t]);
+      e.dxt_f[t] = float(e.fmid[t] - e.xoth[t]);
+      e.dst_f[t] = float(e.dsoth[t]);
     }
+    e.rot_f = float(rot);
+    e.area_f = float(area);
     return e;
   }
 
@@ -702,6 +711,14 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   }
   hoff[nc] = int32_t(halves.size());
   eoff[nc] = int32_t(ents.size());
+  // float copies for the FP32 advection form
+  for (auto& sd : B.sl)
+    for (int k = 0; k < 2; ++k) {
+      sd.rds_f[k] = float(sd.rds[k]);
+      sd.inv_cnt_f[k] = float(sd.inv_cnt[k]);
+    }
+  for (auto& e : B.slnb)
+    for (int t = 0; t < 3; ++t) e.dm_f[t] = float(e.dm[t]);
   KAH_TRY(C->upload(C->d_sl, B.sl.data(), B.sl.size(), "slope descriptors"));
   KAH_TRY(C->upload(C->d_slnb, B.slnb.data(), B.slnb.size(), "slope neighbours"));
   KAH_TRY(C->upload(C->d_hoff, hoff.data(), hoff.size(), "half offsets"));

@@ -326,14 +326,15 @@ inline hipError_t launch_overlap_map_t(const DevArgs& A, const OvJob* jobs, int 
 // LIMITED is positivity_preserving_reconstruct (src/Flux/CAIDVM.jl:127-141); UNLIMITED the
 // plain reconstruction used next to solid cells and at domain walls/inflow (CAIDVM.jl:13-14,
 // :110-112); PLAIN f*vn (UniformOutflow, CAIDVM.jl:53-67).
-// T is the arithmetic type: double, or float in the FP32 advection form. There the
-// offset face midpoint - reconstruction point is formed in double before the rounding (it is
-// a difference of two coordinates of order one).
+// T is the arithmetic type: double, or float in the FP32 advection form. There the offset
+// face midpoint - reconstruction point and the limiter's cell size come from the
+// descriptor's float copies (dxf, dsf; the offset was formed in double before the rounding,
+// being a difference of two coordinates of order one).
 template <int D, int NDF, class T>
 __device__ inline void micro_recon(int mode, const T (&f)[NDF], const T (&s)[D][NDF],
                                    const T (&u)[D], T dt, const double* xrec,
-                                   const double* dsrec, const double* fmid, int dir,
-                                   T (&out)[NDF]) {
+                                   const double* dsrec, const double* fmid, const float* dxf,
+                                   const float* dsf, int dir, T (&out)[NDF]) {
   const T vn = pick<D>(u, dir);
   if (mode == MI_PLAIN || mode == MI_SELF_PLAIN) {
 #pragma unroll
@@ -344,7 +345,7 @@ __device__ inline void micro_recon(int mode, const T (&f)[NDF], const T (&s)[D][
 #pragma unroll
   for (int t = 0; t < D; ++t) {
     if constexpr (sizeof(T) == 8) dx[t] = fmid[t] - u[t] * dt - xrec[t];
-    else dx[t] = T(fmid[t] - xrec[t]) - u[t] * dt;
+    else dx[t] = dxf[t] - u[t] * dt;
   }
 #pragma unroll
   for (int k = 0; k < NDF; ++k) {
@@ -352,9 +353,9 @@ __device__ inline void micro_recon(int mode, const T (&f)[NDF], const T (&s)[D][
 #pragma unroll
     for (int t = 1; t < D; ++t) s_dx = s_dx + dx[t] * s[t][k];
     if (mode == MI_LIMITED) {
-      T s_abs = T(dsrec[0]) * fabs(s[0][k]);
+      T s_abs = desc_val<T>(dsrec[0], dsf[0]) * fabs(s[0][k]);
 #pragma unroll
-      for (int t = 1; t < D; ++t) s_abs = s_abs + T(dsrec[t]) * fabs(s[t][k]);
+      for (int t = 1; t < D; ++t) s_abs = s_abs + desc_val<T>(dsrec[t], dsf[t]) * fabs(s[t][k]);
       const T a = f[k] - T(MACH_EPS), b = T(0.5) * s_abs + T(EPS);
       // b >= EPS > 0, so |a| >= b with b finite means |a/b| >= 1 and the limiter is 1
       // exactly; the quotient is only formed where it can be below 1 (or NaN)
@@ -422,15 +423,15 @@ __device__ inline void micro_maxwell(const double* bc, double rho, bool wall, co
 // src/Flux/Slope.jl:458-945), the diff_vs!/diff_vs_transverse! sums (:29-64, :278-332)
 // taken as key-interval overlaps.
 template <class T>
-__device__ inline T div_ds(T a, double ds, double rds) {
+__device__ inline T div_ds(T a, double ds, double rds, float rds_f) {
   if constexpr (sizeof(T) == 8) return div_by(a, ds, rds);
-  else return a * T(rds);
+  else return a * rds_f;
 }
 
 template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int cnt,
-                                double dsx, double rdsx, int i, int li, uint32_t ki,
-                                const T (&f)[NDF], T (&a)[NDF]) {
+                                double dsx, double rdsx, float rdsx_f, int i, int li,
+                                uint32_t ki, const T (&f)[NDF], T (&a)[NDF]) {
 #pragma unroll
   for (int k = 0; k < NDF; ++k) a[k] = T(0);
   for (int e = 0; e < cnt; ++e) {
@@ -443,11 +444,11 @@ __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int
         T pj = T(N.f(j, k));
         if (nb.proj) {
 #pragma unroll
-          for (int t = 0; t < D; ++t) pj += T(nb.dm[t]) * T(N.s(j, k, t));
+          for (int t = 0; t < D; ++t) pj += desc_val<T>(nb.dm[t], nb.dm_f[t]) * T(N.s(j, k, t));
         }
         T term = f[k] - pj;
         if (den != 1.0) term = term * T(rden);
-        a[k] += div_ds(term, dsx, rdsx);
+        a[k] += div_ds(term, dsx, rdsx, rdsx_f);
       }
     });
   }
@@ -471,18 +472,19 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
     return;
   }
   T aL[NDF];
-  side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0], i, li,
-                               ki, f, aL);
+  side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0],
+                               sd.rds_f[0], i, li, ki, f, aL);
+  const T ic0 = desc_val<T>(sd.inv_cnt[0], sd.inv_cnt_f[0]);
   if (sd.mode == SL_INNER) {
     T aR[NDF];
-    side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1], i,
-                                 li, ki, f, aR);
+    side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1],
+                                 sd.rds_f[1], i, li, ki, f, aR);
+    const T ic1 = desc_val<T>(sd.inv_cnt[1], sd.inv_cnt_f[1]);
 #pragma unroll
-    for (int k = 0; k < NDF; ++k)
-      s[k] = minmod_sel(aL[k] * T(sd.inv_cnt[0]), aR[k] * T(sd.inv_cnt[1]));
+    for (int k = 0; k < NDF; ++k) s[k] = minmod_sel(aL[k] * ic0, aR[k] * ic1);
   } else {
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) s[k] = aL[k] * T(sd.inv_cnt[0]);
+    for (int k = 0; k < NDF; ++k) s[k] = aL[k] * ic0;
   }
 #pragma unroll
   for (int k = 0; k < NDF; ++k) {
@@ -502,7 +504,7 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
 #pragma unroll
   for (int q = 0; q < NW; ++q) m[q] = 0.0;
   const T ud = pick<D>(u, hd.dir);
-  const T ru = T(hd.rot) * ud;
+  const T ru = desc_val<T>(hd.rot, hd.rot_f) * ud;
   const bool in = hd.sel ? (ru > T(0)) : (ru <= T(0));
   if (hd.mode == MI_WALL) {
     const double* bc = &A.bc_prim[hd.bc * NW];
@@ -519,7 +521,7 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
 #pragma unroll
       for (int t = 0; t < D; ++t) {
         if constexpr (sizeof(T) == 8) dx[t] = hd.fmid[t] - u[t] * A.dt - hd.xrec[t];
-        else dx[t] = T(hd.fmid[t] - hd.xrec[t]) - u[t] * T(A.dt);
+        else dx[t] = hd.dx_f[t] - u[t] * T(A.dt);
       }
       T hrec = dx[0] * s[0][0];
 #pragma unroll
@@ -536,7 +538,8 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
   } else if (hd.mode == MI_INTERP) {
     micro_interp<D, NDF>(f, s, u, T(A.dt), hd.xrec, hd.fmid, hd.dir, mic);
   } else if (hd.data_cell == c) {
-    micro_recon<D, NDF>(hd.mode, f, s, u, T(A.dt), hd.xrec, hd.dsrec, hd.fmid, hd.dir, mic);
+    micro_recon<D, NDF>(hd.mode, f, s, u, T(A.dt), hd.xrec, hd.dsrec, hd.fmid, hd.dx_f, hd.ds_f,
+                        hd.dir, mic);
   } else {
     // the there side of a face whose other cell is solid: f of that cell * vn (shared
     // velocity grids only; checked on the host)
@@ -714,12 +717,13 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
   for (int e = 0; e < ne; ++e) {
     const EntryDesc& ed = A.ents[eb + e];
     const T ud = pick<D>(u, ed.dir);
-    const T ru = T(ed.rot) * ud;
+    const T ru = desc_val<T>(ed.rot, ed.rot_f) * ud;
     const bool own = ed.side ? (ru > T(0)) : (ru <= T(0));
-    const T area = T(ed.area);
+    const T area = desc_val<T>(ed.area, ed.area_f);
     T mic[NDF];
     if (own) {
-      micro_recon<D, NDF>(ed.own_mode, f, s, u, dt, ed.xown, ed.dsown, ed.fmid, ed.dir, mic);
+      micro_recon<D, NDF>(ed.own_mode, f, s, u, dt, ed.xown, ed.dsown, ed.fmid, ed.dxo_f,
+                          ed.dso_f, ed.dir, mic);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
         T t = area * mic[k];
@@ -762,8 +766,8 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
               sj[t][k] = (ed.oth_mode == MI_PLAIN) ? T(0) : T(N.s(j, k, t));
           }
           T mj[NDF];
-          micro_recon<D, NDF>(ed.oth_mode, fj, sj, uj, dt, ed.xoth, ed.dsoth, ed.fmid, ed.dir,
-                              mj);
+          micro_recon<D, NDF>(ed.oth_mode, fj, sj, uj, dt, ed.xoth, ed.dsoth, ed.fmid, ed.dxt_f,
+                              ed.dst_f, ed.dir, mj);
 #pragma unroll
           for (int k = 0; k < NDF; ++k) {
             T t = area * mj[k];

@@ -42,6 +42,8 @@ struct SlopeNb {        // one neighbour in a slope stencil
   int32_t map;          // overlap-map slot of (owner cell, this neighbour), -1: none
   int32_t pad;
   double dm[3];         // transverse offsets (dm[dir] = 0)
+  float dm_f[3];        // the same rounded to float (FP32 advection form)
+  float pad_f;
 };
 
 // Overlap map of adapted velocity grids: for owner cell c, its map slot k (one per distinct
@@ -64,6 +66,8 @@ struct SlopeDesc {      // per (fluid cell, direction)
   double ds[2];         // dsL, dsR
   double rds[2];        // 1/dsL, 1/dsR (correctly rounded)
   double inv_cnt[2];    // 1/nb_cnt (exact: the counts are powers of two)
+  float rds_f[2];       // rds and inv_cnt rounded to float (FP32 advection form)
+  float inv_cnt_f[2];
 };
 
 struct HalfDesc {       // one half flux computed by its owner cell in K1
@@ -77,6 +81,11 @@ struct HalfDesc {       // one half flux computed by its owner cell in K1
   double xrec[3];       // cell-record midpoint used for the reconstruction
   double dsrec[3];      // cell-record size used by the limiter
   double fmid[3];       // face midpoint
+  // FP32 advection form: rot, fmid - xrec (formed in double, then rounded) and dsrec as float
+  float rot_f;
+  float dx_f[3];
+  float ds_f[3];
+  float pad_f;
 };
 
 struct EntryDesc {      // one face side of a fluid cell, gathered in K2
@@ -93,7 +102,18 @@ struct EntryDesc {      // one face side of a fluid cell, gathered in K2
   double fmid[3];
   double xown[3], dsown[3];
   double xoth[3], dsoth[3];
+  // FP32 advection form: rot, area, fmid - x (formed in double, then rounded) and ds as float
+  float rot_f, area_f;
+  float dxo_f[3], dso_f[3];
+  float dxt_f[3], dst_f[3];
 };
+
+// the double or the float copy of a descriptor value, for arithmetic type T
+template <class T>
+__host__ __device__ inline T desc_val(double d, float f) {
+  if constexpr (sizeof(T) == 8) return d;
+  else return f;
+}
 
 // a / b given rb = RN(1/b): q = RN(a*rb) corrected by one FMA step on the exact residual,
 // which is the correctly rounded quotient (Markstein's theorem; no overflow or underflow
