@@ -181,6 +181,44 @@ int kah_pack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind, do
 int kah_unpack_mirrors(kah_ctx* ctx, const int32_t* cells, int64_t n, int kind,
                        const double* in);
 
+/* ---- velocity-space adaptation on the device ------------------------------------- */
+/* vs_adaptive_mesh_refinement! (src/Velocity_space/AMR.jl:168-181) with the distribution
+ * kept in HBM: the criteria run on the device and only one flag byte per velocity cell
+ * goes to the host, which rebuilds the grids (index work) and hands them back; df is
+ * remapped and projected on the device. Parameters as in Solver (src/Solver/Types.jl:79-100). */
+enum { KAH_VS_LOHNER = 0,          /* ADAPT_VS_MODE = :lohner (the default)            */
+       KAH_VS_CONTRIBUTION = 1 };  /* the contribution criteria                         */
+enum { KAH_VS_REFINE = 1, KAH_VS_COARSEN = 2 };   /* flag bits */
+typedef struct kah_vs_adapt {
+  int32_t mode;          /* KAH_VS_*                                                    */
+  int32_t reserved;
+  double coeff_local;    /* ADAPT_COEFFI_VS_LOCAL  (1e-2)                               */
+  double coeff_global;   /* ADAPT_COEFFI_VS_GLOBAL (0.125)                              */
+  double coeff_lohner;   /* ADAPT_COEFFI_VS_LOHNER (0.6)                                */
+} kah_vs_adapt;
+/* vr[0..1] = Velocity_Resolution (density, energy) of the current state, the maxima over
+ * the fluid cells of vs_resolution (src/Velocity_space/AMR.jl:133-167). */
+int kah_vs_resolution(kah_ctx* ctx, double* vr);
+/* flags[P]: per velocity cell, KAH_VS_REFINE as vs_refine! decides it (level below
+ * vs_maxlevel and the criterion, src/Velocity_space/AMR.jl:24-65) and KAH_VS_COARSEN as
+ * vs_coarsen!'s per-cell willingness (:69-111), both on the current grids, state, slopes,
+ * w and prim, with the given vr (vs_adaptive_mesh_refinement! computes vr once, before
+ * refining, and uses it for both passes). */
+int kah_vs_flags(kah_ctx* ctx, const kah_vs_adapt* params, const double* vr, uint8_t* flags);
+/* Replaces the velocity grids (new_topo: the same cells, faces and neighbours, new vs_*
+ * arrays) and carries the state over on the device: for new velocity cell i of cell c,
+ * src[off_new[c] + i] = j >= 0 takes the old cell j of c (kept cell, or a refined parent
+ * injected into its children: refine_grid_stream!, src/Velocity_space/Rebuild.jl:44-85),
+ * src = -(j+1) takes the mean of the 2^dim old cells j.. (coarsen_grid_stream!, :96-163).
+ * Slopes of cells with changed[c] != 0 are zero afterwards (the rebuilt VsData), others
+ * keep theirs; w and prim are kept; flux accumulators are zero. */
+int kah_vs_remap(kah_ctx* ctx, const kah_topology* new_topo, const int32_t* src,
+                 const uint8_t* changed);
+/* conserved_I_porjection! of the listed cells' h onto their w (vs_conserved_correction!,
+ * src/Velocity_space/AMR.jl:117-131; src/Theory/I-projection.jl:144-158); cells inside a
+ * solid are skipped. */
+int kah_vs_project(kah_ctx* ctx, const int32_t* cells, int64_t n);
+
 /* Per-rank phase-space arrays of a restart checkpoint, as save_for_restart collects them
  * (src/IO/Restart.jl:80-117) and _restart_build_ps_list reads them (:306-321): N = n_cells,
  * Nv = sum of vs_num over the cells outside solids; matrices column-major:

@@ -80,6 +80,9 @@ struct kah_ctx {
   // kah_pack_mirrors / kah_unpack_mirrors (grown on demand)
   std::vector<int64_t> voff_h;
   DBuf d_cmid, d_benc, d_xcells, d_xoff, d_xbuf, d_xfluid;
+  // velocity-space adaptation: the root lattice of the velocity grids (valid when vs_ok)
+  VsArgs vsa{};
+  bool vs_ok = false;
   bool ov_enable = true;   // overlap maps for adapted grids
   int64_t part_rows = 0;
   int cur = 0;
@@ -597,6 +600,29 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       C->vt_n0 = n0;
       C->vt_n1 = n1;
     }
+  }
+
+  // root lattice for velocity-space adaptation (VsNeighborIndex, src/Velocity_space/
+  // Neighbor.jl:61-124: ds0 = quadrature extent / vs_trees_num, h_fine = ds0 / 2^maxlevel)
+  C->vs_ok = T->vs_maxlevel >= 0 && T->vs_maxlevel < 16;
+  C->vsa = VsArgs{};
+  {
+    double du = 1.0;
+    int64_t tn = 1, stride = 1;
+    for (int d = 0; d < D; ++d) {
+      if (T->vs_trees_num[d] <= 0 || !(T->vs_hi[d] > T->vs_lo[d])) C->vs_ok = false;
+      if (!C->vs_ok) break;
+      C->vsa.vmin[d] = T->vs_lo[d];
+      C->vsa.hf[d] = ((T->vs_hi[d] - T->vs_lo[d]) / double(T->vs_trees_num[d])) /
+                     std::ldexp(1.0, T->vs_maxlevel);
+      C->vsa.tn[d] = T->vs_trees_num[d];
+      C->vsa.stride[d] = stride;
+      stride *= T->vs_trees_num[d];
+      du = d == 0 ? (T->vs_hi[d] - T->vs_lo[d]) : du * (T->vs_hi[d] - T->vs_lo[d]);
+      tn *= T->vs_trees_num[d];
+    }
+    C->vsa.L = T->vs_maxlevel;
+    if (C->vs_ok) C->vsa.wfine = du / double(tn) / double(int64_t(1) << (D * T->vs_maxlevel));
   }
 
   // velocity grids: upload, and keys for adapted grids
@@ -1274,6 +1300,141 @@ int kah_load_restart(kah_ctx* C, const int64_t* vs_nums, const int64_t* bound_en
   KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "restart load"));
   C->have_state = true;
   return KAH_OK;
+}
+
+int kah_vs_resolution(kah_ctx* C, double* vr) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (!vr) return C->fail(KAH_EINVAL, "null vr");
+  if (!C->vs_ok) return C->fail(KAH_EINVAL, "velocity root grid (vs_trees_num / vs_lo / vs_hi) not set");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  DBuf d_res;
+  KAH_TRY(C->alloc(d_res, sizeof(double) * size_t(2 * C->nc), "vs resolution"));
+  DevArgs A = make_args(C, 0.0);
+  KAH_TRY(C->hip(launch_vs_res(A, C->D, C->uni, C->f32, C->vsa, int(C->nc), d_res.as<double>(),
+                               C->stream), "vs resolution launch"));
+  std::vector<double> r(size_t(2 * C->nc));
+  KAH_TRY(C->hip(hipMemcpyAsync(r.data(), d_res.p, d_res.bytes, hipMemcpyDeviceToHost, C->stream),
+                 "vs resolution download"));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "vs resolution"));
+  // maxima over the fluid cells (vs_resolution(trees), src/Velocity_space/AMR.jl:137-151)
+  double dres = 0.0, eres = 0.0;
+  auto jmax = [](double a, double b) { return (a != a) ? a : (b != b) ? b : (b > a ? b : a); };
+  for (int64_t c = 0; c < C->nc; ++c) {
+    if (C->benc[c] < 0) continue;
+    dres = jmax(r[size_t(2 * c)], dres);
+    eres = jmax(r[size_t(2 * c + 1)], eres);
+  }
+  vr[0] = dres;
+  vr[1] = eres;
+  return KAH_OK;
+}
+
+int kah_vs_flags(kah_ctx* C, const kah_vs_adapt* p, const double* vr, uint8_t* flags) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (!p || !vr || !flags) return C->fail(KAH_EINVAL, "null argument");
+  if (p->mode != KAH_VS_LOHNER && p->mode != KAH_VS_CONTRIBUTION)
+    return C->fail(KAH_EINVAL, "unknown velocity adaptation mode " + std::to_string(p->mode));
+  if (!C->vs_ok) return C->fail(KAH_EINVAL, "velocity root grid (vs_trees_num / vs_lo / vs_hi) not set");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  VsArgs P = C->vsa;
+  P.mode = p->mode;
+  P.cl = p->coeff_local;
+  P.cg = p->coeff_global;
+  P.tl = p->coeff_lohner;
+  P.vr[0] = vr[0];
+  P.vr[1] = vr[1];
+  DBuf d_fl;
+  KAH_TRY(C->alloc(d_fl, size_t(C->np), "vs flags"));
+  DevArgs A = make_args(C, 0.0);
+  KAH_TRY(C->hip(launch_vs_flags(A, C->D, C->uni, C->f32, P, int(C->nc), d_fl.as<uint8_t>(),
+                                 C->stream), "vs flags launch"));
+  KAH_TRY(C->hip(hipMemcpyAsync(flags, d_fl.p, size_t(C->np), hipMemcpyDeviceToHost, C->stream),
+                 "vs flags download"));
+  return C->hip(hipStreamSynchronize(C->stream), "vs flags");
+}
+
+int kah_vs_remap(kah_ctx* C, const kah_topology* T, const int32_t* src, const uint8_t* changed) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (!T || !src || !changed) return C->fail(KAH_EINVAL, "null argument");
+  if (T->n_cells != C->nc) return C->fail(KAH_EINVAL, "the new topology has another cell count");
+  if (!T->vs_offset) return C->fail(KAH_EINVAL, "null array in topology");
+  // every source stays inside its cell's old grid
+  const int64_t nc = C->nc, nch = int64_t(1) << C->D;
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t n_old = C->voff_h[c + 1] - C->voff_h[c];
+    const int64_t b = T->vs_offset[c], e = T->vs_offset[c + 1];
+    if (e <= b) return C->fail(KAH_EINVAL, "empty or unordered velocity grid of cell " + std::to_string(c));
+    if (!changed[c] && e - b != n_old)
+      return C->fail(KAH_EINVAL, "cell " + std::to_string(c) + " is marked unchanged but its grid size changed");
+    for (int64_t i = b; i < e; ++i) {
+      const int64_t j = src[i];
+      const bool ok = j >= 0 ? j < n_old : (-j - 1) + nch <= n_old;
+      if (!ok)
+        return C->fail(KAH_EINVAL, "remap source out of range in cell " + std::to_string(c));
+      if (!changed[c] && j != i - b)
+        return C->fail(KAH_EINVAL, "cell " + std::to_string(c) + " is marked unchanged but remapped");
+    }
+  }
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "remap"));
+  // the old state leaves the context; kah_set_topology allocates the new buffers
+  DBuf o_df, o_sdf, o_voff, o_w, o_prim, o_qf;
+  std::swap(o_df, C->d_df[C->cur]);
+  std::swap(o_sdf, C->d_sdf);
+  std::swap(o_voff, C->d_voff);
+  std::swap(o_w, C->d_w);
+  std::swap(o_prim, C->d_prim);
+  std::swap(o_qf, C->d_qf);
+  KAH_TRY(kah_set_topology(C, T));
+  const int64_t NW = C->NW;
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_w.p, o_w.p, sizeof(double) * nc * NW, hipMemcpyDeviceToDevice,
+                                C->stream), "w copy"));
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_prim.p, o_prim.p, sizeof(double) * nc * NW,
+                                hipMemcpyDeviceToDevice, C->stream), "prim copy"));
+  KAH_TRY(C->hip(hipMemcpyAsync(C->d_qf.p, o_qf.p, sizeof(double) * nc * C->D,
+                                hipMemcpyDeviceToDevice, C->stream), "qf copy"));
+  DBuf d_src, d_ch;
+  KAH_TRY(C->upload(d_src, src, size_t(C->np), "remap sources"));
+  KAH_TRY(C->upload(d_ch, changed, size_t(nc), "remap changed flags"));
+  DevArgs A = make_args(C, 0.0);
+  A.df_out = C->d_df[C->cur].as<double>();   // the remapped state is the current f
+  KAH_TRY(C->hip(launch_vs_remap(A, C->D, C->f32, o_voff.as<int64_t>(), o_df.p, o_sdf.p,
+                                 d_src.as<int32_t>(), d_ch.as<uint8_t>(), int(nc), C->stream),
+                 "remap launch"));
+  KAH_TRY(C->zero(C->d_psflux));
+  KAH_TRY(C->zero(C->d_vflux));
+  KAH_TRY(C->hip(hipStreamSynchronize(C->stream), "remap"));
+  C->have_state = true;
+  return KAH_OK;
+}
+
+int kah_vs_project(kah_ctx* C, const int32_t* cells, int64_t n) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (n < 0 || (n > 0 && !cells)) return C->fail(KAH_EINVAL, "null or negative cell list");
+  std::vector<int32_t> list;
+  for (int64_t k = 0; k < n; ++k) {
+    if (cells[k] < 0 || cells[k] >= C->nc) return C->fail(KAH_EINVAL, "cell id out of range");
+    if (C->benc[cells[k]] >= 0) list.push_back(cells[k]);   // solid cells are skipped
+  }
+  if (list.empty()) return KAH_OK;
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  DBuf d_cells, d_scr;
+  KAH_TRY(C->upload(d_cells, list.data(), list.size(), "projection cells"));
+  const size_t lds = sizeof(double) * size_t(C->NDF + 1) * size_t(C->max_n);
+  double* scr = nullptr;
+  if (lds > 64 * 1024) {
+    KAH_TRY(C->alloc(d_scr, sizeof(double) * size_t(C->np) * size_t(C->NDF + 1), "projection scratch"));
+    scr = d_scr.as<double>();
+  }
+  DevArgs A = make_args(C, 0.0);
+  KAH_TRY(C->hip(launch_vs_project(A, C->D, C->uni, C->f32, d_cells.as<int32_t>(),
+                                   int(list.size()), lds, scr, C->d_df[C->cur].p, C->stream),
+                 "projection launch"));
+  return C->hip(hipStreamSynchronize(C->stream), "projection");
 }
 
 int kah_get_flux(kah_ctx* C, double* flux, double* vs_flux) {

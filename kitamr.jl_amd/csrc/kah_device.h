@@ -840,66 +840,20 @@ __device__ inline void block_min2(double a, double b, double* red, double* out) 
   __syncthreads();
 }
 
-// The CIP part of iterate!(CIP_Marching) (src/Theory/I-projection.jl:161-192) after the
-// convection pass, which left f* in sc and the partial sums of the heat flux (w.r.t. prim_c),
-// of e_int and the minima of h: g+ = M(prim_c) + Shakhov(q), the conserved I-projection of h
-// (conserved_I_porjection!, :144-158, with solve_I_projection, :55-141: shave, Newton on the
-// dual, Armijo backtracking evaluated without cancellation), then f = f*tau/(tau+dt) +
-// dt/(tau+dt)*g+, written to the next f. The same algorithm as the oracle's cip_projection;
-// thread 0 takes the per-cell decisions, all threads loop uniformly on them.
+// The iterative part of solve_I_projection (src/Theory/I-projection.jl:74-141) on h in
+// sc[0..n): the shave of negative entries, then Newton on the dual with Armijo backtracking.
+// st holds W[0..5) and lam[5..10) (in: W, lam = 0; out: lam), dl[10..15) and the scalars set
+// up by thread 0: 16 tolerance, 17 previous |G|, 18 stall count, 19 done, 24 f_min,
+// 25 fp_min, 26 d, 27 1/d, 28 shave; 20-23 and 40.. are work space. sc[NDF*n..) receives the
+// c_k of the last Newton step.
 template <int D, int NDF, bool UNI, class FT, int NWV>
-__device__ inline void relax_cip(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int n,
-                                 double* sc, double* red, double* cell_sh,
-                                 const double (&qa)[(D == 2) ? 4 : 3], double eint, double fmn,
-                                 double fmp, double (&q)[D], double (&pc)[D + 2]) {
-  constexpr int NW = D + 2, M = D + 2, NQ = (D == 2) ? 4 : 3, NJ = M * (M + 1) / 2;
+__device__ inline void cip_shave_newton(const Grid<D, NDF, UNI, FT>& G, int n, double* sc,
+                                        double* red, double* st) {
+  constexpr int M = D + 2, NJ = M * (M + 1) / 2;
   constexpr int BS = NWV * 64;
-  __shared__ double st[64];   // W[5] lam[5] dl[5] | scalars from 16 on
   double* W = st;
   double* lam = st + 5;
   double* dl = st + 10;
-  // scalars: 16 tol_eff, 17 g_prev, 18 stall, 19 done, 20 slope, 21 dW, 22 alpha, 23 accept,
-  // 24 f_min, 25 fp_min, 26 d, 27 1/d, 28 shave, 29.. q, 40.. sums
-  {
-    double v[NQ + 1];
-#pragma unroll
-    for (int m = 0; m < NQ; ++m) v[m] = qa[m];
-    v[NQ] = eint;
-    block_reduce<NQ + 1, NWV>(v, NQ + 1, red, &st[40]);
-    block_min2<NWV>(fmn, fmp, red, &st[46]);
-  }
-  if (threadIdx.x == 0) {
-    double qs[NQ], qq[D];
-#pragma unroll
-    for (int m = 0; m < NQ; ++m) qs[m] = st[40 + m];
-    heat_finish<D>(qs, qq);
-#pragma unroll
-    for (int d = 0; d < D; ++d) st[29 + d] = qq[d];
-    double wn = 0.0;
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      W[m] = cell_sh[m];
-      lam[m] = 0.0;
-    }
-    if constexpr (NDF == 2) W[M - 1] -= st[40 + NQ] / 2;   // e_int = dot(weight, b)/2
-#pragma unroll
-    for (int m = 0; m < M; ++m) wn += W[m] * W[m];
-    st[16] = 1e-10 * fmax(1.0, sqrt(wn));
-    st[17] = HUGE_VAL;
-    st[18] = 0.0;
-    st[19] = 0.0;
-    const double f_min = 1.1 * st[46];
-    st[24] = f_min;
-    st[25] = st[47];
-    st[26] = st[47] - f_min;
-    st[27] = 1.0 / st[26];
-    st[28] = f_min < 0 ? 1.0 : 0.0;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int d = 0; d < D; ++d) q[d] = st[29 + d];
-#pragma unroll
-  for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
   // shave negative entries of h: (h - f_min)/d*fp_min
   if (st[28] != 0.0) {
     const double f_min = st[24], fp_min = st[25], dd = st[26], rd = st[27];
@@ -1007,6 +961,69 @@ __device__ inline void relax_cip(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
       for (int m = 0; m < M; ++m) lam[m] += st[22] * dl[m];
     __syncthreads();
   }
+}
+
+// The CIP part of iterate!(CIP_Marching) (src/Theory/I-projection.jl:161-192) after the
+// convection pass, which left f* in sc and the partial sums of the heat flux (w.r.t. prim_c),
+// of e_int and the minima of h: g+ = M(prim_c) + Shakhov(q), the conserved I-projection of h
+// (conserved_I_porjection!, :144-158, with solve_I_projection, :55-141: shave, Newton on the
+// dual, Armijo backtracking evaluated without cancellation), then f = f*tau/(tau+dt) +
+// dt/(tau+dt)*g+, written to the next f. The same algorithm as the oracle's cip_projection;
+// thread 0 takes the per-cell decisions, all threads loop uniformly on them.
+template <int D, int NDF, bool UNI, class FT, int NWV>
+__device__ inline void relax_cip(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int n,
+                                 double* sc, double* red, double* cell_sh,
+                                 const double (&qa)[(D == 2) ? 4 : 3], double eint, double fmn,
+                                 double fmp, double (&q)[D], double (&pc)[D + 2]) {
+  constexpr int NW = D + 2, M = D + 2, NQ = (D == 2) ? 4 : 3, NJ = M * (M + 1) / 2;
+  constexpr int BS = NWV * 64;
+  __shared__ double st[64];   // W[5] lam[5] dl[5] | scalars from 16 on
+  double* W = st;
+  double* lam = st + 5;
+  double* dl = st + 10;
+  // scalars: 16 tol_eff, 17 g_prev, 18 stall, 19 done, 20 slope, 21 dW, 22 alpha, 23 accept,
+  // 24 f_min, 25 fp_min, 26 d, 27 1/d, 28 shave, 29.. q, 40.. sums
+  {
+    double v[NQ + 1];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) v[m] = qa[m];
+    v[NQ] = eint;
+    block_reduce<NQ + 1, NWV>(v, NQ + 1, red, &st[40]);
+    block_min2<NWV>(fmn, fmp, red, &st[46]);
+  }
+  if (threadIdx.x == 0) {
+    double qs[NQ], qq[D];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) qs[m] = st[40 + m];
+    heat_finish<D>(qs, qq);
+#pragma unroll
+    for (int d = 0; d < D; ++d) st[29 + d] = qq[d];
+    double wn = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      W[m] = cell_sh[m];
+      lam[m] = 0.0;
+    }
+    if constexpr (NDF == 2) W[M - 1] -= st[40 + NQ] / 2;   // e_int = dot(weight, b)/2
+#pragma unroll
+    for (int m = 0; m < M; ++m) wn += W[m] * W[m];
+    st[16] = 1e-10 * fmax(1.0, sqrt(wn));
+    st[17] = HUGE_VAL;
+    st[18] = 0.0;
+    st[19] = 0.0;
+    const double f_min = 1.1 * st[46];
+    st[24] = f_min;
+    st[25] = st[47];
+    st[26] = st[47] - f_min;
+    st[27] = 1.0 / st[26];
+    st[28] = f_min < 0 ? 1.0 : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = st[29 + d];
+#pragma unroll
+  for (int m = 0; m < NW; ++m) pc[m] = cell_sh[NW + m];
+  cip_shave_newton<D, NDF, UNI, FT, NWV>(G, n, sc, red, st);
   // h *= exp(lam.psi); g+ = M(prim_c) + Shakhov; f = f*tau/(tau+dt) + dt/(tau+dt)*g+
   double lm[M];
 #pragma unroll
@@ -1506,6 +1523,352 @@ inline hipError_t launch_mirror_t(const DevArgs& A, const int32_t* cells, const 
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_mirror<D, NDF, UNI, FT>), dim3(n), dim3(256), 0, st, A, cells, off, kind,
                      unpack, buf, mid, benc, sw, static_cast<FT*>(dfw));
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ velocity-space adaptation
+// vs_adaptive_mesh_refinement! (src/Velocity_space/AMR.jl:133-181) on the device: the
+// resolution, the per-velocity-cell criteria (Criteria.jl), the remap of the state onto the
+// rebuilt grids (Rebuild.jl) and the conservative I-projection. The face neighbours of the
+// Loehner indicator come from a binary search over the cell's keys (the grid is in key
+// order), as VsNeighborIndex locates them (Neighbor.jl:126-201).
+template <int D>
+__device__ inline uint64_t vs_code(const VsArgs& P, const int64_t (&g)[D]) {
+  int64_t root = 0;
+  uint64_t morton = 0;
+  const int64_t mask = (int64_t(1) << P.L) - 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    root += (g[d] >> P.L) * P.stride[d];
+    const int64_t q = g[d] & mask;
+    for (int b = 0; b < P.L; ++b) morton |= uint64_t((q >> b) & 1) << (b * D + d);
+  }
+  return (uint64_t(root) << (D * P.L)) | morton;
+}
+
+template <int D, int NDF, bool UNI, class FT>
+struct VsGrid {
+  const Grid<D, NDF, UNI, FT>& G;
+  const VsArgs& P;
+  __device__ int lev(int j) const { return int(G.A.glev[G.goff() + j]); }
+  __device__ double size(int j, int d) const {
+    return P.hf[d] * double(int64_t(1) << (P.L - lev(j)));
+  }
+  __device__ uint64_t key(int j) const {   // _corner_index, Neighbor.jl:38-41
+    int64_t g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      g[d] = int64_t(rint((G.u(j, d) - 0.5 * size(j, d) - P.vmin[d]) / P.hf[d]));
+    return vs_code<D>(P, g);
+  }
+  __device__ int locate(uint64_t code) const {   // _vs_locate_code, Neighbor.jl:126-139
+    int lo = 0, hi = G.n - 1, k = -1;
+    while (lo <= hi) {
+      const int m = (lo + hi) >> 1;
+      if (key(m) <= code) {
+        k = m;
+        lo = m + 1;
+      } else {
+        hi = m - 1;
+      }
+    }
+    if (k < 0) return -1;
+    const uint64_t kk = key(k), span = uint64_t(1) << (D * (P.L - lev(k)));
+    return (code >= kk && code < kk + span) ? k : -1;
+  }
+  __device__ int face(int i, int dim, int dir) const {   // vs_face_neighbor, :186-201
+    const double cell = size(i, dim);
+    int64_t g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double coord = G.u(i, d);
+      if (d == dim) coord += double(dir) * (0.5 * cell + 0.5 * P.hf[d]);
+      const double x = floor((coord - P.vmin[d]) / P.hf[d]);
+      if (x < 0 || x >= double(P.tn[d] << P.L)) return -1;
+      g[d] = int64_t(x);
+    }
+    return locate(vs_code<D>(P, g));
+  }
+};
+
+__device__ inline double jl_maxd(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return b > a ? b : a;
+}
+
+// _lohner_ratio, src/Velocity_space/Criteria.jl:221-227
+__device__ inline double lohner_ratio(double L, double C, double R, double dsL, double dsR,
+                                      double scale) {
+  const double eflt = 1e-2, eabs = 1e-3;
+  const double num = fabs(dsR * L - (dsL + dsR) * C + dsL * R);
+  const double den = dsR * fabs(L - C) + dsL * fabs(R - C) +
+                     eflt * (dsR * fabs(L) + (dsL + dsR) * fabs(C) + dsL * fabs(R)) +
+                     eabs * scale * (dsL + dsR);
+  return den > 0 ? num / den : 0.0;
+}
+
+// vs_resolution of one cell (AMR.jl:153-167): out[2c] = max(df)*w_fine,
+// out[2c+1] = 0.5*max(h*c^2 (+ b))*w_fine
+template <int D, int NDF, bool UNI, class FT>
+__global__ __launch_bounds__(256) void k_vs_res(DevArgs A, VsArgs P, double* __restrict__ out) {
+  constexpr int NW = D + 2;
+  __shared__ double red[2 * 4];
+  __shared__ double res[2];
+  const int c = blockIdx.x;
+  const Grid<D, NDF, UNI, FT> G(A, c);
+  const double* U = &A.prim[int64_t(c) * NW + 1];
+  double dm = -HUGE_VAL, em = -HUGE_VAL;
+  for (int i = threadIdx.x; i < G.n; i += 256) {
+    double c2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double t = U[d] - G.u(i, d);
+      c2 = d == 0 ? t * t : c2 + t * t;
+    }
+    const double h = G.f(i, 0);
+    dm = jl_maxd(dm, h);
+    if constexpr (NDF == 2) {
+      const double b = G.f(i, 1);
+      dm = jl_maxd(dm, b);
+      em = jl_maxd(em, h * c2 + b);
+    } else {
+      em = jl_maxd(em, h * c2);
+    }
+  }
+  block_min2<4>(-dm, -em, red, res);
+  if (threadIdx.x == 0) {
+    out[2 * c] = -res[0] * P.wfine;
+    out[2 * c + 1] = 0.5 * -res[1] * P.wfine;
+  }
+}
+
+// refine / coarsen flags of every velocity cell of one cell (vs_refine!, vs_coarsen!,
+// AMR.jl:24-111, with the criteria of Criteria.jl)
+template <int D, int NDF, bool UNI, class FT>
+__global__ __launch_bounds__(256) void k_vs_flags(DevArgs A, VsArgs P,
+                                                  uint8_t* __restrict__ flags) {
+  constexpr int NW = D + 2;
+  __shared__ double red[2 * 4];
+  __shared__ double scl[2];
+  const int c = blockIdx.x;
+  const Grid<D, NDF, UNI, FT> G(A, c);
+  const VsGrid<D, NDF, UNI, FT> X{G, P};
+  const int n = G.n;
+  const bool lohner = P.mode == VS_LOHNER;
+  // vs_lohner_scales: peaks of |h| and |b|
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double x = fabs(G.f(i, 0));
+    if (x > a1) a1 = x;
+    if constexpr (NDF == 2) {
+      const double y = fabs(G.f(i, 1));
+      if (y > a2) a2 = y;
+    }
+  }
+  block_min2<4>(-a1, -a2, red, scl);
+  const double s1 = -scl[0], s2 = -scl[1];
+  const double* w = &A.w[int64_t(c) * NW];
+  const double* U = &A.prim[int64_t(c) * NW + 1];
+  const double* ds = &A.rec_ds[int64_t(c) * D];
+  double SU = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) SU = d == 0 ? U[d] * U[d] : SU + U[d] * U[d];
+  const double den = w[NW - 1] - 0.5 * w[0] * SU;
+  const double P2 = double(1 << D);
+  const double tauC = 0.3 * P.tl;   // VS_LOHNER_COARSEN_RATIO * ADAPT_COEFFI_VS_LOHNER
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double cdf[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {   // _criterion_cell!: df + max_d |sdf*ds|
+      double m = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double x = fabs(G.s(i, k, d) * ds[d]);
+        if (x > m) m = x;
+      }
+      cdf[k] = G.f(i, k) + m;
+    }
+    const double wt = G.wt(i);
+    double S = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double t = U[d] - G.u(i, d);
+      S = d == 0 ? t * t : S + t * t;
+    }
+    const double c1w = cdf[0] * wt;
+    double loc_r, loc_c, e_w;
+    if constexpr (NDF == 2) {
+      e_w = 0.5 * (S * cdf[0] + cdf[1]) * wt;
+      loc_r = jl_maxd(fabs(e_w) / den, c1w / w[0]);
+      loc_c = jl_maxd(e_w / den, c1w / w[0]);
+    } else {
+      loc_r = jl_maxd(fabs(0.5 * S * cdf[0] * wt) / den, c1w / w[0]);
+      e_w = 0.5 * (S * cdf[0]) * wt;
+      loc_c = jl_maxd(e_w / den, c1w / w[0]);
+    }
+    const bool local_refine = loc_r > P.cl;
+    const bool local_coarsen = loc_c < P.cl / P2;
+    const bool global_refine =
+        (NDF == 2 ? c1w > P.cg * P.vr[0] : c1w > P.vr[0] * P.cg) || e_w > P.vr[1] * P.cg;
+    const bool global_coarsen =
+        (NDF == 2 ? c1w < P.cg * P.vr[0] / P2 : c1w < P.vr[0] * P.cg / P2) &&
+        e_w < P.vr[1] * P.cg / P2;
+    double eta = 0.0;
+    if (lohner) {   // vs_lohner_indicator, Criteria.jl:254-283
+      const double h0 = G.f(i, 0), b0 = NDF == 2 ? G.f(i, NDF - 1) : 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double hi = X.size(i, d);
+        const int Ln = X.face(i, d, -1), Rn = X.face(i, d, +1);
+        const double dsL = Ln < 0 ? hi : 0.5 * (hi + X.size(Ln, d));
+        const double dsR = Rn < 0 ? hi : 0.5 * (hi + X.size(Rn, d));
+        const double hL = Ln < 0 ? 0.0 : G.f(Ln, 0), hR = Rn < 0 ? 0.0 : G.f(Rn, 0);
+        eta = jl_maxd(eta, lohner_ratio(hL, h0, hR, dsL, dsR, s1));
+        if constexpr (NDF == 2) {
+          const double bL = Ln < 0 ? 0.0 : G.f(Ln, 1), bR = Rn < 0 ? 0.0 : G.f(Rn, 1);
+          eta = jl_maxd(eta, lohner_ratio(bL, b0, bR, dsL, dsR, s2));
+        }
+      }
+    }
+    const bool base = lohner ? (eta > P.tl || local_refine) : (local_refine || global_refine);
+    const bool refine = X.lev(i) < P.L && base;
+    const bool coarsen = lohner ? (eta < tauC && local_coarsen) : (local_coarsen && global_coarsen);
+    flags[G.off + i] = uint8_t((refine ? VS_REFINE : 0) | (coarsen ? VS_COARSEN : 0));
+  }
+}
+
+// The state on rebuilt grids: new velocity cell i of cell c takes old cell src >= 0, or the
+// mean of the 2^D old cells from -(src+1) (summed in order, then divided, as
+// coarsen_grid_stream! does); slopes are copied for unchanged cells and zero otherwise.
+template <int D, int NDF, class FT>
+__global__ __launch_bounds__(256) void k_vs_remap(DevArgs A, const int64_t* __restrict__ voff_old,
+                                                  const FT* __restrict__ df_old,
+                                                  const FT* __restrict__ sdf_old,
+                                                  const int32_t* __restrict__ src,
+                                                  const uint8_t* __restrict__ changed) {
+  const int c = blockIdx.x;
+  const int64_t on = A.voff[c], nn = A.voff[c + 1] - on;
+  const int64_t oo = voff_old[c], no = voff_old[c + 1] - oo;
+  FT* df = reinterpret_cast<FT*>(A.df_out);
+  FT* sdf = reinterpret_cast<FT*>(A.sdf);
+  const bool ch = changed[c] != 0;
+  for (int64_t i = threadIdx.x; i < nn; i += 256) {
+    const int32_t j = src[on + i];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      double v;
+      if (j >= 0) {
+        v = double(df_old[oo * NDF + k * no + j]);
+      } else {
+        const int64_t j0 = -int64_t(j) - 1;
+        double sum = 0.0;
+        for (int g = 0; g < (1 << D); ++g) sum += double(df_old[oo * NDF + k * no + j0 + g]);
+        v = sum / double(1 << D);
+      }
+      df[on * NDF + k * nn + i] = FT(v);
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        sdf[on * NDF * D + (d * NDF + k) * nn + i] =
+            ch ? FT(0) : sdf_old[oo * NDF * D + (d * NDF + k) * no + i];
+    }
+  }
+}
+
+// conserved_I_porjection! of one listed cell's h onto its w (src/Theory/I-projection.jl:
+// 144-158): e_int removed from the energy for 2D2F, shave and Newton as in CIP_Marching,
+// then h *= exp(lam.psi), in place.
+template <int D, int NDF, bool UNI, class FT, int NWV>
+__global__ __launch_bounds__(NWV * 64) void k_vs_project(DevArgs A, const int32_t* __restrict__ cells,
+                                                         double* __restrict__ scratch,
+                                                         FT* __restrict__ dfw) {
+  constexpr int M = D + 2, BS = NWV * 64;
+  __shared__ double red[NWV * (D + 2) * (D + 5) / 2];   // G and J of the Newton step
+  __shared__ double st[64];
+  extern __shared__ double lds[];
+  const int c = cells[blockIdx.x];
+  const Grid<D, NDF, UNI, FT> G(A, c);
+  const int n = G.n;
+  double* sc = scratch ? &scratch[G.off * (NDF + 1)] : lds;
+  double eint = 0.0, fmn = HUGE_VAL, fmp = HUGE_VAL;
+  for (int p = threadIdx.x; p < n; p += BS) {
+    const int i = G.vi(p);
+    const double h = G.f(i, 0);
+    sc[i] = h;
+    if constexpr (NDF == 2) eint += G.wt(i) * G.f(i, 1);
+    fmn = h < fmn ? h : fmn;
+    if (h > 0.0 && h < fmp) fmp = h;
+  }
+  {
+    double v[1] = {eint};
+    block_reduce<1, NWV>(v, 1, red, &st[40]);
+    block_min2<NWV>(fmn, fmp, red, &st[46]);
+  }
+  if (threadIdx.x == 0) {
+    double wn = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      st[m] = A.w[int64_t(c) * M + m];
+      st[5 + m] = 0.0;
+    }
+    if constexpr (NDF == 2) st[M - 1] -= st[40] / 2;   // e_int = dot(weight, b)/2
+#pragma unroll
+    for (int m = 0; m < M; ++m) wn += st[m] * st[m];
+    st[16] = 1e-10 * fmax(1.0, sqrt(wn));
+    st[17] = HUGE_VAL;
+    st[18] = 0.0;
+    st[19] = 0.0;
+    const double f_min = 1.1 * st[46];
+    st[24] = f_min;
+    st[25] = st[47];
+    st[26] = st[47] - f_min;
+    st[27] = 1.0 / st[26];
+    st[28] = f_min < 0 ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  cip_shave_newton<D, NDF, UNI, FT, NWV>(G, n, sc, red, st);
+  double lm[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) lm[m] = st[5 + m];
+  FT* fo = dfw + G.off * NDF;
+  for (int p = threadIdx.x; p < n; p += BS) {
+    const int i = G.vi(p);
+    double u[D], ps[M];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+    cip_psi<D>(u, ps);
+    double dot = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) dot += lm[m] * ps[m];
+    G.at(fo, i) = FT(sc[i] * exp(dot));
+  }
+}
+
+template <int D, int NDF, bool UNI, class FT>
+inline hipError_t launch_vs_t(const DevArgs& A, int which, const VsArgs& P, int nc, double* out,
+                              uint8_t* flags, hipStream_t st) {
+  if (nc <= 0) return hipSuccess;
+  if (which == 0)
+    hipLaunchKernelGGL((k_vs_res<D, NDF, UNI, FT>), dim3(nc), dim3(256), 0, st, A, P, out);
+  else
+    hipLaunchKernelGGL((k_vs_flags<D, NDF, UNI, FT>), dim3(nc), dim3(256), 0, st, A, P, flags);
+  return hipGetLastError();
+}
+template <int D, int NDF, class FT>
+inline hipError_t launch_vs_remap_t(const DevArgs& A, const int64_t* voff_old, const void* df_old,
+                                    const void* sdf_old, const int32_t* src,
+                                    const uint8_t* changed, int nc, hipStream_t st) {
+  if (nc <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_vs_remap<D, NDF, FT>), dim3(nc), dim3(256), 0, st, A, voff_old,
+                     static_cast<const FT*>(df_old), static_cast<const FT*>(sdf_old), src,
+                     changed);
+  return hipGetLastError();
+}
+template <int D, int NDF, bool UNI, class FT>
+inline hipError_t launch_vs_project_t(const DevArgs& A, const int32_t* cells, int n, size_t lds,
+                                      double* scratch, void* dfw, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_vs_project<D, NDF, UNI, FT, 8>), dim3(n), dim3(512),
+                     scratch ? 0 : lds, st, A, cells, scratch, static_cast<FT*>(dfw));
   return hipGetLastError();
 }
 

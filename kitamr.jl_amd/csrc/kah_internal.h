@@ -36,6 +36,9 @@ enum : int8_t {
   MI_SELF_PLAIN = 6,  // UniformOutflow: the cell's own f * vn on the incoming set
 };
 
+// velocity-space adaptation: criteria mode and flag bits (the values of KAH_VS_*)
+enum { VS_LOHNER = 0, VS_CONTRIBUTION = 1, VS_REFINE = 1, VS_COARSEN = 2 };
+
 struct SlopeNb {        // one neighbour in a slope stencil
   int32_t cell;         // data cell
   int32_t proj;         // 1: project with the neighbour's transverse slopes

@@ -101,6 +101,25 @@ hipError_t launch_mirror(const DevArgs& A, int dim, bool uni, bool f32, const in
                          const int64_t* off, int n, int kind, int unpack, double* buf,
                          const double* mid, const int32_t* benc, double* sw, void* dfw,
                          hipStream_t st);
+// velocity-space adaptation: the root lattice and the criteria parameters
+struct VsArgs {
+  double vmin[3], hf[3];   // velocity-domain lower corner, finest cell size
+  int64_t tn[3], stride[3];
+  int32_t L;               // vs_maxlevel
+  int32_t mode;            // KAH_VS_*
+  double cl, cg, tl;       // ADAPT_COEFFI_VS_LOCAL / _GLOBAL / _LOHNER
+  double vr[2];            // Velocity_Resolution
+  double wfine;            // weight of a finest velocity cell
+};
+hipError_t launch_vs_res(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P, int nc,
+                         double* out, hipStream_t st);
+hipError_t launch_vs_flags(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P,
+                           int nc, uint8_t* flags, hipStream_t st);
+hipError_t launch_vs_remap(const DevArgs& A, int dim, bool f32, const int64_t* voff_old,
+                           const void* df_old, const void* sdf_old, const int32_t* src,
+                           const uint8_t* changed, int nc, hipStream_t st);
+hipError_t launch_vs_project(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                             int n, size_t lds, double* scratch, void* dfw, hipStream_t st);
 // gathers (load = 0) or scatters the per-rank restart arrays of all cells; row[c] is the first
 // velocity row of cell c, -1 for a solid cell
 hipError_t launch_restart(const DevArgs& A, int dim, bool uni, bool f32, const int64_t* row,

@@ -27,6 +27,18 @@ KAH_RESTART_DECL(launch_restart_3d)
 KAH_RESTART_DECL(launch_restart_2d_f32)
 KAH_RESTART_DECL(launch_restart_3d_f32)
 #undef KAH_RESTART_DECL
+#define KAH_VS_DECL(sfx)                                                                        \
+  hipError_t launch_vs_##sfx(const DevArgs&, bool, int, const VsArgs&, int, double*, uint8_t*,  \
+                             hipStream_t);                                                      \
+  hipError_t launch_vs_remap_##sfx(const DevArgs&, const int64_t*, const void*, const void*,    \
+                                   const int32_t*, const uint8_t*, int, hipStream_t);           \
+  hipError_t launch_vs_project_##sfx(const DevArgs&, bool, const int32_t*, int, size_t,         \
+                                     double*, void*, hipStream_t);
+KAH_VS_DECL(2d)
+KAH_VS_DECL(3d)
+KAH_VS_DECL(2d_f32)
+KAH_VS_DECL(3d_f32)
+#undef KAH_VS_DECL
 hipError_t launch_overlap_map_3d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
 hipError_t launch_slope_2d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
 hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
@@ -117,6 +129,38 @@ hipError_t launch_restart(const DevArgs& A, int dim, bool uni, bool f32, const i
                     : launch_restart_3d_f32(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st);
   return dim == 2 ? launch_restart_2d(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st)
                   : launch_restart_3d(A, uni, row, N, Nv, load, ws, lev, vmid, vdf, dfw, st);
+}
+
+hipError_t launch_vs_res(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P, int nc,
+                         double* out, hipStream_t st) {
+  if (f32) return dim == 2 ? launch_vs_2d_f32(A, uni, 0, P, nc, out, nullptr, st)
+                           : launch_vs_3d_f32(A, uni, 0, P, nc, out, nullptr, st);
+  return dim == 2 ? launch_vs_2d(A, uni, 0, P, nc, out, nullptr, st)
+                  : launch_vs_3d(A, uni, 0, P, nc, out, nullptr, st);
+}
+hipError_t launch_vs_flags(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P,
+                           int nc, uint8_t* flags, hipStream_t st) {
+  if (f32) return dim == 2 ? launch_vs_2d_f32(A, uni, 1, P, nc, nullptr, flags, st)
+                           : launch_vs_3d_f32(A, uni, 1, P, nc, nullptr, flags, st);
+  return dim == 2 ? launch_vs_2d(A, uni, 1, P, nc, nullptr, flags, st)
+                  : launch_vs_3d(A, uni, 1, P, nc, nullptr, flags, st);
+}
+hipError_t launch_vs_remap(const DevArgs& A, int dim, bool f32, const int64_t* voff_old,
+                           const void* df_old, const void* sdf_old, const int32_t* src,
+                           const uint8_t* changed, int nc, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_vs_remap_2d_f32(A, voff_old, df_old, sdf_old, src, changed, nc, st)
+                    : launch_vs_remap_3d_f32(A, voff_old, df_old, sdf_old, src, changed, nc, st);
+  return dim == 2 ? launch_vs_remap_2d(A, voff_old, df_old, sdf_old, src, changed, nc, st)
+                  : launch_vs_remap_3d(A, voff_old, df_old, sdf_old, src, changed, nc, st);
+}
+hipError_t launch_vs_project(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
+                             int n, size_t lds, double* scratch, void* dfw, hipStream_t st) {
+  if (f32)
+    return dim == 2 ? launch_vs_project_2d_f32(A, uni, cells, n, lds, scratch, dfw, st)
+                    : launch_vs_project_3d_f32(A, uni, cells, n, lds, scratch, dfw, st);
+  return dim == 2 ? launch_vs_project_2d(A, uni, cells, n, lds, scratch, dfw, st)
+                  : launch_vs_project_3d(A, uni, cells, n, lds, scratch, dfw, st);
 }
 
 hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int njobs,

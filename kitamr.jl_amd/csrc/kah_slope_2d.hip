@@ -45,4 +45,20 @@ hipError_t launch_restart_2d(const DevArgs& A, bool uni, const int64_t* row, int
                                                        dfw, st);
 }
 
+hipError_t launch_vs_2d(const DevArgs& A, bool uni, int which, const VsArgs& P, int nc,
+                          double* out, uint8_t* flags, hipStream_t st) {
+  return uni ? launch_vs_t<2, 2, true, double>(A, which, P, nc, out, flags, st)
+             : launch_vs_t<2, 2, false, double>(A, which, P, nc, out, flags, st);
+}
+hipError_t launch_vs_remap_2d(const DevArgs& A, const int64_t* voff_old, const void* df_old,
+                                const void* sdf_old, const int32_t* src, const uint8_t* changed,
+                                int nc, hipStream_t st) {
+  return launch_vs_remap_t<2, 2, double>(A, voff_old, df_old, sdf_old, src, changed, nc, st);
+}
+hipError_t launch_vs_project_2d(const DevArgs& A, bool uni, const int32_t* cells, int n,
+                                  size_t lds, double* scratch, void* dfw, hipStream_t st) {
+  return uni ? launch_vs_project_t<2, 2, true, double>(A, cells, n, lds, scratch, dfw, st)
+             : launch_vs_project_t<2, 2, false, double>(A, cells, n, lds, scratch, dfw, st);
+}
+
 }  // namespace kah

@@ -40,4 +40,20 @@ hipError_t launch_restart_3d_f32(const DevArgs& A, bool uni, const int64_t* row,
                                                        dfw, st);
 }
 
+hipError_t launch_vs_3d_f32(const DevArgs& A, bool uni, int which, const VsArgs& P, int nc,
+                          double* out, uint8_t* flags, hipStream_t st) {
+  return uni ? launch_vs_t<3, 1, true, float>(A, which, P, nc, out, flags, st)
+             : launch_vs_t<3, 1, false, float>(A, which, P, nc, out, flags, st);
+}
+hipError_t launch_vs_remap_3d_f32(const DevArgs& A, const int64_t* voff_old, const void* df_old,
+                                const void* sdf_old, const int32_t* src, const uint8_t* changed,
+                                int nc, hipStream_t st) {
+  return launch_vs_remap_t<3, 1, float>(A, voff_old, df_old, sdf_old, src, changed, nc, st);
+}
+hipError_t launch_vs_project_3d_f32(const DevArgs& A, bool uni, const int32_t* cells, int n,
+                                  size_t lds, double* scratch, void* dfw, hipStream_t st) {
+  return uni ? launch_vs_project_t<3, 1, true, float>(A, cells, n, lds, scratch, dfw, st)
+             : launch_vs_project_t<3, 1, false, float>(A, cells, n, lds, scratch, dfw, st);
+}
+
 }  // namespace kah

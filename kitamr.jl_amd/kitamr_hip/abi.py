@@ -26,7 +26,8 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_get_state", "kah_get_flux", "kah_slope", "kah_flux", "kah_iterate", "kah_step",
            "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
            "kah_copy_bandwidth", "kah_set_option", "kah_mirror_layout", "kah_pack_mirrors",
-           "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart", "kah_load_restart"]
+           "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart", "kah_load_restart",
+           "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap", "kah_vs_project"]
 MIRROR = {"data": 0, "slope": 1}
 OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5}
 
@@ -37,6 +38,19 @@ class KahParams(C.Structure):
                 ("st_check_interval", C.c_int32), ("device", C.c_int32),
                 ("reserved", C.c_int32), ("K", C.c_double), ("gamma", C.c_double),
                 ("Pr", C.c_double), ("mu_ref", C.c_double), ("omega", C.c_double)]
+
+
+class KahVsAdapt(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("reserved", C.c_int32), ("coeff_local", C.c_double),
+                ("coeff_global", C.c_double), ("coeff_lohner", C.c_double)]
+
+
+VS_MODE = {"lohner": 0, "contribution": 1}
+
+
+def vs_adapt_params(mode="lohner", coeff_local=1e-2, coeff_global=0.125, coeff_lohner=0.6):
+    """kah_vs_adapt with the reference's defaults (src/Solver/Types.jl:97-100)."""
+    return KahVsAdapt(VS_MODE[mode], 0, coeff_local, coeff_global, coeff_lohner)
 
 
 _P = C.c_void_p
@@ -154,11 +168,17 @@ def load_library(path=None):
     lib.kah_restart_sizes.argtypes = [C.c_void_p, lp, lp]
     lib.kah_save_restart.argtypes = [C.c_void_p, lp, lp, dp, C.POINTER(C.c_int8), dp, dp]
     lib.kah_load_restart.argtypes = [C.c_void_p, lp, lp, dp, dp]
+    u8p = C.POINTER(C.c_uint8)
+    lib.kah_vs_resolution.argtypes = [C.c_void_p, dp]
+    lib.kah_vs_flags.argtypes = [C.c_void_p, C.POINTER(KahVsAdapt), dp, u8p]
+    lib.kah_vs_remap.argtypes = [C.c_void_p, C.POINTER(KahTopology), ip, u8p]
+    lib.kah_vs_project.argtypes = [C.c_void_p, ip, i64]
     for name in ["kah_set_topology", "kah_set_state", "kah_get_state", "kah_get_flux",
                  "kah_slope", "kah_flux", "kah_iterate", "kah_step", "kah_sync",
                  "kah_kernel_times", "kah_mirror_layout", "kah_pack_mirrors",
                  "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart",
-                 "kah_load_restart"]:
+                 "kah_load_restart", "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap",
+                 "kah_vs_project"]:
         getattr(lib, name).restype = C.c_int
     del i32
     if path is None:
@@ -308,6 +328,37 @@ class HipContext:
         lp = lambda x: x.ctypes.data_as(C.POINTER(C.c_int64))
         self._check(self.lib.kah_load_restart(self.h, lp(vn), lp(be), _ptr(ws), _ptr(df)),
                     "kah_load_restart")
+
+    def vs_resolution(self):
+        """Velocity_Resolution (density, energy) of the current state."""
+        vr = np.empty(2)
+        self._check(self.lib.kah_vs_resolution(self.h, _ptr(vr)), "kah_vs_resolution")
+        return vr
+
+    def vs_flags(self, params, vr):
+        """Per-velocity-cell flag bytes (1: refine, 2: coarsen) of the current grids."""
+        out = np.empty(self.np, dtype=np.uint8)
+        v = np.ascontiguousarray(vr, dtype=np.float64)
+        self._check(self.lib.kah_vs_flags(self.h, C.byref(params), _ptr(v),
+                                          out.ctypes.data_as(C.POINTER(C.c_uint8))),
+                    "kah_vs_flags")
+        return out
+
+    def vs_remap(self, topo, src, changed):
+        """Replaces the velocity grids by those of topo, carrying the state over."""
+        s, keep = make_topology_struct(topo)
+        sv = np.ascontiguousarray(src, dtype=np.int32)
+        ch = np.ascontiguousarray(changed, dtype=np.uint8)
+        self._check(self.lib.kah_vs_remap(self.h, C.byref(s), sv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          ch.ctypes.data_as(C.POINTER(C.c_uint8))),
+                    "kah_vs_remap")
+        del keep
+        self.nc = int(topo["n_cells"])
+        self.np = int(np.asarray(topo["vs_offset"])[-1])
+
+    def vs_project(self, cells):
+        c, cp = self._cells(cells)
+        self._check(self.lib.kah_vs_project(self.h, cp, len(c)), "kah_vs_project")
 
     def slope(self, dt):
         self._check(self.lib.kah_slope(self.h, float(dt)), "kah_slope")

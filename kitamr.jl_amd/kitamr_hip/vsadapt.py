@@ -1,0 +1,172 @@
+"""Velocity-space adaptation with the distribution kept on the device.
+
+vs_adaptive_mesh_refinement! (src/Velocity_space/AMR.jl:168-181) in four moves: the device
+evaluates the criteria (``kah_vs_resolution`` / ``kah_vs_flags``), this module rebuilds the
+grids from one flag byte per velocity cell (refine_grid_stream! / coarsen_grid_stream!,
+src/Velocity_space/Rebuild.jl:44-163, on levels, midpoints and weights only, plus the source
+of every new velocity cell), the device remaps f onto the new grids (``kah_vs_remap``) and
+applies the conservative I-projection to the changed cells (``kah_vs_project``). f never
+leaves HBM.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .vspace import RMT
+
+
+def cell_grids(topo, dim):
+    """(level, midpoint [n, dim], weight) of every cell's velocity grid."""
+    off = np.asarray(topo["vs_offset"])
+    uni = int(topo.get("vs_uniform", 0))
+    lev = np.asarray(topo["vs_level"])
+    mid = np.asarray(topo["vs_mid"])
+    wt = np.asarray(topo["vs_weight"])
+    out = []
+    for c in range(len(off) - 1):
+        n = int(off[c + 1] - off[c])
+        g = 0 if uni else int(off[c])
+        out.append((lev[g:g + n].copy(), mid[g * dim:(g + n) * dim].reshape(dim, n).T.copy(),
+                    wt[g:g + n].copy()))
+    return out
+
+
+def with_grids(topo, grids, dim):
+    """The topology with per-cell velocity grids replaced (vs_uniform off)."""
+    t = dict(topo)
+    sizes = np.array([len(g[0]) for g in grids], dtype=np.int64)
+    t["vs_offset"] = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    t["vs_level"] = np.concatenate([g[0] for g in grids]).astype(np.int8)
+    t["vs_mid"] = np.concatenate([np.ascontiguousarray(g[1].T).reshape(-1) for g in grids])
+    t["vs_weight"] = np.concatenate([g[2] for g in grids]).astype(np.float64)
+    t["vs_uniform"] = 0
+    return t
+
+
+def root_ds(topo, dim):
+    """Root cell size per dimension, as vs_refine! forms it (quadrature extent / trees)."""
+    return np.array([(topo["vs_hi"][d] - topo["vs_lo"][d]) / topo["vs_trees_num"][d]
+                     for d in range(dim)])
+
+
+def refine_stream(lev, mid, wt, flags, ds, dim):
+    """refine_grid_stream! (Rebuild.jl:44-85): children in RMT order, the parent's f
+    injected. Returns (changed, level, midpoint, weight, src) with src the old index of each
+    new cell."""
+    sel = np.asarray(flags, dtype=bool)
+    n = len(lev)
+    if not sel.any():
+        return False, lev, mid, wt, np.arange(n, dtype=np.int32)
+    nch = 1 << dim
+    counts = np.where(sel, nch, 1)
+    src = np.repeat(np.arange(n), counts)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    child = np.arange(len(src)) - np.repeat(first, counts)
+    ref = sel[src]
+    L = lev[src].astype(np.int64)
+    nlev = (L + ref).astype(np.int8)
+    nwt = np.where(ref, wt[src] / nch, wt[src])
+    step = 0.5 * (ds[None, :] / (2.0 ** (L + 1))[:, None])
+    nmid = np.where(ref[:, None], mid[src] + step * RMT[dim][child % nch], mid[src])
+    return True, nlev, nmid, nwt, src.astype(np.int32)
+
+
+def coarsen_stream(lev, mid, wt, ok, ds, maxlevel, dim):
+    """coarsen_grid_stream! (Rebuild.jl:96-163): a 2^dim-aligned same-level sibling group is
+    merged iff every member is willing, with the same alignment walk. Returns (changed,
+    level, midpoint, weight, src) with src = old index of a kept cell or -(first+1) for a
+    merged group."""
+    ok = np.asarray(ok, dtype=bool)
+    n = len(lev)
+    if not ok.any():
+        return False, lev, mid, wt, np.arange(n, dtype=np.int32)
+    nch = 1 << dim
+    flag = [0.0] * (maxlevel + 1)
+    nl, nm, nw, src = [], [], [], []
+    changed = False
+    index = 0
+    while index < n:
+        fl = int(lev[index])
+        if fl > 0:
+            if (flag[fl] % 1 == 0.0 and index + nch - 1 <= n - 1 and
+                    all(int(lev[index + g]) == fl for g in range(1, nch))):
+                if all(ok[index + g] for g in range(nch)):
+                    nl.append(fl - 1)
+                    nw.append(wt[index] * nch)
+                    nm.append([mid[index, d] - 0.5 * (ds[d] / 2.0 ** fl) * RMT[dim][0, d]
+                               for d in range(dim)])
+                    src.append(-(index + 1))
+                    changed = True
+                else:
+                    for g in range(nch):
+                        nl.append(int(lev[index + g]))
+                        nw.append(wt[index + g])
+                        nm.append(list(mid[index + g]))
+                        src.append(index + g)
+                index += nch
+                for i in range(1, fl):
+                    flag[i] += 1 / 2 ** (dim * (fl - i))
+            else:
+                nl.append(fl)
+                nw.append(wt[index])
+                nm.append(list(mid[index]))
+                src.append(index)
+                for i in range(1, fl + 1):
+                    flag[i] += 1 / 2 ** (dim * (fl - i + 1))
+                index += 1
+        else:
+            nl.append(fl)
+            nw.append(wt[index])
+            nm.append(list(mid[index]))
+            src.append(index)
+            index += 1
+    return (changed, np.asarray(nl, dtype=np.int8), np.asarray(nm, dtype=np.float64).reshape(-1, dim),
+            np.asarray(nw, dtype=np.float64), np.asarray(src, dtype=np.int32))
+
+
+def plan_refine(topo, flags, dim):
+    """New grids, sources and changed cells of the refine pass."""
+    off = np.asarray(topo["vs_offset"])
+    ds = root_ds(topo, dim)
+    grids, srcs, changed = [], [], []
+    for c, (lv, md, wt) in enumerate(cell_grids(topo, dim)):
+        ch, nl, nm, nw, s = refine_stream(lv, md, wt, flags[off[c]:off[c + 1]] & 1, ds, dim)
+        grids.append((nl, nm, nw))
+        srcs.append(s)
+        changed.append(ch)
+    return with_grids(topo, grids, dim), np.concatenate(srcs), np.array(changed, dtype=np.uint8)
+
+
+def plan_coarsen(topo, flags, dim):
+    """New grids, sources and changed cells of the coarsen pass."""
+    off = np.asarray(topo["vs_offset"])
+    ds = root_ds(topo, dim)
+    maxlevel = int(topo["vs_maxlevel"])
+    grids, srcs, changed = [], [], []
+    for c, (lv, md, wt) in enumerate(cell_grids(topo, dim)):
+        ch, nl, nm, nw, s = coarsen_stream(lv, md, wt, (flags[off[c]:off[c + 1]] >> 1) & 1, ds,
+                                           maxlevel, dim)
+        grids.append((nl, nm, nw))
+        srcs.append(s)
+        changed.append(ch)
+    return with_grids(topo, grids, dim), np.concatenate(srcs), np.array(changed, dtype=np.uint8)
+
+
+def vs_adapt(dev, topo, params, dim):
+    """One vs_adaptive_mesh_refinement! (without vs_balance) on a context: returns the new
+    topology and the per-cell changed flags. vr is computed once, before refining, and used
+    by both passes, as the reference does."""
+    vr = dev.vs_resolution()
+    flags = dev.vs_flags(params, vr)
+    t1, src1, ch1 = plan_refine(topo, flags, dim)
+    if ch1.any():
+        dev.vs_remap(t1, src1, ch1)
+    else:
+        t1 = topo
+    flags = dev.vs_flags(params, vr)
+    t2, src2, ch2 = plan_coarsen(t1, flags, dim)
+    # coarsen_grid_stream! rebuilds every grid, so every cell's slopes are zero afterwards
+    dev.vs_remap(t2, src2, np.ones(len(ch2), dtype=np.uint8))
+    va = (ch1 | ch2).astype(bool)
+    dev.vs_project(np.flatnonzero(va).astype(np.int32))
+    return t2, va

@@ -71,6 +71,9 @@ struct orc_ctx {
   std::vector<double> prot, parea, pmid;
   std::vector<int32_t> bck;
   std::vector<double> bcp;
+  int vs_maxlevel = 0;
+  int32_t vs_tn[3] = {1, 1, 1};
+  double vs_lo[3] = {0, 0, 0}, vs_hi[3] = {0, 0, 0};
   std::vector<std::vector<int64_t>> cell_pairs;   // per cell: pair*2 + side, in pair order
   // state
   std::vector<double> w, prim, qf, sw, flux;
@@ -1045,6 +1048,12 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
   copy_in(o->pbc, T->pair_bc, npair);
   copy_in(o->bck, T->bc_kind, T->n_bc);
   copy_in(o->bcp, T->bc_prim, int64_t(T->n_bc) * (D + 2));
+  o->vs_maxlevel = T->vs_maxlevel;
+  for (int d = 0; d < 3; ++d) {
+    o->vs_tn[d] = T->vs_trees_num[d];
+    o->vs_lo[d] = T->vs_lo[d];
+    o->vs_hi[d] = T->vs_hi[d];
+  }
   int NW = D + 2;
   o->w.assign(size_t(o->nc * NW), 0.0);
   o->prim.assign(size_t(o->nc * NW), 0.0);
@@ -1058,6 +1067,235 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
 }
 
 void orc_destroy(orc_ctx* o) { delete o; }
+
+// ------------------------------------------------------------- velocity-space adaptation
+namespace {
+
+double jl_max(double a, double b) {   // Julia max: NaN wins
+  if (a != a) return a;
+  if (b != b) return b;
+  return b > a ? b : a;
+}
+
+// VsNeighborIndex of one cell's grid (src/Velocity_space/Neighbor.jl:61-124): the keys
+// (cellkey) of its velocity cells, which are in key order.
+struct VsIndex {
+  int D = 2, L = 0, mb = 0;
+  double vmin[3], hf[3];
+  int64_t tn[3], stride[3];
+  std::vector<uint64_t> keys;
+  const std::vector<int8_t>* lev = nullptr;
+  int64_t g0 = 0, n = 0;
+  void init(const orc_ctx& o) {
+    D = o.dim;
+    L = o.vs_maxlevel;
+    mb = D * L;
+    int64_t s = 1;
+    for (int d = 0; d < D; ++d) {
+      vmin[d] = o.vs_lo[d];
+      // ds0 = (quad_hi - quad_lo) / vs_trees_num, h_fine = ds0 / 2.0^maxlevel
+      hf[d] = ((o.vs_hi[d] - o.vs_lo[d]) / double(o.vs_tn[d])) / std::ldexp(1.0, L);
+      tn[d] = o.vs_tn[d];
+      stride[d] = s;
+      s *= tn[d];
+    }
+  }
+  uint64_t code_of(const int64_t* g) const {
+    int64_t root = 0;
+    uint64_t morton = 0;
+    const int64_t mask = (int64_t(1) << L) - 1;
+    for (int d = 0; d < D; ++d) {
+      root += (g[d] >> L) * stride[d];
+      const int64_t q = g[d] & mask;
+      for (int b = 0; b < L; ++b) morton |= uint64_t((q >> b) & 1) << (b * D + d);
+    }
+    return (uint64_t(root) << mb) | morton;
+  }
+  void build(const orc_ctx& o, int64_t c) {
+    g0 = o.goff(c);
+    n = o.nv(c);
+    keys.resize(size_t(n));
+    for (int64_t i = 0; i < n; ++i) {
+      const int lv = o.vlev[g0 + i];
+      int64_t g[3];
+      for (int d = 0; d < D; ++d) {
+        const double cell = hf[d] * double(int64_t(1) << (L - lv));
+        g[d] = int64_t(std::nearbyint((o.vmid[(g0 * D) + d * n + i] - 0.5 * cell - vmin[d]) / hf[d]));
+      }
+      keys[size_t(i)] = code_of(g);
+    }
+  }
+  double size(const orc_ctx& o, int64_t j, int d) const {
+    return hf[d] * double(int64_t(1) << (L - o.vlev[g0 + j]));
+  }
+  // leaf containing the finest-level cell `code`, -1 if none (_vs_locate_code)
+  int64_t locate(const orc_ctx& o, uint64_t code) const {
+    int64_t lo = 0, hi = n - 1, k = -1;
+    while (lo <= hi) {
+      const int64_t m = (lo + hi) / 2;
+      if (keys[size_t(m)] <= code) { k = m; lo = m + 1; } else { hi = m - 1; }
+    }
+    if (k < 0) return -1;
+    const uint64_t span = uint64_t(1) << (D * (L - o.vlev[g0 + k]));
+    return (code >= keys[size_t(k)] && code < keys[size_t(k)] + span) ? k : -1;
+  }
+  // vs_face_neighbor (Neighbor.jl:186-201): the leaf across the face of leaf i in dimension
+  // dim, direction dir; -1 at the velocity-domain boundary
+  int64_t face(const orc_ctx& o, int64_t i, int dim, int dir) const {
+    const double cell = size(o, i, dim);
+    int64_t g[3];
+    for (int d = 0; d < D; ++d) {
+      double coord = o.vmid[(g0 * D) + d * n + i];
+      if (d == dim) coord += double(dir) * (0.5 * cell + 0.5 * hf[d]);
+      const double x = std::floor((coord - vmin[d]) / hf[d]);
+      if (x < 0 || x >= double(tn[d] << L)) return -1;
+      g[d] = int64_t(x);
+    }
+    return locate(o, code_of(g));
+  }
+};
+
+// _lohner_ratio, src/Velocity_space/Criteria.jl:221-227
+double lohner_ratio(double L, double C, double R, double dsL, double dsR, double scale) {
+  const double eflt = 1e-2, eabs = 1e-3;
+  const double num = std::fabs(dsR * L - (dsL + dsR) * C + dsL * R);
+  const double den = dsR * std::fabs(L - C) + dsL * std::fabs(R - C) +
+                     eflt * (dsR * std::fabs(L) + (dsL + dsR) * std::fabs(C) + dsL * std::fabs(R)) +
+                     eabs * scale * (dsL + dsR);
+  return den > 0 ? num / den : 0.0;
+}
+
+}  // namespace
+
+int orc_set_sdf(orc_ctx* o, const double* sdf) {
+  if (!o || !sdf) return KAH_EINVAL;
+  o->sdf.assign(sdf, sdf + o->np * o->ndf * o->dim);
+  return 0;
+}
+
+int orc_vs_resolution(orc_ctx* o, double* vr) {
+  if (!o || !vr) return KAH_EINVAL;
+  const int D = o->dim, NDF = o->ndf, NW = D + 2;
+  // weight of a finest cell: prod(du)/prod(vs_trees_num)/2^(DIM*maxlevel)
+  double du = o->vs_hi[0] - o->vs_lo[0];
+  int64_t tn = o->vs_tn[0];
+  for (int d = 1; d < D; ++d) {
+    du = du * (o->vs_hi[d] - o->vs_lo[d]);
+    tn *= o->vs_tn[d];
+  }
+  const double weight = du / double(tn) / double(int64_t(1) << (D * o->vs_maxlevel));
+  double dres = 0.0, eres = 0.0;
+  for (int64_t c = 0; c < o->nc; ++c) {
+    if (o->benc[c] < 0) continue;
+    const int64_t n = o->nv(c), g0 = o->goff(c);
+    const double* f = &o->df[size_t(o->off(c) * NDF)];
+    const double* U = &o->prim[size_t(c * NW) + 1];
+    double dmax = -HUGE_VAL, emax = -HUGE_VAL;
+    for (int64_t x = 0; x < n * NDF; ++x) dmax = jl_max(dmax, f[x]);
+    for (int64_t i = 0; i < n; ++i) {
+      double c2 = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double t = U[d] - o->vmid[(g0 * D) + d * n + i];
+        c2 = d == 0 ? t * t : c2 + t * t;
+      }
+      const double e = NDF == 1 ? f[i] * c2 : f[i] * c2 + f[n + i];
+      emax = jl_max(emax, e);
+    }
+    dres = jl_max(dmax * weight, dres);
+    eres = jl_max(0.5 * emax * weight, eres);
+  }
+  vr[0] = dres;
+  vr[1] = eres;
+  return 0;
+}
+
+int orc_vs_flags(orc_ctx* o, const kah_vs_adapt* p, const double* vr, uint8_t* flags) {
+  if (!o || !p || !vr || !flags) return KAH_EINVAL;
+  const int D = o->dim, NDF = o->ndf, NW = D + 2;
+  const bool lohner = p->mode == KAH_VS_LOHNER;
+  const double CL = p->coeff_local, CG = p->coeff_global, tauL = p->coeff_lohner;
+  const double tauC = 0.3 * tauL;   // VS_LOHNER_COARSEN_RATIO * ADAPT_COEFFI_VS_LOHNER
+  const double P2 = double(1 << D);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t c = 0; c < o->nc; ++c) {
+    VsIndex X;
+    X.init(*o);
+    X.build(*o, c);
+    const int64_t n = o->nv(c), g0 = o->goff(c), off = o->off(c);
+    const double* f = &o->df[size_t(off * NDF)];
+    const double* s = &o->sdf[size_t(off * NDF * D)];
+    const double* w = &o->w[size_t(c * NW)];
+    const double* U = &o->prim[size_t(c * NW) + 1];
+    const double* ds = o->rds(c);
+    double s1 = 0.0, s2 = 0.0;   // vs_lohner_scales
+    for (int64_t i = 0; i < n; ++i) {
+      const double a1 = std::fabs(f[i]);
+      if (a1 > s1) s1 = a1;
+      if (NDF == 2) {
+        const double a2 = std::fabs(f[n + i]);
+        if (a2 > s2) s2 = a2;
+      }
+    }
+    double SU = 0.0;
+    for (int d = 0; d < D; ++d) SU = d == 0 ? U[d] * U[d] : SU + U[d] * U[d];
+    const double den = w[NW - 1] - 0.5 * w[0] * SU;
+    for (int64_t i = 0; i < n; ++i) {
+      // _criterion_cell!: cdf = df + max_d |sdf*ds|
+      double cdf[2];
+      for (int k = 0; k < NDF; ++k) {
+        double m = 0.0;
+        for (int d = 0; d < D; ++d) {
+          const double a = std::fabs(s[(d * NDF + k) * n + i] * ds[d]);
+          if (a > m) m = a;
+        }
+        cdf[k] = f[k * n + i] + m;
+      }
+      const double wt = o->vw[g0 + i];
+      double S = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double t = U[d] - o->vmid[(g0 * D) + d * n + i];
+        S = d == 0 ? t * t : S + t * t;
+      }
+      const double c1w = cdf[0] * wt;
+      double loc_r, loc_c, e_w;
+      if (NDF == 2) {
+        e_w = 0.5 * (S * cdf[0] + cdf[1]) * wt;
+        loc_r = jl_max(std::fabs(e_w) / den, c1w / w[0]);
+        loc_c = jl_max(e_w / den, c1w / w[0]);
+      } else {
+        loc_r = jl_max(std::fabs(0.5 * S * cdf[0] * wt) / den, c1w / w[0]);
+        e_w = 0.5 * (S * cdf[0]) * wt;
+        loc_c = jl_max(e_w / den, c1w / w[0]);
+      }
+      const bool local_refine = loc_r > CL;
+      const bool local_coarsen = loc_c < CL / P2;
+      const bool global_refine = (NDF == 2 ? c1w > CG * vr[0] : c1w > vr[0] * CG) || e_w > vr[1] * CG;
+      const bool global_coarsen =
+          (NDF == 2 ? c1w < CG * vr[0] / P2 : c1w < vr[0] * CG / P2) && e_w < vr[1] * CG / P2;
+      double eta = 0.0;
+      if (lohner) {
+        const double h0 = f[i], b0 = NDF == 2 ? f[n + i] : 0.0;
+        for (int d = 0; d < D; ++d) {
+          const double hi = X.size(*o, i, d);
+          const int64_t Ln = X.face(*o, i, d, -1), Rn = X.face(*o, i, d, +1);
+          const double dsL = Ln < 0 ? hi : 0.5 * (hi + X.size(*o, Ln, d));
+          const double dsR = Rn < 0 ? hi : 0.5 * (hi + X.size(*o, Rn, d));
+          const double hL = Ln < 0 ? 0.0 : f[Ln], hR = Rn < 0 ? 0.0 : f[Rn];
+          eta = jl_max(eta, lohner_ratio(hL, h0, hR, dsL, dsR, s1));
+          if (NDF == 2) {
+            const double bL = Ln < 0 ? 0.0 : f[n + Ln], bR = Rn < 0 ? 0.0 : f[n + Rn];
+            eta = jl_max(eta, lohner_ratio(bL, b0, bR, dsL, dsR, s2));
+          }
+        }
+      }
+      const bool base = lohner ? (eta > tauL || local_refine) : (local_refine || global_refine);
+      const bool refine = o->vlev[g0 + i] < o->vs_maxlevel && base;
+      const bool coarsen = lohner ? (eta < tauC && local_coarsen) : (local_coarsen && global_coarsen);
+      flags[off + i] = uint8_t((refine ? KAH_VS_REFINE : 0) | (coarsen ? KAH_VS_COARSEN : 0));
+    }
+  }
+  return 0;
+}
 const char* orc_last_error(const orc_ctx* o) { return o ? o->err.c_str() : "null context"; }
 
 int orc_set_threads(int n) {

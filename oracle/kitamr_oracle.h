@@ -47,6 +47,12 @@ void orc_shakhov_part(int dim, int ndf, int64_t n, const double* vs_mid, const d
  * that its moments (with b's internal energy, ndf == 2) equal w. */
 void orc_i_projection(int dim, int ndf, int64_t n, const double* vs_mid, const double* weight,
                       double* df, const double* w);
+/* vs_resolution and the per-velocity-cell refine / coarsen flags of vs_refine! /
+ * vs_coarsen! (src/Velocity_space/AMR.jl:24-167, Criteria.jl, Neighbor.jl) on the current
+ * state; flags as kah_vs_flags. */
+int orc_vs_resolution(orc_ctx* o, double* vr);
+int orc_set_sdf(orc_ctx* o, const double* sdf);   /* VsData.sdf of all cells (tests) */
+int orc_vs_flags(orc_ctx* o, const kah_vs_adapt* p, const double* vr, uint8_t* flags);
 void orc_get_prim(int dim, const double* w, double gamma, double* prim);
 void orc_get_conserved(int dim, const double* prim, double gamma, double* w);
 double orc_get_tau(int dim, const double* prim, double mu, double omega);

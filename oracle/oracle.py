@@ -59,6 +59,10 @@ def lib():
         L.orc_get_tau.restype = C.c_double
         L.orc_ref_vhs_vis.argtypes = [C.c_double] * 4
         L.orc_ref_vhs_vis.restype = C.c_double
+        L.orc_vs_resolution.argtypes = [C.c_void_p, dp]
+        L.orc_vs_flags.argtypes = [C.c_void_p, C.POINTER(abi.KahVsAdapt), dp,
+                                   C.POINTER(C.c_uint8)]
+        L.orc_set_sdf.argtypes = [C.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -112,6 +116,20 @@ class Oracle:
                                                 ["w", "prim", "qf", "sw", "df", "sdf"]]),
                  "get_state")
         return a
+
+    def set_sdf(self, sdf):
+        self._ck(self.L.orc_set_sdf(self.h, _p(_f(sdf))), "set_sdf")
+
+    def vs_resolution(self):
+        vr = np.empty(2)
+        self._ck(self.L.orc_vs_resolution(self.h, _p(vr)), "vs_resolution")
+        return vr
+
+    def vs_flags(self, params, vr):
+        out = np.empty(self.np, dtype=np.uint8)
+        self._ck(self.L.orc_vs_flags(self.h, C.byref(params), _p(_f(vr)),
+                                     out.ctypes.data_as(C.POINTER(C.c_uint8))), "vs_flags")
+        return out
 
     def get_flux(self):
         f = np.empty(self.nc * (self.dim + 2))

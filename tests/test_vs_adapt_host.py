@@ -1,0 +1,184 @@
+"""Host side of velocity-space adaptation (kitamr_hip.vsadapt): the grid streams of
+src/Velocity_space/Rebuild.jl on levels, midpoints and weights, and the sources that let the
+device move f. Pinned by: refine equals VelocityGrid.refine (an independent loop form of
+refine_grid_stream!); coarsening a refined grid with every cell willing restores it exactly;
+a group is only merged when all 2^dim siblings are willing and aligned; mass and energy
+moments of a remapped f are preserved by refine (injection keeps sum(weight*f)) and by coarsen
+(the mean times 2^dim weight); the oracle's criteria agree with a brute-force restatement
+whose face neighbours come from a geometric search."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kitamr_hip import abi, cases, vsadapt
+from kitamr_hip.solver import kah_params
+from kitamr_hip.vspace import VelocityGrid
+
+
+def _grid(dim=2, n=4, maxlevel=2, seed=0):
+    g = VelocityGrid(dim, [-4.0, 4.0] * dim, [n] * dim, maxlevel=maxlevel)
+    rng = np.random.default_rng(seed)
+    for _ in range(maxlevel):
+        g.refine(rng.random(g.n) < 0.4)
+    return g
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_refine_stream_equals_grid_refine(dim):
+    g = _grid(dim, n=4, maxlevel=2)
+    rng = np.random.default_rng(5)
+    flags = (rng.random(g.n) < 0.3) & (g.level < 2)
+    ch, lv, md, wt, src = vsadapt.refine_stream(g.level, g.midpoint, g.weight, flags, g.ds, dim)
+    h = g.copy().refine(flags)
+    assert ch
+    assert np.array_equal(lv, h.level) and np.array_equal(md, h.midpoint)
+    assert np.array_equal(wt, h.weight)
+    # injection: every child takes its parent's value
+    f = rng.random(g.n)
+    assert np.array_equal(f[src][np.repeat(~flags, np.where(flags, 1 << dim, 1))],
+                          f[~flags])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_coarsen_all_willing_restores_the_refined_parent_grid(dim):
+    base = VelocityGrid(dim, [-4.0, 4.0] * dim, [4] * dim, maxlevel=1)
+    flags = np.zeros(base.n, bool)
+    flags[::3] = True
+    fine = base.copy().refine(flags)
+    ch, lv, md, wt, src = vsadapt.coarsen_stream(fine.level, fine.midpoint, fine.weight,
+                                                 np.ones(fine.n, bool), fine.ds, 1, dim)
+    assert ch
+    # every level-1 group merges: the result is the root grid
+    root = VelocityGrid(dim, [-4.0, 4.0] * dim, [4] * dim, maxlevel=1)
+    assert np.array_equal(lv, root.level)
+    assert np.array_equal(md, root.midpoint)
+    assert np.allclose(wt, root.weight, rtol=0, atol=0)
+    # mass of f is kept by the mean: sum(w_new*mean) = sum over children
+    f = np.random.default_rng(1).random(fine.n)
+    fn = np.array([f[s] if s >= 0 else f[-s - 1:-s - 1 + (1 << dim)].sum() / (1 << dim)
+                   for s in src])
+    assert np.isclose((fn * wt).sum(), (f * fine.weight).sum(), rtol=1e-14)
+
+
+def test_coarsen_needs_every_sibling():
+    g = VelocityGrid(2, [-4.0, 4.0] * 2, [2, 2], maxlevel=1)
+    fine = g.copy().refine(np.ones(g.n, bool))          # 16 cells, 4 groups
+    ok = np.ones(fine.n, bool)
+    ok[5] = False                                        # second group not willing
+    ch, lv, md, wt, src = vsadapt.coarsen_stream(fine.level, fine.midpoint, fine.weight, ok,
+                                                 fine.ds, 1, 2)
+    assert ch and len(lv) == 3 + 4
+    assert list(src[:1]) == [-1] and list(src[1:5]) == [4, 5, 6, 7]
+
+
+def _brute_flags(orc, case, params, vr):
+    """Lohner / contribution flags with face neighbours found geometrically."""
+    topo, dim, ndf = case.topo, case.config.solver.DIM, case.config.solver.NDF
+    st = orc.get_state()
+    off = np.asarray(topo["vs_offset"])
+    L = int(topo["vs_maxlevel"])
+    hf = np.array([(topo["vs_hi"][d] - topo["vs_lo"][d]) / topo["vs_trees_num"][d]
+                   for d in range(dim)]) / 2.0 ** L
+    out = np.zeros(off[-1], np.uint8)
+    nw = dim + 2
+    for c, (lv, md, wt) in enumerate(vsadapt.cell_grids(topo, dim)):
+        n = len(lv)
+        f = st["df"][off[c] * ndf:off[c + 1] * ndf].reshape(ndf, n)
+        s = st["sdf"][off[c] * ndf * dim:off[c + 1] * ndf * dim].reshape(dim, ndf, n)
+        w = st["w"][c * nw:(c + 1) * nw]
+        U = st["prim"][c * nw + 1:c * nw + 1 + dim]
+        ds = np.asarray(topo["cell_ds"]).reshape(-1, dim)[c]
+        size = hf[None, :] * (2.0 ** (L - lv.astype(np.int64)))[:, None]
+        lo, hi = md - 0.5 * size, md + 0.5 * size
+        s1 = np.abs(f[0]).max(initial=0.0)
+        s2 = np.abs(f[1]).max(initial=0.0) if ndf == 2 else 0.0
+
+        def locate(p):
+            inside = np.all((p[None, :] >= lo) & (p[None, :] < hi), axis=1)
+            k = np.flatnonzero(inside)
+            return int(k[0]) if len(k) else -1
+
+        def ratio(a, b, cc, dl, dr, sc):
+            num = abs(dr * a - (dl + dr) * b + dl * cc)
+            den = (dr * abs(a - b) + dl * abs(cc - b) +
+                   1e-2 * (dr * abs(a) + (dl + dr) * abs(b) + dl * abs(cc)) + 1e-3 * sc * (dl + dr))
+            return num / den if den > 0 else 0.0
+
+        SU = U[0] * U[0]
+        for d in range(1, dim):
+            SU = SU + U[d] * U[d]
+        den = w[-1] - 0.5 * w[0] * SU
+        for i in range(n):
+            cdf = [f[k, i] + max([abs(s[d, k, i] * ds[d]) for d in range(dim)] + [0.0])
+                   for k in range(ndf)]
+            S = (U[0] - md[i, 0]) ** 2
+            for d in range(1, dim):
+                S = S + (U[d] - md[i, d]) ** 2
+            c1w = cdf[0] * wt[i]
+            if ndf == 2:
+                e = 0.5 * (S * cdf[0] + cdf[1]) * wt[i]
+                lr = max(abs(e) / den, c1w / w[0])
+            else:
+                lr = max(abs(0.5 * S * cdf[0] * wt[i]) / den, c1w / w[0])
+                e = 0.5 * (S * cdf[0]) * wt[i]
+            lc = max(e / den, c1w / w[0])
+            P2 = 2.0 ** dim
+            loc_r, loc_c = lr > params.coeff_local, lc < params.coeff_local / P2
+            cg = params.coeff_global
+            glob_r = c1w > cg * vr[0] or e > vr[1] * cg
+            glob_c = c1w < cg * vr[0] / P2 and e < vr[1] * cg / P2
+            eta = 0.0
+            if params.mode == 0:
+                for d in range(dim):
+                    nb = []
+                    for sgn in (-1, 1):
+                        p = md[i].copy()
+                        p[d] += sgn * (0.5 * size[i, d] + 0.5 * hf[d])
+                        nb.append(locate(p))
+                    dl = size[i, d] if nb[0] < 0 else 0.5 * (size[i, d] + size[nb[0], d])
+                    dr = size[i, d] if nb[1] < 0 else 0.5 * (size[i, d] + size[nb[1], d])
+                    for k, sc in ((0, s1), (1, s2))[:ndf]:
+                        a = 0.0 if nb[0] < 0 else f[k, nb[0]]
+                        b = 0.0 if nb[1] < 0 else f[k, nb[1]]
+                        eta = max(eta, ratio(a, f[k, i], b, dl, dr, sc))
+                base = eta > params.coeff_lohner or loc_r
+                coarse = eta < 0.3 * params.coeff_lohner and loc_c
+            else:
+                base = loc_r or glob_r
+                coarse = loc_c and glob_c
+            out[off[c] + i] = (1 if (lv[i] < L and base) else 0) | (2 if coarse else 0)
+    return out
+
+
+@pytest.mark.parametrize("mode", ["lohner", "contribution"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_flags_equal_brute_force(mode, dim):
+    case = (cases.riemann_2d(n=4, nv=8, vs_levels=2) if dim == 2 else
+            cases.box_3d(n=2, nv=4, vs_levels=1))
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    o.step(case.dt_xi, 0)
+    vr = o.vs_resolution()
+    # a lower local threshold in contribution mode so that both decisions occur
+    p = abi.vs_adapt_params(mode, coeff_local=1e-2 if mode == "lohner" else 1e-3)
+    got = o.vs_flags(p, vr)
+    exp = _brute_flags(o, case, p, vr)
+    assert np.array_equal(got, exp), np.flatnonzero(got != exp)[:10]
+    assert (got & 2).any()
+    if dim == 2:    # the 3D grid is already at its maximum level where it matters
+        assert (got & 1).any()
+    o.close()
+
+
+def test_oracle_resolution_is_the_peak_contribution():
+    case = cases.riemann_2d(n=4, nv=8, vs_levels=2)
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    vr = o.vs_resolution()
+    st = o.get_state()
+    L = int(case.topo["vs_maxlevel"])
+    wf = 100.0 / 64 / 4 ** L
+    off = np.asarray(case.topo["vs_offset"])
+    dmax = max(st["df"][off[c] * 2:off[c + 1] * 2].max() for c in range(case.n_cells))
+    assert vr[0] == dmax * wf
+    o.close()
