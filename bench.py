@@ -26,16 +26,40 @@ sys.path.insert(0, str(ROOT / "oracle"))
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes(s_f, adaptive=False):
-    """Algorithmic bytes per phase-space update of the reference's 3-phase dataflow, 2D2F,
-    with s_f bytes per stored f value (8: FP64, 4: FP32-f): slope R df (+ R level) + W sdf;
+def alg_bytes(s_f, adaptive=False, dim=2, ndf=2):
+    """Algorithmic bytes per phase-space update of the reference's 3-phase dataflow, with
+    s_f bytes per stored f value (8: FP64, 4: FP32 form): slope R df (+ R level) + W sdf;
     flux R df, R sdf (+ R u, R weight, R level), W flux; iterate R df, R flux (+ R u,
-    R weight), W df. The geometry terms (8-byte u and weight, 1-byte level) count only for
-    per-cell adapted velocity grids; a shared grid is read once."""
-    g_slope, g_flux, g_iter = (1, 2 * 8 + 8 + 1, 2 * 8 + 8) if adaptive else (0, 0, 0)
-    b_slope = 2 * s_f * (1 + 2) + g_slope
-    b_flux_iter = 2 * s_f * (1 + 2 + 1) + g_flux + 2 * s_f * 3 + g_iter
-    return b_slope, b_flux_iter, b_slope + b_flux_iter   # FP64: 48, 112, 160 (adaptive 210)
+    R weight), W df. The geometry terms (8-byte u components and weight, 1-byte level)
+    count only for per-cell adapted velocity grids; a shared grid is read once.
+    2D2F FP64: 48, 112, 160 (adaptive 210); 3D1F FP64: 32, 64, 96 (adaptive 162)."""
+    g_slope, g_flux, g_iter = ((1, dim * 8 + 8 + 1, dim * 8 + 8) if adaptive else (0, 0, 0))
+    b_slope = ndf * s_f * (1 + dim) + g_slope
+    b_flux_iter = ndf * s_f * (1 + dim + 1) + g_flux + ndf * s_f * 3 + g_iter
+    return b_slope, b_flux_iter, b_slope + b_flux_iter
+
+
+WORKLOADS = {
+    "riemann2d": "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), uniform 48x48 "
+                 "velocity grid, CAIDVM + CAIDVM_Marching",
+    "riemann2d-vsamr": "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), per-cell "
+                       "adapted velocity grids (20x20 base on [-15,15]^2, 2 levels, 1567-1651 "
+                       "cells each), CAIDVM + CAIDVM_Marching",
+    "box3d-vsamr": "3D1F box, 24^3 cells (13824 leaves), per-cell adapted velocity grids "
+                   "(16^3 base on [-10,10]^3, 1 level, 9430-9731 cells each), CAIDVM + "
+                   "CAIDVM_Marching",
+}
+
+
+def make_case(workload, precision):
+    from kitamr_hip import cases
+    if workload == "riemann2d-vsamr":
+        return cases.riemann_2d(n=64, nv=20, vmax=15.0, ps_levels=3, vs_levels=2,
+                                vs_variants=6, precision=precision)
+    if workload == "box3d-vsamr":
+        return cases.box_3d(n=24, nv=16, vmax=10.0, vs_levels=1, vs_variants=6,
+                            precision=precision)
+    return cases.riemann_2d(n=64, nv=48, ps_levels=3, precision=precision)
 
 
 def log(msg):
@@ -145,7 +169,8 @@ def main():
                     help="waves per workgroup of the flux/relaxation kernel (0: library default)")
     ap.add_argument("--slope-vpt", type=int, default=0,
                     help="velocity cells per thread of the slope kernel (0: library default)")
-    ap.add_argument("--workload", default="riemann2d", choices=["riemann2d", "riemann2d-vsamr"],
+    ap.add_argument("--workload", default="riemann2d",
+                    choices=["riemann2d", "riemann2d-vsamr", "box3d-vsamr"],
                     help="riemann2d: shared 48x48 velocity grid (the headline workload); "
                          "riemann2d-vsamr: per-cell adapted grids (20x20 base on [-15,15]^2, "
                          "2 levels)")
@@ -170,13 +195,12 @@ def main():
     if args.small:
         case = cases.riemann_2d(n=32, nv=48, ps_levels=2, halo=(4.0, 2.0),
                                 precision=args.precision)
-    elif args.workload == "riemann2d-vsamr":
-        case = cases.riemann_2d(n=64, nv=20, vmax=15.0, ps_levels=3, vs_levels=2, vs_variants=6,
-                                precision=args.precision)
     else:
-        case = cases.riemann_2d(n=64, nv=48, ps_levels=3, precision=args.precision)
-    adaptive = args.workload == "riemann2d-vsamr"
-    B_SLOPE, B_FLUX_ITER, B_STEP = alg_bytes(8 if args.precision == "fp64" else 4, adaptive)
+        case = make_case(args.workload, args.precision)
+    adaptive = args.workload != "riemann2d"
+    DIM, NDF = case.config.solver.DIM, case.config.solver.NDF
+    B_SLOPE, B_FLUX_ITER, B_STEP = alg_bytes(8 if args.precision == "fp64" else 4, adaptive,
+                                             DIM, NDF)
     P = case.phase_cells
     log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
         f"pairs={case.topo['n_pairs']} P={P}")
@@ -254,11 +278,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": ("reduced 32x32 + 2 levels (debug)" if args.small else
-                         "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
-                         "per-cell adapted velocity grids (20x20 base on [-15,15]^2, 2 levels, "
-                         "1567-1651 cells each), CAIDVM + CAIDVM_Marching" if adaptive else
-                         "2D Riemann, 64x64 base + 3 static PS levels (49312 leaves), "
-                         "uniform 48x48 velocity grid, CAIDVM + CAIDVM_Marching"),
+                         WORKLOADS[args.workload]),
             "precision": args.precision,
             "launch_options": tuned or "library defaults",
             "leaves": case.n_cells,
@@ -287,10 +307,8 @@ def main():
     if args.precision == "fp64" and not args.no_fp32 and not args.small:
         # the FP32 form on the same workload: f stored and advected in FP32, moment sums,
         # prim, tau, q and the Maxwellian parameters in FP64
-        c32 = (cases.riemann_2d(n=64, nv=20, vmax=15.0, ps_levels=3, vs_levels=2, vs_variants=6,
-                                precision="fp32f") if adaptive else
-               cases.riemann_2d(n=64, nv=48, ps_levels=3, precision="fp32f"))
-        b_s32, b_fi32, b_st32 = alg_bytes(4, adaptive)
+        c32 = make_case(args.workload, "fp32f")
+        b_s32, b_fi32, b_st32 = alg_bytes(4, adaptive, DIM, NDF)
         e32, ms32, fin32, _ = timed_run(c32, {}, args.steps, args.warmup)
         v32 = c32.phase_cells * args.steps / e32
         a32 = b_fi32 * c32.phase_cells / (ms32[1] * 1e-3) / 1e9

@@ -818,20 +818,22 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     for (int32_t c : C->fluid) Dmax = std::max(Dmax, depth[size_t(c)]);
   }
   for (int L = 0; L <= Dmax; ++L) {
-    const int caps[4] = {4, 8, 16, 24};
-    std::vector<int32_t> lists[4];
-    std::vector<int2> works[4];
+    // half-count bins: the per-thread accumulators of K1 are MAXH*(dim+2) doubles, so a bin
+    // sized to the common counts (4: 2D interior, 6: 3D interior) keeps registers low
+    const int caps[5] = {4, 6, 8, 16, 24};
+    std::vector<int32_t> lists[5];
+    std::vector<int2> works[5];
     for (int32_t c : C->fluid) {
       if (depth[size_t(c)] != L) continue;
       int nh = hoff[c + 1] - hoff[c];
-      int b = nh <= 4 ? 0 : (nh <= 8 ? 1 : (nh <= 16 ? 2 : 3));
+      int b = nh <= 4 ? 0 : (nh <= 6 ? 1 : (nh <= 8 ? 2 : (nh <= 16 ? 3 : 4)));
       if (walled[c]) {
         lists[b].push_back(c);
       } else {
         for (int ch = 0; ch < nchunks(c); ++ch) works[b].push_back(make_int2(c, ch));
       }
     }
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < 5; ++b) {
       for (int kind = 0; kind < 2; ++kind) {
         size_t cnt = kind ? works[b].size() : lists[b].size();
         if (cnt == 0) continue;

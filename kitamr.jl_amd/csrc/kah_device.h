@@ -1928,6 +1928,9 @@ inline hipError_t launch_slope_chunk_v(const DevArgs& A, const int2* work, int n
   if (maxh <= 4)
     hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));
+  else if (maxh <= 6)
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 6, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+                       A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 8)
     hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 8, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));

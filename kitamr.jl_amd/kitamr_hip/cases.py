@@ -122,7 +122,9 @@ def wall_channel(n=16, nv=16, flux="CAIDVM", marching="CAIDVM_Marching", ps_leve
 
 
 def box_3d(n=4, nv=8, vmax=6.0, ps_levels=0, bcs=("UniformOutflow",) * 6, vs_levels=0,
-           marching="CAIDVM_Marching", flux="CAIDVM", seed=7):
+           marching="CAIDVM_Marching", flux="CAIDVM", seed=7, precision="fp64", vs_variants=0):
+    """3D1F box. vs_variants > 0: the adapted grid of cell c is built once per
+    c mod vs_variants (from that cell's initial state), as in riemann_2d."""
     solver = Solver(DIM=3, NDF=1, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=vs_levels,
                     flux=flux, time_marching=marching, CFL=0.5)
     gas = Gas(K=0.0, Kn=0.05, omega=0.81)
@@ -139,13 +141,22 @@ def box_3d(n=4, nv=8, vmax=6.0, ps_levels=0, bcs=("UniformOutflow",) * 6, vs_lev
         r = 1.0 + 0.5 * math.tanh(8 * (mid[0] - 0.5)) + 0.2 * math.sin(2 * math.pi * mid[1])
         return [r, 0.1 * mid[2], -0.05, 0.02 * mid[0], 1.0 / (1.0 + 0.2 * mid[1])]
     cfg = Configure(solver, gas, [0.0, 1.0] * 3, [n, n, n], [-vmax, vmax] * 3, [nv] * 3,
-                    PCoordFn(ic), dom)
+                    PCoordFn(ic), dom, precision=precision)
     refine = None
     if ps_levels > 0:
         def refine(mid, ds, level):
             return np.abs(mid[:, 0] - 0.5) < 0.2
     vs_fn = None
     if vs_levels > 0:
+        cache = {}
+
         def vs_fn(c, prim, base, _seed=seed):
-            return refine_toward(base.copy(), prim, rounds=vs_levels, frac=0.15, rng=_seed + c)
+            if vs_variants <= 0:
+                return refine_toward(base.copy(), prim, rounds=vs_levels, frac=0.15,
+                                     rng=_seed + c)
+            r = c % vs_variants
+            if r not in cache:
+                cache[r] = refine_toward(base.copy(), prim, rounds=vs_levels, frac=0.15,
+                                         rng=_seed + r)
+            return cache[r]
     return build_case(cfg, ps_refine=refine, vs_grid_fn=vs_fn)
