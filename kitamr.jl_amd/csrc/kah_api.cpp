@@ -68,6 +68,7 @@ struct kah_ctx {
   int lmax = 0;
   int max_n = 0;
   size_t lds_bytes = 0;
+  size_t lds_limit = 56 * 1024;   // dynamic LDS per workgroup (set from the device)
   int64_t phase_cells = 0;
   std::vector<int32_t> fluid;
   std::vector<int32_t> benc;
@@ -507,6 +508,12 @@ int kah_create(const kah_params* params, kah_ctx** out) {
     delete C;
     return KAH_EDEVICE;
   }
+  // dynamic LDS per workgroup: the device's limit (160 KiB on gfx950) less the kernels'
+  // static LDS
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            params->device) == hipSuccess && lds_max > 16 * 1024)
+    C->lds_limit = size_t(lds_max) - 8 * 1024;
   *out = C;
   return KAH_OK;
 }
@@ -884,7 +891,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   const size_t sc_elem =
       (C->f32 && C->P.marching != KAH_MARCH_CIP) ? sizeof(float) : sizeof(double);
   C->lds_bytes = sc_elem * size_t(NDF + 1) * size_t(C->max_n);
-  if (C->lds_bytes > 64 * 1024) {
+  if (C->lds_bytes > C->lds_limit) {
     C->lds_bytes = 0;
     KAH_TRY(C->alloc(C->d_scratch, sizeof(double) * C->np * (NDF + 1), "scratch"));
   } else {
@@ -1428,7 +1435,7 @@ int kah_vs_project(kah_ctx* C, const int32_t* cells, int64_t n) {
   KAH_TRY(C->upload(d_cells, list.data(), list.size(), "projection cells"));
   const size_t lds = sizeof(double) * size_t(C->NDF + 1) * size_t(C->max_n);
   double* scr = nullptr;
-  if (lds > 64 * 1024) {
+  if (lds > C->lds_limit) {
     KAH_TRY(C->alloc(d_scr, sizeof(double) * size_t(C->np) * size_t(C->NDF + 1), "projection scratch"));
     scr = d_scr.as<double>();
   }
