@@ -211,7 +211,9 @@ int kah_vs_flags(kah_ctx* ctx, const kah_vs_adapt* params, const double* vr, uin
  * injected into its children: refine_grid_stream!, src/Velocity_space/Rebuild.jl:44-85),
  * src = -(j+1) takes the mean of the 2^dim old cells j.. (coarsen_grid_stream!, :96-163).
  * Slopes of cells with changed[c] != 0 are zero afterwards (the rebuilt VsData), others
- * keep theirs; w and prim are kept; flux accumulators are zero. */
+ * keep theirs; w and prim are kept; flux accumulators are zero. The sources are checked
+ * before anything changes (KAH_EINVAL); a failure after that (device memory) leaves the
+ * context without a state, to be set again. */
 int kah_vs_remap(kah_ctx* ctx, const kah_topology* new_topo, const int32_t* src,
                  const uint8_t* changed);
 /* conserved_I_porjection! of the listed cells' h onto their w (vs_conserved_correction!,

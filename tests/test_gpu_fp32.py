@@ -1,7 +1,8 @@
-"""The FP32-f form (f and its slopes stored in float, all arithmetic, moments and w in FP64)
-against the FP64 oracle: moments rho, rho*U, rho*E and the heat flux q within 1e-5
-relative after several steps, and mass, momentum and energy conserved to round-off on a
-periodic box (the macro face fluxes stay antisymmetric FP64 sums)."""
+"""The FP32 form (f and its slopes stored and advected in FP32; moment sums, w, prim, tau,
+q and the Maxwellian parameters in FP64) against the FP64 oracle: moments rho, rho*U, rho*E
+and the heat flux q within 1e-5 relative after several steps (CAIDVM_Marching, and
+CIP_Marching whose projection stays FP64), and mass, momentum and energy conserved to
+round-off on a periodic box (the macro face fluxes stay antisymmetric FP64 sums)."""
 import numpy as np
 import pytest
 
@@ -17,6 +18,9 @@ CASES = {
     "riemann16_amr2": lambda: cases.riemann_2d(n=16, nv=12, ps_levels=2, halo=(3.0, 2.0),
                                                precision="fp32f"),
     "riemann8_vsamr": lambda: cases.riemann_2d(n=8, nv=8, vs_levels=2, precision="fp32f"),
+    "riemann8_cip": lambda: cases.riemann_2d(n=8, nv=16, marching="CIP_Marching",
+                                             precision="fp32f"),
+    "box3d_cip": lambda: cases.box_3d(n=4, nv=12, marching="CIP_Marching", precision="fp32f"),
     "wall_channel": lambda: cases.wall_channel(n=8, nv=8),
     "box3d_nv8": lambda: cases.box_3d(n=4, nv=8),
     "box3d_amr_vsamr": lambda: cases.box_3d(n=4, nv=6, ps_levels=1, vs_levels=1),
