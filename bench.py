@@ -93,7 +93,7 @@ def profiled_traffic():
 def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
     """A short stream of the serving workload: 2D Riemann cases (outflow or Maxwellian-inflow
     boundaries) with Kn log-uniform in [1e-3, 1], 256-8.3k physical cells (16^2-48^2 base,
-    0-2 refinement levels), 16^2-48^2 velocity grids, FP64 or FP32-f, Poisson arrivals;
+    0-2 refinement levels), 16^2-48^2 velocity grids, FP64 or FP32 form, Poisson arrivals;
     each runs to the residual tolerance or its step budget.
     Reports p50/p99 request latency (arrival to result) and the aggregate updates/s."""
     import numpy as np
@@ -124,7 +124,7 @@ def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
     rep = srv.run(reqs, keep_df=False)
     rep["workload"] = (f"{n_req} requests, Poisson arrivals at {rate}/s, 2D Riemann cases "
                        "(outflow or Maxwellian-inflow boundaries), 256-8.3k cells, 16^2-48^2 "
-                       "velocity grids, FP64 or FP32-f, 40-160 steps each (step budget reduced "
+                       "velocity grids, FP64 or FP32 form, 40-160 steps each (step budget reduced "
                        "from 2000 to keep the run short)")
     rep["slots"] = slots
     return rep

@@ -63,7 +63,7 @@ struct kah_ctx {
   int D = 2, NDF = 2, NW = 4;
   int64_t nc = 0, np = 0, npairs = 0;
   bool uni = true;
-  bool f32 = false;   // FP32-f form: f, sdf stored as float
+  bool f32 = false;   // FP32 form: f, sdf stored as float and advected in FP32
   int n0 = 0;
   int lmax = 0;
   int max_n = 0;
@@ -163,7 +163,7 @@ struct kah_ctx {
     return hip(hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice), what);
   }
   size_t fsize() const { return f32 ? sizeof(float) : sizeof(double); }
-  // host double array -> device storage type (float in the FP32-f form: rounded to nearest)
+  // host double array -> device storage type (float in the FP32 form: rounded to nearest)
   int upload_f(DBuf& b, const double* src, size_t count, const char* what) {
     if (!f32)
       return hip(hipMemcpyAsync(b.p, src, count * sizeof(double), hipMemcpyHostToDevice, stream),

@@ -1,18 +1,21 @@
 // kah_device.h -- gfx950 device code of the per-step phase-space update (included by the
 // per-dimension kernel translation units kah_slope_{2d,3d}.hip and kah_flux_{2d,3d}.hip).
 //
-// K1 k_slope_chunk one workgroup per (cell, chunk of velocity cells) of one physical level:
-//                  velocity slopes and per-chunk partial half fluxes
+// K1 k_slope_chunk one workgroup per (cell, chunk of velocity cells) of one projection
+//                  depth: velocity slopes and per-chunk partial half fluxes
 //    k_slope       one workgroup per cell, for cells with a Maxwellian wall face (the wall
 //                  density needs the whole cell's sums before the half flux)
 // K2 k_flux_relax  one workgroup per fluid cell (MODE_FLUX / MODE_ITER / MODE_FUSED)
 // K3 k_macro_slope on demand: PsData.sw = moments of sdf (src/Flux/Slope.jl:1022-1036)
+// and, off the per-step path: k_overlap_map (cross-grid overlap maps, once per topology),
+// k_mirror (ghost-exchange buffers), k_restart (restart arrays), k_vs_res / k_vs_flags /
+// k_vs_remap / k_vs_project (velocity-space adaptation).
 //
 // Velocity index fastest in memory, so every load and store of f, sdf and the flux is
 // coalesced; per-cell moment sums are reduced deterministically (fixed-order trees through
-// wave shuffles or LDS, then fixed-order sums of the partials), with no atomics anywhere. Private arrays are only ever indexed with
-// compile-time indices (direction-dependent picks are selects), so nothing spills to
-// scratch memory.
+// wave shuffles or LDS, then fixed-order sums of the partials), with no atomics anywhere.
+// Private arrays are only ever indexed with compile-time indices (direction-dependent picks
+// are selects), so nothing spills to scratch memory.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -217,8 +220,8 @@ struct Grid {
   __device__ double wt(int i) const { return at(gwb, i); }
   __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[off + i]); }
   __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
-  // f, sdf and the next f are stored as FT (double, or float in the FP32-f form); all
-  // arithmetic is FP64
+  // f, sdf and the next f are stored as FT (double, or float in the FP32 form); the
+  // accessors widen to double, and the FP32 form's arithmetic narrows where it advects
   __device__ double f(int i, int k) const { return double(at(fb, k * n + i)); }
   __device__ double s(int i, int k, int d) const { return double(at(sb, (d * NDF + k) * n + i)); }
   __device__ void set_s(int i, int k, int d, double v) const {

@@ -1,4 +1,4 @@
-// kah_flux_2d_f32.hip -- flux + relaxation kernel (K2) of the 2D2F form, FP32-f storage.
+// kah_flux_2d_f32.hip -- flux + relaxation kernel (K2) of the 2D2F form, FP32 form (f stored and advected in FP32).
 #include "kah_device.h"
 
 namespace kah {

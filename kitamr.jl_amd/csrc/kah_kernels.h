@@ -85,7 +85,7 @@ constexpr int SLOPE_CHUNK = 256;   // threads per workgroup of the chunked slope
 constexpr int SLOPE_MAX_VPT = 4;   // velocity cells per thread of the chunked slope kernel: 1..4
 constexpr int MAX_ENTRIES = 24;    // face sides per cell (2*DIM sides, up to 2^(DIM-1) each)
 
-// f32: f, sdf and the next f stored as float (FP32-f form), else double
+// f32: f, sdf and the next f stored as float (FP32 form), else double
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
                         int n, int maxh, hipStream_t st);
 // work = (cell, chunk) pairs, chunk = vpt*SLOPE_CHUNK velocity cells

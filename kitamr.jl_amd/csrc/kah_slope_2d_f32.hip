@@ -1,4 +1,4 @@
-// kah_slope_2d_f32.hip -- slope kernels (K1, K3) of the 2D2F form, FP32-f storage.
+// kah_slope_2d_f32.hip -- slope kernels (K1, K3) of the 2D2F form, FP32 form (f stored and advected in FP32).
 #include "kah_device.h"
 
 namespace kah {

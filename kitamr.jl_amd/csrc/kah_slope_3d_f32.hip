@@ -1,4 +1,4 @@
-// kah_slope_3d_f32.hip -- slope kernels (K1, K3) of the 3D1F form, FP32-f storage.
+// kah_slope_3d_f32.hip -- slope kernels (K1, K3) of the 3D1F form, FP32 form (f stored and advected in FP32).
 #include "kah_device.h"
 
 namespace kah {

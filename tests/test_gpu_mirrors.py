@@ -78,7 +78,7 @@ def test_pack_matches_reference_layout(name, precision):
         expect = host_pack(case.topo, st, cells, kind, dim, ndf)
         assert got.shape == expect.shape
         assert np.array_equal(got, expect), (kind, np.flatnonzero(got != expect)[:8])
-    # the same buffers from the oracle's state: the step's parity tolerance (FP32-f: the
+    # the same buffers from the oracle's state: the step's parity tolerance (FP32 form: the
     # moment bound of the float storage)
     tol = 1e-12 if precision == "fp64" else 1e-5
     got, _ = dev.pack_mirrors(cells, "data")
