@@ -194,3 +194,26 @@ def test_dvm_maxwellian_wall_raises_like_the_reference():
     o.slope()
     with pytest.raises(RuntimeError, match="here_weight"):
         o.flux(case.dt_xi)
+
+
+def test_maxwellian_walls_conserve_mass():
+    """Diffuse (Maxwellian) walls on every side of a box: the wall density rho_w
+    (calc_rho_w, src/Theory/Math.jl:251-282) makes the mass flux through each wall face zero,
+    and interior faces cancel in pairs, so the total mass stays constant to round-off. A
+    wrong wall density or a wrong wall-face sign leaks mass at O(dt)."""
+    case = cases.riemann_2d(n=8, nv=16, bc="Maxwellian")
+    n = case.n_cells
+    V = np.prod(np.asarray(case.topo["cell_ds"])[:n].reshape(n, 2), axis=1)
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(case.w, case.prim, case.df)
+    m0 = (np.asarray(case.w).reshape(n, 4)[:, 0] * V).sum()
+    for s in range(6):
+        o.step(case.dt_xi, s)
+    st = o.get_state()
+    m = (st["w"].reshape(n, 4)[:, 0] * V).sum()
+    assert abs(m - m0) <= 1e-13 * m0, (m - m0) / m0
+    # energy and momentum do exchange with the walls (a sanity check that the walls act)
+    e0 = (np.asarray(case.w).reshape(n, 4)[:, 3] * V).sum()
+    e = (st["w"].reshape(n, 4)[:, 3] * V).sum()
+    assert abs(e - e0) > 1e-10 * e0
+    o.close()
