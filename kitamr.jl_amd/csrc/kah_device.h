@@ -569,7 +569,10 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   __shared__ double tot[CNT];
   __shared__ double rho_sh[MAXH];
 
-  using CT = typename Arith<FT>::type;
+  // FP64 arithmetic in both forms: the wall density zeroes the wall face's mass flux only
+  // if its sums and the cell's own half fluxes round alike (in the FP32 form the slopes
+  // computed here are still rounded to float when stored)
+  using CT = double;
   double acc[CNT];
 #pragma unroll
   for (int q = 0; q < CNT; ++q) acc[q] = 0.0;

@@ -164,3 +164,21 @@ def test_autotune_on_device_picks_a_timed_setting():
     best, ms, trials = autotune.autotune(case, steps=2, warmup=1, log=say)
     assert ms > 0 and best in [t[0] for t in trials]
     assert ms == min(t[1] for t in trials)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_walled_box_conserves_mass_on_device(precision):
+    """Maxwellian walls on every side: the device's wall density keeps the total mass
+    (FP64 and the FP32 form, whose w update stays FP64)."""
+    case = cases.riemann_2d(n=8, nv=16, bc="Maxwellian")
+    say(f"[parity] walled box mass, precision {precision}")
+    dev, _ = pair(case, precision=precision)
+    n = case.n_cells
+    V = np.prod(np.asarray(case.topo["cell_ds"])[:n].reshape(n, 2), axis=1)
+    m0 = (np.asarray(case.w).reshape(n, 4)[:, 0] * V).sum()
+    for s in range(6):
+        dev.step(case.dt_xi, s)
+    m = (dev.get_state()["w"].reshape(n, 4)[:, 0] * V).sum()
+    assert abs(m - m0) <= 1e-13 * m0, (m - m0) / m0
+    dev.close()
