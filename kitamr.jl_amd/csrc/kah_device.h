@@ -63,8 +63,8 @@ __device__ inline void block_reduce(const double (&vals)[CNT], int cnt, double* 
 // partials, and one thread adds the last 4; fixed order throughout. buf needs
 // CHUNK_QB*SLOPE_CHUNK doubles, part CHUNK_QB*36, out cnt.
 constexpr int CHUNK_QB = 8;
-template <int CNT, int B0 = 0>
-__device__ inline void chunk_sum(const double (&vals)[CNT], int cnt, double* buf, double* part,
+template <int CNT, int B0 = 0, class V>
+__device__ inline void chunk_sum(const V (&vals)[CNT], int cnt, double* buf, double* part,
                                  double* out) {
   static_assert(SLOPE_CHUNK == 256, "chunk_sum assumes 256 threads");
   constexpr int NT = SLOPE_CHUNK, QB = CHUNK_QB, TPV = NT / QB, RPT = NT / TPV;
@@ -101,7 +101,7 @@ __device__ inline void chunk_sum(const double (&vals)[CNT], int cnt, double* buf
   if (t < QB && B0 + t < cnt)
     out[B0 + t] = ((part2[4 * t] + part2[4 * t + 1]) + part2[4 * t + 2]) + part2[4 * t + 3];
   __syncthreads();
-  if constexpr (B0 + QB < CNT) chunk_sum<CNT, B0 + QB>(vals, cnt, buf, part, out);
+  if constexpr (B0 + QB < CNT) chunk_sum<CNT, B0 + QB, V>(vals, cnt, buf, part, out);
 }
 
 // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs; give every XCD
@@ -674,7 +674,10 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
   __shared__ double part[CHUNK_QB * 36];
   __shared__ double tot[CNT];
 
-  double acc[CNT];
+  // Per-thread partials (VPT <= 3 terms each) in the arithmetic type: FP32 in the FP32 form,
+  // which halves their registers; the chunk sum over threads is FP64 in both forms.
+  using AT = typename Arith<FT>::type;
+  AT acc[CNT];
 #pragma unroll
   for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
   // velocity cells ch*VPT*SLOPE_CHUNK + r*SLOPE_CHUNK + threadIdx.x, r < VPT
