@@ -126,6 +126,38 @@ typedef struct kah_topology {
   int32_t reserved;
   double  vs_lo[3];
   double  vs_hi[3];
+
+  /* Immersed boundary (ImmersedBoundary, src/Boundary/Types.jl:185-199); n_ib_faces = 0
+   * and n_ib_solid = 0 when there is none. The host computes the geometry once per mesh
+   * (solid and donor flags in bound_enc, calc_intersect, the image stencils and the cut
+   * velocity cells of initialize_cutted_velocity_cell); the library runs
+   * update_solid_cell! and update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:
+   * 122-148, 436-500) on the device every step, between the slopes and the flux. Needs a
+   * shared velocity grid (vs_uniform), the CAIDVM flux and a marching other than
+   * CIP_Marching (whose positivity_preserving_ib! is not built).
+   *
+   * Solid cells: the bound_enc < 0 cells whose f update_solid_cell! rebuilds from their
+   * fluid face neighbours (the first neighbour of each face id, in face-id order). */
+  int64_t n_ib_solid;
+  const int64_t* ib_solid;        /* [n_ib_solid]                                      */
+  const int64_t* ib_solid_nb_off; /* [n_ib_solid+1]                                    */
+  const int64_t* ib_solid_nb;     /* [ib_solid_nb_off[n_ib_solid]]                     */
+  /* IB faces: one per (donor cell, solid face neighbour) = one SolidNeighbor. ib_face_pair
+   * names the flux pair of the face (here = the donor, there = the solid cell). The image
+   * stencil lists the donor's fluid neighbours (first per face id, face-id order) and then
+   * the donor itself (image_df, :367-395). Cut cells: the velocity cells of the donor's
+   * grid that u.n = 0 splits, with the areas of their gas (u.n < 0) and solid parts. */
+  int64_t n_ib_faces;
+  const int64_t* ib_face_pair;    /* [n_ib_faces]                                      */
+  const double*  ib_aux_point;    /* [n_ib_faces*dim]  SolidNeighbor.aux_point          */
+  const double*  ib_normal;       /* [n_ib_faces*dim]  SolidNeighbor.normal             */
+  const double*  ib_bc;           /* [n_ib_faces*(dim+2)]  wall prim (get_bc); [0] unused */
+  const int64_t* ib_image_off;    /* [n_ib_faces+1]                                    */
+  const int64_t* ib_image_cell;   /* [ib_image_off[n_ib_faces]]                        */
+  const int64_t* ib_cut_off;      /* [n_ib_faces+1]                                    */
+  const int32_t* ib_cut_index;    /* [ib_cut_off[n_ib_faces]]  ascending per face       */
+  const double*  ib_cut_gas;      /* CuttedVelocityCells.gas_weights                    */
+  const double*  ib_cut_solid;    /* CuttedVelocityCells.solid_weights                  */
 } kah_topology;
 
 typedef struct kah_ctx kah_ctx;
@@ -157,6 +189,12 @@ int kah_get_state(kah_ctx* ctx, double* w, double* prim, double* qf, double* sw,
  * (NULL skips). They hold the face fluxes between kah_flux and kah_iterate and are zero
  * after kah_iterate / kah_step, as after the reference's iterate!. */
 int kah_get_flux(kah_ctx* ctx, double* flux, double* vs_flux);
+
+/* Downloads the distribution of every SolidNeighbor [n_ib_faces*n0*ndf] (column k of face
+ * j at (j*ndf + k)*n0), as update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:
+ * 436-483) left it in the last flux: the wall-side data of the IB faces. KAH_ESTATE before
+ * the first flux or without immersed boundary. */
+int kah_get_ib(kah_ctx* ctx, double* aux_df);
 
 /* Ghost-exchange buffers of a list of cells, in the reference's wire format, one buffer per
  * listed cell concatenated in list order (a cell may be listed more than once):

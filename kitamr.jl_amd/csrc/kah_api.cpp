@@ -81,6 +81,12 @@ struct kah_ctx {
   // kah_pack_mirrors / kah_unpack_mirrors (grown on demand)
   std::vector<int64_t> voff_h;
   DBuf d_cmid, d_benc, d_xcells, d_xoff, d_xbuf, d_xfluid;
+  // immersed boundary: solid cells with their fluid neighbours, the IB faces (their
+  // SolidNeighbor df lives in the rows of the cells nc + k of the state buffers)
+  int ib_ns = 0, ib_nf = 0;
+  bool ib_done = false;
+  std::vector<int64_t> ib_of_pair;   // IB face of each pair, or -1
+  DBuf d_ib_solid, d_ib_snoff, d_ib_snb, d_ib_faces, d_ib_img, d_ib_cut, d_ib_cutw;
   // velocity-space adaptation: the root lattice of the velocity grids (valid when vs_ok)
   VsArgs vsa{};
   bool vs_ok = false;
@@ -122,7 +128,9 @@ struct kah_ctx {
                           &d_sdf, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
                           &d_res, &d_scratch, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
                           &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
-                          &d_moff, &d_cmid, &d_benc, &d_xcells, &d_xoff, &d_xbuf, &d_xfluid};
+                          &d_moff, &d_cmid, &d_benc, &d_xcells, &d_xoff, &d_xbuf, &d_xfluid,
+                          &d_ib_solid, &d_ib_snoff, &d_ib_snb, &d_ib_faces, &d_ib_img,
+                          &d_ib_cut, &d_ib_cutw};
     int64_t t = 0;
     for (const DBuf* b : bufs) t += int64_t(b->bytes);
     for (const LevelBin* b : bins) t += int64_t(b->items.bytes);
@@ -176,12 +184,15 @@ struct kah_ctx {
     return hip(hipStreamSynchronize(stream), what);
   }
   int download_f(double* dst, const DBuf& b, size_t count, const char* what) {
+    return download_f(dst, b.p, count, what);
+  }
+  int download_f(double* dst, const void* src, size_t count, const char* what) {
     if (!dst) return 0;
     if (!f32)
-      return hip(hipMemcpyAsync(dst, b.p, count * sizeof(double), hipMemcpyDeviceToHost, stream),
+      return hip(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, stream),
                  what);
     std::vector<float> tmp(count);
-    int rc = hip(hipMemcpyAsync(tmp.data(), b.p, count * sizeof(float), hipMemcpyDeviceToHost,
+    int rc = hip(hipMemcpyAsync(tmp.data(), src, count * sizeof(float), hipMemcpyDeviceToHost,
                                 stream), what);
     if (!rc) rc = hip(hipStreamSynchronize(stream), what);
     if (rc) return rc;
@@ -458,12 +469,18 @@ struct Builder {
                         "faces with solid cells need a shared velocity grid in this version");
         int8_t mh = solid ? MI_UNLIMITED : MI_LIMITED;
         int8_t mt = solid ? (caidvm ? MI_PLAIN : MI_UNLIMITED) : MI_LIMITED;
+        // an IB face reads its there side from the SolidNeighbor rows (cell nc + k), whose
+        // half flux the IB kernel writes after the slopes
+        const int64_t ibk = C.ib_of_pair.empty() ? -1 : C.ib_of_pair[size_t(p)];
         if (caidvm) {
           hal[h].push_back(half(s0, int32_t(h), mh, 0, dir, rot, h, fm, -1));
-          if (solid) hal[h].push_back(half(s1, int32_t(t), MI_PLAIN, 1, dir, rot, trec, fm, -1));
-          else hal[t].push_back(half(s1, int32_t(t), MI_LIMITED, 1, dir, rot, trec, fm, -1));
+          if (solid && ibk < 0)
+            hal[h].push_back(half(s1, int32_t(t), MI_PLAIN, 1, dir, rot, trec, fm, -1));
+          else if (!solid)
+            hal[t].push_back(half(s1, int32_t(t), MI_LIMITED, 1, dir, rot, trec, fm, -1));
         }
-        ent[h].push_back(entry(int32_t(p), 0, mh, mt, int32_t(t), dir, rot, area, fm, h, trec, -1));
+        const int32_t oth = ibk >= 0 ? int32_t(nc + ibk) : int32_t(t);
+        ent[h].push_back(entry(int32_t(p), 0, mh, mt, oth, dir, rot, area, fm, h, trec, -1));
         if (!solid)
           ent[t].push_back(entry(int32_t(p), 1, MI_LIMITED, MI_LIMITED, int32_t(h), dir, rot, area,
                                  fm, trec, h, -1));
@@ -472,6 +489,127 @@ struct Builder {
     return 0;
   }
 };
+
+// Immersed-boundary arrays of the topology: checked, then uploaded as device descriptors.
+// Fills C->ib_of_pair for the descriptor builder.
+int setup_ib(kah_ctx* C, const kah_topology* T) {
+  const int D = C->D, NW = C->NW;
+  const int64_t nc = T->n_cells;
+  C->ib_ns = 0;
+  C->ib_nf = 0;
+  C->ib_done = false;
+  C->ib_of_pair.clear();
+  for (DBuf* b : {&C->d_ib_solid, &C->d_ib_snoff, &C->d_ib_snb, &C->d_ib_faces, &C->d_ib_img,
+                  &C->d_ib_cut, &C->d_ib_cutw})
+    b->release();
+  const int64_t ns = T->n_ib_solid, nf = T->n_ib_faces;
+  if (ns == 0 && nf == 0) return 0;
+  if (ns < 0 || nf < 0 || ns > nc || nf > T->n_pairs)
+    return C->fail(KAH_EINVAL, "immersed boundary counts out of range");
+  if (!C->uni)
+    return C->fail(KAH_EUNSUPPORTED, "the immersed boundary needs a shared velocity grid");
+  if (C->P.flux_kind != KAH_FLUX_CAIDVM)
+    return C->fail(KAH_EUNSUPPORTED, "the immersed boundary is built for the CAIDVM flux");
+  if ((ns > 0 && (!T->ib_solid || !T->ib_solid_nb_off || !T->ib_solid_nb)) ||
+      (nf > 0 && (!T->ib_face_pair || !T->ib_aux_point || !T->ib_normal || !T->ib_bc ||
+                  !T->ib_image_off || !T->ib_image_cell || !T->ib_cut_off ||
+                  (T->ib_cut_off[nf] > 0 &&
+                   (!T->ib_cut_index || !T->ib_cut_gas || !T->ib_cut_solid)))))
+    return C->fail(KAH_EINVAL, "null immersed-boundary array");
+  auto fluid = [&](int64_t c) { return c >= 0 && c < nc && T->bound_enc[c] >= 0; };
+  std::vector<int32_t> sol(static_cast<size_t>(ns)), soff(static_cast<size_t>(ns + 1), 0), snb;
+  for (int64_t q = 0; q < ns; ++q) {
+    const int64_t s = T->ib_solid[q];
+    if (s < 0 || s >= nc || T->bound_enc[s] >= 0)
+      return C->fail(KAH_EINVAL, "ib_solid[" + std::to_string(q) + "] is not a solid cell");
+    const int64_t b = T->ib_solid_nb_off[q], e = T->ib_solid_nb_off[q + 1];
+    if (b != int64_t(snb.size()) || e - b < 1 || e - b > 8)
+      return C->fail(KAH_EINVAL, "solid cell " + std::to_string(s) +
+                                     ": 1 to 8 fluid neighbours in order expected");
+    for (int64_t x = b; x < e; ++x) {
+      if (!fluid(T->ib_solid_nb[x]))
+        return C->fail(KAH_EINVAL, "solid cell " + std::to_string(s) + ": neighbour " +
+                                       std::to_string(T->ib_solid_nb[x]) + " is not fluid");
+      snb.push_back(int32_t(T->ib_solid_nb[x]));
+    }
+    sol[q] = int32_t(s);
+    soff[q + 1] = int32_t(snb.size());
+  }
+  C->ib_of_pair.assign(size_t(T->n_pairs), -1);
+  std::vector<IbFaceDesc> fd(static_cast<size_t>(nf));
+  std::vector<int32_t> img, cut;
+  std::vector<double2> cutw;
+  for (int64_t k = 0; k < nf; ++k) {
+    const int64_t p = T->ib_face_pair[k];
+    if (p < 0 || p >= T->n_pairs || C->ib_of_pair[size_t(p)] >= 0)
+      return C->fail(KAH_EINVAL, "ib_face_pair[" + std::to_string(k) + "] out of range or repeated");
+    const int64_t h = T->pair_here[p], t = T->pair_there[p];
+    if (h < 0 || h >= nc || T->bound_enc[h] <= 0 || t < 0 || t >= nc || T->bound_enc[t] >= 0)
+      return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) +
+                                     ": pair must join a donor (here) and a solid cell (there)");
+    C->ib_of_pair[size_t(p)] = k;
+    IbFaceDesc& F = fd[size_t(k)];
+    F.donor = int32_t(h);
+    F.solid = int32_t(t);
+    F.pseudo = int32_t(nc + k);
+    F.dir = T->pair_dir[p];
+    F.slot = int32_t(2 * p + 1);
+    F.rot = T->pair_rot[p];
+    const int64_t ib = T->ib_image_off[k], ie = T->ib_image_off[k + 1];
+    if (ib != int64_t(img.size()) || ie - ib < 1 || ie - ib > 8)
+      return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) + ": 1 to 8 image cells in order expected");
+    for (int64_t x = ib; x < ie; ++x) {
+      if (!fluid(T->ib_image_cell[x]))
+        return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) + ": image cell " +
+                                       std::to_string(T->ib_image_cell[x]) + " is not fluid");
+      img.push_back(int32_t(T->ib_image_cell[x]));
+    }
+    F.img_beg = int32_t(ib);
+    F.img_cnt = int32_t(ie - ib);
+    const int64_t cb = T->ib_cut_off[k], ce = T->ib_cut_off[k + 1];
+    if (cb != int64_t(cut.size()) || ce < cb)
+      return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) + ": cut offsets out of order");
+    for (int64_t x = cb; x < ce; ++x) {
+      const int32_t i = T->ib_cut_index[x];
+      if (i < 0 || i >= C->n0 || (x > cb && i <= T->ib_cut_index[x - 1]) ||
+          !(T->ib_cut_gas[x] > 0.0) || !(T->ib_cut_solid[x] > 0.0))
+        return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) +
+                                       ": cut cells must be ascending, in range, with positive areas");
+      cut.push_back(i);
+      cutw.push_back(make_double2(T->ib_cut_gas[x], T->ib_cut_solid[x]));
+    }
+    F.cut_beg = int32_t(cb);
+    F.cut_cnt = int32_t(ce - cb);
+    const double* ap = &T->ib_aux_point[k * D];
+    const double* xd = &T->cell_mid[h * D];
+    const double* xs = &T->cell_mid[t * D];
+    for (int d = 0; d < 3; ++d) {
+      F.ip[d] = d < D ? ap[d] + xd[d] - xs[d] : 0.0;
+      F.nrm[d] = d < D ? T->ib_normal[k * D + d] : 0.0;
+    }
+    F.dxs = xd[F.dir] - xs[F.dir];
+    F.dxf = F.ip[F.dir] - xd[F.dir];
+    F.dxL = ap[F.dir] - F.ip[F.dir];
+    for (int m = 0; m < 5; ++m) F.bc[m] = m < NW ? T->ib_bc[k * NW + m] : 0.0;
+    F.bc[0] = 1.0;
+  }
+  C->ib_ns = int(ns);
+  C->ib_nf = int(nf);
+  if (ns > 0) {
+    KAH_TRY(C->upload(C->d_ib_solid, sol.data(), sol.size(), "ib solid cells"));
+    KAH_TRY(C->upload(C->d_ib_snoff, soff.data(), soff.size(), "ib solid neighbour offsets"));
+    KAH_TRY(C->upload(C->d_ib_snb, snb.data(), snb.size(), "ib solid neighbours"));
+  }
+  if (nf > 0) {
+    KAH_TRY(C->upload(C->d_ib_faces, fd.data(), fd.size(), "ib faces"));
+    KAH_TRY(C->upload(C->d_ib_img, img.data(), img.size(), "ib image cells"));
+    if (!cut.empty()) {
+      KAH_TRY(C->upload(C->d_ib_cut, cut.data(), cut.size(), "ib cut cells"));
+      KAH_TRY(C->upload(C->d_ib_cutw, cutw.data(), cutw.size(), "ib cut areas"));
+    }
+  }
+  return 0;
+}
 
 }  // namespace
 
@@ -672,7 +810,8 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   if (!C->uni) KAH_TRY(C->upload(C->d_gkey, keys.data(), keys.size(), "vs_key"));
   else C->d_gkey.release();
 
-  // descriptors
+  // immersed boundary, then the descriptors
+  KAH_TRY(setup_ib(C, T));
   Builder B(*C, *T);
   KAH_TRY(B.build());
 
@@ -863,6 +1002,8 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   for (int64_t c = 0; c < nc; ++c)
     for (int h = hoff[c]; h < hoff[c + 1]; ++h)
       pcnt[halves[h].slot] = walled[c] ? 1 : nchunks(c);
+  for (int64_t p = 0; p < int64_t(C->ib_of_pair.size()); ++p)
+    if (C->ib_of_pair[size_t(p)] >= 0) pcnt[size_t(2 * p + 1)] = 1;   // written by the IB kernel
   int64_t rows = 0;
   for (size_t s = 0; s < poff.size(); ++s) {
     poff[s] = rows;
@@ -875,8 +1016,14 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   // state buffers
   const int64_t NDF = C->NDF, NW = C->NW;
   const size_t fs = C->fsize();
-  KAH_TRY(C->alloc(C->d_df[0], fs * C->np * NDF, "df"));
-  KAH_TRY(C->alloc(C->d_df[1], fs * C->np * NDF, "df"));
+  // the state buffers hold the SolidNeighbor rows of the IB faces after the cells' rows
+  const int64_t prow = C->np + int64_t(C->ib_nf) * C->n0;
+  KAH_TRY(C->alloc(C->d_df[0], fs * prow * NDF, "df"));
+  KAH_TRY(C->alloc(C->d_df[1], fs * prow * NDF, "df"));
+  if (C->ib_nf > 0) {
+    KAH_TRY(C->zero(C->d_df[0]));
+    KAH_TRY(C->zero(C->d_df[1]));
+  }
   KAH_TRY(C->alloc(C->d_sdf, fs * C->np * NDF * D, "sdf"));
   KAH_TRY(C->alloc(C->d_psflux, sizeof(double) * nc * NW, "flux"));
   KAH_TRY(C->alloc(C->d_w, sizeof(double) * nc * NW, "w"));
@@ -992,6 +1139,20 @@ int run_slope(kah_ctx* C, const DevArgs& A) {
                                   C->stream), "slope kernel launch"));
   }
   if (C->timing) KAH_TRY(ev_end(C, e));
+  return 0;
+}
+
+// update_solid_cell! and update_solid_neighbor! (src/Flux/Flux.jl:461-482)
+int run_ib(kah_ctx* C, const DevArgs& A) {
+  if (C->ib_ns == 0 && C->ib_nf == 0) return 0;
+  KAH_TRY(C->hip(launch_ib(A, C->D, C->f32, C->ib_ns, C->d_ib_solid.as<int32_t>(),
+                           C->d_ib_snoff.as<int32_t>(), C->d_ib_snb.as<int32_t>(), C->ib_nf,
+                           C->d_ib_faces.as<IbFaceDesc>(), C->d_ib_img.as<int32_t>(),
+                           C->d_ib_cut.as<int32_t>(), C->d_ib_cutw.as<double2>(),
+                           C->d_cmid.as<double>(), C->d_df[C->cur].p, C->d_df[1 - C->cur].p,
+                           C->stream),
+                 "immersed-boundary kernel launch"));
+  if (C->ib_nf > 0) C->ib_done = true;
   return 0;
 }
 
@@ -1479,8 +1640,23 @@ int kah_flux(kah_ctx* C, double dt) {
     KAH_TRY(C->zero(C->d_vflux));
   }
   DevArgs A = make_args(C, dt);
+  KAH_TRY(run_ib(C, A));
   KAH_TRY(run_flux_relax(C, A, MODE_FLUX));
   return C->hip(hipStreamSynchronize(C->stream), "flux");
+}
+
+int kah_get_ib(kah_ctx* C, double* aux_df) {
+  if (!C) return KAH_EINVAL;
+  if (!aux_df) return C->fail(KAH_EINVAL, "null output");
+  if (C->ib_nf == 0 || !C->ib_done)
+    return C->fail(KAH_ESTATE, "no immersed-boundary faces computed yet");
+  KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
+  // both state buffers hold the rows of the last flux
+  const size_t off = size_t(C->np) * C->NDF * C->fsize();
+  const size_t cnt = size_t(C->ib_nf) * C->n0 * C->NDF;
+  KAH_TRY(C->download_f(aux_df, static_cast<const char*>(C->d_df[C->cur].p) + off, cnt,
+                        "ib download"));
+  return C->hip(hipStreamSynchronize(C->stream), "ib download");
 }
 
 int kah_iterate(kah_ctx* C, double dt, int64_t step, double* sum_res, double* sum_avg) {
@@ -1511,6 +1687,7 @@ int kah_step(kah_ctx* C, double dt, int64_t step, double* sum_res, double* sum_a
       KAH_TRY(C->hip(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal),
                      "hipStreamBeginCapture"));
       int rc = run_slope(C, A);
+      if (!rc) rc = run_ib(C, A);
       if (!rc) rc = run_flux_relax(C, A, MODE_FUSED);
       hipError_t ec = hipStreamEndCapture(C->stream, &g);
       if (rc) {
@@ -1526,6 +1703,7 @@ int kah_step(kah_ctx* C, double dt, int64_t step, double* sum_res, double* sum_a
     KAH_TRY(C->hip(hipGraphLaunch(C->gexec[k], C->stream), "hipGraphLaunch"));
   } else {
     KAH_TRY(run_slope(C, A));
+    KAH_TRY(run_ib(C, A));
     KAH_TRY(run_flux_relax(C, A, MODE_FUSED));
   }
   bool check = (step % C->P.st_check_interval) == 0 && (sum_res || sum_avg);

@@ -2006,6 +2006,267 @@ inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int
   return launch_fr_w<D, NDF, UNI, FT, 8>(A, cells, n, mode, march, lds, st);
 }
 
+// ------------------------------------------------------------------ immersed boundary
+// update_solid_cell! and update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:122-148,
+// 367-483) between the slopes and the flux, on a shared velocity grid: one workgroup per
+// solid cell or IB face, the threads striding over the velocity cells. Arithmetic in FP64.
+
+// sum of one value over the 256-thread workgroup (every thread gets the sum)
+__device__ inline double block_sum256(double v, double* sh) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// sum_j wn_j (f_j + sdf_j . (x - x_j)) over the listed cells at velocity cell i, with
+// wn_j = w_j / sum w, w_j = max(0, u.l_j/|u|)^2, l_j = (x - x_j)/|x - x_j| (1/N when the sum
+// is zero): vs_extrapolate! with the weights of update_solid_cell! / image_df (:24-47,
+// :122-141, :378-395). Sums of products in index order, as Julia's dot and norm.
+template <int D, int NDF, class FT>
+__device__ inline void ib_extrapolate(const DevArgs& A, const int32_t* cells, int N,
+                                      const double* cmid, const double* x, int i,
+                                      const double (&u)[D], double (&out)[NDF]) {
+  double nu = u[0] * u[0];
+#pragma unroll
+  for (int t = 1; t < D; ++t) nu += u[t] * u[t];
+  nu = sqrt(nu);
+  double w[8], ws = 0.0;
+  for (int j = 0; j < N; ++j) {
+    const double* xj = &cmid[int64_t(cells[j]) * D];
+    double l[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) l[t] = x[t] - xj[t];
+    double nl = l[0] * l[0];
+#pragma unroll
+    for (int t = 1; t < D; ++t) nl += l[t] * l[t];
+    nl = sqrt(nl);
+#pragma unroll
+    for (int t = 0; t < D; ++t) l[t] = l[t] / nl;
+    double d = u[0] * l[0];
+#pragma unroll
+    for (int t = 1; t < D; ++t) d += u[t] * l[t];
+    const double r = d / nu;
+    const double c = (r != r) ? r : (r > 0.0 ? r : 0.0);
+    w[j] = c * c;
+    ws = (j == 0) ? w[j] : ws + w[j];
+  }
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) out[k] = 0.0;
+  for (int j = 0; j < N; ++j) {
+    const double wn = ws == 0.0 ? 1.0 / double(N) : w[j] / ws;
+    const Grid<D, NDF, true, FT> G(A, cells[j]);
+    const double* xj = &cmid[int64_t(cells[j]) * D];
+    double dx[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) dx[t] = x[t] - xj[t];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      double ddf = G.s(i, k, 0) * dx[0];
+#pragma unroll
+      for (int t = 1; t < D; ++t) ddf += G.s(i, k, t) * dx[t];
+      out[k] += (G.f(i, k) + ddf) * wn;
+    }
+  }
+}
+
+// update_solid_cell!: f of solid cell s from its fluid face neighbours, written to both
+// state buffers (this step's IB faces read it; the next step keeps it), then w and prim
+template <int D, int NDF, class FT>
+__global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __restrict__ solid,
+                                                  const int32_t* __restrict__ nb_off,
+                                                  const int32_t* __restrict__ nb,
+                                                  const double* __restrict__ cmid, FT* df_a,
+                                                  FT* df_b) {
+  constexpr int NW = D + 2;
+  __shared__ double red[4];
+  const int q = blockIdx.x;
+  const int s = solid[q];
+  const int b = nb_off[q], N = nb_off[q + 1] - b;
+  const int n = A.uni_n;
+  const Grid<D, NDF, true, FT> G(A, s);
+  const double* xs = &cmid[int64_t(s) * D];
+  double m[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) m[r] = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double u[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
+    double f[NDF];
+    ib_extrapolate<D, NDF, FT>(A, nb + b, N, cmid, xs, i, u, f);
+    double fs[NDF];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      const FT v = FT(f[k]);
+      df_a[(int64_t(s) * NDF + k) * n + i] = v;
+      df_b[(int64_t(s) * NDF + k) * n + i] = v;
+      fs[k] = double(v);
+    }
+    double mt[NW];
+    moment_terms<D, NDF, double>(G.wt(i), u, fs, mt);
+#pragma unroll
+    for (int r = 0; r < NW; ++r) m[r] += mt[r];
+  }
+  double w[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) w[r] = block_sum256(m[r], red);
+  w[NW - 1] = 0.5 * w[NW - 1];
+  if (threadIdx.x == 0) {
+    double p[NW];
+    get_prim<D>(w, A.gamma, p);
+#pragma unroll
+    for (int r = 0; r < NW; ++r) {
+      A.w[int64_t(s) * NW + r] = w[r];
+      A.prim[int64_t(s) * NW + r] = p[r];
+    }
+  }
+}
+
+// update_solid_neighbor! of one IB face: image_df, boundary_slope! and its positivity
+// scaling, the wall Maxwellian of zero net mass flux through the wall (cvc_Mu /
+// cvc_density) and the cut-cell mixing (cvc_correction!). The SolidNeighbor's df goes to
+// the pseudo cell's rows of both state buffers; the face's there half (df .* u[dir] over
+// rot*u[dir] > 0, calc_flux with a solid there side, src/Flux/CAIDVM.jl:110-112) goes to
+// its half_fw row, which the flux kernel sums like any other.
+template <int D, int NDF, class FT>
+__global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __restrict__ faces,
+                                                 const int32_t* __restrict__ img,
+                                                 const int32_t* __restrict__ cut_idx,
+                                                 const double2* __restrict__ cut_w,
+                                                 const double* __restrict__ cmid, FT* df_a,
+                                                 FT* df_b) {
+  constexpr int NW = D + 2;
+  __shared__ double red[4];
+  const IbFaceDesc F = faces[blockIdx.x];
+  const int n = A.uni_n;
+  const Grid<D, NDF, true, FT> G(A, F.donor);
+  const Grid<D, NDF, true, FT> S(A, F.solid);
+  FT* pa = df_a + int64_t(F.pseudo) * NDF * n;
+  FT* pb = df_b + int64_t(F.pseudo) * NDF * n;
+  const int32_t* ci = cut_idx + F.cut_beg;
+  // the cut-cell slot of velocity cell i, or -1 (the list is ascending)
+  auto cut_of = [&](int i) {
+    int lo = 0, hi = F.cut_cnt - 1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = ci[mid];
+      if (v == i) return mid;
+      if (v < i) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+  };
+  // pass 1: the extrapolated df at the wall, and the two flux sums of the wall density
+  double sf = 0.0, mur = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double u[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
+    double im[NDF];
+    ib_extrapolate<D, NDF, FT>(A, img + F.img_beg, F.img_cnt, cmid, F.ip, i, u, im);
+    double aux[NDF];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      const double fc = G.f(i, k);
+      const double sL = (fc - S.f(i, k)) / F.dxs;
+      double sl = minmod(sL, (im[k] - fc) / F.dxf);
+      sl = jl_min(fabs((im[k] - MACH_EPS) / (sl * F.dxL + MACH_EPS)), 1.0) * sl;
+      aux[k] = im[k] + sl * F.dxL;
+      pa[k * n + i] = FT(aux[k]);
+    }
+    double vn = u[0] * F.nrm[0];
+#pragma unroll
+    for (int t = 1; t < D; ++t) vn += u[t] * F.nrm[t];
+    double M[2];
+    maxwell<D>(u, F.bc, A.K, M);
+    const int j = cut_of(i);
+    if (j >= 0) {
+      const double2 gw = cut_w[F.cut_beg + j];
+      mur += gw.y * vn * M[0];
+      sf += gw.x * vn * aux[0];
+    } else if (vn >= 0.0) {
+      mur += G.wt(i) * vn * M[0];
+    } else {
+      sf += G.wt(i) * vn * aux[0];
+    }
+  }
+  const double SF = block_sum256(sf, red);
+  const double MuR = block_sum256(mur, red);
+  const double rho = -SF / MuR;
+  // pass 2: the wall Maxwellian on u.n >= 0, the cut-cell mix, the there-half moments. Each
+  // thread revisits the velocity cells it wrote in pass 1, so no barrier is needed here.
+  double hm[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) hm[r] = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double u[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
+    double vn = u[0] * F.nrm[0];
+#pragma unroll
+    for (int t = 1; t < D; ++t) vn += u[t] * F.nrm[t];
+    double M[2];
+    maxwell<D>(u, F.bc, A.K, M);
+    const int j = cut_of(i);
+    double out[NDF];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      const double Mk = M[k] * rho;
+      const double g = double(pa[k * n + i]);
+      if (j >= 0) {
+        const double2 gw = cut_w[F.cut_beg + j];
+        out[k] = (gw.x * g + gw.y * Mk) / (gw.x + gw.y);
+      } else {
+        out[k] = vn >= 0.0 ? Mk : g;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      const FT v = FT(out[k]);
+      pa[k * n + i] = v;
+      pb[k * n + i] = v;
+      out[k] = double(v);
+    }
+    const double ud = pick<D>(u, F.dir);
+    if (F.rot * ud > 0.0) {
+      double mic[NDF], mt[NW];
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) mic[k] = out[k] * ud;
+      moment_terms<D, NDF, double>(G.wt(i), u, mic, mt);
+#pragma unroll
+      for (int r = 0; r < NW; ++r) hm[r] += mt[r];
+    }
+  }
+  double tot[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) tot[r] = block_sum256(hm[r], red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NW; ++r) A.half_fw[A.part_off[F.slot] * NW + r] = tot[r];
+  }
+}
+
+template <int D, int NDF, class FT>
+inline hipError_t launch_ib_t(const DevArgs& A, int ns, const int32_t* solid,
+                              const int32_t* nb_off, const int32_t* nb, int nf,
+                              const IbFaceDesc* faces, const int32_t* img, const int32_t* cut_idx,
+                              const double2* cut_w, const double* cmid, void* df_a, void* df_b,
+                              hipStream_t st) {
+  if (ns > 0) {
+    hipLaunchKernelGGL((k_ib_solid<D, NDF, FT>), dim3(ns), dim3(256), 0, st, A, solid, nb_off, nb,
+                       cmid, static_cast<FT*>(df_a), static_cast<FT*>(df_b));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (nf > 0) {
+    hipLaunchKernelGGL((k_ib_face<D, NDF, FT>), dim3(nf), dim3(256), 0, st, A, faces, img, cut_idx,
+                       cut_w, cmid, static_cast<FT*>(df_a), static_cast<FT*>(df_b));
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
 template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_macro_slope_t(const DevArgs& A, const int32_t* cells, int n, double* sw,
                                        hipStream_t st) {

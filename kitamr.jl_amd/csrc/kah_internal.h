@@ -111,6 +111,21 @@ struct EntryDesc {      // one face side of a fluid cell, gathered in K2
   float dxt_f[3], dst_f[3];
 };
 
+struct IbFaceDesc {     // one IB face (donor, SolidNeighbor), rebuilt every step
+  int32_t donor, solid; // cells of the face
+  int32_t pseudo;       // cell index whose f rows hold the SolidNeighbor's df
+  int32_t dir;
+  int32_t img_beg, img_cnt;   // image stencil (fluid neighbours, then the donor)
+  int32_t cut_beg, cut_cnt;   // cut velocity cells
+  int32_t slot;         // half_fw row of the face's there half (pair*2 + 1)
+  int32_t pad;
+  double rot;
+  double ip[3];         // image point aux + x_donor - x_solid
+  double nrm[3];        // wall normal
+  double dxs, dxf, dxL; // boundary_slope! distances along dir
+  double bc[5];         // wall prim, density 1
+};
+
 // the double or the float copy of a descriptor value, for arithmetic type T
 template <class T>
 __host__ __device__ inline T desc_val(double d, float f) {

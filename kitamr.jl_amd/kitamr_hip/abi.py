@@ -27,7 +27,7 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
            "kah_copy_bandwidth", "kah_set_option", "kah_mirror_layout", "kah_pack_mirrors",
            "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart", "kah_load_restart",
-           "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap", "kah_vs_project"]
+           "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap", "kah_vs_project", "kah_get_ib"]
 MIRROR = {"data": 0, "slope": 1}
 OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5}
 
@@ -66,7 +66,12 @@ class KahTopology(C.Structure):
                 ("pair_dir", _P), ("pair_rot", _P), ("pair_area", _P), ("pair_mid", _P),
                 ("pair_bc", _P), ("n_bc", C.c_int32), ("vs_maxlevel", C.c_int32),
                 ("bc_kind", _P), ("bc_prim", _P), ("vs_trees_num", C.c_int32 * 3),
-                ("reserved", C.c_int32), ("vs_lo", C.c_double * 3), ("vs_hi", C.c_double * 3)]
+                ("reserved", C.c_int32), ("vs_lo", C.c_double * 3), ("vs_hi", C.c_double * 3),
+                ("n_ib_solid", C.c_int64), ("ib_solid", _P), ("ib_solid_nb_off", _P),
+                ("ib_solid_nb", _P), ("n_ib_faces", C.c_int64), ("ib_face_pair", _P),
+                ("ib_aux_point", _P), ("ib_normal", _P), ("ib_bc", _P), ("ib_image_off", _P),
+                ("ib_image_cell", _P), ("ib_cut_off", _P), ("ib_cut_index", _P),
+                ("ib_cut_gas", _P), ("ib_cut_solid", _P)]
 
 
 _TOPO_ARRAYS = {
@@ -77,6 +82,11 @@ _TOPO_ARRAYS = {
     "pair_there": np.int64, "pair_dir": np.int32, "pair_rot": np.float64,
     "pair_area": np.float64, "pair_mid": np.float64, "pair_bc": np.int32,
     "bc_kind": np.int32, "bc_prim": np.float64,
+    "ib_solid": np.int64, "ib_solid_nb_off": np.int64, "ib_solid_nb": np.int64,
+    "ib_face_pair": np.int64, "ib_aux_point": np.float64, "ib_normal": np.float64,
+    "ib_bc": np.float64, "ib_image_off": np.int64, "ib_image_cell": np.int64,
+    "ib_cut_off": np.int64, "ib_cut_index": np.int32, "ib_cut_gas": np.float64,
+    "ib_cut_solid": np.float64,
 }
 
 
@@ -98,6 +108,8 @@ def make_topology_struct(topo):
     t.n_bc = int(len(topo.get("bc_kind", [])))
     t.vs_uniform = int(topo.get("vs_uniform", 0))
     t.vs_maxlevel = int(topo.get("vs_maxlevel", 0))
+    t.n_ib_solid = int(len(topo.get("ib_solid", [])))
+    t.n_ib_faces = int(len(topo.get("ib_face_pair", [])))
     vtn = list(topo.get("vs_trees_num", [0, 0, 0])) + [0, 0, 0]
     lo = list(topo.get("vs_lo", [0.0, 0.0, 0.0])) + [0.0, 0.0, 0.0]
     hi = list(topo.get("vs_hi", [0.0, 0.0, 0.0])) + [0.0, 0.0, 0.0]
@@ -146,6 +158,7 @@ def load_library(path=None):
     lib.kah_set_state.argtypes = [C.c_void_p, dp, dp, dp]
     lib.kah_get_state.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp]
     lib.kah_get_flux.argtypes = [C.c_void_p, dp, dp]
+    lib.kah_get_ib.argtypes = [C.c_void_p, dp]
     lib.kah_slope.argtypes = [C.c_void_p, d]
     lib.kah_flux.argtypes = [C.c_void_p, d]
     lib.kah_iterate.argtypes = [C.c_void_p, d, i64, dp, dp]
@@ -178,7 +191,7 @@ def load_library(path=None):
                  "kah_kernel_times", "kah_mirror_layout", "kah_pack_mirrors",
                  "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart",
                  "kah_load_restart", "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap",
-                 "kah_vs_project"]:
+                 "kah_vs_project", "kah_get_ib"]:
         getattr(lib, name).restype = C.c_int
     del i32
     if path is None:
@@ -234,6 +247,8 @@ class HipContext:
         self.nc = int(topo["n_cells"])
         off = np.asarray(topo["vs_offset"])
         self.np = int(off[-1])
+        self.n_ib = int(len(topo.get("ib_face_pair", [])))
+        self.n0 = int(off[1]) if self.nc else 0
 
     def set_state(self, w, prim, df):
         w = np.ascontiguousarray(w, dtype=np.float64)
@@ -265,6 +280,13 @@ class HipContext:
         vflux = np.empty(self.np * self.ndf)
         self._check(self.lib.kah_get_flux(self.h, _ptr(flux), _ptr(vflux)), "kah_get_flux")
         return flux, vflux
+
+    def get_ib(self):
+        """The SolidNeighbor distributions of the last flux, [n_ib_faces, ndf, n0]."""
+        n_ib, n0 = getattr(self, "n_ib", 0), getattr(self, "n0", 0)
+        out = np.empty(max(n_ib * self.ndf * n0, 1))
+        self._check(self.lib.kah_get_ib(self.h, _ptr(out)), "kah_get_ib")
+        return out[:n_ib * self.ndf * n0].reshape(n_ib, self.ndf, n0)
 
     def _cells(self, cells):
         c = np.ascontiguousarray(cells, dtype=np.int32)

@@ -8,6 +8,8 @@ wall_channel  the reference's own smoke case (test/runtests.jl:1-42): Maxwellian
 box_3d        3D1F analogue on a small octree brick.
 periodic_box_2d  the Riemann initial state (or a uniform state) with periodic boundaries on
               all sides, where mass, momentum and energy are conserved exactly.
+cylinder_2d   flow past a circular immersed boundary (example/cylinder/cylinder.jl):
+              supersonic inflow at xmin, outflow elsewhere, a Maxwellian wall on the circle.
 """
 from __future__ import annotations
 
@@ -160,3 +162,27 @@ def box_3d(n=4, nv=8, vmax=6.0, ps_levels=0, bcs=("UniformOutflow",) * 6, vs_lev
                                          rng=_seed + r)
             return cache[r]
     return build_case(cfg, ps_refine=refine, vs_grid_fn=vs_fn)
+
+
+def cylinder_2d(n=32, nv=16, vmax=10.0, L=4.0, radius=1.0, Ma=1.5, Kn=0.1, omega=0.81,
+                ps_levels=0, band=0.6, flux="CAIDVM", marching="CAIDVM_Marching",
+                precision="fp64", wall=(1.0, 0.0, 0.0, 1.0), CFL=0.8):
+    """Circle(Maxwellian, [0,0], radius, true, ., wall) in [-L, L]^2 with an inflow of Mach
+    Ma (U = Ma*sqrt(5/6) at T = 1) from xmin; ps_levels > 0 refines the band |r - R| < band
+    (and 2:1 balances), so that the cells on both sides of the circle share one level."""
+    from .ib import immerse_circle
+    solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=0, flux=flux,
+                    time_marching=marching, CFL=CFL)
+    gas = Gas(K=1.0, gamma=5.0 / 3.0, Pr=2.0 / 3.0, Kn=Kn, omega=omega, omega_ref=omega)
+    U = Ma * math.sqrt(5.0 / 6.0)
+    free = [1.0, U, 0.0, 1.0]
+    dom = [Domain("SuperSonicInflow", 1, free)] + [Domain("UniformOutflow", i) for i in (2, 3, 4)]
+    cfg = Configure(solver, gas, [-L, L, -L, L], [n, n], [-vmax, vmax, -vmax, vmax], [nv, nv],
+                    Uniform(free), dom, precision=precision)
+    refine = None
+    if ps_levels > 0:
+        def refine(mid, ds, level):
+            return np.abs(np.hypot(mid[:, 0], mid[:, 1]) - radius) < band
+    case = build_case(cfg, ps_refine=refine)
+    case.topo = immerse_circle(case.topo, (0.0, 0.0), radius, True, wall)
+    return case

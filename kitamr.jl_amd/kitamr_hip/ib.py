@@ -1,0 +1,248 @@
+"""Immersed-boundary geometry on the host: a circle in a 2D mesh.
+
+The geometry is computed once per mesh and handed to the library with the topology; the
+per-step work (update_solid_cell!, update_solid_neighbor! and the IB face fluxes) runs on the
+device. This module restates, for ``Circle`` (src/Boundary/Circle.jl, Types.jl:83-113):
+
+* the cell roles: solid_flag / boundary_flag (Circle.jl:1-23): a cell whose midpoint is in
+  the solid is a solid cell (bound_enc < 0) when one of its four neighbour points crosses the
+  circle, else it lies inside the solid (InsideSolidData: no faces, no update); a fluid cell
+  with a solid face neighbour is a donor (bound_enc > 0,
+  initialize_solid_neighbor!, src/Boundary/Immersed_boundary.jl:258-275);
+* the face list: a face between a solid cell and a fluid cell is owned by the fluid cell
+  (initialize_full_face!, src/Solver/Initialize.jl:61-84), faces between two solid cells are
+  dropped;
+* calc_intersect (Circle.jl:33-52): the intersection point and normal of each donor/solid
+  face;
+* the image stencil of each face (image_df, Immersed_boundary.jl:367-377) and the fluid
+  neighbours of each solid cell (update_solid_cell!, :142-146): the first neighbour of each
+  face id, in face-id order, when it is a fluid cell;
+* the cut velocity cells (initialize_cutted_velocity_cell, :224-250, with cut_rect,
+  src/Velocity_space/Cut_cell.jl:13-75): the velocity cells that the line u.n = 0 splits,
+  with the areas of their two parts, computed here by clipping the cell square with the
+  half plane u.n < 0 (the gas side; the solid side u.n > 0 is the rest of the square).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+EPS = 1e-12           # src/Abstract/Types.jl:3
+NMT2 = np.array([[-1, 0], [1, 0], [0, -1], [0, 1]], dtype=np.float64)   # Types.jl:12
+
+
+def solid_flag(mid, center, radius, solid=True):
+    """Midpoints inside the solid: xor(|x - c| > r, solid) (Circle.jl:17-19)."""
+    out = np.linalg.norm(mid - np.asarray(center)[None, :], axis=1) > radius
+    return np.logical_xor(out, solid)
+
+
+def boundary_flag(mid, ds, center, radius):
+    """A neighbour point x + ds*NMT[i] lies on the other side of the circle (Circle.jl:1-8)."""
+    c = np.asarray(center)
+    flag = np.zeros(len(mid), dtype=np.int64)
+    for i in range(4):
+        p = mid + ds * NMT2[i][None, :]
+        flag += np.where(np.linalg.norm(p - c[None, :], axis=1) > radius, 1, -1)
+    return np.abs(flag) != 4
+
+
+def calc_intersect(f_mid, s_mid, center, radius, solid=True):
+    """Intersection point and unit normal of the donor/solid face (Circle.jl:33-52). As in
+    the reference, the point is measured from the circle's centre (it is the geometric
+    point when the centre is the origin) and the normal points out of the solid."""
+    c, r = center, radius
+    sgn = r if solid else -r
+    if abs(f_mid[0] - s_mid[0]) < EPS:
+        t = math.acos((f_mid[0] - c[0]) / r)
+        if f_mid[1] > c[1]:
+            ap = [r * math.cos(t), r * math.sin(t)]
+        else:
+            ap = [r * math.cos(t), -r * math.sin(t)]
+    else:
+        t = math.asin((f_mid[1] - c[1]) / r)
+        if f_mid[0] > c[0]:
+            ap = [r * math.cos(t), r * math.sin(t)]
+        else:
+            ap = [-r * math.cos(t), r * math.sin(t)]
+    ap = np.array(ap)
+    return ap, ap / sgn
+
+
+def _clip_area(x0, x1, y0, y1, n):
+    """Area of the rectangle [x0,x1]x[y0,y1] on the side u.n < 0 (Sutherland-Hodgman with
+    one half plane, then the shoelace formula)."""
+    poly = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+    out = []
+    for k in range(4):
+        p, q = poly[k], poly[(k + 1) % 4]
+        sp = p[0] * n[0] + p[1] * n[1]
+        sq = q[0] * n[0] + q[1] * n[1]
+        if sp < 0:
+            out.append(p)
+        if (sp < 0) != (sq < 0) and sp != sq:
+            t = sp / (sp - sq)
+            out.append((p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])))
+    if len(out) < 3:
+        return 0.0
+    a = 0.0
+    for k in range(len(out)):
+        p, q = out[k], out[(k + 1) % len(out)]
+        a += p[0] * q[1] - q[0] * p[1]
+    return 0.5 * abs(a)
+
+
+def cut_cells_2d(normal, vs_mid, vs_level, du):
+    """Cut velocity cells of a 2D grid for the line u.n = 0: (index, gas area, solid area),
+    index ascending. No cut cells when n is within 1e-6 of an axis (the line then runs along
+    cell edges, Immersed_boundary.jl:225)."""
+    n = np.asarray(normal, dtype=np.float64)
+    idx, gas, sol = [], [], []
+    if np.any(np.abs(n) < 1e-6):
+        return np.zeros(0, np.int32), np.zeros(0), np.zeros(0)
+    for i in range(len(vs_level)):
+        ddu = np.asarray(du) / 2.0 ** int(vs_level[i])
+        m = vs_mid[i]
+        if abs(m[0] * n[0] + m[1] * n[1]) > 0.5 * math.sqrt(ddu[0] * ddu[0] + ddu[1] * ddu[1]):
+            continue
+        x0, x1 = m[0] - 0.5 * ddu[0], m[0] + 0.5 * ddu[0]
+        y0, y1 = m[1] - 0.5 * ddu[1], m[1] + 0.5 * ddu[1]
+        full = (x1 - x0) * (y1 - y0)
+        g = _clip_area(x0, x1, y0, y1, n)
+        if g <= 0.0 or g >= full:
+            continue
+        idx.append(i)
+        gas.append(g)
+        sol.append(full - g)
+    return np.asarray(idx, np.int32), np.asarray(gas), np.asarray(sol)
+
+
+def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 0.0, 1.0)):
+    """The topology with a circular immersed boundary: bound_enc, the face list and the IB
+    arrays of kah_topology. Needs a shared velocity grid, equal-level cells on both sides of
+    every solid/fluid face and no solid cell on the domain boundary or a periodic image."""
+    dim = int(topo["dim"])
+    if dim != 2:
+        raise ValueError("immerse_circle: 2D meshes only")
+    if not int(topo.get("vs_uniform", 0)):
+        raise ValueError("immerse_circle: the immersed boundary needs a shared velocity grid")
+    n = int(topo["n_cells"])
+    mid = np.asarray(topo["cell_mid"]).reshape(-1, dim)[:n]
+    ds = np.asarray(topo["cell_ds"]).reshape(-1, dim)[:n]
+    level = np.asarray(topo["ps_level"])
+    img = np.asarray(topo.get("image_src", np.zeros(0, np.int64)))
+    inside = solid_flag(mid, center, radius, solid)
+    ghost = inside & boundary_flag(mid, ds, center, radius)
+    benc = np.zeros(n, dtype=np.int32)
+    benc[inside] = -1
+
+    # faces: the fluid side owns a solid/fluid face; solid/solid faces are dropped
+    keys = ("pair_here", "pair_there", "pair_dir", "pair_rot", "pair_area", "pair_bc")
+    P = {k: np.array(topo[k], copy=True) for k in keys}
+    pmid = np.asarray(topo["pair_mid"]).reshape(-1, dim).copy()
+    keep = np.ones(len(P["pair_here"]), dtype=bool)
+    for p in range(len(keep)):
+        h, t = int(P["pair_here"][p]), int(P["pair_there"][p])
+        if t >= n:
+            if inside[h] or inside[int(img[t - n])]:
+                raise ValueError("immerse_circle: a solid cell touches a periodic image")
+            continue
+        if t < 0:
+            if inside[h]:
+                raise ValueError("immerse_circle: a solid cell lies on the domain boundary")
+            continue
+        if inside[h] and inside[t]:
+            keep[p] = False
+            continue
+        if inside[h] or inside[t]:
+            s, f = (h, t) if inside[h] else (t, h)
+            if not ghost[s]:
+                raise ValueError(f"immerse_circle: fluid cell {f} touches cell {s} inside "
+                                 "the solid")
+            if level[s] != level[f]:
+                raise ValueError("immerse_circle: solid/fluid faces need equal cell levels")
+            if inside[h]:
+                P["pair_here"][p], P["pair_there"][p] = t, h
+                P["pair_rot"][p] = -P["pair_rot"][p]
+                P["pair_area"][p] = -P["pair_area"][p]
+    out = dict(topo)
+    for k in keys:
+        out[k] = P[k][keep]
+    out["pair_mid"] = pmid[keep].reshape(-1)
+    out["n_pairs"] = int(keep.sum())
+
+    nbs = np.asarray(topo["nb_state"]).reshape(n, 2 * dim)
+    nboff = np.asarray(topo["nb_offset"])
+    nbc = np.asarray(topo["nb_cell"])
+
+    def first_nb(c, f):
+        if nbs[c, f] == 0:
+            return -1
+        return int(nbc[nboff[c * 2 * dim + f]])
+
+    def fluid_nbs(c):
+        lst = []
+        for f in range(2 * dim):
+            x = first_nb(c, f)
+            if x >= 0 and not inside[x]:
+                lst.append(x)
+        return lst
+
+    # donors and their IB faces (initialize_solid_neighbor!, face-id order)
+    pair_of = {}
+    for p in range(out["n_pairs"]):
+        pair_of[(int(out["pair_here"][p]), int(out["pair_there"][p]))] = p
+    quad = np.asarray(topo["vs_lo"], dtype=np.float64)[:dim], np.asarray(topo["vs_hi"])[:dim]
+    du = (np.asarray(quad[1], dtype=np.float64) - quad[0]) / np.asarray(topo["vs_trees_num"][:dim])
+    vlev = np.asarray(topo["vs_level"])
+    vmid = np.asarray(topo["vs_mid"]).reshape(dim, -1).T
+    faces = dict(pair=[], aux=[], normal=[], img_off=[0], img=[], cut_off=[0], cut_idx=[],
+                 cut_gas=[], cut_sol=[])
+    for c in range(n):
+        if inside[c]:
+            continue
+        for f in range(2 * dim):
+            s = first_nb(c, f)
+            if s < 0 or not inside[s]:
+                continue
+            benc[c] = 1
+            if (c, s) not in pair_of:
+                raise ValueError(f"immerse_circle: no face between donor {c} and solid {s}")
+            faces["pair"].append(pair_of[(c, s)])
+            ap, nrm = calc_intersect(mid[c], mid[s], center, radius, solid)
+            faces["aux"].append(ap)
+            faces["normal"].append(nrm)
+            st = fluid_nbs(c) + [c]
+            faces["img"].extend(st)
+            faces["img_off"].append(len(faces["img"]))
+            ci, cg, cs = cut_cells_2d(nrm, vmid, vlev, du)
+            faces["cut_idx"].extend(ci.tolist())
+            faces["cut_gas"].extend(cg.tolist())
+            faces["cut_sol"].extend(cs.tolist())
+            faces["cut_off"].append(len(faces["cut_idx"]))
+    solid_cells = np.flatnonzero(ghost)
+    snb_off, snb = [0], []
+    for s in solid_cells:
+        lst = fluid_nbs(int(s))
+        if not lst:
+            raise ValueError(f"immerse_circle: solid cell {s} has no fluid face neighbour")
+        snb.extend(lst)
+        snb_off.append(len(snb))
+    out["bound_enc"] = benc
+    out["ib_solid"] = solid_cells.astype(np.int64)
+    out["ib_solid_nb_off"] = np.asarray(snb_off, np.int64)
+    out["ib_solid_nb"] = np.asarray(snb, np.int64)
+    out["ib_face_pair"] = np.asarray(faces["pair"], np.int64)
+    nf = len(faces["pair"])
+    out["ib_aux_point"] = np.asarray(faces["aux"], np.float64).reshape(-1)
+    out["ib_normal"] = np.asarray(faces["normal"], np.float64).reshape(-1)
+    out["ib_bc"] = np.tile(np.asarray(bc, np.float64), nf)
+    out["ib_image_off"] = np.asarray(faces["img_off"], np.int64)
+    out["ib_image_cell"] = np.asarray(faces["img"], np.int64)
+    out["ib_cut_off"] = np.asarray(faces["cut_off"], np.int64)
+    out["ib_cut_index"] = np.asarray(faces["cut_idx"], np.int32)
+    out["ib_cut_gas"] = np.asarray(faces["cut_gas"], np.float64)
+    out["ib_cut_solid"] = np.asarray(faces["cut_sol"], np.float64)
+    out["ib_inside"] = inside
+    return out

@@ -75,6 +75,13 @@ struct orc_ctx {
   int32_t vs_tn[3] = {1, 1, 1};
   double vs_lo[3] = {0, 0, 0}, vs_hi[3] = {0, 0, 0};
   std::vector<std::vector<int64_t>> cell_pairs;   // per cell: pair*2 + side, in pair order
+  // immersed boundary (kah_topology's ib_* arrays) and the SolidNeighbor distributions
+  std::vector<int64_t> ib_solid, ib_snb_off, ib_snb, ib_pair, ib_img_off, ib_img, ib_cut_off;
+  std::vector<double> ib_aux, ib_nrm, ib_bc, ib_cut_gas, ib_cut_sol;
+  std::vector<int32_t> ib_cut_idx;
+  std::vector<int64_t> ib_of_pair;                 // IB face of each pair, or -1
+  std::vector<double> ib_df;                       // [face][k][i], the SolidNeighbor df
+  bool ib_done = false;
   // state
   std::vector<double> w, prim, qf, sw, flux;
   std::vector<double> df, sdf, vflux;
@@ -708,13 +715,16 @@ void reconstruct(orc_ctx& o, SideMicro& s, const double* xc, const double* dsc,
     }
   }
 }
-void plain_micro(orc_ctx& o, SideMicro& s, int dir) {   // df .* vn
+// df .* vn; fsrc: the df of a SolidNeighbor ([k][i] on the cell's grid) instead of the cell's
+void plain_micro(orc_ctx& o, SideMicro& s, int dir, const double* fsrc = nullptr) {
   int64_t m = int64_t(s.rows.size());
+  int64_t n = o.nv(s.cell);
   s.micro.assign(size_t(m * o.ndf), 0.0);
   for (int64_t j = 0; j < m; ++j) {
     int64_t i = s.rows[j];
     double vn = o.U(s.cell, i, dir);
-    for (int k = 0; k < o.ndf; ++k) s.micro[k * m + j] = o.F(s.cell, i, k) * vn;
+    for (int k = 0; k < o.ndf; ++k)
+      s.micro[k * m + j] = (fsrc ? fsrc[k * n + i] : o.F(s.cell, i, k)) * vn;
   }
 }
 
@@ -956,7 +966,10 @@ int inner_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
     if (rot * o.U(t, j, dir) > 0.0) ts.rows.push_back(j);
   if (there_solid) {
     reconstruct(o, hs, o.rmid(h), o.rds(h), fm, dir, dt, false);
-    if (o.flux_kind == KAH_FLUX_CAIDVM)
+    const int64_t k = o.ib_of_pair.empty() ? -1 : o.ib_of_pair[p];
+    if (k >= 0)   // IB face: the there side is the SolidNeighbor (calc_flux, CAIDVM.jl:110-112)
+      plain_micro(o, ts, dir, &o.ib_df[size_t(k) * o.ndf * nt]);
+    else if (o.flux_kind == KAH_FLUX_CAIDVM)
       plain_micro(o, ts, dir);
     else
       reconstruct(o, ts, o.rmid(trec), o.rds(trec), fm, dir, dt, false);
@@ -996,6 +1009,159 @@ void apply_face(orc_ctx& o, const FaceWork& W, bool do_h, bool do_t) {
       for (int q = 0; q < NW; ++q) o.flux[W.t * NW + q] -= W.fw[q];
   }
   update_micro_flux(o, W.hs, W.ts, W.area, W.h, W.t, W.there_solid, W.heavi, do_h, do_t);
+}
+
+
+// ---------------------------------------------------------------- immersed boundary
+// Julia's norm / dot of short vectors: plain sums of products, in index order
+double vnorm(int dim, const double* x) {
+  double a = x[0] * x[0];
+  for (int t = 1; t < dim; ++t) a += x[t] * x[t];
+  return std::sqrt(a);
+}
+double vdot(int dim, const double* x, const double* y) {
+  double a = x[0] * y[0];
+  for (int t = 1; t < dim; ++t) a += x[t] * y[t];
+  return a;
+}
+
+// Direction-weighted extrapolation of the listed fluid cells' f to the point x, at velocity
+// cell i of the shared grid: sum_j wn_j (f_j + sdf_j . (x - x_j)), wn_j = w_j / sum w,
+// w_j = max(0, u.l_j/|u|)^2 with l_j = (x - x_j)/|x - x_j| (1/N when sum w = 0).
+// update_solid_cell! / image_df with vs_extrapolate! (src/Boundary/Immersed_boundary.jl:
+// 24-47, 122-141, 384-395).
+void extrapolate(orc_ctx& o, const int64_t* cells, int64_t N, const double* x, int64_t i,
+                 double* out) {
+  const int D = o.dim, NDF = o.ndf;
+  double u[3] = {0.0, 0.0, 0.0}, w[8], ws = 0.0;
+  for (int t = 0; t < D; ++t) u[t] = o.U(cells[0], i, t);
+  const double nu = vnorm(D, u);
+  for (int64_t j = 0; j < N; ++j) {
+    double l[3] = {0.0, 0.0, 0.0};
+    const double* xj = o.rmid(cells[j]);
+    for (int t = 0; t < D; ++t) l[t] = x[t] - xj[t];
+    const double nl = vnorm(D, l);
+    for (int t = 0; t < D; ++t) l[t] = l[t] / nl;
+    const double r = vdot(D, u, l) / nu;
+    const double c = (r != r) ? r : (r > 0.0 ? r : 0.0);   // max(0., r)
+    w[j] = c * c;
+    ws = (j == 0) ? w[j] : ws + w[j];
+  }
+  for (int k = 0; k < NDF; ++k) out[k] = 0.0;
+  for (int64_t j = 0; j < N; ++j) {
+    const double wn = ws == 0.0 ? 1.0 / double(N) : w[j] / ws;
+    double dx[3];
+    const double* xj = o.rmid(cells[j]);
+    for (int t = 0; t < D; ++t) dx[t] = x[t] - xj[t];
+    for (int k = 0; k < NDF; ++k) {
+      double ddf = o.S(cells[j], i, k, 0) * dx[0];
+      for (int t = 1; t < D; ++t) ddf += o.S(cells[j], i, k, t) * dx[t];
+      out[k] += (o.F(cells[j], i, k) + ddf) * wn;
+    }
+  }
+}
+
+// update_solid_cell! (src/Boundary/Immersed_boundary.jl:122-148): f of each solid cell from
+// its fluid face neighbours, then w and prim
+void ib_solid_cells(orc_ctx& o) {
+  const int NDF = o.ndf, NW = o.NW();
+  const int64_t ns = int64_t(o.ib_solid.size());
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t q = 0; q < ns; ++q) {
+    const int64_t s = o.ib_solid[q];
+    const int64_t* nb = &o.ib_snb[o.ib_snb_off[q]];
+    const int64_t N = o.ib_snb_off[q + 1] - o.ib_snb_off[q];
+    const int64_t n = o.nv(s);
+    for (int64_t i = 0; i < n; ++i) {
+      double f[2];
+      extrapolate(o, nb, N, o.rmid(s), i, f);
+      for (int k = 0; k < NDF; ++k) o.F(s, i, k) = f[k];
+    }
+    double wv[5];
+    moments_full(o, s, &o.df[o.off(s) * NDF], n, wv);
+    for (int m = 0; m < NW; ++m) o.w[s * NW + m] = wv[m];
+    get_prim(o.dim, &o.w[s * NW], o.gamma, &o.prim[s * NW]);
+  }
+}
+
+// update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:436-483) with image_df (:367-395),
+// boundary_slope! (:396-435), the positivity scaling of the boundary slope, the cut-cell
+// corrections cvc_gas_correction! / cvc_Mu / cvc_density / cvc_correction! (:332-366) and
+// the wall Maxwellian of zero net mass flux: the df of each SolidNeighbor.
+void ib_solid_neighbors(orc_ctx& o) {
+  const int D = o.dim, NDF = o.ndf;
+  const int64_t nf = int64_t(o.ib_pair.size());
+#pragma omp parallel for schedule(dynamic, 2)
+  for (int64_t k = 0; k < nf; ++k) {
+    const int64_t p = o.ib_pair[k];
+    const int64_t c = o.ph[p], s = o.src(o.pt[p]);
+    const int dir = o.pd[p];
+    const int64_t n = o.nv(c);
+    const double* ap = &o.ib_aux[k * D];
+    const double* nr = &o.ib_nrm[k * D];
+    const double* xc = o.rmid(c);
+    const double* xs = o.rmid(s);
+    double ip[3];
+    for (int t = 0; t < D; ++t) ip[t] = ap[t] + xc[t] - xs[t];
+    const int64_t* img = &o.ib_img[o.ib_img_off[k]];
+    const int64_t N = o.ib_img_off[k + 1] - o.ib_img_off[k];
+    const double dxs = xc[dir] - xs[dir];
+    const double dxf = ip[dir] - xc[dir];
+    const double dxL = ap[dir] - ip[dir];
+    double* aux = &o.ib_df[size_t(k) * NDF * n];
+    std::vector<double> vn(static_cast<size_t>(n)), M(static_cast<size_t>(n * NDF));
+    std::vector<char> th(static_cast<size_t>(n));
+    double pw[5];
+    for (int m = 0; m < D + 2; ++m) pw[m] = o.ib_bc[k * (D + 2) + m];
+    pw[0] = 1.0;
+    for (int64_t i = 0; i < n; ++i) {
+      double img_f[2];
+      extrapolate(o, img, N, ip, i, img_f);
+      double u[3];
+      for (int t = 0; t < D; ++t) u[t] = o.U(c, i, t);
+      for (int q = 0; q < NDF; ++q) {
+        const double fc = o.F(c, i, q);
+        const double sL = (fc - o.F(s, i, q)) / dxs;
+        double sl = minmod(sL, (img_f[q] - fc) / dxf);
+        sl = jl_min(std::fabs((img_f[q] - MACH_EPS) / (sl * dxL + MACH_EPS)), 1.0) * sl;
+        aux[q * n + i] = img_f[q] + sl * dxL;
+      }
+      vn[i] = vdot(D, u, nr);
+      th[i] = vn[i] >= 0.0;
+      double m2[2];
+      maxwell(D, u, pw, o.K, m2);
+      for (int q = 0; q < NDF; ++q) M[q * n + i] = m2[q];
+    }
+    const int64_t c0 = o.ib_cut_off[k], c1 = o.ib_cut_off[k + 1];
+    std::vector<char> cut(static_cast<size_t>(n), 0);
+    for (int64_t j = c0; j < c1; ++j) cut[o.ib_cut_idx[j]] = 1;
+    Neumaier sf, mur;
+    for (int64_t i = 0; i < n; ++i) {
+      if (cut[i]) continue;   // cvc.weight is zero on the cut cells
+      const double wv = o.Wt(c, i) * vn[i];
+      if (th[i]) mur.add(wv * M[i]);
+      else sf.add(wv * aux[i]);
+    }
+    double SF = sf.get(), MuR = mur.get();
+    std::vector<double> gas(static_cast<size_t>((c1 - c0) * NDF));
+    for (int64_t j = c0; j < c1; ++j) {
+      const int64_t i = o.ib_cut_idx[j];
+      MuR += o.ib_cut_sol[j] * vn[i] * M[i];
+      SF += o.ib_cut_gas[j] * vn[i] * aux[i];
+      for (int q = 0; q < NDF; ++q) gas[(j - c0) * NDF + q] = aux[q * n + i];
+    }
+    const double rho = -SF / MuR;
+    for (int64_t i = 0; i < n * NDF; ++i) M[i] *= rho;
+    for (int64_t i = 0; i < n; ++i)
+      if (th[i])
+        for (int q = 0; q < NDF; ++q) aux[q * n + i] = M[q * n + i];
+    for (int64_t j = c0; j < c1; ++j) {
+      const int64_t i = o.ib_cut_idx[j];
+      const double g = o.ib_cut_gas[j], sw = o.ib_cut_sol[j];
+      for (int q = 0; q < NDF; ++q)
+        aux[q * n + i] = (g * gas[(j - c0) * NDF + q] + sw * M[q * n + i]) / (g + sw);
+    }
+  }
 }
 
 }  // namespace
@@ -1053,6 +1219,26 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
     o->vs_tn[d] = T->vs_trees_num[d];
     o->vs_lo[d] = T->vs_lo[d];
     o->vs_hi[d] = T->vs_hi[d];
+  }
+  if (T->n_ib_faces > 0 || T->n_ib_solid > 0) {
+    if (!o->vs_uniform || o->flux_kind != KAH_FLUX_CAIDVM) { delete o; return nullptr; }
+    const int64_t ns = T->n_ib_solid, nf = T->n_ib_faces;
+    copy_in(o->ib_solid, T->ib_solid, ns);
+    copy_in(o->ib_snb_off, T->ib_solid_nb_off, ns + 1);
+    copy_in(o->ib_snb, T->ib_solid_nb, o->ib_snb_off[ns]);
+    copy_in(o->ib_pair, T->ib_face_pair, nf);
+    copy_in(o->ib_aux, T->ib_aux_point, nf * D);
+    copy_in(o->ib_nrm, T->ib_normal, nf * D);
+    copy_in(o->ib_bc, T->ib_bc, nf * (D + 2));
+    copy_in(o->ib_img_off, T->ib_image_off, nf + 1);
+    copy_in(o->ib_img, T->ib_image_cell, o->ib_img_off[nf]);
+    copy_in(o->ib_cut_off, T->ib_cut_off, nf + 1);
+    copy_in(o->ib_cut_idx, T->ib_cut_index, o->ib_cut_off[nf]);
+    copy_in(o->ib_cut_gas, T->ib_cut_gas, o->ib_cut_off[nf]);
+    copy_in(o->ib_cut_sol, T->ib_cut_solid, o->ib_cut_off[nf]);
+    o->ib_of_pair.assign(size_t(npair), -1);
+    for (int64_t k = 0; k < nf; ++k) o->ib_of_pair[o->ib_pair[k]] = k;
+    o->ib_df.assign(size_t(nf * o->ndf * ng), 0.0);
   }
   int NW = D + 2;
   o->w.assign(size_t(o->nc * NW), 0.0);
@@ -1296,6 +1482,13 @@ int orc_vs_flags(orc_ctx* o, const kah_vs_adapt* p, const double* vr, uint8_t* f
   }
   return 0;
 }
+int orc_get_ib(orc_ctx* o, double* aux_df) {
+  if (!o || !aux_df) return KAH_EINVAL;
+  if (!o->ib_done) return KAH_ESTATE;
+  std::copy(o->ib_df.begin(), o->ib_df.end(), aux_df);
+  return 0;
+}
+
 const char* orc_last_error(const orc_ctx* o) { return o ? o->err.c_str() : "null context"; }
 
 int orc_set_threads(int n) {
@@ -1383,12 +1576,19 @@ int orc_slope(orc_ctx* o) {
   return 0;
 }
 
-// flux!(p4est,ka), src/Flux/Flux.jl:458-495 (no immersed boundaries). Faces are computed in
+// flux!(p4est,ka), src/Flux/Flux.jl:458-495. Faces are computed in
 // parallel; each cell then applies its side of its faces in face-list order, which gives
 // every accumulator the same sequence of additions as the reference's serial face loop.
 int orc_flux(orc_ctx* o, double dt) {
   o->err.clear();
   const int64_t npair = int64_t(o->ph.size());
+  // update_solid_cell! and update_solid_neighbor! (src/Flux/Flux.jl:461-482): the solid
+  // cells, then the SolidNeighbor df that the IB faces read
+  if (!o->ib_solid.empty()) ib_solid_cells(*o);
+  if (!o->ib_pair.empty()) {
+    ib_solid_neighbors(*o);
+    o->ib_done = true;
+  }
   if (o->cell_pairs.empty() && o->nc > 0) {
     o->cell_pairs.assign(size_t(o->nc), {});
     for (int64_t p = 0; p < npair; ++p) {

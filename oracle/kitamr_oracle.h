@@ -28,6 +28,7 @@ int orc_set_state(orc_ctx* o, const double* w, const double* prim, const double*
 int orc_get_state(orc_ctx* o, double* w, double* prim, double* qf, double* sw, double* df,
                   double* sdf);
 int orc_get_flux(orc_ctx* o, double* flux, double* vs_flux);
+int orc_get_ib(orc_ctx* o, double* aux_df);   /* SolidNeighbor df [face][k][i], last flux */
 int orc_set_threads(int n);
 int orc_slope(orc_ctx* o);
 int orc_flux(orc_ctx* o, double dt);

@@ -63,6 +63,7 @@ def lib():
         L.orc_vs_flags.argtypes = [C.c_void_p, C.POINTER(abi.KahVsAdapt), dp,
                                    C.POINTER(C.c_uint8)]
         L.orc_set_sdf.argtypes = [C.c_void_p, dp]
+        L.orc_get_ib.argtypes = [C.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -88,6 +89,8 @@ class Oracle:
         self.dim, self.ndf = params.dim, params.ndf
         self.nc = int(topo["n_cells"])
         self.np = int(np.asarray(topo["vs_offset"])[-1])
+        self.n_ib = int(len(topo.get("ib_face_pair", [])))
+        self.n0 = int(np.asarray(topo["vs_offset"])[1]) if self.nc else 0
 
     def close(self):
         if self.h:
@@ -130,6 +133,12 @@ class Oracle:
         self._ck(self.L.orc_vs_flags(self.h, C.byref(params), _p(_f(vr)),
                                      out.ctypes.data_as(C.POINTER(C.c_uint8))), "vs_flags")
         return out
+
+    def get_ib(self):
+        """SolidNeighbor distributions of the last flux, [n_ib_faces, ndf, n0]."""
+        out = np.empty(self.n_ib * self.ndf * self.n0)
+        self._ck(self.L.orc_get_ib(self.h, _p(out)), "get_ib")
+        return out.reshape(self.n_ib, self.ndf, self.n0)
 
     def get_flux(self):
         f = np.empty(self.nc * (self.dim + 2))

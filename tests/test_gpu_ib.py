@@ -1,0 +1,167 @@
+"""Immersed boundary on the device (update_solid_cell!, update_solid_neighbor! and the IB
+faces, src/Boundary/Immersed_boundary.jl and src/Flux/Flux.jl:458-487) against the oracle:
+the split phases and the fused step in FP64, graph replay, the FP32 form, the zero wall mass
+flux of every SolidNeighbor and the rejected configurations.
+
+Cells inside the solid (no faces, never updated) are left out of the state comparisons: the
+device does not carry their f across the two state buffers.
+"""
+import numpy as np
+import pytest
+
+from _util import pair, rel_err, rel_err_cols, rel_err_rows, say
+from kitamr_hip import abi, cases
+from kitamr_hip.solver import kah_params
+
+TOL = 1e-12
+
+CASES = {
+    "cyl16": lambda: cases.cylinder_2d(n=16, nv=12),
+    "cyl32_nv16": lambda: cases.cylinder_2d(n=32, nv=16),
+    "cyl16_amr1": lambda: cases.cylinder_2d(n=16, nv=8, ps_levels=1),
+    "cyl16_euler": lambda: cases.cylinder_2d(n=16, nv=8, marching="Euler"),
+}
+
+
+def live_cells(case):
+    """Fluid cells and the solid cells that update_solid_cell! rebuilds."""
+    benc = case.topo["bound_enc"]
+    keep = benc >= 0
+    keep[case.topo["ib_solid"]] = True
+    return keep
+
+
+def check_state(case, g, o, tol, what):
+    keep = live_cells(case)
+    nw = 4
+    n0 = int(case.topo["vs_offset"][1])
+    gdf = g["df"].reshape(case.n_cells, -1)[keep]
+    odf = o["df"].reshape(case.n_cells, -1)[keep]
+    assert rel_err(gdf, odf) <= tol, f"{what}: df"
+    for k in ("w", "prim"):
+        assert rel_err_rows(g[k].reshape(-1, nw)[keep], o[k].reshape(-1, nw)[keep], nw) <= tol, \
+            f"{what}: {k}"
+    assert n0 > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_split_phases(name):
+    case = CASES[name]()
+    say(f"[ib split] {name}: cells={case.n_cells} faces={len(case.topo['ib_face_pair'])}")
+    dev, orc = pair(case)
+    dt = case.dt_xi
+    for s in range(2):
+        dev.slope(dt)
+        orc.slope()
+        dev.flux(dt)
+        orc.flux(dt)
+        assert rel_err(dev.get_ib(), orc.get_ib()) <= TOL, "SolidNeighbor df"
+        gf, gv = dev.get_flux()
+        of, ov = orc.get_flux()
+        assert rel_err(gv, ov) <= TOL, "vs flux"
+        assert rel_err_cols(gf, of, 4) <= 1e-11, "macro flux"
+        rg = dev.iterate(dt, s)
+        ro = orc.iterate(dt, s)
+        check_state(case, dev.get_state(), orc.get_state(), TOL, f"step {s}")
+        assert rel_err(rg[1], ro[1]) <= 1e-11
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_steps(name):
+    case = CASES[name]()
+    say(f"[ib fused] {name}: cells={case.n_cells} P={case.phase_cells}")
+    dev, orc = pair(case)
+    for s in range(4):
+        dev.step(case.dt_xi, s)
+        orc.step(case.dt_xi, s)
+    check_state(case, dev.get_state(), orc.get_state(), TOL, "4 steps")
+    assert rel_err(dev.get_ib(), orc.get_ib()) <= TOL
+    dev.close()
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_launches():
+    case = CASES["cyl32_nv16"]()
+    say("[ib graphs]")
+    a, _ = pair(case)
+    b, _ = pair(case, options={"graphs": 1})
+    for s in range(5):
+        a.step(case.dt_xi, s)
+        b.step(case.dt_xi, s)
+        ga, gb = a.get_state(), b.get_state()
+        for k in ("df", "w", "prim"):
+            assert np.array_equal(ga[k], gb[k]), (s, k)
+        assert np.array_equal(a.get_ib(), b.get_ib()), s
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16"])
+def test_wall_carries_no_mass(name):
+    case = CASES[name]()
+    say(f"[ib wall] {name}")
+    dev = abi.HipContext(kah_params(case.config))
+    dev.set_topology(case.topo)
+    dev.set_state(case.w, case.prim, case.df)
+    for s in range(3):
+        dev.step(case.dt_xi, s)
+    aux = dev.get_ib()
+    t = case.topo
+    vm = np.asarray(t["vs_mid"]).reshape(2, -1).T
+    wt = np.asarray(t["vs_weight"])
+    nr = t["ib_normal"].reshape(-1, 2)
+    for k in range(len(t["ib_face_pair"])):
+        flux = wt * (vm @ nr[k]) * aux[k, 0]
+        assert abs(flux.sum()) < 1e-13 * np.abs(flux).sum(), k
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16"])
+def test_fp32_form(name):
+    """FP32 form (f stored and advected in FP32) against the FP64 oracle after 3 steps:
+    moments within 1e-5 of each cell's scale, the SolidNeighbor df within 1e-5."""
+    case = CASES[name]()
+    say(f"[ib fp32f] {name}")
+    dev, orc = pair(case, precision=abi.PREC["fp32f"])
+    for s in range(3):
+        dev.step(case.dt_xi, s)
+        orc.step(case.dt_xi, s)
+    g, o = dev.get_state(), orc.get_state()
+    keep = live_cells(case)
+    for k in ("w", "prim"):
+        assert rel_err_rows(g[k].reshape(-1, 4)[keep], o[k].reshape(-1, 4)[keep], 4) <= 1e-5, k
+    assert rel_err(dev.get_ib(), orc.get_ib()) <= 1e-5
+    dev.close()
+
+
+@pytest.mark.gpu
+def test_rejections():
+    say("[ib rejects]")
+    case = CASES["cyl16"]()
+    p = kah_params(case.config)
+    dev = abi.HipContext(p)
+    with pytest.raises(abi.KahError, match="no immersed-boundary faces"):
+        dev.get_ib()
+    t = dict(case.topo)
+    img = np.array(t["ib_image_cell"], copy=True)
+    img[0] = int(t["ib_solid"][0])
+    t["ib_image_cell"] = img
+    with pytest.raises(abi.KahError, match="is not fluid"):
+        dev.set_topology(t)
+    dev.set_topology(case.topo)
+    dev.set_state(case.w, case.prim, case.df)
+    with pytest.raises(abi.KahError, match="no immersed-boundary faces"):
+        dev.get_ib()
+    dev.close()
+    for kw, msg in ((dict(flux="DVM"), "CAIDVM flux"),
+                    (dict(marching="CIP_Marching"), "positivity_preserving_ib")):
+        c = cases.cylinder_2d(n=16, nv=8, **kw)
+        d = abi.HipContext(kah_params(c.config))
+        with pytest.raises(abi.KahError, match=msg):
+            d.set_topology(c.topo)
+        d.close()

@@ -1,0 +1,195 @@
+"""Immersed boundary on the host: the circle geometry (kitamr_hip.ib) and the oracle's
+update_solid_cell! / update_solid_neighbor! restatement (src/Boundary/Immersed_boundary.jl),
+pinned without the reference's outputs:
+
+- cut cells: the two parts fill the velocity cell, a line through the cell's centre at 45
+  degrees halves it, a line through a corner cuts off the triangle of area 1/8 or 7/8, and
+  the solid part is the one on the side u.n > 0 (the side the wall Maxwellian fills);
+- roles and faces: no face joins two solid cells, every solid/fluid face is owned by the
+  fluid cell with its rot and area sign flipped, every donor has a solid face neighbour;
+- calc_intersect: the point lies on the circle, the normal is the unit outward radius;
+- update_solid_cell! / image_df: for f linear in x with exact slopes the direction-weighted
+  extrapolation is exact (weights sum to one; a wrong dx sign or midpoint breaks it), so the
+  solid cell holds f(x_solid) and the gas side of every SolidNeighbor holds f(aux point)
+  (boundary_slope! then equals the exact gradient, and dxL carries f from the image point
+  to the wall);
+- the wall density: after the Maxwellian fill and the cut-cell mix the SolidNeighbor carries
+  no net mass through the wall, sum_i weight_i (u_i.n) f_i = 0, and the filled cells hold
+  rho_w times the unit-density wall Maxwellian.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from kitamr_hip import cases, ib
+from kitamr_hip.solver import discrete_maxwell, kah_params
+
+
+def test_cut_areas_fill_the_cell_and_halve_on_the_diagonal():
+    n = np.array([1.0, 1.0]) / math.sqrt(2.0)
+    mids = np.array([[0.0, 0.0], [0.5, 0.5], [-0.5, 0.5], [3.0, 3.0]])
+    idx, gas, sol = ib.cut_cells_2d(n, mids, np.zeros(4, np.int8), np.array([1.0, 1.0]))
+    # [0.5, 0.5]: the line u.n = 0 passes through its corner (0, 0) only; [3, 3] is far away
+    assert idx.tolist() == [0, 2]
+    assert np.allclose(gas + sol, 1.0, rtol=0, atol=1e-15)
+    assert gas[0] == pytest.approx(0.5, abs=1e-15)       # through the centre
+    assert gas[1] == pytest.approx(0.5, abs=1e-15)       # [-0.5, 0.5]: the diagonal of the cell
+    # a steeper line through the corner (0, 0) of the cell [0, 1] x [0, 1]
+    n2 = np.array([2.0, -1.0]) / math.sqrt(5.0)          # u.n = 0 on y = 2x
+    idx, gas, sol = ib.cut_cells_2d(n2, np.array([[0.5, 0.5]]), np.zeros(1, np.int8),
+                                    np.array([1.0, 1.0]))
+    # gas side 2x - y < 0: the triangle above y = 2x within the square, area 1/4
+    assert idx.tolist() == [0]
+    assert gas[0] == pytest.approx(0.25, abs=1e-15) and sol[0] == pytest.approx(0.75, abs=1e-15)
+
+
+def test_cut_solid_part_is_on_the_normal_side():
+    n = np.array([0.6, 0.8])
+    # a cell straddling the line, shifted toward +n: its solid (u.n > 0) part is the larger
+    m = np.array([[0.1 * 0.6, 0.1 * 0.8]])
+    idx, gas, sol = ib.cut_cells_2d(n, m, np.zeros(1, np.int8), np.array([1.0, 1.0]))
+    assert idx.tolist() == [0] and sol[0] > gas[0]
+    idx, gas, sol = ib.cut_cells_2d(n, -m, np.zeros(1, np.int8), np.array([1.0, 1.0]))
+    assert sol[0] < gas[0]
+    # axis-aligned normals: no cut cells (the line runs along cell edges)
+    idx, _, _ = ib.cut_cells_2d(np.array([0.0, 1.0]), m, np.zeros(1, np.int8), np.ones(2))
+    assert len(idx) == 0
+
+
+def test_cut_areas_of_a_refined_cell_scale_with_its_level():
+    n = np.array([1.0, 2.0]) / math.sqrt(5.0)
+    m = np.array([[0.05, -0.02]])
+    for lev in range(3):
+        idx, g, s = ib.cut_cells_2d(n, m, np.full(1, lev, np.int8), np.array([0.4, 0.4]))
+        assert idx.tolist() == [0]
+        assert g[0] + s[0] == pytest.approx((0.4 / 2 ** lev) ** 2, rel=1e-14)
+
+
+@pytest.mark.parametrize("ps_levels", [0, 1])
+def test_roles_faces_and_intersections(ps_levels):
+    case = cases.cylinder_2d(n=16, nv=8, ps_levels=ps_levels)
+    t = case.topo
+    n = case.n_cells
+    benc = t["bound_enc"]
+    mid = t["cell_mid"].reshape(-1, 2)[:n]
+    inside = np.hypot(mid[:, 0], mid[:, 1]) < 1.0
+    assert np.array_equal(benc < 0, inside)
+    ph, pt = t["pair_here"], t["pair_there"]
+    assert np.all(benc[ph] >= 0), "a pair is owned by a solid cell"
+    real = pt >= 0
+    assert not np.any(inside[ph[real]] & inside[pt[real]])
+    # the faces towards solid cells are exactly the IB faces, one per donor/solid neighbour
+    sf = np.flatnonzero(real & (benc[np.where(real, pt, 0)] < 0))
+    assert sorted(sf.tolist()) == sorted(t["ib_face_pair"].tolist())
+    assert set(ph[sf].tolist()) == set(np.flatnonzero(benc > 0).tolist())
+    # flipped faces keep the geometric face: midpoint between here and there along dir
+    for p in sf:
+        d = int(t["pair_dir"][p])
+        xm = t["pair_mid"].reshape(-1, 2)[p, d]
+        xh, xt = mid[ph[p], d], mid[pt[p], d]
+        assert xm == pytest.approx(0.5 * (xh + xt), abs=1e-14)
+        # rot = +1 on the here cell's low face (the solid lies below), -1 on its high face
+        assert t["pair_rot"][p] == (1.0 if xt < xh else -1.0)
+        assert np.sign(t["pair_area"][p]) == t["pair_rot"][p]
+    ap = t["ib_aux_point"].reshape(-1, 2)
+    nr = t["ib_normal"].reshape(-1, 2)
+    assert np.allclose(np.hypot(ap[:, 0], ap[:, 1]), 1.0, atol=1e-14)
+    assert np.allclose(nr, ap, atol=1e-15)
+    # the image stencil ends with the donor, the solid cells' neighbours are fluid
+    ioff = t["ib_image_off"]
+    for k, p in enumerate(t["ib_face_pair"]):
+        assert t["ib_image_cell"][ioff[k + 1] - 1] == ph[p]
+        assert np.all(benc[t["ib_image_cell"][ioff[k]:ioff[k + 1]]] >= 0)
+    assert np.all(benc[t["ib_solid_nb"]] >= 0)
+    assert np.all(benc[t["ib_solid"]] < 0)
+
+
+def test_geometry_rejects_unsupported_meshes():
+    case = cases.riemann_2d(n=8, nv=8)
+    with pytest.raises(ValueError, match="domain boundary"):
+        ib.immerse_circle(case.topo, (0.0, 0.0), 0.3)
+    case = cases.riemann_2d(n=8, nv=8, vs_levels=1)
+    with pytest.raises(ValueError, match="shared velocity grid"):
+        ib.immerse_circle(case.topo, (0.5, 0.5), 0.2)
+
+
+def _linear_ib_state(case, seed=3):
+    """f(x, u) = a(u) + b(u).x on every cell (a > 0, small b), with the exact slopes b."""
+    rng = np.random.default_rng(seed)
+    t = case.topo
+    n = case.n_cells
+    n0 = int(t["vs_offset"][1])
+    mid = t["cell_mid"].reshape(-1, 2)[:n]
+    a = rng.uniform(0.5, 1.5, (2, n0))
+    g = rng.uniform(-0.02, 0.02, (2, 2, n0))          # [d, k, v]
+    df = a[None] + np.einsum("cd,dkv->ckv", mid, g)
+    sdf = np.broadcast_to(g[None], (n, 2, 2, n0))   # [c][d][k][v]
+    return df, a, g, np.ascontiguousarray(sdf).reshape(-1)
+
+
+def test_oracle_ib_is_exact_on_linear_fields():
+    case = cases.cylinder_2d(n=16, nv=8)
+    t = case.topo
+    n = case.n_cells
+    df, a, g, sdf = _linear_ib_state(case)
+    o = O.Oracle(kah_params(case.config), t)
+    o.set_state(case.w, case.prim, df.reshape(-1))
+    o.set_sdf(sdf)
+    o.flux(case.dt_xi)
+    st = o.get_state()
+    got = st["df"].reshape(n, 2, -1)
+    mid = t["cell_mid"].reshape(-1, 2)[:n]
+    for s in t["ib_solid"]:
+        exact = a + np.einsum("d,dkv->kv", mid[s], g)
+        assert np.max(np.abs(got[s] - exact)) < 1e-14
+    aux = o.get_ib()
+    vm = np.asarray(t["vs_mid"]).reshape(2, -1).T
+    ap = t["ib_aux_point"].reshape(-1, 2)
+    nr = t["ib_normal"].reshape(-1, 2)
+    coff = t["ib_cut_off"]
+    checked = 0
+    for k in range(len(t["ib_face_pair"])):
+        vn = vm @ nr[k]
+        cut = np.zeros(len(vn), bool)
+        cut[t["ib_cut_index"][coff[k]:coff[k + 1]]] = True
+        gas = (vn < 0) & ~cut
+        exact = a + np.einsum("d,dkv->kv", ap[k], g)
+        assert np.max(np.abs(aux[k][:, gas] - exact[:, gas])) < 1e-13
+        checked += gas.sum()
+    assert checked > 0
+
+
+def test_oracle_wall_carries_no_mass_and_fills_a_maxwellian():
+    case = cases.cylinder_2d(n=16, nv=12)
+    t = case.topo
+    o = O.Oracle(kah_params(case.config), t)
+    o.set_state(case.w, case.prim, case.df)
+    for s in range(3):
+        o.slope()
+        o.flux(case.dt_xi)
+        o.iterate(case.dt_xi, s)
+    aux = o.get_ib()
+    vm = np.asarray(t["vs_mid"]).reshape(2, -1).T
+    wt = np.asarray(t["vs_weight"])
+    nr = t["ib_normal"].reshape(-1, 2)
+    coff = t["ib_cut_off"]
+    bc = t["ib_bc"].reshape(-1, 4)
+    ncut = 0
+    for k in range(len(t["ib_face_pair"])):
+        vn = vm @ nr[k]
+        flux = wt * vn * aux[k, 0]
+        assert abs(flux.sum()) < 1e-13 * np.abs(flux).sum()
+        cut = np.zeros(len(vn), bool)
+        cut[t["ib_cut_index"][coff[k]:coff[k + 1]]] = True
+        ncut += cut.sum()
+        fill = (vn >= 0) & ~cut
+        pw = bc[k].copy()
+        pw[0] = 1.0
+        M = discrete_maxwell(vm, pw, case.config.gas.K)     # [k, v]
+        ratio = aux[k][:, fill] / M[:, fill]
+        # one density rho_w per face (negative where the extrapolated gas side carries mass
+        # toward the wall out of an expansion, as the reference's formula allows)
+        assert np.ptp(ratio) < 1e-12 * np.abs(ratio).max()
+    assert ncut > 0
