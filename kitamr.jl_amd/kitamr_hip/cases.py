@@ -10,6 +10,7 @@ periodic_box_2d  the Riemann initial state (or a uniform state) with periodic bo
               all sides, where mass, momentum and energy are conserved exactly.
 cylinder_2d   flow past a circular immersed boundary (example/cylinder/cylinder.jl):
               supersonic inflow at xmin, outflow elsewhere, a Maxwellian wall on the circle.
+sphere_3d     the 3D1F analogue with a spherical immersed boundary (example/sphere/sphere.jl).
 """
 from __future__ import annotations
 
@@ -185,4 +186,24 @@ def cylinder_2d(n=32, nv=16, vmax=10.0, L=4.0, radius=1.0, Ma=1.5, Kn=0.1, omega
             return np.abs(np.hypot(mid[:, 0], mid[:, 1]) - radius) < band
     case = build_case(cfg, ps_refine=refine)
     case.topo = immerse_circle(case.topo, (0.0, 0.0), radius, True, wall)
+    return case
+
+
+def sphere_3d(n=12, nv=8, vmax=7.28, L=3.0, radius=1.0, Ma=1.5, Kn=0.05, omega=0.81,
+              flux="CAIDVM", marching="CAIDVM_Marching", precision="fp64", CFL=0.5):
+    """Sphere(Maxwellian, [0,0,0], radius, true, ., [1,0,0,0,1]) in [-L, L]^3 with an inflow
+    of Mach Ma (U = Ma*sqrt(5/6) at T = 1) from xmin, outflow elsewhere, on a shared grid."""
+    from .ib import immerse_sphere
+    solver = Solver(DIM=3, NDF=1, AMR_PS_MAXLEVEL=0, AMR_VS_MAXLEVEL=0, flux=flux,
+                    time_marching=marching, CFL=CFL)
+    gas = Gas(K=0.0, Kn=Kn, omega=omega, omega_ref=omega)
+    U = Ma * math.sqrt(5.0 / 6.0)
+    free = [1.0, U, 0.0, 0.0, 1.0]
+    dom = [Domain("SuperSonicInflow", 1, free)] + [Domain("UniformOutflow", i)
+                                                  for i in range(2, 7)]
+    cfg = Configure(solver, gas, [-L, L] * 3, [n, n, n], [-vmax, vmax] * 3, [nv] * 3,
+                    Uniform(free), dom, precision=precision)
+    case = build_case(cfg)
+    case.topo = immerse_sphere(case.topo, (0.0, 0.0, 0.0), radius, True,
+                               (1.0, 0.0, 0.0, 0.0, 1.0))
     return case

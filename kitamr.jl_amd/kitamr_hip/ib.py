@@ -1,8 +1,9 @@
-"""Immersed-boundary geometry on the host: a circle in a 2D mesh.
+"""Immersed-boundary geometry on the host: a circle in a 2D mesh or a sphere in a 3D mesh.
 
 The geometry is computed once per mesh and handed to the library with the topology; the
 per-step work (update_solid_cell!, update_solid_neighbor! and the IB face fluxes) runs on the
-device. This module restates, for ``Circle`` (src/Boundary/Circle.jl, Types.jl:83-113):
+device. This module restates, for ``Circle`` and ``Sphere`` (src/Boundary/Circle.jl,
+Types.jl:83-130):
 
 * the cell roles: solid_flag / boundary_flag (Circle.jl:1-23): a cell whose midpoint is in
   the solid is a solid cell (bound_enc < 0) when one of its four neighbour points crosses the
@@ -17,10 +18,13 @@ device. This module restates, for ``Circle`` (src/Boundary/Circle.jl, Types.jl:8
 * the image stencil of each face (image_df, Immersed_boundary.jl:367-377) and the fluid
   neighbours of each solid cell (update_solid_cell!, :142-146): the first neighbour of each
   face id, in face-id order, when it is a fluid cell;
-* the cut velocity cells (initialize_cutted_velocity_cell, :224-250, with cut_rect,
-  src/Velocity_space/Cut_cell.jl:13-75): the velocity cells that the line u.n = 0 splits,
-  with the areas of their two parts, computed here by clipping the cell square with the
-  half plane u.n < 0 (the gas side; the solid side u.n > 0 is the rest of the square).
+* the cut velocity cells (initialize_cutted_velocity_cell, :224-275, with cut_rect and
+  cut_cube, src/Velocity_space/Cut_cell.jl:13-200): the velocity cells that the line or plane
+  u.n = 0 splits, with the sizes of their two parts. Here the gas part (u.n < 0) is measured
+  by clipping: in 2D the square against the half plane; in 3D each face of the cube against
+  the half space, with the volume from the divergence theorem, V = (1/3) sum over faces of
+  (x.N) times the clipped face area (the cut face lies on a plane through the origin and adds
+  nothing). The solid part (u.n > 0, the side the wall Maxwellian fills) is the rest.
 """
 from __future__ import annotations
 
@@ -29,7 +33,9 @@ import math
 import numpy as np
 
 EPS = 1e-12           # src/Abstract/Types.jl:3
-NMT2 = np.array([[-1, 0], [1, 0], [0, -1], [0, 1]], dtype=np.float64)   # Types.jl:12
+NMT = {2: np.array([[-1, 0], [1, 0], [0, -1], [0, 1]], dtype=np.float64),   # Types.jl:12
+       3: np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]],
+                   dtype=np.float64)}
 
 
 def solid_flag(mid, center, radius, solid=True):
@@ -39,13 +45,15 @@ def solid_flag(mid, center, radius, solid=True):
 
 
 def boundary_flag(mid, ds, center, radius):
-    """A neighbour point x + ds*NMT[i] lies on the other side of the circle (Circle.jl:1-8)."""
+    """A neighbour point x + ds*NMT[i] lies on the other side of the circle or sphere
+    (Circle.jl:1-16)."""
     c = np.asarray(center)
+    nm = NMT[mid.shape[1]]
     flag = np.zeros(len(mid), dtype=np.int64)
-    for i in range(4):
-        p = mid + ds * NMT2[i][None, :]
+    for i in range(len(nm)):
+        p = mid + ds * nm[i][None, :]
         flag += np.where(np.linalg.norm(p - c[None, :], axis=1) > radius, 1, -1)
-    return np.abs(flag) != 4
+    return np.abs(flag) != len(nm)
 
 
 def calc_intersect(f_mid, s_mid, center, radius, solid=True):
@@ -70,15 +78,35 @@ def calc_intersect(f_mid, s_mid, center, radius, solid=True):
     return ap, ap / sgn
 
 
-def _clip_area(x0, x1, y0, y1, n):
-    """Area of the rectangle [x0,x1]x[y0,y1] on the side u.n < 0 (Sutherland-Hodgman with
+def calc_intersect_sphere(f_mid, s_mid, center, radius, solid=True):
+    """Intersection point and normal of the donor/solid face of a Sphere (Circle.jl:54-65):
+    the point moves along the face's direction onto the sphere; as in the reference the
+    normal takes the donor's midpoint components across the face direction (the unit outward
+    radius when the centre is the origin)."""
+    c, r = np.asarray(center, dtype=np.float64), radius
+    f = np.asarray(f_mid, dtype=np.float64)
+    s = np.asarray(s_mid, dtype=np.float64)
+    d = int(np.flatnonzero(np.abs(f - s) > EPS)[0])
+    rm = f - c
+    x, y = rm[(d + 1) % 3], rm[(d + 2) % 3]
+    rz = math.sqrt(r * r - x * x - y * y)
+    dz = rz - rm[d] if abs((rz - rm[d]) / (s[d] - f[d])) < 1 else -rz - rm[d]
+    ap = f.copy()
+    nrm = f.copy()
+    ap[d] = dz + f[d]
+    nrm[d] = dz + rm[d]
+    return ap, nrm / (r if solid else -r)
+
+
+def _clip_area(x0, x1, y0, y1, n, b=0.0):
+    """Area of the rectangle [x0,x1]x[y0,y1] on the side n.y < b (Sutherland-Hodgman with
     one half plane, then the shoelace formula)."""
     poly = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
     out = []
     for k in range(4):
         p, q = poly[k], poly[(k + 1) % 4]
-        sp = p[0] * n[0] + p[1] * n[1]
-        sq = q[0] * n[0] + q[1] * n[1]
+        sp = p[0] * n[0] + p[1] * n[1] - b
+        sq = q[0] * n[0] + q[1] * n[1] - b
         if sp < 0:
             out.append(p)
         if (sp < 0) != (sq < 0) and sp != sq:
@@ -118,15 +146,53 @@ def cut_cells_2d(normal, vs_mid, vs_level, du):
     return np.asarray(idx, np.int32), np.asarray(gas), np.asarray(sol)
 
 
-def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 0.0, 1.0)):
-    """The topology with a circular immersed boundary: bound_enc, the face list and the IB
-    arrays of kah_topology. Needs a shared velocity grid, equal-level cells on both sides of
-    every solid/fluid face and no solid cell on the domain boundary or a periodic image."""
+def _gas_volume(lo, hi, n):
+    """Volume of the box [lo, hi] on the side u.n < 0: (1/3) sum over the six faces of
+    (x.N) times the face area on that side (the cut face lies on u.n = 0 and adds nothing)."""
+    v = 0.0
+    for i in range(3):
+        j, k = (i + 1) % 3, (i + 2) % 3
+        for c, sgn in ((lo[i], -1.0), (hi[i], 1.0)):
+            a = _clip_area(lo[j], hi[j], lo[k], hi[k], (n[j], n[k]), -n[i] * c)
+            v += sgn * c * a
+    return v / 3.0
+
+
+def cut_cells_3d(normal, vs_mid, vs_level, du):
+    """Cut velocity cells of a 3D grid for the plane u.n = 0: (index, gas volume, solid
+    volume), index ascending. No cut cells when two components of n are within 1e-6 of zero
+    (Immersed_boundary.jl:252-254)."""
+    n = np.asarray(normal, dtype=np.float64)
+    idx, gas, sol = [], [], []
+    if np.count_nonzero(np.abs(n) < 1e-6) > 1:
+        return np.zeros(0, np.int32), np.zeros(0), np.zeros(0)
+    for i in range(len(vs_level)):
+        ddu = np.asarray(du) / 2.0 ** int(vs_level[i])
+        m = vs_mid[i]
+        if abs(m[0] * n[0] + m[1] * n[1] + m[2] * n[2]) > \
+                0.5 * math.sqrt(ddu[0] * ddu[0] + ddu[1] * ddu[1] + ddu[2] * ddu[2]):
+            continue
+        lo, hi = m - 0.5 * ddu, m + 0.5 * ddu
+        full = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2])
+        g = _gas_volume(lo, hi, n)
+        if g <= 1e-14 * full or g >= (1.0 - 1e-14) * full:
+            continue
+        idx.append(i)
+        gas.append(g)
+        sol.append(full - g)
+    return np.asarray(idx, np.int32), np.asarray(gas), np.asarray(sol)
+
+
+def immerse_ball(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 0.0, 1.0)):
+    """The topology with a circular (2D) or spherical (3D) immersed boundary: bound_enc, the
+    face list and the IB arrays of kah_topology. Needs a shared velocity grid, equal-level
+    cells on both sides of every solid/fluid face and no solid cell on the domain boundary or
+    a periodic image."""
     dim = int(topo["dim"])
-    if dim != 2:
-        raise ValueError("immerse_circle: 2D meshes only")
+    if dim not in (2, 3):
+        raise ValueError("immerse_ball: 2D or 3D meshes")
     if not int(topo.get("vs_uniform", 0)):
-        raise ValueError("immerse_circle: the immersed boundary needs a shared velocity grid")
+        raise ValueError("immerse_ball: the immersed boundary needs a shared velocity grid")
     n = int(topo["n_cells"])
     mid = np.asarray(topo["cell_mid"]).reshape(-1, dim)[:n]
     ds = np.asarray(topo["cell_ds"]).reshape(-1, dim)[:n]
@@ -146,11 +212,11 @@ def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0
         h, t = int(P["pair_here"][p]), int(P["pair_there"][p])
         if t >= n:
             if inside[h] or inside[int(img[t - n])]:
-                raise ValueError("immerse_circle: a solid cell touches a periodic image")
+                raise ValueError("immerse_ball: a solid cell touches a periodic image")
             continue
         if t < 0:
             if inside[h]:
-                raise ValueError("immerse_circle: a solid cell lies on the domain boundary")
+                raise ValueError("immerse_ball: a solid cell lies on the domain boundary")
             continue
         if inside[h] and inside[t]:
             keep[p] = False
@@ -158,10 +224,10 @@ def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0
         if inside[h] or inside[t]:
             s, f = (h, t) if inside[h] else (t, h)
             if not ghost[s]:
-                raise ValueError(f"immerse_circle: fluid cell {f} touches cell {s} inside "
+                raise ValueError(f"immerse_ball: fluid cell {f} touches cell {s} inside "
                                  "the solid")
             if level[s] != level[f]:
-                raise ValueError("immerse_circle: solid/fluid faces need equal cell levels")
+                raise ValueError("immerse_ball: solid/fluid faces need equal cell levels")
             if inside[h]:
                 P["pair_here"][p], P["pair_there"][p] = t, h
                 P["pair_rot"][p] = -P["pair_rot"][p]
@@ -208,15 +274,18 @@ def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0
                 continue
             benc[c] = 1
             if (c, s) not in pair_of:
-                raise ValueError(f"immerse_circle: no face between donor {c} and solid {s}")
+                raise ValueError(f"immerse_ball: no face between donor {c} and solid {s}")
             faces["pair"].append(pair_of[(c, s)])
-            ap, nrm = calc_intersect(mid[c], mid[s], center, radius, solid)
+            if dim == 2:
+                ap, nrm = calc_intersect(mid[c], mid[s], center, radius, solid)
+            else:
+                ap, nrm = calc_intersect_sphere(mid[c], mid[s], center, radius, solid)
             faces["aux"].append(ap)
             faces["normal"].append(nrm)
             st = fluid_nbs(c) + [c]
             faces["img"].extend(st)
             faces["img_off"].append(len(faces["img"]))
-            ci, cg, cs = cut_cells_2d(nrm, vmid, vlev, du)
+            ci, cg, cs = (cut_cells_2d if dim == 2 else cut_cells_3d)(nrm, vmid, vlev, du)
             faces["cut_idx"].extend(ci.tolist())
             faces["cut_gas"].extend(cg.tolist())
             faces["cut_sol"].extend(cs.tolist())
@@ -226,7 +295,7 @@ def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0
     for s in solid_cells:
         lst = fluid_nbs(int(s))
         if not lst:
-            raise ValueError(f"immerse_circle: solid cell {s} has no fluid face neighbour")
+            raise ValueError(f"immerse_ball: solid cell {s} has no fluid face neighbour")
         snb.extend(lst)
         snb_off.append(len(snb))
     out["bound_enc"] = benc
@@ -246,3 +315,7 @@ def immerse_circle(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0
     out["ib_cut_solid"] = np.asarray(faces["cut_sol"], np.float64)
     out["ib_inside"] = inside
     return out
+
+
+immerse_circle = immerse_ball   # Circle (2D)
+immerse_sphere = immerse_ball   # Sphere (3D)

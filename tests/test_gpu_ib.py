@@ -1,7 +1,8 @@
 """Immersed boundary on the device (update_solid_cell!, update_solid_neighbor! and the IB
 faces, src/Boundary/Immersed_boundary.jl and src/Flux/Flux.jl:458-487) against the oracle:
-the split phases and the fused step in FP64, graph replay, the FP32 form, the zero wall mass
-flux of every SolidNeighbor and the rejected configurations.
+the split phases and the fused step in FP64 (cylinders in 2D, spheres in 3D), graph replay,
+the FP32 form, the zero wall mass flux of every SolidNeighbor and the rejected
+configurations.
 
 Cells inside the solid (no faces, never updated) are left out of the state comparisons: the
 device does not carry their f across the two state buffers.
@@ -20,6 +21,8 @@ CASES = {
     "cyl32_nv16": lambda: cases.cylinder_2d(n=32, nv=16),
     "cyl16_amr1": lambda: cases.cylinder_2d(n=16, nv=8, ps_levels=1),
     "cyl16_euler": lambda: cases.cylinder_2d(n=16, nv=8, marching="Euler"),
+    "sph8": lambda: cases.sphere_3d(n=8, nv=6, L=2.0),
+    "sph12_nv8": lambda: cases.sphere_3d(n=12, nv=8),
 }
 
 
@@ -33,7 +36,7 @@ def live_cells(case):
 
 def check_state(case, g, o, tol, what):
     keep = live_cells(case)
-    nw = 4
+    nw = case.config.solver.DIM + 2
     n0 = int(case.topo["vs_offset"][1])
     gdf = g["df"].reshape(case.n_cells, -1)[keep]
     odf = o["df"].reshape(case.n_cells, -1)[keep]
@@ -60,7 +63,7 @@ def test_split_phases(name):
         gf, gv = dev.get_flux()
         of, ov = orc.get_flux()
         assert rel_err(gv, ov) <= TOL, "vs flux"
-        assert rel_err_cols(gf, of, 4) <= 1e-11, "macro flux"
+        assert rel_err_cols(gf, of, case.config.solver.DIM + 2) <= 1e-11, "macro flux"
         rg = dev.iterate(dt, s)
         ro = orc.iterate(dt, s)
         check_state(case, dev.get_state(), orc.get_state(), TOL, f"step {s}")
@@ -100,7 +103,7 @@ def test_graph_replay_equals_launches():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16"])
+@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16", "sph12_nv8"])
 def test_wall_carries_no_mass(name):
     case = CASES[name]()
     say(f"[ib wall] {name}")
@@ -111,9 +114,10 @@ def test_wall_carries_no_mass(name):
         dev.step(case.dt_xi, s)
     aux = dev.get_ib()
     t = case.topo
-    vm = np.asarray(t["vs_mid"]).reshape(2, -1).T
+    dim = case.config.solver.DIM
+    vm = np.asarray(t["vs_mid"]).reshape(dim, -1).T
     wt = np.asarray(t["vs_weight"])
-    nr = t["ib_normal"].reshape(-1, 2)
+    nr = t["ib_normal"].reshape(-1, dim)
     for k in range(len(t["ib_face_pair"])):
         flux = wt * (vm @ nr[k]) * aux[k, 0]
         assert abs(flux.sum()) < 1e-13 * np.abs(flux).sum(), k
@@ -121,7 +125,7 @@ def test_wall_carries_no_mass(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16"])
+@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16", "sph12_nv8"])
 def test_fp32_form(name):
     """FP32 form (f stored and advected in FP32) against the FP64 oracle after 3 steps:
     moments within 1e-5 of each cell's scale, the SolidNeighbor df within 1e-5."""
@@ -133,8 +137,9 @@ def test_fp32_form(name):
         orc.step(case.dt_xi, s)
     g, o = dev.get_state(), orc.get_state()
     keep = live_cells(case)
+    nw = case.config.solver.DIM + 2
     for k in ("w", "prim"):
-        assert rel_err_rows(g[k].reshape(-1, 4)[keep], o[k].reshape(-1, 4)[keep], 4) <= 1e-5, k
+        assert rel_err_rows(g[k].reshape(-1, nw)[keep], o[k].reshape(-1, nw)[keep], nw) <= 1e-5, k
     assert rel_err(dev.get_ib(), orc.get_ib()) <= 1e-5
     dev.close()
 

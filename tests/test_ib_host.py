@@ -2,12 +2,15 @@
 update_solid_cell! / update_solid_neighbor! restatement (src/Boundary/Immersed_boundary.jl),
 pinned without the reference's outputs:
 
-- cut cells: the two parts fill the velocity cell, a line through the cell's centre at 45
-  degrees halves it, a line through a corner cuts off the triangle of area 1/8 or 7/8, and
-  the solid part is the one on the side u.n > 0 (the side the wall Maxwellian fills);
+- cut cells: the two parts fill the velocity cell, a line (plane) through the cell's centre
+  halves it, a line through a corner cuts off its closed-form triangle, a plane cuts off the
+  corner tetrahedron of volume l^3/6, a normal with one zero component gives the prism over
+  the 2D cut, and the solid part is the one on the side u.n > 0 (the side the wall
+  Maxwellian fills);
 - roles and faces: no face joins two solid cells, every solid/fluid face is owned by the
   fluid cell with its rot and area sign flipped, every donor has a solid face neighbour;
-- calc_intersect: the point lies on the circle, the normal is the unit outward radius;
+- calc_intersect (circle and sphere): the point lies on the boundary between donor and solid
+  cell, the normal is the unit outward radius;
 - update_solid_cell! / image_df: for f linear in x with exact slopes the direction-weighted
   extrapolation is exact (weights sum to one; a wrong dx sign or midpoint breaks it), so the
   solid cell holds f(x_solid) and the gas side of every SolidNeighbor holds f(aux point)
@@ -67,6 +70,53 @@ def test_cut_areas_of_a_refined_cell_scale_with_its_level():
         assert g[0] + s[0] == pytest.approx((0.4 / 2 ** lev) ** 2, rel=1e-14)
 
 
+def test_cut_volumes_of_cubes():
+    lvl = np.zeros(1, np.int8)
+    # a plane through the centre halves the cube, for any normal
+    for n in ([1.0, 1.0, 1.0], [1.0, -2.0, 0.5], [0.3, 0.0, 1.0]):
+        n = np.asarray(n) / np.linalg.norm(n)
+        idx, g, s_ = ib.cut_cells_3d(n, np.zeros((1, 3)), lvl, np.ones(3))
+        assert idx.tolist() == [0] and g[0] == pytest.approx(0.5, abs=1e-15)
+    # the corner tetrahedron x + y + z < 0 of the cube [-0.25, 0.75]^3: legs 0.75
+    n = np.ones(3) / math.sqrt(3.0)
+    idx, g, s_ = ib.cut_cells_3d(n, np.full((1, 3), 0.25), lvl, np.ones(3))
+    assert g[0] == pytest.approx(0.75 ** 3 / 6.0, rel=1e-14)
+    assert s_[0] == pytest.approx(1.0 - 0.75 ** 3 / 6.0, rel=1e-14)
+    # the solid part lies on u.n > 0
+    idx, g, s_ = ib.cut_cells_3d(-n, np.full((1, 3), 0.25), lvl, np.ones(3))
+    assert s_[0] == pytest.approx(0.75 ** 3 / 6.0, rel=1e-14)
+    # one zero component: the cube is a prism over the 2D cut, volume = extent x area
+    n3 = np.array([0.6, 0.8, 0.0])
+    m = np.array([[0.1, -0.05, 0.3]])
+    idx, g, s_ = ib.cut_cells_3d(n3, m, lvl, np.array([1.0, 1.0, 0.5]))
+    i2, g2, s2 = ib.cut_cells_2d(n3[:2], m[:, :2], lvl, np.array([1.0, 1.0]))
+    assert g[0] == pytest.approx(0.5 * g2[0], rel=1e-14)
+    # touching a corner only, and normals along an axis: no cut cell
+    assert len(ib.cut_cells_3d(n, np.full((1, 3), 0.5), lvl, np.ones(3))[0]) == 0
+    assert len(ib.cut_cells_3d(np.array([0.0, 0.0, 1.0]), m, lvl, np.ones(3))[0]) == 0
+
+
+def test_sphere_roles_and_intersections():
+    case = cases.sphere_3d(n=8, nv=6, L=2.0)
+    t = case.topo
+    n = case.n_cells
+    mid = t["cell_mid"].reshape(-1, 3)[:n]
+    inside = np.linalg.norm(mid, axis=1) < 1.0
+    assert np.array_equal(t["bound_enc"] < 0, inside)
+    ap = t["ib_aux_point"].reshape(-1, 3)
+    nr = t["ib_normal"].reshape(-1, 3)
+    assert np.allclose(np.linalg.norm(ap, axis=1), 1.0, atol=1e-14)
+    assert np.allclose(nr, ap, atol=1e-15)
+    ph, pt = t["pair_here"], t["pair_there"]
+    for k, p in enumerate(t["ib_face_pair"]):
+        d = int(t["pair_dir"][p])
+        # the intersection lies between donor and solid along the face direction
+        lo, hi = sorted((mid[ph[p], d], mid[pt[p], d]))
+        assert lo <= ap[k, d] <= hi
+        other = [e for e in range(3) if e != d]
+        assert np.array_equal(ap[k, other], mid[ph[p], other])
+
+
 @pytest.mark.parametrize("ps_levels", [0, 1])
 def test_roles_faces_and_intersections(ps_levels):
     case = cases.cylinder_2d(n=16, nv=8, ps_levels=ps_levels)
@@ -120,34 +170,43 @@ def _linear_ib_state(case, seed=3):
     rng = np.random.default_rng(seed)
     t = case.topo
     n = case.n_cells
+    dim, ndf = case.config.solver.DIM, case.config.solver.NDF
     n0 = int(t["vs_offset"][1])
-    mid = t["cell_mid"].reshape(-1, 2)[:n]
-    a = rng.uniform(0.5, 1.5, (2, n0))
-    g = rng.uniform(-0.02, 0.02, (2, 2, n0))          # [d, k, v]
+    mid = t["cell_mid"].reshape(-1, dim)[:n]
+    a = rng.uniform(0.5, 1.5, (ndf, n0))
+    g = rng.uniform(-0.02, 0.02, (dim, ndf, n0))          # [d, k, v]
     df = a[None] + np.einsum("cd,dkv->ckv", mid, g)
-    sdf = np.broadcast_to(g[None], (n, 2, 2, n0))   # [c][d][k][v]
+    sdf = np.broadcast_to(g[None], (n, dim, ndf, n0))   # [c][d][k][v]
     return df, a, g, np.ascontiguousarray(sdf).reshape(-1)
 
 
-def test_oracle_ib_is_exact_on_linear_fields():
-    case = cases.cylinder_2d(n=16, nv=8)
+IB_CASES = {
+    "cylinder": lambda: cases.cylinder_2d(n=16, nv=8),
+    "sphere": lambda: cases.sphere_3d(n=8, nv=6, L=2.0),
+}
+
+
+@pytest.mark.parametrize("name", list(IB_CASES))
+def test_oracle_ib_is_exact_on_linear_fields(name):
+    case = IB_CASES[name]()
     t = case.topo
     n = case.n_cells
+    dim, ndf = case.config.solver.DIM, case.config.solver.NDF
     df, a, g, sdf = _linear_ib_state(case)
     o = O.Oracle(kah_params(case.config), t)
     o.set_state(case.w, case.prim, df.reshape(-1))
     o.set_sdf(sdf)
     o.flux(case.dt_xi)
     st = o.get_state()
-    got = st["df"].reshape(n, 2, -1)
-    mid = t["cell_mid"].reshape(-1, 2)[:n]
+    got = st["df"].reshape(n, ndf, -1)
+    mid = t["cell_mid"].reshape(-1, dim)[:n]
     for s in t["ib_solid"]:
         exact = a + np.einsum("d,dkv->kv", mid[s], g)
         assert np.max(np.abs(got[s] - exact)) < 1e-14
     aux = o.get_ib()
-    vm = np.asarray(t["vs_mid"]).reshape(2, -1).T
-    ap = t["ib_aux_point"].reshape(-1, 2)
-    nr = t["ib_normal"].reshape(-1, 2)
+    vm = np.asarray(t["vs_mid"]).reshape(dim, -1).T
+    ap = t["ib_aux_point"].reshape(-1, dim)
+    nr = t["ib_normal"].reshape(-1, dim)
     coff = t["ib_cut_off"]
     checked = 0
     for k in range(len(t["ib_face_pair"])):
@@ -161,9 +220,12 @@ def test_oracle_ib_is_exact_on_linear_fields():
     assert checked > 0
 
 
-def test_oracle_wall_carries_no_mass_and_fills_a_maxwellian():
-    case = cases.cylinder_2d(n=16, nv=12)
+@pytest.mark.parametrize("name", ["cylinder", "sphere"])
+def test_oracle_wall_carries_no_mass_and_fills_a_maxwellian(name):
+    case = (cases.cylinder_2d(n=16, nv=12) if name == "cylinder"
+            else cases.sphere_3d(n=8, nv=6, L=2.0))
     t = case.topo
+    dim = case.config.solver.DIM
     o = O.Oracle(kah_params(case.config), t)
     o.set_state(case.w, case.prim, case.df)
     for s in range(3):
@@ -171,11 +233,11 @@ def test_oracle_wall_carries_no_mass_and_fills_a_maxwellian():
         o.flux(case.dt_xi)
         o.iterate(case.dt_xi, s)
     aux = o.get_ib()
-    vm = np.asarray(t["vs_mid"]).reshape(2, -1).T
+    vm = np.asarray(t["vs_mid"]).reshape(dim, -1).T
     wt = np.asarray(t["vs_weight"])
-    nr = t["ib_normal"].reshape(-1, 2)
+    nr = t["ib_normal"].reshape(-1, dim)
     coff = t["ib_cut_off"]
-    bc = t["ib_bc"].reshape(-1, 4)
+    bc = t["ib_bc"].reshape(-1, dim + 2)
     ncut = 0
     for k in range(len(t["ib_face_pair"])):
         vn = vm @ nr[k]
