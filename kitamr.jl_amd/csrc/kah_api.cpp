@@ -84,6 +84,8 @@ struct kah_ctx {
   // immersed boundary: solid cells with their fluid neighbours, the IB faces (their
   // SolidNeighbor df lives in the rows of the cells nc + k of the state buffers)
   int ib_ns = 0, ib_nf = 0;
+  int64_t ib_rows = 0;               // velocity rows of all SolidNeighbors (after the cells')
+  std::vector<int64_t> ib_donor;     // donor cell of each IB face
   bool ib_done = false;
   std::vector<int64_t> ib_of_pair;   // IB face of each pair, or -1
   DBuf d_ib_solid, d_ib_snoff, d_ib_snb, d_ib_faces, d_ib_img, d_ib_cut, d_ib_cutw;
@@ -464,14 +466,15 @@ struct Builder {
         if (trec >= nrec) return C.fail(KAH_EINVAL, "pair there out of range");
         int64_t t = src(trec);
         bool solid = T.bound_enc[t] < 0;
-        if (solid && !C.uni)
-          return C.fail(KAH_EUNSUPPORTED,
-                        "faces with solid cells need a shared velocity grid in this version");
-        int8_t mh = solid ? MI_UNLIMITED : MI_LIMITED;
-        int8_t mt = solid ? (caidvm ? MI_PLAIN : MI_UNLIMITED) : MI_LIMITED;
         // an IB face reads its there side from the SolidNeighbor rows (cell nc + k), whose
         // half flux the IB kernel writes after the slopes
         const int64_t ibk = C.ib_of_pair.empty() ? -1 : C.ib_of_pair[size_t(p)];
+        if (solid && !C.uni && ibk < 0)
+          return C.fail(KAH_EUNSUPPORTED,
+                        "faces with solid cells outside an immersed boundary need a shared "
+                        "velocity grid");
+        int8_t mh = solid ? MI_UNLIMITED : MI_LIMITED;
+        int8_t mt = solid ? (caidvm ? MI_PLAIN : MI_UNLIMITED) : MI_LIMITED;
         if (caidvm) {
           hal[h].push_back(half(s0, int32_t(h), mh, 0, dir, rot, h, fm, -1));
           if (solid && ibk < 0)
@@ -497,6 +500,8 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
   const int64_t nc = T->n_cells;
   C->ib_ns = 0;
   C->ib_nf = 0;
+  C->ib_rows = 0;
+  C->ib_donor.clear();
   C->ib_done = false;
   C->ib_of_pair.clear();
   for (DBuf* b : {&C->d_ib_solid, &C->d_ib_snoff, &C->d_ib_snb, &C->d_ib_faces, &C->d_ib_img,
@@ -506,8 +511,6 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
   if (ns == 0 && nf == 0) return 0;
   if (ns < 0 || nf < 0 || ns > nc || nf > T->n_pairs)
     return C->fail(KAH_EINVAL, "immersed boundary counts out of range");
-  if (!C->uni)
-    return C->fail(KAH_EUNSUPPORTED, "the immersed boundary needs a shared velocity grid");
   if (C->P.flux_kind != KAH_FLUX_CAIDVM)
     return C->fail(KAH_EUNSUPPORTED, "the immersed boundary is built for the CAIDVM flux");
   if ((ns > 0 && (!T->ib_solid || !T->ib_solid_nb_off || !T->ib_solid_nb)) ||
@@ -548,6 +551,9 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
       return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) +
                                      ": pair must join a donor (here) and a solid cell (there)");
     C->ib_of_pair[size_t(p)] = k;
+    C->ib_donor.push_back(h);
+    const int64_t nd = T->vs_offset[h + 1] - T->vs_offset[h];
+    C->ib_rows += nd;
     IbFaceDesc& F = fd[size_t(k)];
     F.donor = int32_t(h);
     F.solid = int32_t(t);
@@ -571,7 +577,7 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
       return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) + ": cut offsets out of order");
     for (int64_t x = cb; x < ce; ++x) {
       const int32_t i = T->ib_cut_index[x];
-      if (i < 0 || i >= C->n0 || (x > cb && i <= T->ib_cut_index[x - 1]) ||
+      if (i < 0 || i >= nd || (x > cb && i <= T->ib_cut_index[x - 1]) ||
           !(T->ib_cut_gas[x] > 0.0) || !(T->ib_cut_solid[x] > 0.0))
         return C->fail(KAH_EINVAL, "IB face " + std::to_string(k) +
                                        ": cut cells must be ascending, in range, with positive areas");
@@ -803,15 +809,35 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       }
     }
   }
-  KAH_TRY(C->upload(C->d_voff, T->vs_offset, size_t(nc + 1), "vs_offset"));
-  KAH_TRY(C->upload(C->d_gu, T->vs_mid, size_t(ng * D), "vs_mid"));
-  KAH_TRY(C->upload(C->d_gw, T->vs_weight, size_t(ng), "vs_weight"));
-  KAH_TRY(C->upload(C->d_glev, T->vs_level, size_t(ng), "vs_level"));
+  // immersed boundary: on adapted grids each SolidNeighbor (cell nc + k) carries a copy of its
+  // donor's grid after the cells' grids, so kernels read it like any cell
+  KAH_TRY(setup_ib(C, T));
+  if (!C->uni && C->ib_nf > 0) {
+    std::vector<int64_t> voff(T->vs_offset, T->vs_offset + nc + 1);
+    std::vector<double> gu(T->vs_mid, T->vs_mid + ng * D), gw(T->vs_weight, T->vs_weight + ng);
+    std::vector<int8_t> gl(T->vs_level, T->vs_level + ng);
+    for (int64_t h : C->ib_donor) {
+      const int64_t o = T->vs_offset[h], n = T->vs_offset[h + 1] - o;
+      voff.push_back(voff.back() + n);
+      gu.insert(gu.end(), T->vs_mid + o * D, T->vs_mid + (o + n) * D);
+      gw.insert(gw.end(), T->vs_weight + o, T->vs_weight + o + n);
+      gl.insert(gl.end(), T->vs_level + o, T->vs_level + o + n);
+      keys.insert(keys.end(), keys.begin() + o, keys.begin() + o + n);
+    }
+    KAH_TRY(C->upload(C->d_voff, voff.data(), voff.size(), "vs_offset"));
+    KAH_TRY(C->upload(C->d_gu, gu.data(), gu.size(), "vs_mid"));
+    KAH_TRY(C->upload(C->d_gw, gw.data(), gw.size(), "vs_weight"));
+    KAH_TRY(C->upload(C->d_glev, gl.data(), gl.size(), "vs_level"));
+  } else {
+    KAH_TRY(C->upload(C->d_voff, T->vs_offset, size_t(nc + 1), "vs_offset"));
+    KAH_TRY(C->upload(C->d_gu, T->vs_mid, size_t(ng * D), "vs_mid"));
+    KAH_TRY(C->upload(C->d_gw, T->vs_weight, size_t(ng), "vs_weight"));
+    KAH_TRY(C->upload(C->d_glev, T->vs_level, size_t(ng), "vs_level"));
+  }
   if (!C->uni) KAH_TRY(C->upload(C->d_gkey, keys.data(), keys.size(), "vs_key"));
   else C->d_gkey.release();
 
-  // immersed boundary, then the descriptors
-  KAH_TRY(setup_ib(C, T));
+  // descriptors
   Builder B(*C, *T);
   KAH_TRY(B.build());
 
@@ -1017,7 +1043,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   const int64_t NDF = C->NDF, NW = C->NW;
   const size_t fs = C->fsize();
   // the state buffers hold the SolidNeighbor rows of the IB faces after the cells' rows
-  const int64_t prow = C->np + int64_t(C->ib_nf) * C->n0;
+  const int64_t prow = C->np + C->ib_rows;
   KAH_TRY(C->alloc(C->d_df[0], fs * prow * NDF, "df"));
   KAH_TRY(C->alloc(C->d_df[1], fs * prow * NDF, "df"));
   if (C->ib_nf > 0) {
@@ -1145,7 +1171,7 @@ int run_slope(kah_ctx* C, const DevArgs& A) {
 // update_solid_cell! and update_solid_neighbor! (src/Flux/Flux.jl:461-482)
 int run_ib(kah_ctx* C, const DevArgs& A) {
   if (C->ib_ns == 0 && C->ib_nf == 0) return 0;
-  KAH_TRY(C->hip(launch_ib(A, C->D, C->f32, C->ib_ns, C->d_ib_solid.as<int32_t>(),
+  KAH_TRY(C->hip(launch_ib(A, C->D, C->uni, C->f32, C->ib_ns, C->d_ib_solid.as<int32_t>(),
                            C->d_ib_snoff.as<int32_t>(), C->d_ib_snb.as<int32_t>(), C->ib_nf,
                            C->d_ib_faces.as<IbFaceDesc>(), C->d_ib_img.as<int32_t>(),
                            C->d_ib_cut.as<int32_t>(), C->d_ib_cutw.as<double2>(),
@@ -1653,7 +1679,7 @@ int kah_get_ib(kah_ctx* C, double* aux_df) {
   KAH_TRY(C->hip(hipSetDevice(C->P.device), "hipSetDevice"));
   // both state buffers hold the rows of the last flux
   const size_t off = size_t(C->np) * C->NDF * C->fsize();
-  const size_t cnt = size_t(C->ib_nf) * C->n0 * C->NDF;
+  const size_t cnt = size_t(C->ib_rows) * C->NDF;
   KAH_TRY(C->download_f(aux_df, static_cast<const char*>(C->d_df[C->cur].p) + off, cnt,
                         "ib download"));
   return C->hip(hipStreamSynchronize(C->stream), "ib download");

@@ -2008,8 +2008,9 @@ inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int
 
 // ------------------------------------------------------------------ immersed boundary
 // update_solid_cell! and update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:122-148,
-// 367-483) between the slopes and the flux, on a shared velocity grid: one workgroup per
-// solid cell or IB face, the threads striding over the velocity cells. Arithmetic in FP64.
+// 367-483) between the slopes and the flux: one workgroup per solid cell or IB face, the
+// threads striding over the velocity cells of the target grid; sources on other grids are
+// read through their key-interval overlap. Arithmetic in FP64.
 
 // sum of one value over the 256-thread workgroup (every thread gets the sum)
 __device__ inline double block_sum256(double v, double* sh) {
@@ -2024,10 +2025,10 @@ __device__ inline double block_sum256(double v, double* sh) {
 // wn_j = w_j / sum w, w_j = max(0, u.l_j/|u|)^2, l_j = (x - x_j)/|x - x_j| (1/N when the sum
 // is zero): vs_extrapolate! with the weights of update_solid_cell! / image_df (:24-47,
 // :122-141, :378-395). Sums of products in index order, as Julia's dot and norm.
-template <int D, int NDF, class FT>
+template <int D, int NDF, bool UNI, class FT>
 __device__ inline void ib_extrapolate(const DevArgs& A, const int32_t* cells, int N,
-                                      const double* cmid, const double* x, int i,
-                                      const double (&u)[D], double (&out)[NDF]) {
+                                      const double* cmid, const double* x, int i, int li,
+                                      uint32_t ki, const double (&u)[D], double (&out)[NDF]) {
   double nu = u[0] * u[0];
 #pragma unroll
   for (int t = 1; t < D; ++t) nu += u[t] * u[t];
@@ -2056,24 +2057,30 @@ __device__ inline void ib_extrapolate(const DevArgs& A, const int32_t* cells, in
   for (int k = 0; k < NDF; ++k) out[k] = 0.0;
   for (int j = 0; j < N; ++j) {
     const double wn = ws == 0.0 ? 1.0 / double(N) : w[j] / ws;
-    const Grid<D, NDF, true, FT> G(A, cells[j]);
+    const Grid<D, NDF, UNI, FT> G(A, cells[j]);
     const double* xj = &cmid[int64_t(cells[j]) * D];
     double dx[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) dx[t] = x[t] - xj[t];
+    // the source cells over target cell i: same level or coarser (injection) with factor 1,
+    // finer ones with 2^-D per level (vs_extrapolate!'s walk)
+    for_overlap<D, NDF, UNI, FT>(G, i, li, ki, A.lmax, [&](int jj, double den, double rden) {
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) {
-      double ddf = G.s(i, k, 0) * dx[0];
+      for (int k = 0; k < NDF; ++k) {
+        double ddf = G.s(jj, k, 0) * dx[0];
 #pragma unroll
-      for (int t = 1; t < D; ++t) ddf += G.s(i, k, t) * dx[t];
-      out[k] += (G.f(i, k) + ddf) * wn;
-    }
+        for (int t = 1; t < D; ++t) ddf += G.s(jj, k, t) * dx[t];
+        double v = G.f(jj, k) + ddf;
+        if (den != 1.0) v = v * rden;   // exact: den is a power of two
+        out[k] += v * wn;
+      }
+    });
   }
 }
 
 // update_solid_cell!: f of solid cell s from its fluid face neighbours, written to both
 // state buffers (this step's IB faces read it; the next step keeps it), then w and prim
-template <int D, int NDF, class FT>
+template <int D, int NDF, bool UNI, class FT>
 __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __restrict__ solid,
                                                   const int32_t* __restrict__ nb_off,
                                                   const int32_t* __restrict__ nb,
@@ -2084,8 +2091,8 @@ __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __re
   const int q = blockIdx.x;
   const int s = solid[q];
   const int b = nb_off[q], N = nb_off[q + 1] - b;
-  const int n = A.uni_n;
-  const Grid<D, NDF, true, FT> G(A, s);
+  const Grid<D, NDF, UNI, FT> G(A, s);
+  const int n = G.n;
   const double* xs = &cmid[int64_t(s) * D];
   double m[NW];
 #pragma unroll
@@ -2095,13 +2102,13 @@ __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __re
 #pragma unroll
     for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
     double f[NDF];
-    ib_extrapolate<D, NDF, FT>(A, nb + b, N, cmid, xs, i, u, f);
+    ib_extrapolate<D, NDF, UNI, FT>(A, nb + b, N, cmid, xs, i, G.lev(i), G.key(i), u, f);
     double fs[NDF];
 #pragma unroll
     for (int k = 0; k < NDF; ++k) {
       const FT v = FT(f[k]);
-      df_a[(int64_t(s) * NDF + k) * n + i] = v;
-      df_b[(int64_t(s) * NDF + k) * n + i] = v;
+      df_a[G.off * NDF + int64_t(k) * n + i] = v;
+      df_b[G.off * NDF + int64_t(k) * n + i] = v;
       fs[k] = double(v);
     }
     double mt[NW];
@@ -2130,7 +2137,7 @@ __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __re
 // the pseudo cell's rows of both state buffers; the face's there half (df .* u[dir] over
 // rot*u[dir] > 0, calc_flux with a solid there side, src/Flux/CAIDVM.jl:110-112) goes to
 // its half_fw row, which the flux kernel sums like any other.
-template <int D, int NDF, class FT>
+template <int D, int NDF, bool UNI, class FT>
 __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __restrict__ faces,
                                                  const int32_t* __restrict__ img,
                                                  const int32_t* __restrict__ cut_idx,
@@ -2140,11 +2147,12 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
   constexpr int NW = D + 2;
   __shared__ double red[4];
   const IbFaceDesc F = faces[blockIdx.x];
-  const int n = A.uni_n;
-  const Grid<D, NDF, true, FT> G(A, F.donor);
-  const Grid<D, NDF, true, FT> S(A, F.solid);
-  FT* pa = df_a + int64_t(F.pseudo) * NDF * n;
-  FT* pb = df_b + int64_t(F.pseudo) * NDF * n;
+  const Grid<D, NDF, UNI, FT> G(A, F.donor);
+  const Grid<D, NDF, UNI, FT> S(A, F.solid);
+  const Grid<D, NDF, UNI, FT> PS(A, F.pseudo);   // the donor's grid, copied
+  const int n = G.n;
+  FT* pa = df_a + PS.off * NDF;
+  FT* pb = df_b + PS.off * NDF;
   const int32_t* ci = cut_idx + F.cut_beg;
   // the cut-cell slot of velocity cell i, or -1 (the list is ascending)
   auto cut_of = [&](int i) {
@@ -2163,13 +2171,27 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
     double u[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
+    const int li = G.lev(i);
+    const uint32_t ki = G.key(i);
     double im[NDF];
-    ib_extrapolate<D, NDF, FT>(A, img + F.img_beg, F.img_cnt, cmid, F.ip, i, u, im);
+    ib_extrapolate<D, NDF, UNI, FT>(A, img + F.img_beg, F.img_cnt, cmid, F.ip, i, li, ki, u, im);
+    // boundary_slope!'s one-sided difference with the solid cell, walked across the grids
+    double sLv[NDF];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) sLv[k] = 0.0;
+    for_overlap<D, NDF, UNI, FT>(S, i, li, ki, A.lmax, [&](int jj, double den, double rden) {
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        double d = G.f(i, k) - S.f(jj, k);
+        if (den != 1.0) d = d * rden;
+        sLv[k] += d / F.dxs;
+      }
+    });
     double aux[NDF];
 #pragma unroll
     for (int k = 0; k < NDF; ++k) {
       const double fc = G.f(i, k);
-      const double sL = (fc - S.f(i, k)) / F.dxs;
+      const double sL = sLv[k];
       double sl = minmod(sL, (im[k] - fc) / F.dxf);
       sl = jl_min(fabs((im[k] - MACH_EPS) / (sl * F.dxL + MACH_EPS)), 1.0) * sl;
       aux[k] = im[k] + sl * F.dxL;
@@ -2247,20 +2269,20 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
   }
 }
 
-template <int D, int NDF, class FT>
+template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_ib_t(const DevArgs& A, int ns, const int32_t* solid,
                               const int32_t* nb_off, const int32_t* nb, int nf,
                               const IbFaceDesc* faces, const int32_t* img, const int32_t* cut_idx,
                               const double2* cut_w, const double* cmid, void* df_a, void* df_b,
                               hipStream_t st) {
   if (ns > 0) {
-    hipLaunchKernelGGL((k_ib_solid<D, NDF, FT>), dim3(ns), dim3(256), 0, st, A, solid, nb_off, nb,
+    hipLaunchKernelGGL((k_ib_solid<D, NDF, UNI, FT>), dim3(ns), dim3(256), 0, st, A, solid, nb_off, nb,
                        cmid, static_cast<FT*>(df_a), static_cast<FT*>(df_b));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (nf > 0) {
-    hipLaunchKernelGGL((k_ib_face<D, NDF, FT>), dim3(nf), dim3(256), 0, st, A, faces, img, cut_idx,
+    hipLaunchKernelGGL((k_ib_face<D, NDF, UNI, FT>), dim3(nf), dim3(256), 0, st, A, faces, img, cut_idx,
                        cut_w, cmid, static_cast<FT*>(df_a), static_cast<FT*>(df_b));
     return hipGetLastError();
   }

@@ -132,7 +132,7 @@ hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, bool f32, con
                               int n, double* sw, hipStream_t st);
 // immersed boundary on a shared velocity grid: the ns solid cells, then the nf IB faces;
 // both write f to the two state buffers df_a and df_b
-hipError_t launch_ib(const DevArgs& A, int dim, bool f32, int ns, const int32_t* solid,
+hipError_t launch_ib(const DevArgs& A, int dim, bool uni, bool f32, int ns, const int32_t* solid,
                      const int32_t* nb_off, const int32_t* nb, int nf, const IbFaceDesc* faces,
                      const int32_t* img, const int32_t* cut_idx, const double2* cut_w,
                      const double* cmid, void* df_a, void* df_b, hipStream_t st);

@@ -28,7 +28,7 @@ KAH_RESTART_DECL(launch_restart_2d_f32)
 KAH_RESTART_DECL(launch_restart_3d_f32)
 #undef KAH_RESTART_DECL
 #define KAH_IB_DECL(name)                                                                     \
-  hipError_t name(const DevArgs&, int, const int32_t*, const int32_t*, const int32_t*, int,     \
+  hipError_t name(const DevArgs&, bool, int, const int32_t*, const int32_t*, const int32_t*, int, \
                   const IbFaceDesc*, const int32_t*, const int32_t*, const double2*,            \
                   const double*, void*, void*, hipStream_t);
 KAH_IB_DECL(launch_ib_2d)
@@ -130,12 +130,12 @@ hipError_t launch_mirror(const DevArgs& A, int dim, bool uni, bool f32, const in
                                      st);
 }
 
-hipError_t launch_ib(const DevArgs& A, int dim, bool f32, int ns, const int32_t* solid,
+hipError_t launch_ib(const DevArgs& A, int dim, bool uni, bool f32, int ns, const int32_t* solid,
                      const int32_t* nb_off, const int32_t* nb, int nf, const IbFaceDesc* faces,
                      const int32_t* img, const int32_t* cut_idx, const double2* cut_w,
                      const double* cmid, void* df_a, void* df_b, hipStream_t st) {
   auto fn = dim == 2 ? (f32 ? launch_ib_2d_f32 : launch_ib_2d) : (f32 ? launch_ib_3d_f32 : launch_ib_3d);
-  return fn(A, ns, solid, nb_off, nb, nf, faces, img, cut_idx, cut_w, cmid, df_a, df_b, st);
+  return fn(A, uni, ns, solid, nb_off, nb, nf, faces, img, cut_idx, cut_w, cmid, df_a, df_b, st);
 }
 
 hipError_t launch_restart(const DevArgs& A, int dim, bool uni, bool f32, const int64_t* row,

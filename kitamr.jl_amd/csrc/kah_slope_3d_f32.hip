@@ -56,12 +56,14 @@ hipError_t launch_vs_project_3d_f32(const DevArgs& A, bool uni, const int32_t* c
              : launch_vs_project_t<3, 1, false, float>(A, cells, n, lds, scratch, dfw, st);
 }
 
-hipError_t launch_ib_3d_f32(const DevArgs& A, int ns, const int32_t* solid, const int32_t* nb_off,
-                     const int32_t* nb, int nf, const IbFaceDesc* faces, const int32_t* img,
-                     const int32_t* cut_idx, const double2* cut_w, const double* cmid,
-                     void* df_a, void* df_b, hipStream_t st) {
-  return launch_ib_t<3, 1, float>(A, ns, solid, nb_off, nb, nf, faces, img, cut_idx, cut_w,
-                                     cmid, df_a, df_b, st);
+hipError_t launch_ib_3d_f32(const DevArgs& A, bool uni, int ns, const int32_t* solid,
+                     const int32_t* nb_off, const int32_t* nb, int nf, const IbFaceDesc* faces,
+                     const int32_t* img, const int32_t* cut_idx, const double2* cut_w,
+                     const double* cmid, void* df_a, void* df_b, hipStream_t st) {
+  return uni ? launch_ib_t<3, 1, true, float>(A, ns, solid, nb_off, nb, nf, faces, img,
+                                                 cut_idx, cut_w, cmid, df_a, df_b, st)
+             : launch_ib_t<3, 1, false, float>(A, ns, solid, nb_off, nb, nf, faces, img,
+                                                  cut_idx, cut_w, cmid, df_a, df_b, st);
 }
 
 }  // namespace kah

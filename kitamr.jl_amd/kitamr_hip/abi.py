@@ -120,6 +120,28 @@ def make_topology_struct(topo):
     return t, keep
 
 
+def ib_sizes(topo):
+    """Velocity-grid size of each IB face's donor (the SolidNeighbor's grid)."""
+    if not len(topo.get("ib_face_pair", [])):
+        return np.zeros(0, np.int64)
+    off = np.asarray(topo["vs_offset"])
+    h = np.asarray(topo["pair_here"])[np.asarray(topo["ib_face_pair"])]
+    return (off[h + 1] - off[h]).astype(np.int64)
+
+
+def split_ib(flat_arr, sizes, ndf, flat=False):
+    """Per-face [ndf, n] blocks of the concatenated SolidNeighbor distributions."""
+    if flat:
+        return flat_arr
+    if len(sizes) and np.all(sizes == sizes[0]):
+        return flat_arr.reshape(len(sizes), ndf, int(sizes[0]))
+    out, o = [], 0
+    for n in sizes:
+        out.append(flat_arr[o:o + ndf * n].reshape(ndf, n))
+        o += ndf * n
+    return out
+
+
 def make_params(dim, ndf, flux="CAIDVM", marching="CAIDVM_Marching", precision="fp64",
                 K=1.0, gamma=5.0 / 3.0, Pr=2.0 / 3.0, mu_ref=0.0, omega=0.5,
                 st_check_interval=100, device=0):
@@ -249,6 +271,7 @@ class HipContext:
         self.np = int(off[-1])
         self.n_ib = int(len(topo.get("ib_face_pair", [])))
         self.n0 = int(off[1]) if self.nc else 0
+        self.ib_sizes = ib_sizes(topo)
 
     def set_state(self, w, prim, df):
         w = np.ascontiguousarray(w, dtype=np.float64)
@@ -281,12 +304,15 @@ class HipContext:
         self._check(self.lib.kah_get_flux(self.h, _ptr(flux), _ptr(vflux)), "kah_get_flux")
         return flux, vflux
 
-    def get_ib(self):
-        """The SolidNeighbor distributions of the last flux, [n_ib_faces, ndf, n0]."""
-        n_ib, n0 = getattr(self, "n_ib", 0), getattr(self, "n0", 0)
-        out = np.empty(max(n_ib * self.ndf * n0, 1))
+    def get_ib(self, flat=False):
+        """The SolidNeighbor distributions of the last flux: per face [ndf, n] on the donor's
+        grid (an array [n_ib_faces, ndf, n0] for a shared grid); flat=True: the
+        concatenation."""
+        sizes = getattr(self, "ib_sizes", np.zeros(0, np.int64))
+        tot = int(sizes.sum()) * self.ndf
+        out = np.empty(max(tot, 1))
         self._check(self.lib.kah_get_ib(self.h, _ptr(out)), "kah_get_ib")
-        return out[:n_ib * self.ndf * n0].reshape(n_ib, self.ndf, n0)
+        return split_ib(out[:tot], sizes, self.ndf, flat)
 
     def _cells(self, cells):
         c = np.ascontiguousarray(cells, dtype=np.int32)

@@ -167,13 +167,17 @@ def box_3d(n=4, nv=8, vmax=6.0, ps_levels=0, bcs=("UniformOutflow",) * 6, vs_lev
 
 def cylinder_2d(n=32, nv=16, vmax=10.0, L=4.0, radius=1.0, Ma=1.5, Kn=0.1, omega=0.81,
                 ps_levels=0, band=0.6, flux="CAIDVM", marching="CAIDVM_Marching",
-                precision="fp64", wall=(1.0, 0.0, 0.0, 1.0), CFL=0.8):
+                precision="fp64", wall=(1.0, 0.0, 0.0, 1.0), CFL=0.8, vs_levels=0,
+                vs_variants=6, seed=20261203):
     """Circle(Maxwellian, [0,0], radius, true, ., wall) in [-L, L]^2 with an inflow of Mach
     Ma (U = Ma*sqrt(5/6) at T = 1) from xmin; ps_levels > 0 refines the band |r - R| < band
-    (and 2:1 balances), so that the cells on both sides of the circle share one level."""
+    (and 2:1 balances), so that the cells on both sides of the circle share one level.
+    vs_levels > 0: per-cell adapted velocity grids, seeded refinements toward the local
+    Maxwellian peak, one of vs_variants per cell (c mod vs_variants), so that neighbouring
+    cells carry different grids."""
     from .ib import immerse_circle
-    solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=0, flux=flux,
-                    time_marching=marching, CFL=CFL)
+    solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=ps_levels, AMR_VS_MAXLEVEL=vs_levels,
+                    flux=flux, time_marching=marching, CFL=CFL)
     gas = Gas(K=1.0, gamma=5.0 / 3.0, Pr=2.0 / 3.0, Kn=Kn, omega=omega, omega_ref=omega)
     U = Ma * math.sqrt(5.0 / 6.0)
     free = [1.0, U, 0.0, 1.0]
@@ -184,7 +188,16 @@ def cylinder_2d(n=32, nv=16, vmax=10.0, L=4.0, radius=1.0, Ma=1.5, Kn=0.1, omega
     if ps_levels > 0:
         def refine(mid, ds, level):
             return np.abs(np.hypot(mid[:, 0], mid[:, 1]) - radius) < band
-    case = build_case(cfg, ps_refine=refine)
+    vs_fn = None
+    if vs_levels > 0:
+        cache = {}
+
+        def vs_fn(c, prim, base, _seed=seed):
+            r = c % vs_variants
+            if r not in cache:
+                cache[r] = refine_toward(base.copy(), prim, rounds=vs_levels, rng=_seed + r)
+            return cache[r]
+    case = build_case(cfg, ps_refine=refine, vs_grid_fn=vs_fn)
     case.topo = immerse_circle(case.topo, (0.0, 0.0), radius, True, wall)
     return case
 

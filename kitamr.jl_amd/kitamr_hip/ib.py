@@ -183,16 +183,59 @@ def cut_cells_3d(normal, vs_mid, vs_level, du):
     return np.asarray(idx, np.int32), np.asarray(gas), np.asarray(sol)
 
 
+def _cell_grid(topo, c, dim):
+    """(level, midpoint [n, dim]) of cell c's velocity grid."""
+    off = np.asarray(topo["vs_offset"])
+    n = int(off[c + 1] - off[c])
+    g = 0 if int(topo.get("vs_uniform", 0)) else int(off[c])
+    lev = np.asarray(topo["vs_level"])[g:g + n]
+    mid = np.asarray(topo["vs_mid"])[g * dim:(g + n) * dim].reshape(dim, n).T
+    return lev, mid
+
+
+def _root_du(topo, dim):
+    lo = np.asarray(topo["vs_lo"], dtype=np.float64)[:dim]
+    hi = np.asarray(topo["vs_hi"], dtype=np.float64)[:dim]
+    return (hi - lo) / np.asarray(topo["vs_trees_num"][:dim], dtype=np.float64)
+
+
+def cut_lists(topo, dim):
+    """The cut velocity cells of every IB face on its donor's current grid (offsets, index,
+    gas, solid), as initialize_solid_neighbor! rebuilds them after each adaptation
+    (src/Solver/AMR.jl:62-63)."""
+    du = _root_du(topo, dim)
+    nrm = np.asarray(topo["ib_normal"]).reshape(-1, dim)
+    here = np.asarray(topo["pair_here"])
+    off, idx, gas, sol = [0], [], [], []
+    for k, p in enumerate(np.asarray(topo["ib_face_pair"])):
+        lev, mid = _cell_grid(topo, int(here[p]), dim)
+        ci, cg, cs = (cut_cells_2d if dim == 2 else cut_cells_3d)(nrm[k], mid, lev, du)
+        idx.extend(ci.tolist())
+        gas.extend(cg.tolist())
+        sol.extend(cs.tolist())
+        off.append(len(idx))
+    return (np.asarray(off, np.int64), np.asarray(idx, np.int32), np.asarray(gas, np.float64),
+            np.asarray(sol, np.float64))
+
+
+def refresh_cuts(topo):
+    """The topology with the cut cells recomputed for the current donor grids (after a
+    velocity-space adaptation changed them); topologies without IB faces pass through."""
+    if not len(topo.get("ib_face_pair", [])):
+        return topo
+    t = dict(topo)
+    t["ib_cut_off"], t["ib_cut_index"], t["ib_cut_gas"], t["ib_cut_solid"] = \
+        cut_lists(t, int(t["dim"]))
+    return t
+
+
 def immerse_ball(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 0.0, 1.0)):
     """The topology with a circular (2D) or spherical (3D) immersed boundary: bound_enc, the
-    face list and the IB arrays of kah_topology. Needs a shared velocity grid, equal-level
-    cells on both sides of every solid/fluid face and no solid cell on the domain boundary or
-    a periodic image."""
+    face list and the IB arrays of kah_topology. Needs equal-level cells on both sides of
+    every solid/fluid face and no solid cell on the domain boundary or a periodic image."""
     dim = int(topo["dim"])
     if dim not in (2, 3):
         raise ValueError("immerse_ball: 2D or 3D meshes")
-    if not int(topo.get("vs_uniform", 0)):
-        raise ValueError("immerse_ball: the immersed boundary needs a shared velocity grid")
     n = int(topo["n_cells"])
     mid = np.asarray(topo["cell_mid"]).reshape(-1, dim)[:n]
     ds = np.asarray(topo["cell_ds"]).reshape(-1, dim)[:n]
@@ -259,12 +302,7 @@ def immerse_ball(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 
     pair_of = {}
     for p in range(out["n_pairs"]):
         pair_of[(int(out["pair_here"][p]), int(out["pair_there"][p]))] = p
-    quad = np.asarray(topo["vs_lo"], dtype=np.float64)[:dim], np.asarray(topo["vs_hi"])[:dim]
-    du = (np.asarray(quad[1], dtype=np.float64) - quad[0]) / np.asarray(topo["vs_trees_num"][:dim])
-    vlev = np.asarray(topo["vs_level"])
-    vmid = np.asarray(topo["vs_mid"]).reshape(dim, -1).T
-    faces = dict(pair=[], aux=[], normal=[], img_off=[0], img=[], cut_off=[0], cut_idx=[],
-                 cut_gas=[], cut_sol=[])
+    faces = dict(pair=[], aux=[], normal=[], img_off=[0], img=[])
     for c in range(n):
         if inside[c]:
             continue
@@ -285,11 +323,6 @@ def immerse_ball(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 
             st = fluid_nbs(c) + [c]
             faces["img"].extend(st)
             faces["img_off"].append(len(faces["img"]))
-            ci, cg, cs = (cut_cells_2d if dim == 2 else cut_cells_3d)(nrm, vmid, vlev, du)
-            faces["cut_idx"].extend(ci.tolist())
-            faces["cut_gas"].extend(cg.tolist())
-            faces["cut_sol"].extend(cs.tolist())
-            faces["cut_off"].append(len(faces["cut_idx"]))
     solid_cells = np.flatnonzero(ghost)
     snb_off, snb = [0], []
     for s in solid_cells:
@@ -309,10 +342,8 @@ def immerse_ball(topo, center=(0.0, 0.0), radius=1.0, solid=True, bc=(1.0, 0.0, 
     out["ib_bc"] = np.tile(np.asarray(bc, np.float64), nf)
     out["ib_image_off"] = np.asarray(faces["img_off"], np.int64)
     out["ib_image_cell"] = np.asarray(faces["img"], np.int64)
-    out["ib_cut_off"] = np.asarray(faces["cut_off"], np.int64)
-    out["ib_cut_index"] = np.asarray(faces["cut_idx"], np.int32)
-    out["ib_cut_gas"] = np.asarray(faces["cut_gas"], np.float64)
-    out["ib_cut_solid"] = np.asarray(faces["cut_sol"], np.float64)
+    out["ib_cut_off"], out["ib_cut_index"], out["ib_cut_gas"], out["ib_cut_solid"] = \
+        cut_lists(out, dim)
     out["ib_inside"] = inside
     return out
 

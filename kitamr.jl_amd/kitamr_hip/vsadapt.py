@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from .ib import refresh_cuts
 from .vspace import RMT
 
 
@@ -40,7 +41,8 @@ def with_grids(topo, grids, dim):
     t["vs_mid"] = np.concatenate([np.ascontiguousarray(g[1].T).reshape(-1) for g in grids])
     t["vs_weight"] = np.concatenate([g[2] for g in grids]).astype(np.float64)
     t["vs_uniform"] = 0
-    return t
+    # the cut velocity cells of the IB faces follow the donors' new grids
+    return refresh_cuts(t)
 
 
 def root_ds(topo, dim):

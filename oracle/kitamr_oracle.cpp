@@ -81,6 +81,7 @@ struct orc_ctx {
   std::vector<int32_t> ib_cut_idx;
   std::vector<int64_t> ib_of_pair;                 // IB face of each pair, or -1
   std::vector<double> ib_df;                       // [face][k][i], the SolidNeighbor df
+  std::vector<int64_t> ib_df_off;                  // first velocity row of each face (donor grid)
   bool ib_done = false;
   // state
   std::vector<double> w, prim, qf, sw, flux;
@@ -962,13 +963,21 @@ int inner_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
     heavi[i] = (rot * o.U(h, i, dir) <= 0.0);
     if (heavi[i]) hs.rows.push_back(i);
   }
+  // IB face: the there side is the SolidNeighbor, which carries the donor's grid
+  // (initialize_solid_neighbor!, src/Boundary/Immersed_boundary.jl:264-273)
+  const int64_t ibk = (there_solid && !o.ib_of_pair.empty()) ? o.ib_of_pair[p] : -1;
+  if (ibk >= 0) {
+    t = h;
+    nt = nh;
+    W.t = h;   // the micro-flux walk is then the identity; nothing is added to the solid side
+    ts.cell = h;
+  }
   for (int64_t j = 0; j < nt; ++j)
-    if (rot * o.U(t, j, dir) > 0.0) ts.rows.push_back(j);
+    if (rot * o.U(ts.cell, j, dir) > 0.0) ts.rows.push_back(j);
   if (there_solid) {
     reconstruct(o, hs, o.rmid(h), o.rds(h), fm, dir, dt, false);
-    const int64_t k = o.ib_of_pair.empty() ? -1 : o.ib_of_pair[p];
-    if (k >= 0)   // IB face: the there side is the SolidNeighbor (calc_flux, CAIDVM.jl:110-112)
-      plain_micro(o, ts, dir, &o.ib_df[size_t(k) * o.ndf * nt]);
+    if (ibk >= 0)   // calc_flux with a solid there side, CAIDVM.jl:110-112
+      plain_micro(o, ts, dir, &o.ib_df[size_t(o.ib_df_off[ibk]) * o.ndf]);
     else if (o.flux_kind == KAH_FLUX_CAIDVM)
       plain_micro(o, ts, dir);
     else
@@ -1025,38 +1034,103 @@ double vdot(int dim, const double* x, const double* y) {
   return a;
 }
 
-// Direction-weighted extrapolation of the listed fluid cells' f to the point x, at velocity
-// cell i of the shared grid: sum_j wn_j (f_j + sdf_j . (x - x_j)), wn_j = w_j / sum w,
-// w_j = max(0, u.l_j/|u|)^2 with l_j = (x - x_j)/|x - x_j| (1/N when sum w = 0).
-// update_solid_cell! / image_df with vs_extrapolate! (src/Boundary/Immersed_boundary.jl:
-// 24-47, 122-141, 384-395).
-void extrapolate(orc_ctx& o, const int64_t* cells, int64_t N, const double* x, int64_t i,
+// Direction-weighted extrapolation of the listed fluid cells' f to the point x, onto the
+// velocity grid of cell tc: out[k*n + i] = sum_j wn_j(i) (f_j + sdf_j . (x - x_j)) over the
+// source cells j, with wn_j = w_j / sum w, w_j = max(0, u.l_j/|u|)^2, l_j = (x - x_j)/|x - x_j|
+// (1/N when sum w = 0), u the target's velocity midpoint. Each source is walked against the
+// target grid as vs_extrapolate! does (same level: add; target coarser: the mean of the
+// finer source cells; target finer: injection). update_solid_cell! / image_df
+// (src/Boundary/Immersed_boundary.jl:24-47, 122-141, 378-395).
+void extrapolate(orc_ctx& o, const int64_t* cells, int64_t N, const double* x, int64_t tc,
                  double* out) {
   const int D = o.dim, NDF = o.ndf;
-  double u[3] = {0.0, 0.0, 0.0}, w[8], ws = 0.0;
-  for (int t = 0; t < D; ++t) u[t] = o.U(cells[0], i, t);
-  const double nu = vnorm(D, u);
-  for (int64_t j = 0; j < N; ++j) {
-    double l[3] = {0.0, 0.0, 0.0};
-    const double* xj = o.rmid(cells[j]);
-    for (int t = 0; t < D; ++t) l[t] = x[t] - xj[t];
-    const double nl = vnorm(D, l);
-    for (int t = 0; t < D; ++t) l[t] = l[t] / nl;
-    const double r = vdot(D, u, l) / nu;
-    const double c = (r != r) ? r : (r > 0.0 ? r : 0.0);   // max(0., r)
-    w[j] = c * c;
-    ws = (j == 0) ? w[j] : ws + w[j];
+  const int64_t n = o.nv(tc);
+  std::vector<double> w(static_cast<size_t>(N * n)), ws(static_cast<size_t>(n), 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    double u[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < D; ++t) u[t] = o.U(tc, i, t);
+    const double nu = vnorm(D, u);
+    for (int64_t j = 0; j < N; ++j) {
+      double l[3] = {0.0, 0.0, 0.0};
+      const double* xj = o.rmid(cells[j]);
+      for (int t = 0; t < D; ++t) l[t] = x[t] - xj[t];
+      const double nl = vnorm(D, l);
+      for (int t = 0; t < D; ++t) l[t] = l[t] / nl;
+      const double r = vdot(D, u, l) / nu;
+      const double c = (r != r) ? r : (r > 0.0 ? r : 0.0);   // max(0., r)
+      w[j * n + i] = c * c;
+      ws[i] = (j == 0) ? w[j * n + i] : ws[i] + w[j * n + i];
+    }
   }
-  for (int k = 0; k < NDF; ++k) out[k] = 0.0;
+  for (int64_t x0 = 0; x0 < n * NDF; ++x0) out[x0] = 0.0;
   for (int64_t j = 0; j < N; ++j) {
-    const double wn = ws == 0.0 ? 1.0 / double(N) : w[j] / ws;
-    double dx[3];
-    const double* xj = o.rmid(cells[j]);
+    const int64_t sc = cells[j];
+    double dx[3] = {0.0, 0.0, 0.0};
+    const double* xj = o.rmid(sc);
     for (int t = 0; t < D; ++t) dx[t] = x[t] - xj[t];
-    for (int k = 0; k < NDF; ++k) {
-      double ddf = o.S(cells[j], i, k, 0) * dx[0];
-      for (int t = 1; t < D; ++t) ddf += o.S(cells[j], i, k, t) * dx[t];
-      out[k] += (o.F(cells[j], i, k) + ddf) * wn;
+    auto term = [&](int64_t jj, int k) {   // df + ddf of source velocity cell jj
+      double ddf = o.S(sc, jj, k, 0) * dx[0];
+      for (int t = 1; t < D; ++t) ddf += o.S(sc, jj, k, t) * dx[t];
+      return o.F(sc, jj, k) + ddf;
+    };
+    int64_t jj = 0;
+    double flag = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double wn = ws[i] == 0.0 ? 1.0 / double(N) : w[j * n + i] / ws[i];
+      const int lt = o.L(tc, i);
+      const int ls = o.L(sc, jj);
+      if (lt == ls) {
+        for (int k = 0; k < NDF; ++k) out[k * n + i] += term(jj, k) * wn;
+        jj += 1;
+      } else if (lt < ls) {
+        while (flag != 1.0) {
+          const double den = std::ldexp(1.0, D * (o.L(sc, jj) - lt));
+          for (int k = 0; k < NDF; ++k) out[k * n + i] += term(jj, k) / den * wn;
+          flag += 1 / den;
+          jj += 1;
+        }
+        flag = 0.0;
+      } else {
+        for (int k = 0; k < NDF; ++k) out[k * n + i] += term(jj, k) * wn;
+        flag += 1 / std::ldexp(1.0, D * (lt - ls));
+        if (flag == 1.0) {
+          jj += 1;
+          flag = 0.0;
+        }
+      }
+    }
+  }
+}
+
+// boundary_slope!'s one-sided difference (src/Boundary/Immersed_boundary.jl:396-426): the
+// donor's f minus the solid cell's, walked across the two grids, over dxs
+void boundary_diff(orc_ctx& o, int64_t c, int64_t s, double dxs, double* sL) {
+  const int D = o.dim, NDF = o.ndf;
+  const int64_t n = o.nv(c);
+  for (int64_t x0 = 0; x0 < n * NDF; ++x0) sL[x0] = 0.0;
+  int64_t idx = 0;
+  double flag = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int l = o.L(c, i), ln = o.L(s, idx);
+    if (l == ln) {
+      for (int k = 0; k < NDF; ++k) sL[k * n + i] = (o.F(c, i, k) - o.F(s, idx, k)) / dxs;
+      idx += 1;
+    } else if (l < ln) {
+      while (flag != 1.0) {
+        const double den = std::ldexp(1.0, D * (o.L(s, idx) - l));
+        for (int k = 0; k < NDF; ++k)
+          sL[k * n + i] += (o.F(c, i, k) - o.F(s, idx, k)) / den / dxs;
+        flag += 1 / den;
+        idx += 1;
+      }
+      flag = 0.0;
+    } else {
+      for (int k = 0; k < NDF; ++k) sL[k * n + i] += (o.F(c, i, k) - o.F(s, idx, k)) / dxs;
+      flag += 1 / std::ldexp(1.0, D * (l - ln));
+      if (flag == 1.0) {
+        idx += 1;
+        flag = 0.0;
+      }
     }
   }
 }
@@ -1072,11 +1146,10 @@ void ib_solid_cells(orc_ctx& o) {
     const int64_t* nb = &o.ib_snb[o.ib_snb_off[q]];
     const int64_t N = o.ib_snb_off[q + 1] - o.ib_snb_off[q];
     const int64_t n = o.nv(s);
-    for (int64_t i = 0; i < n; ++i) {
-      double f[2];
-      extrapolate(o, nb, N, o.rmid(s), i, f);
-      for (int k = 0; k < NDF; ++k) o.F(s, i, k) = f[k];
-    }
+    std::vector<double> f(static_cast<size_t>(n * NDF));
+    extrapolate(o, nb, N, o.rmid(s), s, f.data());
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < NDF; ++k) o.F(s, i, k) = f[k * n + i];
     double wv[5];
     moments_full(o, s, &o.df[o.off(s) * NDF], n, wv);
     for (int m = 0; m < NW; ++m) o.w[s * NW + m] = wv[m];
@@ -1108,23 +1181,24 @@ void ib_solid_neighbors(orc_ctx& o) {
     const double dxs = xc[dir] - xs[dir];
     const double dxf = ip[dir] - xc[dir];
     const double dxL = ap[dir] - ip[dir];
-    double* aux = &o.ib_df[size_t(k) * NDF * n];
+    double* aux = &o.ib_df[size_t(o.ib_df_off[k]) * NDF];
     std::vector<double> vn(static_cast<size_t>(n)), M(static_cast<size_t>(n * NDF));
+    std::vector<double> img_f(static_cast<size_t>(n * NDF)), sLv(static_cast<size_t>(n * NDF));
     std::vector<char> th(static_cast<size_t>(n));
     double pw[5];
     for (int m = 0; m < D + 2; ++m) pw[m] = o.ib_bc[k * (D + 2) + m];
     pw[0] = 1.0;
+    extrapolate(o, img, N, ip, c, img_f.data());
+    boundary_diff(o, c, s, dxs, sLv.data());
     for (int64_t i = 0; i < n; ++i) {
-      double img_f[2];
-      extrapolate(o, img, N, ip, i, img_f);
       double u[3];
       for (int t = 0; t < D; ++t) u[t] = o.U(c, i, t);
       for (int q = 0; q < NDF; ++q) {
         const double fc = o.F(c, i, q);
-        const double sL = (fc - o.F(s, i, q)) / dxs;
-        double sl = minmod(sL, (img_f[q] - fc) / dxf);
-        sl = jl_min(std::fabs((img_f[q] - MACH_EPS) / (sl * dxL + MACH_EPS)), 1.0) * sl;
-        aux[q * n + i] = img_f[q] + sl * dxL;
+        const double im = img_f[q * n + i];
+        double sl = minmod(sLv[q * n + i], (im - fc) / dxf);
+        sl = jl_min(std::fabs((im - MACH_EPS) / (sl * dxL + MACH_EPS)), 1.0) * sl;
+        aux[q * n + i] = im + sl * dxL;
       }
       vn[i] = vdot(D, u, nr);
       th[i] = vn[i] >= 0.0;
@@ -1221,7 +1295,7 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
     o->vs_hi[d] = T->vs_hi[d];
   }
   if (T->n_ib_faces > 0 || T->n_ib_solid > 0) {
-    if (!o->vs_uniform || o->flux_kind != KAH_FLUX_CAIDVM) { delete o; return nullptr; }
+    if (o->flux_kind != KAH_FLUX_CAIDVM) { delete o; return nullptr; }
     const int64_t ns = T->n_ib_solid, nf = T->n_ib_faces;
     copy_in(o->ib_solid, T->ib_solid, ns);
     copy_in(o->ib_snb_off, T->ib_solid_nb_off, ns + 1);
@@ -1238,7 +1312,13 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
     copy_in(o->ib_cut_sol, T->ib_cut_solid, o->ib_cut_off[nf]);
     o->ib_of_pair.assign(size_t(npair), -1);
     for (int64_t k = 0; k < nf; ++k) o->ib_of_pair[o->ib_pair[k]] = k;
-    o->ib_df.assign(size_t(nf * o->ndf * ng), 0.0);
+    o->ib_df_off.assign(size_t(nf + 1), 0);
+    for (int64_t k = 0; k < nf; ++k) {
+      const int64_t p = o->ib_pair[k];
+      if (p < 0 || p >= npair) { delete o; return nullptr; }
+      o->ib_df_off[k + 1] = o->ib_df_off[k] + o->nv(o->ph[p]);
+    }
+    o->ib_df.assign(size_t(o->ib_df_off[nf] * o->ndf), 0.0);
   }
   int NW = D + 2;
   o->w.assign(size_t(o->nc * NW), 0.0);

@@ -72,6 +72,10 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def ib_sizes(topo):
+    return abi.ib_sizes(topo)
+
+
 def _f(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -91,6 +95,7 @@ class Oracle:
         self.np = int(np.asarray(topo["vs_offset"])[-1])
         self.n_ib = int(len(topo.get("ib_face_pair", [])))
         self.n0 = int(np.asarray(topo["vs_offset"])[1]) if self.nc else 0
+        self.ib_sizes = ib_sizes(topo)
 
     def close(self):
         if self.h:
@@ -134,11 +139,12 @@ class Oracle:
                                      out.ctypes.data_as(C.POINTER(C.c_uint8))), "vs_flags")
         return out
 
-    def get_ib(self):
-        """SolidNeighbor distributions of the last flux, [n_ib_faces, ndf, n0]."""
-        out = np.empty(self.n_ib * self.ndf * self.n0)
+    def get_ib(self, flat=False):
+        """SolidNeighbor distributions of the last flux: per face [ndf, n] on the donor's grid
+        (an array [n_ib_faces, ndf, n0] for a shared grid); flat=True: the concatenation."""
+        out = np.empty(max(int(self.ib_sizes.sum()) * self.ndf, 1))
         self._ck(self.L.orc_get_ib(self.h, _p(out)), "get_ib")
-        return out.reshape(self.n_ib, self.ndf, self.n0)
+        return abi.split_ib(out[:int(self.ib_sizes.sum()) * self.ndf], self.ib_sizes, self.ndf, flat)
 
     def get_flux(self):
         f = np.empty(self.nc * (self.dim + 2))

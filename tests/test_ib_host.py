@@ -160,9 +160,26 @@ def test_geometry_rejects_unsupported_meshes():
     case = cases.riemann_2d(n=8, nv=8)
     with pytest.raises(ValueError, match="domain boundary"):
         ib.immerse_circle(case.topo, (0.0, 0.0), 0.3)
-    case = cases.riemann_2d(n=8, nv=8, vs_levels=1)
-    with pytest.raises(ValueError, match="shared velocity grid"):
-        ib.immerse_circle(case.topo, (0.5, 0.5), 0.2)
+
+
+def test_cut_cells_follow_each_donors_grid():
+    """Adapted grids: every IB face's cut cells index its donor's grid, and their two parts
+    add up to the weight of that donor's velocity cell (the Newton-Cotes weight is the cell
+    area); refresh_cuts after the grids change gives the cuts of the new grids."""
+    case = cases.cylinder_2d(n=16, nv=8, vs_levels=2)
+    t = case.topo
+    off = np.asarray(t["vs_offset"])
+    here = t["pair_here"][t["ib_face_pair"]]
+    co = t["ib_cut_off"]
+    for k, h in enumerate(here):
+        idx = t["ib_cut_index"][co[k]:co[k + 1]]
+        assert len(idx) and idx.max() < off[h + 1] - off[h]
+        wt = np.asarray(t["vs_weight"])[off[h] + idx]
+        tot = t["ib_cut_gas"][co[k]:co[k + 1]] + t["ib_cut_solid"][co[k]:co[k + 1]]
+        assert np.allclose(tot, wt, rtol=1e-13, atol=0)
+    same = ib.refresh_cuts(t)
+    for key in ("ib_cut_off", "ib_cut_index", "ib_cut_gas", "ib_cut_solid"):
+        assert np.array_equal(same[key], t[key])
 
 
 def _linear_ib_state(case, seed=3):
@@ -255,3 +272,76 @@ def test_oracle_wall_carries_no_mass_and_fills_a_maxwellian(name):
         # toward the wall out of an expansion, as the reference's formula allows)
         assert np.ptp(ratio) < 1e-12 * np.abs(ratio).max()
     assert ncut > 0
+
+
+def _brute_extrapolate(case, sdf, cells, x, tgt):
+    """update_solid_cell!'s extrapolation by geometric overlap of velocity cells: for each
+    target cell, the source cells whose boxes overlap it (a finer source counts 2^-D per level,
+    a coarser one is injected)."""
+    t = case.topo
+    dim, ndf = case.config.solver.DIM, case.config.solver.NDF
+    off = np.asarray(t["vs_offset"])
+    du = ib._root_du(t, dim)
+    mid = t["cell_mid"].reshape(-1, dim)
+    lt, mt = ib._cell_grid(t, tgt, dim)
+    n = len(lt)
+    u = mt
+    nu = np.sqrt((u * u).sum(axis=1))
+    w = []
+    for c in cells:
+        l = x - mid[c]
+        l = l / np.sqrt((l * l).sum())
+        r = (u @ l) / nu
+        w.append(np.maximum(0.0, r) ** 2)
+    w = np.array(w)
+    ws = w.sum(axis=0)
+    out = np.zeros((ndf, n))
+    for j, c in enumerate(cells):
+        wn = np.where(ws == 0.0, 1.0 / len(cells), w[j] / np.where(ws == 0.0, 1.0, ws))
+        ls, ms = ib._cell_grid(t, c, dim)
+        ns = len(ls)
+        f = case.df_now[off[c] * ndf:off[c + 1] * ndf].reshape(ndf, ns)
+        s = sdf[off[c] * ndf * dim:off[c + 1] * ndf * dim].reshape(dim, ndf, ns)
+        val = f + np.einsum("d,dkv->kv", x - mid[c], s)
+        hs = du[None, :] / 2.0 ** ls[:, None].astype(float)
+        for i in range(n):
+            hi = du / 2.0 ** int(lt[i])
+            ov = np.all(np.abs(ms - mt[i][None, :]) < 0.5 * (hs + hi[None, :]) - 1e-12, axis=1)
+            acc = np.zeros(ndf)
+            for jj in np.flatnonzero(ov):
+                fac = 2.0 ** (-dim * (int(ls[jj]) - int(lt[i]))) if ls[jj] > lt[i] else 1.0
+                acc += val[:, jj] * fac
+            out[:, i] += acc * wn[i]
+    return out
+
+
+def test_oracle_solid_cells_on_adapted_grids_match_overlap():
+    case = cases.cylinder_2d(n=16, nv=8, vs_levels=2)
+    t = case.topo
+    dim, ndf = 2, 2
+    rng = np.random.default_rng(11)
+    P = int(t["vs_offset"][-1])
+    sdf = rng.uniform(-0.05, 0.05, P * ndf * dim)
+    o = O.Oracle(kah_params(case.config), t)
+    o.set_state(case.w, case.prim, case.df)
+    o.set_sdf(sdf)
+    o.flux(case.dt_xi)
+    got = o.get_state()["df"]
+    case.df_now = np.asarray(case.df)
+    off = np.asarray(t["vs_offset"])
+    mid = t["cell_mid"].reshape(-1, dim)
+    so = t["ib_solid_nb_off"]
+    for q, s in enumerate(t["ib_solid"]):
+        nbs = t["ib_solid_nb"][so[q]:so[q + 1]]
+        exp = _brute_extrapolate(case, sdf, nbs, mid[s], s)
+        g = got[off[s] * ndf:off[s + 1] * ndf].reshape(ndf, -1)
+        assert np.max(np.abs(g - exp)) <= 1e-13 * np.max(np.abs(exp)), s
+    # and the wall still carries no mass on the donors' adapted grids
+    aux = o.get_ib()
+    nr = t["ib_normal"].reshape(-1, dim)
+    for k, p in enumerate(t["ib_face_pair"]):
+        h = t["pair_here"][p]
+        lev, mv = ib._cell_grid(t, h, dim)
+        wt = np.asarray(t["vs_weight"])[off[h]:off[h + 1]]
+        fl = wt * (mv @ nr[k]) * aux[k][0]
+        assert abs(fl.sum()) < 1e-13 * np.abs(fl).sum()
