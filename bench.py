@@ -51,6 +51,10 @@ WORKLOADS = {
     "cylinder2d": "2D cylinder, Circle(Maxwellian, r=0.5) immersed in [-8,8]^2, 448x448 "
                   "cells, Mach-5 inflow, Kn=0.1, uniform 40x40 velocity grid on [-15,15]^2, "
                   "CAIDVM + CAIDVM_Marching with update_solid_cell! / update_solid_neighbor!",
+    "cylinder2d-vsamr": "2D cylinder, Circle(Maxwellian, r=0.5) immersed in [-8,8]^2, 448x448 "
+                        "cells (200704 leaves), Mach-5 inflow, Kn=0.1, per-cell adapted "
+                        "velocity grids (20x20 base on [-15,15]^2, 2 levels, 1570-1627 cells "
+                        "each), CAIDVM + CAIDVM_Marching with the immersed boundary",
 }
 
 
@@ -62,6 +66,9 @@ def make_case(workload, precision):
     if workload == "box3d-vsamr":
         return cases.box_3d(n=24, nv=16, vmax=10.0, vs_levels=1, vs_variants=6,
                             precision=precision)
+    if workload == "cylinder2d-vsamr":
+        return cases.cylinder_2d(n=448, nv=20, vmax=15.0, L=8.0, radius=0.5, Ma=5.0, Kn=0.1,
+                                 omega=0.81, vs_levels=2, vs_variants=6, precision=precision)
     if workload == "cylinder2d":
         return cases.cylinder_2d(n=448, nv=40, vmax=15.0, L=8.0, radius=0.5, Ma=5.0, Kn=0.1,
                                  omega=0.81, precision=precision)
@@ -179,8 +186,8 @@ def main():
                     choices=list(WORKLOADS),
                     help="riemann2d: shared 48x48 velocity grid (the headline workload); "
                          "riemann2d-vsamr: per-cell adapted grids (20x20 base on [-15,15]^2, "
-                         "2 levels); box3d-vsamr: 3D1F with adapted grids; cylinder2d: "
-                         "immersed circle")
+                         "2 levels); box3d-vsamr: 3D1F with adapted grids; cylinder2d(-vsamr): "
+                         "immersed circle (shared / adapted velocity grids)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32f"],
                     help="fp64 (parity form) or fp32f (f stored in FP32, FP64 moments)")
     ap.add_argument("--autotune", action="store_true",
