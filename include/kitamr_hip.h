@@ -132,8 +132,8 @@ typedef struct kah_topology {
    * (solid and donor flags in bound_enc, calc_intersect, the image stencils and the cut
    * velocity cells of initialize_cutted_velocity_cell); the library runs
    * update_solid_cell! and update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:
-   * 122-148, 436-500) on the device every step, between the slopes and the flux. Needs a
-   * shared velocity grid (vs_uniform), the CAIDVM flux and a marching other than
+   * 122-148, 436-500) on the device every step, between the slopes and the flux, on shared
+   * or per-cell adapted velocity grids. Needs the CAIDVM flux and a marching other than
    * CIP_Marching (whose positivity_preserving_ib! is not built).
    *
    * Solid cells: the bound_enc < 0 cells whose f update_solid_cell! rebuilds from their
@@ -146,7 +146,8 @@ typedef struct kah_topology {
    * names the flux pair of the face (here = the donor, there = the solid cell). The image
    * stencil lists the donor's fluid neighbours (first per face id, face-id order) and then
    * the donor itself (image_df, :367-395). Cut cells: the velocity cells of the donor's
-   * grid that u.n = 0 splits, with the areas of their gas (u.n < 0) and solid parts. */
+   * grid that u.n = 0 splits (indices into the donor's grid), with the areas (volumes in
+   * 3D) of their gas (u.n < 0) and solid parts. */
   int64_t n_ib_faces;
   const int64_t* ib_face_pair;    /* [n_ib_faces]                                      */
   const double*  ib_aux_point;    /* [n_ib_faces*dim]  SolidNeighbor.aux_point          */
@@ -190,10 +191,11 @@ int kah_get_state(kah_ctx* ctx, double* w, double* prim, double* qf, double* sw,
  * after kah_iterate / kah_step, as after the reference's iterate!. */
 int kah_get_flux(kah_ctx* ctx, double* flux, double* vs_flux);
 
-/* Downloads the distribution of every SolidNeighbor [n_ib_faces*n0*ndf] (column k of face
- * j at (j*ndf + k)*n0), as update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:
- * 436-483) left it in the last flux: the wall-side data of the IB faces. KAH_ESTATE before
- * the first flux or without immersed boundary. */
+/* Downloads the distribution of every SolidNeighbor, as update_solid_neighbor!
+ * (src/Boundary/Immersed_boundary.jl:436-483) left it in the last flux: the wall-side data of
+ * the IB faces, face after face, each [ndf][n_d] on its donor's grid (n_d velocity cells;
+ * n0 each on a shared grid). KAH_ESTATE before the first flux or without immersed
+ * boundary. */
 int kah_get_ib(kah_ctx* ctx, double* aux_df);
 
 /* Ghost-exchange buffers of a list of cells, in the reference's wire format, one buffer per
@@ -302,7 +304,8 @@ int64_t kah_phase_cells(const kah_ctx* ctx);
 /* Device bytes held by the context (state + topology). */
 int64_t kah_device_bytes(const kah_ctx* ctx);
 /* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
- * the flux/relaxation kernel (4 or 8). KAH_OPT_SLOPE_VPT: velocity cells per thread
+ * the flux/relaxation kernel (4 or 8; default 4 for the FP32 form on adapted velocity
+ * grids whose per-workgroup LDS copy of f* is at most 40 KiB, else 8). KAH_OPT_SLOPE_VPT: velocity cells per thread
  * of the slope kernel (1..4). KAH_OPT_VEL_TILES: 1 (default) processes a shared
  * tensor-product velocity grid in 8x8 / 4x4x4 tiles per wave, 0 in storage order.
  * KAH_OPT_OVERLAP_MAPS: 1 (default) precomputes, per fluid cell and neighbour with another

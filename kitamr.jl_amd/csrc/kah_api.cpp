@@ -96,7 +96,15 @@ struct kah_ctx {
   int64_t part_rows = 0;
   int cur = 0;
   bool timing = false;
-  int fr_waves = 8;   // waves per workgroup of the flux/relaxation kernel
+  // waves per workgroup of the flux/relaxation kernel; 0: by shape. The FP32 form on adapted
+  // grids whose LDS copy of f* leaves room for four or more workgroups per CU runs 4-wave
+  // workgroups (more of them resident: measured 1.4x on 2D grids of 1.6k cells); larger
+  // grids, whose LDS already limits residency, and the other forms run 8 waves.
+  int fr_waves = 0;
+  int flux_waves() const {
+    if (fr_waves) return fr_waves;
+    return (f32 && !uni && lds_bytes > 0 && lds_bytes <= 40 * 1024) ? 4 : 8;
+  }
   int slope_vpt = 3;  // velocity cells per thread of the chunked slope kernel
   bool vt_enable = true;   // wave-tile order of shared tensor-product velocity grids
   int vt_mode = 0, vt_n0 = 0, vt_n1 = 0;
@@ -1188,7 +1196,7 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER
                : (C->P.marching == KAH_MARCH_CIP ? MARCH_CIP : MARCH_CAIDVM);
   KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
-                                   mode, march, C->lds_bytes, C->fr_waves, C->stream),
+                                   mode, march, C->lds_bytes, C->flux_waves(), C->stream),
                  "flux/relaxation kernel launch"));
   if (C->timing) KAH_TRY(ev_end(C, e));
   return 0;
