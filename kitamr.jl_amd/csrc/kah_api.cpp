@@ -88,7 +88,7 @@ struct kah_ctx {
   std::vector<int64_t> ib_donor;     // donor cell of each IB face
   bool ib_done = false;
   std::vector<int64_t> ib_of_pair;   // IB face of each pair, or -1
-  DBuf d_ib_solid, d_ib_snoff, d_ib_snb, d_ib_faces, d_ib_img, d_ib_cut, d_ib_cutw;
+  DBuf d_ib_solid, d_ib_snoff, d_ib_snb, d_ib_faces, d_ib_img, d_ib_cut, d_ib_cutw, d_ib_ssdf;
   // velocity-space adaptation: the root lattice of the velocity grids (valid when vs_ok)
   VsArgs vsa{};
   bool vs_ok = false;
@@ -140,7 +140,7 @@ struct kah_ctx {
                           &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
                           &d_moff, &d_cmid, &d_benc, &d_xcells, &d_xoff, &d_xbuf, &d_xfluid,
                           &d_ib_solid, &d_ib_snoff, &d_ib_snb, &d_ib_faces, &d_ib_img,
-                          &d_ib_cut, &d_ib_cutw};
+                          &d_ib_cut, &d_ib_cutw, &d_ib_ssdf};
     int64_t t = 0;
     for (const DBuf* b : bufs) t += int64_t(b->bytes);
     for (const LevelBin* b : bins) t += int64_t(b->items.bytes);
@@ -513,7 +513,7 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
   C->ib_done = false;
   C->ib_of_pair.clear();
   for (DBuf* b : {&C->d_ib_solid, &C->d_ib_snoff, &C->d_ib_snb, &C->d_ib_faces, &C->d_ib_img,
-                  &C->d_ib_cut, &C->d_ib_cutw})
+                  &C->d_ib_cut, &C->d_ib_cutw, &C->d_ib_ssdf})
     b->release();
   const int64_t ns = T->n_ib_solid, nf = T->n_ib_faces;
   if (ns == 0 && nf == 0) return 0;
@@ -561,8 +561,9 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
     C->ib_of_pair[size_t(p)] = k;
     C->ib_donor.push_back(h);
     const int64_t nd = T->vs_offset[h + 1] - T->vs_offset[h];
-    C->ib_rows += nd;
     IbFaceDesc& F = fd[size_t(k)];
+    F.row = C->ib_rows;
+    C->ib_rows += nd;
     F.donor = int32_t(h);
     F.solid = int32_t(t);
     F.pseudo = int32_t(nc + k);
@@ -604,6 +605,7 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
     F.dxs = xd[F.dir] - xs[F.dir];
     F.dxf = F.ip[F.dir] - xd[F.dir];
     F.dxL = ap[F.dir] - F.ip[F.dir];
+    F.apd = ap[F.dir];
     for (int m = 0; m < 5; ++m) F.bc[m] = m < NW ? T->ib_bc[k * NW + m] : 0.0;
     F.bc[0] = 1.0;
   }
@@ -614,6 +616,9 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
     KAH_TRY(C->upload(C->d_ib_snoff, soff.data(), soff.size(), "ib solid neighbour offsets"));
     KAH_TRY(C->upload(C->d_ib_snb, snb.data(), snb.size(), "ib solid neighbours"));
   }
+  if (nf > 0 && C->P.marching == KAH_MARCH_CIP)   // the limited boundary slopes
+    KAH_TRY(C->alloc(C->d_ib_ssdf, sizeof(double) * size_t(C->ib_rows) * size_t(C->NDF),
+                     "ib boundary slopes"));
   if (nf > 0) {
     KAH_TRY(C->upload(C->d_ib_faces, fd.data(), fd.size(), "ib faces"));
     KAH_TRY(C->upload(C->d_ib_img, img.data(), img.size(), "ib image cells"));
@@ -723,12 +728,6 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     if (T->nb_cell[x] < 0 || T->nb_cell[x] >= nc)
       return C->fail(KAH_EINVAL, "neighbour id out of range");
   C->benc.assign(T->bound_enc, T->bound_enc + nc);
-  if (C->P.marching == KAH_MARCH_CIP)
-    for (int64_t c = 0; c < nc; ++c)
-      if (T->bound_enc[c] > 0)
-        return C->fail(KAH_EUNSUPPORTED,
-                       "CIP_Marching next to immersed-boundary cells needs "
-                       "positivity_preserving_ib! (src/Boundary/Positivity.jl), not built");
 
   // Wave tiles of a shared tensor-product velocity grid (vs_trees_num cells per direction,
   // first direction fastest): 8x8 (2D) or 4x4x4 (3D) blocks, so that the 64 lanes of a wave
@@ -1139,6 +1138,10 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.omega = C->P.omega;
   A.dt = dt;
   A.flux_kind = C->P.flux_kind;
+  const bool ppib = C->P.marching == KAH_MARCH_CIP && C->ib_nf > 0;
+  A.ib_faces = ppib ? C->d_ib_faces.as<IbFaceDesc>() : nullptr;
+  A.ib_ssdf = ppib ? C->d_ib_ssdf.as<double>() : nullptr;
+  A.ib_base = int32_t(C->nc);
   return A;
 }
 

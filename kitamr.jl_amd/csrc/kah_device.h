@@ -979,6 +979,61 @@ __device__ inline void cip_shave_newton(const Grid<D, NDF, UNI, FT>& G, int n, d
 // dual, Armijo backtracking evaluated without cancellation), then f = f*tau/(tau+dt) +
 // dt/(tau+dt)*g+, written to the next f. The same algorithm as the oracle's cip_projection;
 // thread 0 takes the per-cell decisions, all threads loop uniformly on them.
+// positivity_preserving_ib! (src/Boundary/Positivity.jl:1-43), the CIP_Marching step of a
+// donor cell: the micro flux through IB face ed of the SolidNeighbor's reconstruction
+// perturbance, ssdf*(x_s - aux)[dir] + ndx.ssdf e_dir with ndx = x_face - u dt - x_s, times
+// u[dir] and the signed face area, on rot*u[dir] > 0 (zero elsewhere).
+template <int D, int NDF, bool UNI, class FT>
+__device__ inline bool ib_micro(const DevArgs& A, const EntryDesc& ed, int i, const double (&u)[D],
+                                double dt, double (&mi)[NDF]) {
+  if (!(ed.rot * pick<D>(u, ed.dir) > 0.0)) return false;
+  const IbFaceDesc& F = A.ib_faces[ed.oth_cell - A.ib_base];
+  const Grid<D, NDF, UNI, FT> PS(A, ed.oth_cell);
+  const int n = PS.n;
+  double ndx[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) ndx[t] = ed.fmid[t] - u[t] * dt - ed.xoth[t];
+  const double xsd = ed.xoth[ed.dir];   // a descriptor in memory: a dynamic index is a load
+  const double ud = pick<D>(u, ed.dir);
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) {
+    const double sl = A.ib_ssdf[F.row * NDF + int64_t(k) * n + i];
+    const double pert = sl * (xsd - F.apd);
+    double dot = ndx[0] * (ed.dir == 0 ? sl : 0.0);
+#pragma unroll
+    for (int t = 1; t < D; ++t) dot += ndx[t] * (ed.dir == t ? sl : 0.0);
+    mi[k] = (pert + dot) * ud * ed.area;
+  }
+  return true;
+}
+
+// The limiter of positivity_preserving_ib!: we (the moments of the IB micro fluxes times
+// dt/V) is added to w with the largest theta <= 1 that keeps density and internal energy
+// positive (margin delta = 1e-3). Updates w in place, returns theta.
+template <int D>
+__device__ inline double ib_theta(double* w, double (&we)[D + 2]) {
+  constexpr int NW = D + 2;
+  const double delta = 1e-3;
+  const double th_rho = we[0] > 0.0 ? 1.0
+                                    : jl_min(1.0, (1.0 - delta) * w[0] / (fabs(we[0]) + MACH_EPS));
+  double sb = w[1] * w[1], se = we[1] * we[1], sd = w[1] * we[1];
+#pragma unroll
+  for (int t = 2; t <= D; ++t) {
+    sb += w[t] * w[t];
+    se += we[t] * we[t];
+    sd += w[t] * we[t];
+  }
+  const double eb = w[NW - 1] - sb / (2.0 * w[0]);
+  const double ee = we[NW - 1] - sd / w[0];
+  const double gam = se / (2.0 * w[0]);
+  const double th_e = jl_min(1.0, 2.0 * (1.0 - delta) * eb /
+                                      (sqrt(ee * ee + 4.0 * gam * (1.0 - delta) * eb) - ee + MACH_EPS));
+  const double th = jl_min(th_rho, th_e);
+#pragma unroll
+  for (int q = 0; q < NW; ++q) w[q] += th * we[q];
+  return th;
+}
+
 template <int D, int NDF, bool UNI, class FT, int NWV>
 __device__ inline void relax_cip(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int n,
                                  double* sc, double* red, double* cell_sh,
@@ -1180,6 +1235,52 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   }
   __syncthreads();
   const double dtV = uniform(res[0]);
+  // CIP_Marching next to an immersed boundary: positivity_preserving_ib! on w before prim_c
+  bool ib_cell = false;
+  double ib_th = 0.0;
+  if constexpr (MARCH == MARCH_CIP) {
+    if (A.ib_faces) {
+      for (int e = 0; e < ne; ++e) ib_cell |= A.ents[eb + e].oth_cell >= A.ib_base;
+      if (ib_cell) {
+        double mw[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) mw[q] = 0.0;
+        for (int p = threadIdx.x; p < n; p += BS) {
+          const int i = G.vi(p);
+          double u[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) u[d] = G.u(i, d);
+          for (int e = 0; e < ne; ++e) {
+            const EntryDesc& ed = A.ents[eb + e];
+            if (ed.oth_cell < A.ib_base) continue;
+            double mi[NDF], m[NW];
+            if (!ib_micro<D, NDF, UNI, FT>(A, ed, i, u, dt, mi)) continue;
+            moment_terms<D, NDF, double>(G.wt(i), u, mi, m);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) mw[q] += m[q];
+          }
+        }
+        block_reduce<NW, NWV>(mw, NW, red, &cell_sh[2 * NW]);
+        if (threadIdx.x == 0) {
+          double we[NW], wb[NW], pc[NW];
+#pragma unroll
+          for (int q = 0; q < NW; ++q) {
+            we[q] = (q == NW - 1 ? 0.5 * cell_sh[2 * NW + q] : cell_sh[2 * NW + q]) * dtV;
+            wb[q] = cell_sh[q];
+          }
+          res[1] = ib_theta<D>(wb, we);
+          get_prim<D>(wb, A.gamma, pc);
+#pragma unroll
+          for (int q = 0; q < NW; ++q) {
+            cell_sh[q] = wb[q];
+            cell_sh[NW + q] = pc[q];
+          }
+        }
+        __syncthreads();
+        ib_th = uniform(res[1]);
+      }
+    }
+  }
   if (MARCH != MARCH_CIP && threadIdx.x == 0) {
     // needed after the first reduction; wave 0 computes them before its share of pass 1
     double pc[NW];
@@ -1245,6 +1346,19 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         // convection first, then the heat flux of f* with prim_c (calc_qf after the
         // convection, src/Theory/I-projection.jl:177-179); e_int and the minima of h for the
         // I-projection
+        if (ib_cell) {   // + theta times the IB micro fluxes, face after face
+          double ud[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) ud[d] = G.u(i, d);
+          for (int e = 0; e < ne; ++e) {
+            const EntryDesc& ed = A.ents[eb + e];
+            if (ed.oth_cell < A.ib_base) continue;
+            double mi[NDF];
+            if (!ib_micro<D, NDF, UNI, FT>(A, ed, i, ud, dt, mi)) continue;
+#pragma unroll
+            for (int k = 0; k < NDF; ++k) fl[k] = fl[k] + CT(ib_th * mi[k]);
+          }
+        }
         CT fs[NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -2196,6 +2310,7 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
       sl = jl_min(fabs((im[k] - MACH_EPS) / (sl * F.dxL + MACH_EPS)), 1.0) * sl;
       aux[k] = im[k] + sl * F.dxL;
       pa[k * n + i] = FT(aux[k]);
+      if (A.ib_ssdf) A.ib_ssdf[F.row * NDF + int64_t(k) * n + i] = sl;
     }
     double vn = u[0] * F.nrm[0];
 #pragma unroll

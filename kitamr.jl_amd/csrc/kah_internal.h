@@ -119,6 +119,8 @@ struct IbFaceDesc {     // one IB face (donor, SolidNeighbor), rebuilt every ste
   int32_t cut_beg, cut_cnt;   // cut velocity cells
   int32_t slot;         // half_fw row of the face's there half (pair*2 + 1)
   int32_t pad;
+  int64_t row;          // first row of the face in the per-face velocity arrays (ib_ssdf)
+  double apd;           // aux_point[dir]
   double rot;
   double ip[3];         // image point aux + x_donor - x_solid
   double nrm[3];        // wall normal

@@ -55,10 +55,15 @@ struct DevArgs {
   const int64_t* map_off;   // [nc] first map entry of each owner cell
   const int32_t* vt_base;   // first velocity cell of each tile
   int32_t vt_mode, vt_n0, vt_n1, vt_pad;
+  // immersed boundary under CIP_Marching (positivity_preserving_ib!): the IB faces, whose
+  // SolidNeighbor rows are the cells ib_base + k, and the limited boundary slope of each
+  // (rows as the SolidNeighbor's df, FP64); null otherwise
+  const IbFaceDesc* ib_faces;
+  double* ib_ssdf;
   // parameters
   double K, gamma, Pr, mu, omega, dt;
   int32_t flux_kind;
-  int32_t pad;
+  int32_t ib_base;
 };
 
 // Read-only arrays passed to the kernels as separate __restrict__ arguments.

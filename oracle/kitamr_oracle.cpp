@@ -82,6 +82,8 @@ struct orc_ctx {
   std::vector<int64_t> ib_of_pair;                 // IB face of each pair, or -1
   std::vector<double> ib_df;                       // [face][k][i], the SolidNeighbor df
   std::vector<int64_t> ib_df_off;                  // first velocity row of each face (donor grid)
+  std::vector<double> ib_ssdf;                     // the limited boundary slope, as ib_df
+  std::vector<std::vector<int64_t>> ib_faces_of;   // IB faces of each donor, in face-id order
   bool ib_done = false;
   // state
   std::vector<double> w, prim, qf, sw, flux;
@@ -1199,6 +1201,7 @@ void ib_solid_neighbors(orc_ctx& o) {
         double sl = minmod(sLv[q * n + i], (im - fc) / dxf);
         sl = jl_min(std::fabs((im - MACH_EPS) / (sl * dxL + MACH_EPS)), 1.0) * sl;
         aux[q * n + i] = im + sl * dxL;
+        o.ib_ssdf[size_t(o.ib_df_off[k]) * NDF + q * n + i] = sl;
       }
       vn[i] = vdot(D, u, nr);
       th[i] = vn[i] >= 0.0;
@@ -1236,6 +1239,85 @@ void ib_solid_neighbors(orc_ctx& o) {
         aux[q * n + i] = (g * gas[(j - c0) * NDF + q] + sw * M[q * n + i]) / (g + sw);
     }
   }
+}
+
+// positivity_preserving_ib! (src/Boundary/Positivity.jl:1-43) for a donor cell, after its w
+// took the macroscopic flux: the micro flux through each IB face of the reconstruction
+// perturbance (SolidNeighbor.vs_data.flux = ssdf*(x_s - aux)[dir], with the boundary slope
+// ssdf as the SolidNeighbor's only slope) over rot*u[dir] > 0, its moments we*dt/V, the
+// limiter theta that keeps density and internal energy positive, then w += theta*we and the
+// velocity flux += theta*micro.
+void ib_positivity(orc_ctx& o, int64_t c, double* w, double* vfl, double dt, double volume) {
+  const int D = o.dim, NDF = o.ndf, NW = o.NW();
+  if (o.ib_faces_of.empty() || o.ib_faces_of[size_t(c)].empty()) return;
+  const auto& faces = o.ib_faces_of[size_t(c)];
+  const int64_t n = o.nv(c);
+  std::vector<double> micros(faces.size() * size_t(NDF * n), 0.0);
+  double we[5] = {0, 0, 0, 0, 0};
+  for (size_t a = 0; a < faces.size(); ++a) {
+    const int64_t k = faces[a];
+    const int64_t p = o.ib_pair[k];
+    const int dir = o.pd[p];
+    const double rot = o.prot[p], area = o.parea[p];
+    const double* fm = &o.pmid[p * D];
+    const double* xs = o.rmid(o.src(o.pt[p]));
+    const double apd = o.ib_aux[k * D + dir];
+    const double* ss = &o.ib_ssdf[size_t(o.ib_df_off[k]) * NDF];
+    double* mi = &micros[a * size_t(NDF * n)];
+    Neumaier acc[5];
+    for (int64_t i = 0; i < n; ++i) {
+      double u[3] = {0, 0, 0};
+      for (int t = 0; t < D; ++t) u[t] = o.U(c, i, t);
+      if (!(rot * u[dir] > 0.0)) continue;
+      double ndx[3];
+      for (int t = 0; t < D; ++t) ndx[t] = fm[t] - u[t] * dt - xs[t];
+      for (int q = 0; q < NDF; ++q) {
+        const double sl = ss[q * n + i];
+        const double pert = sl * (xs[dir] - apd);
+        double dot = ndx[0] * (dir == 0 ? sl : 0.0);
+        for (int t = 1; t < D; ++t) dot += ndx[t] * (dir == t ? sl : 0.0);
+        mi[q * n + i] = (pert + dot) * u[dir] * area;
+      }
+      const double wt = o.Wt(c, i);
+      if (D == 2) {
+        const double h = mi[i], b = mi[n + i];
+        acc[0].add(wt * h);
+        acc[1].add(wt * u[0] * h);
+        acc[2].add(wt * u[1] * h);
+        acc[3].add(wt * ((u[0] * u[0] + u[1] * u[1]) * h + b));
+      } else {
+        const double f = mi[i];
+        acc[0].add(wt * f);
+        acc[1].add(wt * u[0] * f);
+        acc[2].add(wt * u[1] * f);
+        acc[3].add(wt * u[2] * f);
+        acc[4].add(wt * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * f);
+      }
+    }
+    for (int q = 0; q < NW; ++q)
+      we[q] += (q == NW - 1) ? 0.5 * acc[q].get() : acc[q].get();
+  }
+  const double dtv = dt / volume;
+  for (int q = 0; q < NW; ++q) we[q] *= dtv;
+  const double delta = 1e-3;
+  const double th_rho = we[0] > 0.0 ? 1.0
+                                    : jl_min(1.0, (1.0 - delta) * w[0] / (std::fabs(we[0]) + MACH_EPS));
+  double sb = w[1] * w[1], se = we[1] * we[1], sd = w[1] * we[1];
+  for (int t = 2; t <= D; ++t) {
+    sb += w[t] * w[t];
+    se += we[t] * we[t];
+    sd += w[t] * we[t];
+  }
+  const double eb = w[NW - 1] - sb / (2.0 * w[0]);
+  const double ee = we[NW - 1] - sd / w[0];
+  const double gam = se / (2.0 * w[0]);
+  const double th_e = jl_min(1.0, 2.0 * (1.0 - delta) * eb /
+                                      (std::sqrt(ee * ee + 4.0 * gam * (1.0 - delta) * eb) - ee +
+                                       MACH_EPS));
+  const double th = jl_min(th_rho, th_e);
+  for (int q = 0; q < NW; ++q) w[q] += th * we[q];
+  for (size_t a = 0; a < faces.size(); ++a)
+    for (int64_t x = 0; x < NDF * n; ++x) vfl[x] += th * micros[a * size_t(NDF * n) + x];
 }
 
 }  // namespace
@@ -1319,6 +1401,9 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
       o->ib_df_off[k + 1] = o->ib_df_off[k] + o->nv(o->ph[p]);
     }
     o->ib_df.assign(size_t(o->ib_df_off[nf] * o->ndf), 0.0);
+    o->ib_ssdf.assign(o->ib_df.size(), 0.0);
+    o->ib_faces_of.assign(size_t(o->nc), {});
+    for (int64_t k = 0; k < nf; ++k) o->ib_faces_of[size_t(o->ph[o->ib_pair[k]])].push_back(k);
   }
   int NW = D + 2;
   o->w.assign(size_t(o->nc * NW), 0.0);
@@ -1765,9 +1850,10 @@ int orc_iterate(orc_ctx* o, double dt, int64_t step, double* sum_res, double* su
         for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / (tau + dt) * fc[x];
         for (int q2 = 0; q2 < NW; ++q2) prim_new[q2] = prim_c[q2];
       } else if (o->marching == KAH_MARCH_CIP) {
-        // iterate!(CIP_Marching), src/Theory/I-projection.jl:161-192 (positivity_preserving_ib!
-        // is a no-op for bound_enc == 0, src/Boundary/Positivity.jl:3-5)
+        // iterate!(CIP_Marching), src/Theory/I-projection.jl:161-192, with
+        // positivity_preserving_ib! for donor cells (a no-op elsewhere)
         for (int q2 = 0; q2 < NW; ++q2) w[q2] += fl[q2] * dt / area;
+        ib_positivity(*o, c, w, vfl, dt, area);
         double prim_c[5];
         get_prim(D, w, o->gamma, prim_c);
         for (int64_t x = 0; x < n * NDF; ++x) f[x] += dt / area * vfl[x];

@@ -1,8 +1,8 @@
 """Immersed boundary on the device (update_solid_cell!, update_solid_neighbor! and the IB
 faces, src/Boundary/Immersed_boundary.jl and src/Flux/Flux.jl:458-487) against the oracle:
 the split phases and the fused step in FP64 (cylinders in 2D, spheres in 3D), graph replay,
-the FP32 form, the zero wall mass flux of every SolidNeighbor and the rejected
-configurations.
+the FP32 form, positivity_preserving_ib! under CIP_Marching, the zero wall mass flux of every
+SolidNeighbor and the rejected configurations.
 
 Cells inside the solid (no faces, never updated) are left out of the state comparisons: the
 device does not carry their f across the two state buffers.
@@ -26,6 +26,11 @@ CASES = {
     # per-cell adapted velocity grids (six variants: neighbours differ)
     "cyl16_vsamr": lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=2),
     "cyl16_vsamr_amr1": lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=1, ps_levels=1),
+    # CIP_Marching: positivity_preserving_ib! on the donors
+    "cyl16_cip": lambda: cases.cylinder_2d(n=16, nv=16, marching="CIP_Marching"),
+    "cyl16_vsamr_cip": lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=1,
+                                                 marching="CIP_Marching"),
+    "sph8_cip": lambda: cases.sphere_3d(n=8, nv=12, L=2.0, marching="CIP_Marching"),
 }
 
 
@@ -198,8 +203,7 @@ def test_rejections():
     with pytest.raises(abi.KahError, match="no immersed-boundary faces"):
         dev.get_ib()
     dev.close()
-    for kw, msg in ((dict(flux="DVM"), "CAIDVM flux"),
-                    (dict(marching="CIP_Marching"), "positivity_preserving_ib")):
+    for kw, msg in ((dict(flux="DVM"), "CAIDVM flux"),):
         c = cases.cylinder_2d(n=16, nv=8, **kw)
         d = abi.HipContext(kah_params(c.config))
         with pytest.raises(abi.KahError, match=msg):
