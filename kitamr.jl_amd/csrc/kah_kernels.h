@@ -135,7 +135,7 @@ hipError_t launch_overlap_map(const DevArgs& A, int dim, const OvJob* jobs, int 
                               uint32_t* map, hipStream_t st);
 hipError_t launch_macro_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
                               int n, double* sw, hipStream_t st);
-// immersed boundary on a shared velocity grid: the ns solid cells, then the nf IB faces;
+// immersed boundary (shared or adapted velocity grids): the ns solid cells, then the nf IB faces;
 // both write f to the two state buffers df_a and df_b
 hipError_t launch_ib(const DevArgs& A, int dim, bool uni, bool f32, int ns, const int32_t* solid,
                      const int32_t* nb_off, const int32_t* nb, int nf, const IbFaceDesc* faces,

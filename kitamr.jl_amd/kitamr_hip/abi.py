@@ -403,6 +403,7 @@ class HipContext:
         del keep
         self.nc = int(topo["n_cells"])
         self.np = int(np.asarray(topo["vs_offset"])[-1])
+        self.ib_sizes = ib_sizes(topo)   # the SolidNeighbors follow their donors' new grids
 
     def vs_project(self, cells):
         c, cp = self._cells(cells)

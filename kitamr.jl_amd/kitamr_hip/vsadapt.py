@@ -154,18 +154,39 @@ def plan_coarsen(topo, flags, dim):
     return with_grids(topo, grids, dim), np.concatenate(srcs), np.array(changed, dtype=np.uint8)
 
 
+def _inside_rows(topo):
+    """Velocity rows of the cells inside a solid (InsideSolidData: no velocity space in the
+    reference, so never adapted), or None."""
+    inside = topo.get("ib_inside")
+    if inside is None:
+        return None
+    deep = np.asarray(inside, dtype=bool).copy()
+    deep[np.asarray(topo.get("ib_solid", []), dtype=np.int64)] = False
+    if not deep.any():
+        return None
+    off = np.asarray(topo["vs_offset"])
+    return np.concatenate([np.arange(off[c], off[c + 1]) for c in np.flatnonzero(deep)])
+
+
 def vs_adapt(dev, topo, params, dim):
     """One vs_adaptive_mesh_refinement! (without vs_balance) on a context: returns the new
     topology and the per-cell changed flags. vr is computed once, before refining, and used
-    by both passes, as the reference does."""
+    by both passes, as the reference does. Cells inside an immersed solid keep their grids
+    (src/Velocity_space/AMR.jl:47, :93)."""
     vr = dev.vs_resolution()
     flags = dev.vs_flags(params, vr)
+    rows = _inside_rows(topo)
+    if rows is not None:
+        flags[rows] = 0
     t1, src1, ch1 = plan_refine(topo, flags, dim)
     if ch1.any():
         dev.vs_remap(t1, src1, ch1)
     else:
         t1 = topo
     flags = dev.vs_flags(params, vr)
+    rows = _inside_rows(t1)
+    if rows is not None:
+        flags[rows] = 0
     t2, src2, ch2 = plan_coarsen(t1, flags, dim)
     # coarsen_grid_stream! rebuilds every grid, so every cell's slopes are zero afterwards
     dev.vs_remap(t2, src2, np.ones(len(ch2), dtype=np.uint8))

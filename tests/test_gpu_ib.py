@@ -10,6 +10,7 @@ device does not carry their f across the two state buffers.
 import numpy as np
 import pytest
 
+import oracle as O
 from _util import pair, rel_err, rel_err_cols, rel_err_rows, say
 from kitamr_hip import abi, cases
 from kitamr_hip.solver import kah_params
@@ -209,3 +210,34 @@ def test_rejections():
         with pytest.raises(abi.KahError, match=msg):
             d.set_topology(c.topo)
         d.close()
+
+
+@pytest.mark.gpu
+def test_velocity_adaptation_with_immersed_boundary():
+    """vs_adaptive_mesh_refinement! on a cylinder with adapted grids: the SolidNeighbors and
+    cut cells follow the donors' new grids, and the adapted state steps in parity with the
+    oracle built on the new topology."""
+    from kitamr_hip import ib, vsadapt
+    case = CASES["cyl16_vsamr"]()
+    say("[ib vs adapt]")
+    params = kah_params(case.config)
+    dev, _ = pair(case)
+    for s in range(2):
+        dev.step(case.dt_xi, s)
+    p = abi.vs_adapt_params("lohner", coeff_local=1e-2)
+    topo2, va = vsadapt.vs_adapt(dev, case.topo, p, 2)
+    assert va.any()
+    cut = ib.cut_lists(topo2, 2)
+    for key, arr in zip(("ib_cut_off", "ib_cut_index", "ib_cut_gas", "ib_cut_solid"), cut):
+        assert np.array_equal(topo2[key], arr), key
+    got = dev.get_state()
+    o2 = O.Oracle(params, topo2)
+    o2.set_state(got["w"], got["prim"], got["df"])
+    case2 = type(case)(case.config, topo2, got["w"], got["prim"], got["df"], case.dt_xi)
+    for s in range(2):
+        dev.step(case.dt_xi, 2 + s)
+        o2.step(case.dt_xi, 2 + s)
+    check_state(case2, dev.get_state(), o2.get_state(), TOL, "after adaptation")
+    assert rel_err(dev.get_ib(flat=True), o2.get_ib(flat=True)) <= TOL
+    dev.close()
+    o2.close()
