@@ -105,9 +105,10 @@ def profiled_traffic():
 
 def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
     """A short stream of the serving workload: 2D Riemann cases (outflow or Maxwellian-inflow
-    boundaries) with Kn log-uniform in [1e-3, 1], 256-8.3k physical cells (16^2-48^2 base,
-    0-2 refinement levels), 16^2-48^2 velocity grids, FP64 or FP32 form, Poisson arrivals;
-    each runs to the residual tolerance or its step budget.
+    boundaries) and cylinders (an immersed circle, Mach uniform in [0.5, 5]), Kn log-uniform
+    in [1e-3, 1], 256-8.3k physical cells (16^2-48^2 base, 0-2 refinement levels for the
+    Riemann cases), 16^2-48^2 velocity grids, FP64 or FP32 form, Poisson arrivals; each runs
+    to the residual tolerance or its step budget.
     Reports p50/p99 request latency (arrival to result) and the aggregate updates/s."""
     import numpy as np
     from kitamr_hip import cases, serve
@@ -121,14 +122,20 @@ def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
         nv = int(rng.choice([16, 24, 32, 48]))
         Kn = float(10 ** rng.uniform(-3, 0))
         prec = "fp32f" if rng.random() < 0.25 else "fp64"
-        if rng.random() < 0.5:
+        kind = rng.random()
+        if kind < 1.0 / 3.0:
             case = cases.riemann_2d(n=n, nv=nv, ps_levels=lv, halo=(3.0, 2.0, 1.5), Kn=Kn,
                                     precision=prec)
             tag = f"riemann n={n} L={lv} nv={nv}"
-        else:
+        elif kind < 2.0 / 3.0:
             case = cases.riemann_2d(n=n, nv=nv, ps_levels=lv, halo=(3.0, 2.0, 1.5), Kn=Kn,
                                     bc="SuperSonicInflow", precision=prec)
             tag = f"inflow n={n} L={lv} nv={nv}"
+        else:
+            Ma = float(rng.uniform(0.5, 5.0))
+            vmax = max(5.0, Ma * (5.0 / 6.0) ** 0.5 + 5.0)
+            case = cases.cylinder_2d(n=n, nv=nv, vmax=vmax, Ma=Ma, Kn=Kn, precision=prec)
+            tag = f"cylinder n={n} nv={nv} Ma={Ma:.2f}"
         steps = int(rng.integers(40, 160))
         reqs.append(serve.Request(rid=k, case=case, arrival=float(arrivals[k]), max_steps=steps,
                                   tag=f"{tag} {prec} Kn={Kn:.2g}"))
@@ -136,9 +143,9 @@ def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
     srv = serve.Server(budget_bytes=int(budget_gb * 1e9), max_slots=slots)
     rep = srv.run(reqs, keep_df=False)
     rep["workload"] = (f"{n_req} requests, Poisson arrivals at {rate}/s, 2D Riemann cases "
-                       "(outflow or Maxwellian-inflow boundaries), 256-8.3k cells, 16^2-48^2 "
-                       "velocity grids, FP64 or FP32 form, 40-160 steps each (step budget reduced "
-                       "from 2000 to keep the run short)")
+                       "(outflow or Maxwellian-inflow boundaries) and cylinders (immersed circle, "
+                       "Ma 0.5-5), 256-8.3k cells, 16^2-48^2 velocity grids, FP64 or FP32 form, "
+                       "40-160 steps each (step budget reduced from 2000 to keep the run short)")
     rep["slots"] = slots
     return rep
 

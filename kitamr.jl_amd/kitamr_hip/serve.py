@@ -56,7 +56,13 @@ def device_bytes_estimate(case):
     per_cell = nc * (8 * (4 * (D + 2) + 2 * (D + 2) + D + 2 * D * 3) + 2 * D * 96 + 8 * 32 * 160)
     halves = npairs * 2 * (D + 2) * 8 * max(1, P // max(nc, 1) // 256 + 1)
     scratch = P * (NDF + 1) * 8 if P // max(nc, 1) * (NDF + 1) * 8 > 64 * 1024 else 0
-    return int(state + grids + per_cell + halves + scratch + (1 << 20))
+    # immersed boundary: the SolidNeighbor rows (f in both buffers, grid copies, slopes)
+    ib = 0
+    if len(case.topo.get("ib_face_pair", [])):
+        from .abi import ib_sizes
+        rows = int(ib_sizes(case.topo).sum())
+        ib = rows * (NDF * (2 * fs + 8) + D * 8 + 8 + 1 + 4) + len(case.topo["ib_face_pair"]) * 256
+    return int(state + grids + per_cell + halves + scratch + ib + (1 << 20))
 
 
 def _default_factory(case, options=None):

@@ -20,6 +20,9 @@ def _mix():
         ("box3d", lambda: cases.box_3d(n=4, nv=8), 0.02, 7),
         ("riemann8_vsamr", lambda: cases.riemann_2d(n=8, nv=8, vs_levels=1, Kn=1e-3), 0.02, 8),
         ("wall", lambda: cases.wall_channel(n=8, nv=8), 0.03, 11),
+        ("cylinder", lambda: cases.cylinder_2d(n=16, nv=16), 0.01, 9),
+        ("cylinder_vsamr_fp32f", lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=1,
+                                                           precision="fp32f"), 0.02, 6),
     ]
 
 
