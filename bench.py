@@ -51,6 +51,10 @@ WORKLOADS = {
     "cylinder2d": "2D cylinder, Circle(Maxwellian, r=0.5) immersed in [-8,8]^2, 448x448 "
                   "cells, Mach-5 inflow, Kn=0.1, uniform 40x40 velocity grid on [-15,15]^2, "
                   "CAIDVM + CAIDVM_Marching with update_solid_cell! / update_solid_neighbor!",
+    "sphere3d-vsamr": "3D1F sphere, Sphere(Maxwellian, r=0.5) immersed in [-3,3]^3, 24^3 "
+                      "cells (13824 leaves), Mach-3 inflow, Kn=0.05, per-cell adapted velocity "
+                      "grids (16^3 base on [-10,10]^3, 1 level), CAIDVM + CAIDVM_Marching with "
+                      "the immersed boundary",
     "cylinder2d-vsamr": "2D cylinder, Circle(Maxwellian, r=0.5) immersed in [-8,8]^2, 448x448 "
                         "cells (200704 leaves), Mach-5 inflow, Kn=0.1, per-cell adapted "
                         "velocity grids (20x20 base on [-15,15]^2, 2 levels, 1570-1627 cells "
@@ -66,6 +70,9 @@ def make_case(workload, precision):
     if workload == "box3d-vsamr":
         return cases.box_3d(n=24, nv=16, vmax=10.0, vs_levels=1, vs_variants=6,
                             precision=precision)
+    if workload == "sphere3d-vsamr":
+        return cases.sphere_3d(n=24, nv=16, vmax=10.0, L=3.0, radius=0.5, Ma=3.0, Kn=0.05,
+                               omega=0.81, vs_levels=1, vs_variants=6, precision=precision)
     if workload == "cylinder2d-vsamr":
         return cases.cylinder_2d(n=448, nv=20, vmax=15.0, L=8.0, radius=0.5, Ma=5.0, Kn=0.1,
                                  omega=0.81, vs_levels=2, vs_variants=6, precision=precision)

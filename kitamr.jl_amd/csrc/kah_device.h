@@ -2126,13 +2126,22 @@ inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int
 // threads striding over the velocity cells of the target grid; sources on other grids are
 // read through their key-interval overlap. Arithmetic in FP64.
 
-// sum of one value over the 256-thread workgroup (every thread gets the sum)
-__device__ inline double block_sum256(double v, double* sh) {
+// threads per workgroup of the immersed-boundary kernels: few workgroups (one per solid cell
+// or IB face) over grids of up to ~10^4 velocity cells, so each gets 8 waves (more would cap
+// the registers below what the 3D kernels need)
+constexpr int IB_THREADS = 512;
+
+// sum of one value over the IB_THREADS-thread workgroup (every thread gets the sum; fixed
+// order: wave sums, then the waves' partials in index order)
+__device__ inline double block_sum_ib(double v, double* sh) {
   v = wave_sum(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  double t = sh[0];
+#pragma unroll
+  for (int k = 1; k < IB_THREADS / 64; ++k) t += sh[k];
+  return t;
 }
 
 // sum_j wn_j (f_j + sdf_j . (x - x_j)) over the listed cells at velocity cell i, with
@@ -2195,13 +2204,13 @@ __device__ inline void ib_extrapolate(const DevArgs& A, const int32_t* cells, in
 // update_solid_cell!: f of solid cell s from its fluid face neighbours, written to both
 // state buffers (this step's IB faces read it; the next step keeps it), then w and prim
 template <int D, int NDF, bool UNI, class FT>
-__global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __restrict__ solid,
+__global__ __launch_bounds__(IB_THREADS) void k_ib_solid(DevArgs A, const int32_t* __restrict__ solid,
                                                   const int32_t* __restrict__ nb_off,
                                                   const int32_t* __restrict__ nb,
                                                   const double* __restrict__ cmid, FT* df_a,
                                                   FT* df_b) {
   constexpr int NW = D + 2;
-  __shared__ double red[4];
+  __shared__ double red[IB_THREADS / 64];
   const int q = blockIdx.x;
   const int s = solid[q];
   const int b = nb_off[q], N = nb_off[q + 1] - b;
@@ -2211,7 +2220,7 @@ __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __re
   double m[NW];
 #pragma unroll
   for (int r = 0; r < NW; ++r) m[r] = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += IB_THREADS) {
     double u[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
@@ -2232,7 +2241,7 @@ __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __re
   }
   double w[NW];
 #pragma unroll
-  for (int r = 0; r < NW; ++r) w[r] = block_sum256(m[r], red);
+  for (int r = 0; r < NW; ++r) w[r] = block_sum_ib(m[r], red);
   w[NW - 1] = 0.5 * w[NW - 1];
   if (threadIdx.x == 0) {
     double p[NW];
@@ -2252,14 +2261,14 @@ __global__ __launch_bounds__(256) void k_ib_solid(DevArgs A, const int32_t* __re
 // rot*u[dir] > 0, calc_flux with a solid there side, src/Flux/CAIDVM.jl:110-112) goes to
 // its half_fw row, which the flux kernel sums like any other.
 template <int D, int NDF, bool UNI, class FT>
-__global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __restrict__ faces,
+__global__ __launch_bounds__(IB_THREADS) void k_ib_face(DevArgs A, const IbFaceDesc* __restrict__ faces,
                                                  const int32_t* __restrict__ img,
                                                  const int32_t* __restrict__ cut_idx,
                                                  const double2* __restrict__ cut_w,
                                                  const double* __restrict__ cmid, FT* df_a,
                                                  FT* df_b) {
   constexpr int NW = D + 2;
-  __shared__ double red[4];
+  __shared__ double red[IB_THREADS / 64];
   const IbFaceDesc F = faces[blockIdx.x];
   const Grid<D, NDF, UNI, FT> G(A, F.donor);
   const Grid<D, NDF, UNI, FT> S(A, F.solid);
@@ -2281,7 +2290,7 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
   };
   // pass 1: the extrapolated df at the wall, and the two flux sums of the wall density
   double sf = 0.0, mur = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += IB_THREADS) {
     double u[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
@@ -2328,15 +2337,15 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
       sf += G.wt(i) * vn * aux[0];
     }
   }
-  const double SF = block_sum256(sf, red);
-  const double MuR = block_sum256(mur, red);
+  const double SF = block_sum_ib(sf, red);
+  const double MuR = block_sum_ib(mur, red);
   const double rho = -SF / MuR;
   // pass 2: the wall Maxwellian on u.n >= 0, the cut-cell mix, the there-half moments. Each
   // thread revisits the velocity cells it wrote in pass 1, so no barrier is needed here.
   double hm[NW];
 #pragma unroll
   for (int r = 0; r < NW; ++r) hm[r] = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += IB_THREADS) {
     double u[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) u[t] = G.u(i, t);
@@ -2377,7 +2386,7 @@ __global__ __launch_bounds__(256) void k_ib_face(DevArgs A, const IbFaceDesc* __
   }
   double tot[NW];
 #pragma unroll
-  for (int r = 0; r < NW; ++r) tot[r] = block_sum256(hm[r], red);
+  for (int r = 0; r < NW; ++r) tot[r] = block_sum_ib(hm[r], red);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int r = 0; r < NW; ++r) A.half_fw[A.part_off[F.slot] * NW + r] = tot[r];
@@ -2391,13 +2400,13 @@ inline hipError_t launch_ib_t(const DevArgs& A, int ns, const int32_t* solid,
                               const double2* cut_w, const double* cmid, void* df_a, void* df_b,
                               hipStream_t st) {
   if (ns > 0) {
-    hipLaunchKernelGGL((k_ib_solid<D, NDF, UNI, FT>), dim3(ns), dim3(256), 0, st, A, solid, nb_off, nb,
+    hipLaunchKernelGGL((k_ib_solid<D, NDF, UNI, FT>), dim3(ns), dim3(IB_THREADS), 0, st, A, solid, nb_off, nb,
                        cmid, static_cast<FT*>(df_a), static_cast<FT*>(df_b));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (nf > 0) {
-    hipLaunchKernelGGL((k_ib_face<D, NDF, UNI, FT>), dim3(nf), dim3(256), 0, st, A, faces, img, cut_idx,
+    hipLaunchKernelGGL((k_ib_face<D, NDF, UNI, FT>), dim3(nf), dim3(IB_THREADS), 0, st, A, faces, img, cut_idx,
                        cut_w, cmid, static_cast<FT*>(df_a), static_cast<FT*>(df_b));
     return hipGetLastError();
   }

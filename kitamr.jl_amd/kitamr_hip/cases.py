@@ -203,11 +203,14 @@ def cylinder_2d(n=32, nv=16, vmax=10.0, L=4.0, radius=1.0, Ma=1.5, Kn=0.1, omega
 
 
 def sphere_3d(n=12, nv=8, vmax=7.28, L=3.0, radius=1.0, Ma=1.5, Kn=0.05, omega=0.81,
-              flux="CAIDVM", marching="CAIDVM_Marching", precision="fp64", CFL=0.5):
+              flux="CAIDVM", marching="CAIDVM_Marching", precision="fp64", CFL=0.5,
+              vs_levels=0, vs_variants=6, seed=7):
     """Sphere(Maxwellian, [0,0,0], radius, true, ., [1,0,0,0,1]) in [-L, L]^3 with an inflow
-    of Mach Ma (U = Ma*sqrt(5/6) at T = 1) from xmin, outflow elsewhere, on a shared grid."""
+    of Mach Ma (U = Ma*sqrt(5/6) at T = 1) from xmin, outflow elsewhere; vs_levels > 0:
+    per-cell adapted velocity grids (one of vs_variants seeded refinements per cell, as in
+    box_3d), else a shared grid."""
     from .ib import immerse_sphere
-    solver = Solver(DIM=3, NDF=1, AMR_PS_MAXLEVEL=0, AMR_VS_MAXLEVEL=0, flux=flux,
+    solver = Solver(DIM=3, NDF=1, AMR_PS_MAXLEVEL=0, AMR_VS_MAXLEVEL=vs_levels, flux=flux,
                     time_marching=marching, CFL=CFL)
     gas = Gas(K=0.0, Kn=Kn, omega=omega, omega_ref=omega)
     U = Ma * math.sqrt(5.0 / 6.0)
@@ -216,7 +219,17 @@ def sphere_3d(n=12, nv=8, vmax=7.28, L=3.0, radius=1.0, Ma=1.5, Kn=0.05, omega=0
                                                   for i in range(2, 7)]
     cfg = Configure(solver, gas, [-L, L] * 3, [n, n, n], [-vmax, vmax] * 3, [nv] * 3,
                     Uniform(free), dom, precision=precision)
-    case = build_case(cfg)
+    vs_fn = None
+    if vs_levels > 0:
+        cache = {}
+
+        def vs_fn(c, prim, base, _seed=seed):
+            r = c % vs_variants
+            if r not in cache:
+                cache[r] = refine_toward(base.copy(), prim, rounds=vs_levels, frac=0.15,
+                                         rng=_seed + r)
+            return cache[r]
+    case = build_case(cfg, vs_grid_fn=vs_fn)
     case.topo = immerse_sphere(case.topo, (0.0, 0.0, 0.0), radius, True,
                                (1.0, 0.0, 0.0, 0.0, 1.0))
     return case
