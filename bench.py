@@ -210,7 +210,7 @@ def main():
                     help="1/0: wave tiles of the velocity grid on/off (-1: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by physical-space domain decomposition, not run here)",
               file=sys.stderr)
         sys.exit(2)
