@@ -133,7 +133,7 @@ typedef struct kah_topology {
    * velocity cells of initialize_cutted_velocity_cell); the library runs
    * update_solid_cell! and update_solid_neighbor! (src/Boundary/Immersed_boundary.jl:
    * 122-148, 436-500) on the device every step, between the slopes and the flux, on shared
-   * or per-cell adapted velocity grids, with the CAIDVM flux; under CIP_Marching the
+   * or per-cell adapted velocity grids, with the CAIDVM or DVM flux; under CIP_Marching the
    * donors' update also runs positivity_preserving_ib! (src/Boundary/Positivity.jl).
    *
    * Solid cells: the bound_enc < 0 cells whose f update_solid_cell! rebuilds from their

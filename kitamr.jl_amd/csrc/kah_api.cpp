@@ -519,8 +519,6 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
   if (ns == 0 && nf == 0) return 0;
   if (ns < 0 || nf < 0 || ns > nc || nf > T->n_pairs)
     return C->fail(KAH_EINVAL, "immersed boundary counts out of range");
-  if (C->P.flux_kind != KAH_FLUX_CAIDVM)
-    return C->fail(KAH_EUNSUPPORTED, "the immersed boundary is built for the CAIDVM flux");
   if ((ns > 0 && (!T->ib_solid || !T->ib_solid_nb_off || !T->ib_solid_nb)) ||
       (nf > 0 && (!T->ib_face_pair || !T->ib_aux_point || !T->ib_normal || !T->ib_bc ||
                   !T->ib_image_off || !T->ib_image_cell || !T->ib_cut_off ||
@@ -1057,7 +1055,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     KAH_TRY(C->zero(C->d_df[0]));
     KAH_TRY(C->zero(C->d_df[1]));
   }
-  KAH_TRY(C->alloc(C->d_sdf, fs * C->np * NDF * D, "sdf"));
+  KAH_TRY(C->alloc(C->d_sdf, fs * prow * NDF * D, "sdf"));   // + the SolidNeighbors' slopes
   KAH_TRY(C->alloc(C->d_psflux, sizeof(double) * nc * NW, "flux"));
   KAH_TRY(C->alloc(C->d_w, sizeof(double) * nc * NW, "w"));
   KAH_TRY(C->alloc(C->d_prim, sizeof(double) * nc * NW, "prim"));

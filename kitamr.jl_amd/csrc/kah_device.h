@@ -2276,6 +2276,7 @@ __global__ __launch_bounds__(IB_THREADS) void k_ib_face(DevArgs A, const IbFaceD
   const int n = G.n;
   FT* pa = df_a + PS.off * NDF;
   FT* pb = df_b + PS.off * NDF;
+  FT* ps = reinterpret_cast<FT*>(A.sdf) + PS.off * NDF * D;
   const int32_t* ci = cut_idx + F.cut_beg;
   // the cut-cell slot of velocity cell i, or -1 (the list is ascending)
   auto cut_of = [&](int i) {
@@ -2320,6 +2321,10 @@ __global__ __launch_bounds__(IB_THREADS) void k_ib_face(DevArgs A, const IbFaceD
       aux[k] = im[k] + sl * F.dxL;
       pa[k * n + i] = FT(aux[k]);
       if (A.ib_ssdf) A.ib_ssdf[F.row * NDF + int64_t(k) * n + i] = sl;
+      // the SolidNeighbor's slopes: the limited boundary slope along the face direction,
+      // zero across it (read by the DVM flux's reconstruction of the wall side)
+#pragma unroll
+      for (int d = 0; d < D; ++d) ps[(d * NDF + k) * n + i] = d == F.dir ? FT(sl) : FT(0);
     }
     double vn = u[0] * F.nrm[0];
 #pragma unroll

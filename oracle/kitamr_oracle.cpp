@@ -731,6 +731,31 @@ void plain_micro(orc_ctx& o, SideMicro& s, int dir, const double* fsrc = nullptr
   }
 }
 
+// The there side of an IB face under the DVM flux (calc_flux, src/Flux/DVM.jl:90-92): the
+// SolidNeighbor's df plus ndx . sdf, its only slope being the limited boundary slope along
+// the face direction, ndx = x_face - u dt - x_solid, times the normal velocity.
+void ib_dvm_micro(orc_ctx& o, SideMicro& s, int64_t k, const double* xs, const double* fm,
+                  int dir, double dt) {
+  const int D = o.dim, NDF = o.ndf;
+  const int64_t m = int64_t(s.rows.size());
+  const int64_t n = o.nv(s.cell);
+  const double* f = &o.ib_df[size_t(o.ib_df_off[k]) * NDF];
+  const double* sl = &o.ib_ssdf[size_t(o.ib_df_off[k]) * NDF];
+  s.micro.assign(size_t(m * NDF), 0.0);
+  for (int64_t j = 0; j < m; ++j) {
+    const int64_t i = s.rows[j];
+    double ndx[3];
+    for (int t = 0; t < D; ++t) ndx[t] = fm[t] - o.U(s.cell, i, t) * dt - xs[t];
+    const double vn = o.U(s.cell, i, dir);
+    for (int q = 0; q < NDF; ++q) {
+      const double sv = sl[q * n + i];
+      double dot = ndx[0] * (dir == 0 ? sv : 0.0);
+      for (int t = 1; t < D; ++t) dot += ndx[t] * (dir == t ? sv : 0.0);
+      s.micro[q * m + j] = (f[q * n + i] + dot) * vn;
+    }
+  }
+}
+
 // update_micro_flux!, src/Flux/Flux.jl:151-281 (there local PsData, both branches)
 void update_micro_flux(orc_ctx& o, const SideMicro& hs, const SideMicro& ts, double area,
                        int64_t h, int64_t t, bool there_solid, const std::vector<char>& heavi,
@@ -978,8 +1003,10 @@ int inner_face(orc_ctx& o, int64_t p, double dt, FaceWork& W) {
     if (rot * o.U(ts.cell, j, dir) > 0.0) ts.rows.push_back(j);
   if (there_solid) {
     reconstruct(o, hs, o.rmid(h), o.rds(h), fm, dir, dt, false);
-    if (ibk >= 0)   // calc_flux with a solid there side, CAIDVM.jl:110-112
+    if (ibk >= 0 && o.flux_kind == KAH_FLUX_CAIDVM)   // CAIDVM.jl:110-112
       plain_micro(o, ts, dir, &o.ib_df[size_t(o.ib_df_off[ibk]) * o.ndf]);
+    else if (ibk >= 0)   // DVM.jl:90-92: the SolidNeighbor's df with its slope (ssdf along dir)
+      ib_dvm_micro(o, ts, ibk, o.rmid(trec), fm, dir, dt);
     else if (o.flux_kind == KAH_FLUX_CAIDVM)
       plain_micro(o, ts, dir);
     else
@@ -1377,7 +1404,6 @@ orc_ctx* orc_create(const kah_params* P, const kah_topology* T) {
     o->vs_hi[d] = T->vs_hi[d];
   }
   if (T->n_ib_faces > 0 || T->n_ib_solid > 0) {
-    if (o->flux_kind != KAH_FLUX_CAIDVM) { delete o; return nullptr; }
     const int64_t ns = T->n_ib_solid, nf = T->n_ib_faces;
     copy_in(o->ib_solid, T->ib_solid, ns);
     copy_in(o->ib_snb_off, T->ib_solid_nb_off, ns + 1);

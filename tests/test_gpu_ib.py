@@ -32,6 +32,9 @@ CASES = {
     "cyl16_vsamr_cip": lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=1,
                                                  marching="CIP_Marching"),
     "sph8_cip": lambda: cases.sphere_3d(n=8, nv=12, L=2.0, marching="CIP_Marching"),
+    # the DVM flux: the wall side reconstructed with the SolidNeighbor's boundary slope
+    "cyl16_dvm": lambda: cases.cylinder_2d(n=16, nv=12, flux="DVM"),
+    "cyl16_vsamr_dvm": lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=1, flux="DVM"),
 }
 
 
@@ -204,12 +207,6 @@ def test_rejections():
     with pytest.raises(abi.KahError, match="no immersed-boundary faces"):
         dev.get_ib()
     dev.close()
-    for kw, msg in ((dict(flux="DVM"), "CAIDVM flux"),):
-        c = cases.cylinder_2d(n=16, nv=8, **kw)
-        d = abi.HipContext(kah_params(c.config))
-        with pytest.raises(abi.KahError, match=msg):
-            d.set_topology(c.topo)
-        d.close()
 
 
 @pytest.mark.gpu
