@@ -17,7 +17,7 @@ def rel_err(a, b):
 
 
 def rel_err_rows(a, b, width):
-    """Per-row (per cell) max |a-b| / max |b|, reduced by max: each cell judged on its own scale."""
+    """Per-row (per cell) max |a-b| / max |b|, reduced by max: each cell measured on its own scale."""
     a = np.asarray(a).reshape(-1, width)
     b = np.asarray(b).reshape(-1, width)
     scale = np.maximum(np.max(np.abs(b), axis=1), 1e-300)

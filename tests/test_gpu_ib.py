@@ -35,6 +35,8 @@ CASES = {
     # the DVM flux: the wall side reconstructed with the SolidNeighbor's boundary slope
     "cyl16_dvm": lambda: cases.cylinder_2d(n=16, nv=12, flux="DVM"),
     "cyl16_vsamr_dvm": lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=1, flux="DVM"),
+    "sph8_dvm": lambda: cases.sphere_3d(n=8, nv=6, L=2.0, flux="DVM"),
+    "sph8_vsamr_dvm": lambda: cases.sphere_3d(n=8, nv=6, L=2.0, vs_levels=1, flux="DVM"),
 }
 
 
