@@ -220,6 +220,9 @@ struct Grid {
   __device__ double wt(int i) const { return at(gwb, i); }
   __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[off + i]); }
   __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
+  // where key(i) is stored (adapted grids): the overlap walks load it only when a
+  // neighbour has no overlap map
+  __device__ const uint32_t* keyp(int i) const { return UNI ? nullptr : A.gkey + off + i; }
   // f, sdf and the next f are stored as FT (double, or float in the FP32 form); the
   // accessors widen to double, and the FP32 form's arithmetic narrows where it advects
   __device__ double f(int i, int k) const { return double(at(fb, k * n + i)); }
@@ -266,8 +269,8 @@ __device__ inline void for_overlap(const Grid<D, NDF, UNI, FT>& G, int i, int li
 // and factors in the same order. Without a map (slot < 0) the search is done in place.
 template <int D, int NDF, bool UNI, class FT, class Fn>
 __device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, FT>& N, int c,
-                                     int n_own, int slot, int i, int li, uint32_t ki,
-                                     Fn&& fn) {
+                                     int n_own, int slot, int i, int li,
+                                     const uint32_t* kp, Fn&& fn) {
   if constexpr (UNI) {
     fn(i, 1.0, 1.0);
   } else {
@@ -284,7 +287,7 @@ __device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, F
       }
       return;
     }
-    for_overlap<D, NDF, UNI, FT>(N, i, li, ki, A.lmax, fn);
+    for_overlap<D, NDF, UNI, FT>(N, i, li, *kp, A.lmax, fn);
   }
 }
 
@@ -434,13 +437,13 @@ __device__ inline T div_ds(T a, double ds, double rds, float rds_f) {
 template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int cnt,
                                 double dsx, double rdsx, float rdsx_f, int i, int li,
-                                uint32_t ki, const T (&f)[NDF], T (&a)[NDF]) {
+                                const uint32_t* kp, const T (&f)[NDF], T (&a)[NDF]) {
 #pragma unroll
   for (int k = 0; k < NDF; ++k) a[k] = T(0);
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
     const Grid<D, NDF, UNI, FT> N(A, nb.cell);
-    for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, nb.map, i, li, ki,
+    for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, nb.map, i, li, kp,
                                    [&](int j, double den, double rden) {
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
@@ -459,7 +462,8 @@ __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int
 
 template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c, int d,
-                                 int i, int li, uint32_t ki, const T (&f)[NDF], T (&s)[NDF]) {
+                                 int i, int li, const uint32_t* kp, const T (&f)[NDF],
+                                 T (&s)[NDF]) {
   const SlopeDesc sd = A.sl[c * D + d];
   if (sd.mode == SL_SKIP) {
 #pragma unroll
@@ -476,12 +480,12 @@ __device__ inline void slope_dir(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
   }
   T aL[NDF];
   side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[0], sd.nb_cnt[0], sd.ds[0], sd.rds[0],
-                               sd.rds_f[0], i, li, ki, f, aL);
+                               sd.rds_f[0], i, li, kp, f, aL);
   const T ic0 = desc_val<T>(sd.inv_cnt[0], sd.inv_cnt_f[0]);
   if (sd.mode == SL_INNER) {
     T aR[NDF];
     side_sum<D, NDF, UNI, FT, T>(A, c, G.n, sd.nb_beg[1], sd.nb_cnt[1], sd.ds[1], sd.rds[1],
-                                 sd.rds_f[1], i, li, ki, f, aR);
+                                 sd.rds_f[1], i, li, kp, f, aR);
     const T ic1 = desc_val<T>(sd.inv_cnt[1], sd.inv_cnt_f[1]);
 #pragma unroll
     for (int k = 0; k < NDF; ++k) s[k] = minmod_sel(aL[k] * ic0, aR[k] * ic1);
@@ -584,9 +588,9 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
 #pragma unroll
     for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
     const int li = G.lev(i);
-    const uint32_t ki = G.key(i);
+    const uint32_t* kp = G.keyp(i);
 #pragma unroll
-    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, ki, f, s[d]);
+    for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, kp, f, s[d]);
     const CT wt = CT(G.wt(i));
 #pragma unroll
     for (int h = 0; h < MAXH; ++h) {
@@ -693,9 +697,9 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
 #pragma unroll
       for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
       const int li = G.lev(i);
-      const uint32_t ki = G.key(i);
+      const uint32_t* kp = G.keyp(i);
 #pragma unroll
-      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, ki, f, s[d]);
+      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, kp, f, s[d]);
       const CT wt = CT(G.wt(i));
 #pragma unroll
       for (int h = 0; h < MAXH; ++h) {
@@ -721,7 +725,7 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
 template <int D, int NDF, bool UNI, class FT, class T>
 __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, int ne, int i,
                                    const T (&u)[D], const T (&f)[NDF], const T (&s)[D][NDF],
-                                   int li, uint32_t ki, T (&fl)[NDF]) {
+                                   int li, const uint32_t* kp, T (&fl)[NDF]) {
   const T dt = T(A.dt);
   for (int e = 0; e < ne; ++e) {
     const EntryDesc& ed = A.ents[eb + e];
@@ -762,7 +766,7 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
       }
       default: {
         const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
-        for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, ki,
+        for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, kp,
                                        [&](int j, double den, double rden) {
           T uj[D], fj[NDF], sj[D][NDF];
 #pragma unroll
@@ -1227,7 +1231,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
         for (int d = 0; d < D; ++d) s[d][k] = CT(G.s(i, k, d));
         fl[k] = CT(A.vflux[G.off * NDF + int64_t(k) * n + i]);
       }
-      gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+      gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.keyp(i), fl);
 #pragma unroll
       for (int k = 0; k < NDF; ++k) A.vflux[G.off * NDF + int64_t(k) * n + i] = fl[k];
     }
@@ -1321,7 +1325,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 #pragma unroll
           for (int d = 0; d < D; ++d) s[d][k] = CT(G.s(i, k, d));
         }
-        gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.key(i), fl);
+        gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.keyp(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {

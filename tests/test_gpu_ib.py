@@ -171,7 +171,8 @@ def test_wall_carries_no_mass(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16", "sph12_nv8", "cyl16_vsamr"])
+@pytest.mark.parametrize("name", ["cyl16", "cyl32_nv16", "sph12_nv8", "cyl16_vsamr", "cyl16_dvm",
+                                  "sph8_vsamr_dvm", "cyl16_vsamr_cip"])
 def test_fp32_form(name):
     """FP32 form (f stored and advected in FP32) against the FP64 oracle after 3 steps:
     moments within 1e-5 of each cell's scale, the SolidNeighbor df within 1e-5."""
