@@ -49,5 +49,26 @@ int main() {
     if (!same(minmod(a, b), minmod_sel(a, b))) ++bad;
   }
   std::printf("minmod_sel: %ld of %ld differ\n", bad, n);
-  return bad ? 1 : 0;
+  // div_by (the reciprocal, a product and one FMA step on the exact residual) against IEEE
+  // division: quotients of operands whose exponents stay clear of overflow and underflow, as
+  // the kernels' divisions by cell sizes, 2 lambda and the shave's d are
+  long dbad = 0, dn = 0;
+  std::uniform_int_distribution<int> ea(-120, 40), eb(-40, 10);
+  std::uniform_int_distribution<uint64_t> mant(0, (uint64_t(1) << 52) - 1);
+  auto draw = [&](int e) {
+    double x = std::ldexp(1.0 + std::ldexp(double(mant(rng)), -52), e);
+    return (bits(rng) & 1) ? -x : x;
+  };
+  for (int t = 0; t < 20000000; ++t) {
+    double a = (t % 7 == 0) ? double(bits(rng) % 1000) * 0.001 - 0.5 : draw(ea(rng));
+    double b = draw(eb(rng));
+    ++dn;
+    if (!same(kah::div_by(a, b, 1.0 / b), a / b)) {
+      if (dbad < 5) std::printf("div_by(%a, %a) = %a, a/b = %a\n", a, b,
+                                kah::div_by(a, b, 1.0 / b), a / b);
+      ++dbad;
+    }
+  }
+  std::printf("div_by: %ld of %ld differ from a/b\n", dbad, dn);
+  return (bad || dbad) ? 1 : 0;
 }

@@ -1,6 +1,7 @@
 """Select-form point helpers of the kernels (kah_internal.h) equal the reference formulas
-they replace: minmod (src/Flux/Slope.jl:20-24) on special values and 2e6 random operands.
-Host build of the shared header with g++; no GPU."""
+they replace: minmod (src/Flux/Slope.jl:20-24) on special values and 2e6 random operands;
+div_by (reciprocal, product, one FMA correction) equal to IEEE division on 2e7 random
+quotients. Host build of the shared header with g++; no GPU."""
 import subprocess
 from pathlib import Path
 
