@@ -154,6 +154,67 @@ def plan_coarsen(topo, flags, dim):
     return with_grids(topo, grids, dim), np.concatenate(srcs), np.array(changed, dtype=np.uint8)
 
 
+def grid_keys(topo, lev, mid, dim):
+    """cellkey (src/Velocity_space/Neighbor.jl:61-124: root_lin * 2^(DIM*Lmax) + morton of
+    the lower corner on the finest lattice) of the velocity cells of one grid."""
+    L = int(topo["vs_maxlevel"])
+    lo = np.array([topo["vs_lo"][d] for d in range(dim)])
+    h_fine = root_ds(topo, dim) / 2.0 ** L
+    cell = h_fine[None, :] * (2.0 ** (L - lev.astype(np.int64)))[:, None]
+    g = np.round((mid - 0.5 * cell - lo[None, :]) / h_fine[None, :]).astype(np.int64)
+    stride = 1
+    root_lin = np.zeros(len(lev), dtype=np.int64)
+    morton = np.zeros(len(lev), dtype=np.int64)
+    for d in range(dim):
+        root_lin += (g[:, d] >> L) * stride
+        stride *= int(topo["vs_trees_num"][d])
+        q = g[:, d] & ((1 << L) - 1)
+        for b in range(L):
+            morton |= ((q >> b) & 1) << (b * dim + d)
+    return (root_lin << (dim * L)) | morton
+
+
+def balance_flags(topo, dim):
+    """Refine flags of one balance round (vs_balance_local!, src/Velocity_space/Balance.jl:
+    72-104): a velocity cell of a fluid cell is flagged when a velocity cell of one of its
+    fluid face neighbours that lies inside it is two or more levels finer (vs_balance_pair!,
+    :12-64, takes the maximum neighbour level over the cell's extent). Faces to a domain
+    boundary or to a solid cell (SolidNeighbor) are skipped, as are cells inside a solid.
+    Every flagged cell has to be refined in any balanced refinement of the current grids, so
+    repeating rounds reaches the same grids as the reference's cell-by-cell passes."""
+    off = np.asarray(topo["vs_offset"])
+    benc = np.asarray(topo["bound_enc"])
+    L = int(topo["vs_maxlevel"])
+    grids = cell_grids(topo, dim)
+    keys = [grid_keys(topo, g[0], g[1], dim) for g in grids]
+    need = [np.full(len(g[0]), -1, dtype=np.int64) for g in grids]
+    here = np.asarray(topo["pair_here"])
+    there = np.asarray(topo["pair_there"])
+    for h, t in zip(here, there):
+        if t < 0 or benc[h] < 0 or benc[t] < 0:
+            continue
+        for a, b in ((int(h), int(t)), (int(t), int(h))):
+            ka, la = keys[a], grids[a][0].astype(np.int64)
+            kb, lb = keys[b], grids[b][0].astype(np.int64)
+            own = np.searchsorted(ka, kb, side="right") - 1
+            inside = (own >= 0)
+            own_c = np.where(inside, own, 0)
+            inside &= kb < ka[own_c] + (np.int64(1) << (dim * (L - la[own_c])))
+            np.maximum.at(need[a], own_c[inside], lb[inside])
+    flags = np.zeros(int(off[-1]), dtype=np.uint8)
+    for c, g in enumerate(grids):
+        if benc[c] < 0:
+            continue
+        flags[off[c]:off[c + 1]] = need[c] > g[0].astype(np.int64) + 1
+    return flags
+
+
+def plan_balance(topo, dim):
+    """New grids, sources and changed cells of one balance round (refine_grid_stream! on the
+    flagged cells, the parent's f injected)."""
+    return plan_refine(topo, balance_flags(topo, dim), dim)
+
+
 def _inside_rows(topo):
     """Velocity rows of the cells inside a solid (InsideSolidData: no velocity space in the
     reference, so never adapted), or None."""
@@ -168,11 +229,13 @@ def _inside_rows(topo):
     return np.concatenate([np.arange(off[c], off[c + 1]) for c in np.flatnonzero(deep)])
 
 
-def vs_adapt(dev, topo, params, dim):
-    """One vs_adaptive_mesh_refinement! (without vs_balance) on a context: returns the new
-    topology and the per-cell changed flags. vr is computed once, before refining, and used
-    by both passes, as the reference does. Cells inside an immersed solid keep their grids
-    (src/Velocity_space/AMR.jl:47, :93)."""
+def vs_adapt(dev, topo, params, dim, balance=False):
+    """One vs_adaptive_mesh_refinement!(ka; vs_balance=balance) on a context: returns the new
+    topology and the per-cell changed flags of the refine and coarsen passes. vr is computed
+    once, before refining, and used by both passes, as the reference does. Cells inside an
+    immersed solid keep their grids (src/Velocity_space/AMR.jl:47, :93). With balance, the
+    balance rounds run after the coarsen pass and before the conservative correction, which
+    (as in the reference) projects only the cells the refine and coarsen passes changed."""
     vr = dev.vs_resolution()
     flags = dev.vs_flags(params, vr)
     rows = _inside_rows(topo)
@@ -190,6 +253,12 @@ def vs_adapt(dev, topo, params, dim):
     t2, src2, ch2 = plan_coarsen(t1, flags, dim)
     # coarsen_grid_stream! rebuilds every grid, so every cell's slopes are zero afterwards
     dev.vs_remap(t2, src2, np.ones(len(ch2), dtype=np.uint8))
+    while balance:
+        t3, src3, ch3 = plan_balance(t2, dim)
+        if not ch3.any():
+            break
+        dev.vs_remap(t3, src3, ch3)
+        t2 = t3
     va = (ch1 | ch2).astype(bool)
     dev.vs_project(np.flatnonzero(va).astype(np.int32))
     return t2, va

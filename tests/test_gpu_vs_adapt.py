@@ -72,12 +72,13 @@ def _remap(f, off_old, off_new, src, ndf, nch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["lohner", "contribution"])
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_adaptation_equals_oracle_sequence(name, mode):
+@pytest.mark.parametrize("name,balance", [(n, False) for n in sorted(CASES)] +
+                         [("riemann8_vsamr", True)])
+def test_adaptation_equals_oracle_sequence(name, mode, balance):
     case = CASES[name]()
     dim, ndf = case.config.solver.DIM, case.config.solver.NDF
     nch = 1 << dim
-    say(f"[vs] adapt {name} {mode}")
+    say(f"[vs] adapt {name} {mode} balance={balance}")
     params = kah_params(case.config)
     dev, orc = pair(case)
     for s in range(2):
@@ -85,7 +86,7 @@ def test_adaptation_equals_oracle_sequence(name, mode):
     st0 = _sync_oracle(dev, orc)
     p = abi.vs_adapt_params(mode, coeff_local=1e-2 if mode == "lohner" else 1e-3)
     # the device path
-    topo2, va = vsadapt.vs_adapt(dev, case.topo, p, dim)
+    topo2, va = vsadapt.vs_adapt(dev, case.topo, p, dim, balance=balance)
     got = dev.get_state(sdf=True)
     # the same sequence with the oracle's criteria, numpy remaps and the oracle's projection
     vr = orc.vs_resolution()
@@ -104,8 +105,18 @@ def test_adaptation_equals_oracle_sequence(name, mode):
     o1.set_sdf(sdf1)
     f2 = o1.vs_flags(p, vr)
     t2, src2, ch2 = vsadapt.plan_coarsen(t1, f2, dim)
-    assert np.array_equal(np.asarray(t2["vs_offset"]), np.asarray(topo2["vs_offset"]))
     df2 = _remap(df1, off1, np.asarray(t2["vs_offset"]), src2, ndf, nch)
+    rounds = 0
+    while balance:   # vs_balance!: injection onto the balanced grids, before the projection
+        t3, src3, ch3 = vsadapt.plan_balance(t2, dim)
+        if not ch3.any():
+            break
+        df2 = _remap(df2, np.asarray(t2["vs_offset"]), np.asarray(t3["vs_offset"]), src3, ndf,
+                     nch)
+        t2, rounds = t3, rounds + 1
+    assert rounds > 0 or not balance, "the case does not exercise the balance"
+    assert np.array_equal(np.asarray(t2["vs_offset"]), np.asarray(topo2["vs_offset"]))
+    assert np.array_equal(np.asarray(t2["vs_level"]), np.asarray(topo2["vs_level"]))
     va_ref = (ch1 | ch2).astype(bool)
     assert np.array_equal(va, va_ref)
     assert va.any(), "no cell changed: the case does not exercise the adaptation"

@@ -182,3 +182,90 @@ def test_oracle_resolution_is_the_peak_contribution():
     dmax = max(st["df"][off[c] * 2:off[c + 1] * 2].max() for c in range(case.n_cells))
     assert vr[0] == dmax * wf
     o.close()
+
+
+def _boxes(topo, c, dim):
+    lev, mid, _ = vsadapt.cell_grids(topo, dim)[c]
+    h = vsadapt.root_ds(topo, dim)[None, :] / 2.0 ** lev.astype(np.float64)[:, None]
+    return lev.astype(np.int64), mid - 0.5 * h, mid + 0.5 * h
+
+
+def _fluid_pairs(topo):
+    benc = np.asarray(topo["bound_enc"])
+    return [(int(h), int(t)) for h, t in zip(topo["pair_here"], topo["pair_there"])
+            if t >= 0 and benc[h] >= 0 and benc[t] >= 0]
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_balance_is_a_forced_two_to_one_refinement(levels):
+    """vs_balance! (src/Velocity_space/Balance.jl): pinned by geometry, with overlaps found by
+    brute force over velocity-cell boxes. After the rounds (a) no two overlapping velocity
+    cells of face-neighbouring fluid cells differ by more than one level; (b) every grid
+    refines its original (each original cell is tiled by cells at least as fine, weights
+    adding up) and each new cell's source is the original cell holding it (injection);
+    (c) every refinement was forced: the parent box of each cell made by balance overlaps a
+    neighbour's cell at least two levels finer than that parent."""
+    dim = 2
+    case = cases.riemann_2d(n=8, nv=8, vs_levels=levels)
+    t0 = case.topo
+    t, rounds = t0, 0
+    while True:
+        t1, src, ch = vsadapt.plan_balance(t, dim)
+        if not ch.any():
+            break
+        off0, off1 = np.asarray(t["vs_offset"]), np.asarray(t1["vs_offset"])
+        for c in np.flatnonzero(ch):
+            _, lo0, hi0 = _boxes(t, c, dim)
+            _, lo1, hi1 = _boxes(t1, c, dim)
+            s = src[off1[c]:off1[c + 1]]
+            assert np.all(lo1 >= lo0[s] - 1e-12) and np.all(hi1 <= hi0[s] + 1e-12)
+        t, rounds = t1, rounds + 1
+    assert rounds >= 1
+    pairs = _fluid_pairs(t)
+    assert pairs
+    # (a) balanced
+    for h, n in pairs:
+        la, loa, hia = _boxes(t, h, dim)
+        lb, lob, hib = _boxes(t, n, dim)
+        ov = np.all((loa[:, None, :] < hib[None, :, :] - 1e-12) &
+                    (lob[None, :, :] < hia[:, None, :] - 1e-12), axis=2)
+        ia, ib = np.nonzero(ov)
+        assert np.all(np.abs(la[ia] - lb[ib]) <= 1), (h, n)
+    # (b) a refinement of the original grids
+    for c in range(case.n_cells):
+        l0, lo0, hi0 = _boxes(t0, c, dim)
+        l1, lo1, hi1 = _boxes(t, c, dim)
+        w0 = vsadapt.cell_grids(t0, dim)[c][2]
+        w1 = vsadapt.cell_grids(t, dim)[c][2]
+        inside = np.all((lo1[:, None, :] >= lo0[None, :, :] - 1e-12) &
+                        (hi1[:, None, :] <= hi0[None, :, :] + 1e-12), axis=2)
+        assert np.all(inside.sum(axis=1) == 1), c
+        home = inside.argmax(axis=1)
+        assert np.all(l1 >= l0[home]), c
+        assert np.allclose(np.bincount(home, weights=w1, minlength=len(w0)), w0, rtol=1e-13)
+    # (c) forced
+    nbrs = {}
+    for h, n in pairs:
+        nbrs.setdefault(h, set()).add(n)
+        nbrs.setdefault(n, set()).add(h)
+    made = 0
+    lo_v = np.asarray(t0["vs_lo"][:dim], dtype=np.float64)
+    ds = vsadapt.root_ds(t0, dim)
+    for c in range(case.n_cells):
+        l0, lo0, hi0 = _boxes(t0, c, dim)
+        l1, lo1, hi1 = _boxes(t, c, dim)
+        inside = np.all((lo1[:, None, :] >= lo0[None, :, :] - 1e-12) &
+                        (hi1[:, None, :] <= hi0[None, :, :] + 1e-12), axis=2)
+        new = np.flatnonzero(l1 > l0[inside.argmax(axis=1)])
+        for i in new:
+            hp = ds / 2.0 ** (l1[i] - 1)
+            plo = lo_v + np.floor((lo1[i] - lo_v) / hp + 1e-9) * hp
+            phi = plo + hp
+            forced = False
+            for n in nbrs.get(c, ()):
+                lb, lob, hib = _boxes(t, n, dim)
+                ov = np.all((plo[None, :] < hib - 1e-12) & (lob < phi[None, :] - 1e-12), axis=1)
+                forced |= bool(np.any(lb[ov] >= l1[i] + 1))
+            assert forced, (c, i)
+            made += 1
+    assert made > 0
