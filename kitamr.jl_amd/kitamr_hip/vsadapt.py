@@ -5,8 +5,9 @@ evaluates the criteria (``kah_vs_resolution`` / ``kah_vs_flags``), this module r
 grids from one flag byte per velocity cell (refine_grid_stream! / coarsen_grid_stream!,
 src/Velocity_space/Rebuild.jl:44-163, on levels, midpoints and weights only, plus the source
 of every new velocity cell), the device remaps f onto the new grids (``kah_vs_remap``) and
-applies the conservative I-projection to the changed cells (``kah_vs_project``). f never
-leaves HBM.
+applies the conservative I-projection to the changed cells (``kah_vs_project``); the
+optional vs_balance! rounds (src/Velocity_space/Balance.jl) plan on the grids and inject f on
+the device the same way. f never leaves HBM.
 """
 from __future__ import annotations
 

@@ -4,8 +4,9 @@ kah_vs_project) against the oracle.
 With the oracle holding exactly the device's state (w, prim, f and slopes), the resolution
 and every velocity cell's refine / coarsen flag agree bit for bit (both criteria modes, 2D
 and 3D, adapted grids, FP64 and the FP32 form). A whole adaptation (refine pass, coarsen
-pass, conservative I-projection of the changed cells) equals the same sequence done with the
-oracle's flags, numpy remaps of the downloaded f and the oracle's I-projection: unprojected
+pass, the vs_balance! rounds where asked, conservative I-projection of the changed cells)
+equals the same sequence done with the oracle's flags, numpy remaps of the downloaded f and
+the oracle's I-projection: unprojected
 cells bit for bit, projected ones within 1e-12 (FP64). The adapted state then steps in parity
 with the oracle on the new grids.
 """

@@ -5,7 +5,8 @@ refine_grid_stream!); coarsening a refined grid with every cell willing restores
 a group is only merged when all 2^dim siblings are willing and aligned; mass and energy
 moments of a remapped f are preserved by refine (injection keeps sum(weight*f)) and by coarsen
 (the mean times 2^dim weight); the oracle's criteria agree with a brute-force restatement
-whose face neighbours come from a geometric search."""
+whose face neighbours come from a geometric search; the vs_balance! rounds leave balanced,
+forced refinements of the original grids (checked on velocity-cell boxes)."""
 import numpy as np
 import pytest
 
