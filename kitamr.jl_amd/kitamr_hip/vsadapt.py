@@ -216,6 +216,18 @@ def plan_balance(topo, dim):
     return plan_refine(topo, balance_flags(topo, dim), dim)
 
 
+def balance_rounds(dev, topo, dim):
+    """vs_balance! on a context: balance rounds until one flags nothing, each refining on the
+    host and injecting f on the device (``kah_vs_remap``). Returns (topology, rounds)."""
+    rounds = 0
+    while True:
+        t, src, ch = plan_balance(topo, dim)
+        if not ch.any():
+            return topo, rounds
+        dev.vs_remap(t, src, ch)
+        topo, rounds = t, rounds + 1
+
+
 def _inside_rows(topo):
     """Velocity rows of the cells inside a solid (InsideSolidData: no velocity space in the
     reference, so never adapted), or None."""
@@ -254,12 +266,8 @@ def vs_adapt(dev, topo, params, dim, balance=False):
     t2, src2, ch2 = plan_coarsen(t1, flags, dim)
     # coarsen_grid_stream! rebuilds every grid, so every cell's slopes are zero afterwards
     dev.vs_remap(t2, src2, np.ones(len(ch2), dtype=np.uint8))
-    while balance:
-        t3, src3, ch3 = plan_balance(t2, dim)
-        if not ch3.any():
-            break
-        dev.vs_remap(t3, src3, ch3)
-        t2 = t3
+    if balance:
+        t2, _ = balance_rounds(dev, t2, dim)
     va = (ch1 | ch2).astype(bool)
     dev.vs_project(np.flatnonzero(va).astype(np.int32))
     return t2, va

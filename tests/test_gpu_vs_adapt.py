@@ -22,6 +22,11 @@ CASES = {
     "riemann8_vsamr": lambda: cases.riemann_2d(n=8, nv=8, vs_levels=2),
     "box3d_vsamr": lambda: cases.box_3d(n=4, nv=6, vs_levels=1),
 }
+# grids whose neighbours break the 2:1 velocity-level balance as built
+BALANCE_CASES = {
+    "riemann8_vsamr3": lambda: cases.riemann_2d(n=8, nv=8, vs_levels=3),
+    "box3d_vsamr2": lambda: cases.box_3d(n=4, nv=6, vs_levels=2),
+}
 
 
 def _sync_oracle(dev, orc):
@@ -203,4 +208,47 @@ def test_remap_rejects_bad_sources():
     after = dev.get_state()
     assert np.array_equal(before["df"], after["df"]) and np.array_equal(before["w"], after["w"])
     dev.step(case.dt_xi, 0)
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32f"])
+@pytest.mark.parametrize("name", sorted(BALANCE_CASES))
+def test_balance_rounds_inject_on_the_device(name, precision):
+    """vs_balance! alone on grids that break the balance as built (2D: three rounds, 3D: one):
+    the device's f after the rounds equals the numpy injection of the downloaded f onto the
+    same host-planned grids, bit for bit, and the balanced state steps in parity with the
+    oracle."""
+    case = BALANCE_CASES[name]()
+    dim, ndf = case.config.solver.DIM, case.config.solver.NDF
+    say(f"[vs] balance {name} {precision}")
+    dev, _ = pair(case, precision=abi.PREC[precision])
+    for s in range(2):
+        dev.step(case.dt_xi, s)
+    st0 = dev.get_state()
+    topo, rounds = vsadapt.balance_rounds(dev, case.topo, dim)
+    assert rounds >= 1
+    got = dev.get_state()
+    t, df, r = case.topo, st0["df"], 0
+    while True:
+        t1, src, ch = vsadapt.plan_balance(t, dim)
+        if not ch.any():
+            break
+        df = _remap(df, np.asarray(t["vs_offset"]), np.asarray(t1["vs_offset"]), src, ndf,
+                    1 << dim)
+        t, r = t1, r + 1
+    assert r == rounds
+    assert np.array_equal(np.asarray(t["vs_offset"]), np.asarray(topo["vs_offset"]))
+    assert np.array_equal(got["df"], df)
+    assert np.array_equal(got["w"], st0["w"])
+    if precision == "fp64":
+        o = O.Oracle(kah_params(case.config), topo)
+        o.set_state(got["w"], got["prim"], got["df"])
+        for s in range(2):
+            dev.step(case.dt_xi, 2 + s)
+            o.step(case.dt_xi, 2 + s)
+        g, oo = dev.get_state(), o.get_state()
+        assert rel_err_rows(g["w"], oo["w"], dim + 2) < 1e-12
+        assert rel_err(g["df"], oo["df"]) < 1e-12
+        o.close()
     dev.close()
