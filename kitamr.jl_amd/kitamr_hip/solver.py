@@ -94,6 +94,12 @@ class PCoordFn:
 
 
 @dataclass
+class PCoordField:
+    """A PCoordFn evaluated on all cells at once: fn(midpoints [n, DIM]) -> prim [n, DIM+2]."""
+    fn: Callable
+
+
+@dataclass
 class Configure:
     solver: Solver
     gas: Gas
@@ -135,6 +141,38 @@ def discrete_maxwell(mid, prim, K):
     return (prim[0] * (prim[4] / PI) ** 1.5 * np.exp(-prim[4] * c2))[None, :]
 
 
+def discrete_maxwell_cells(grids, prim, K, ndf):
+    """discrete_maxwell of every cell's prim on its own grid, concatenated in cell order
+    ([NDF, n_c] blocks), evaluated in chunks of cells with the same arithmetic as
+    discrete_maxwell."""
+    out = []
+    n = len(grids)
+    c0 = 0
+    while c0 < n:
+        c1 = c0 + 1
+        tot = grids[c0].n
+        while c1 < n and tot < (1 << 24):
+            tot += grids[c1].n
+            c1 += 1
+        sizes = np.array([grids[c].n for c in range(c0, c1)])
+        mid = np.concatenate([grids[c].midpoint for c in range(c0, c1)])
+        p = np.repeat(prim[c0:c1], sizes, axis=0)
+        dim = mid.shape[1]
+        c2 = (mid[:, 0] - p[:, 1]) * (mid[:, 0] - p[:, 1])
+        for d in range(1, dim):
+            c2 = c2 + (mid[:, d] - p[:, 1 + d]) * (mid[:, d] - p[:, 1 + d])
+        if dim == 2:
+            h = p[:, 0] * (p[:, 3] / PI) * np.exp(-p[:, 3] * c2)
+            blocks = np.stack([h, h * K / (2.0 * p[:, 3])])
+        else:
+            blocks = (p[:, 0] * (p[:, 4] / PI) ** 1.5 * np.exp(-p[:, 4] * c2))[None, :]
+        ends = np.cumsum(sizes)
+        for k in range(len(sizes)):
+            out.append(blocks[:, ends[k] - sizes[k]:ends[k]].reshape(-1))
+        c0 = c1
+    return np.concatenate(out) if out else np.zeros(0)
+
+
 # ---------------------------------------------------------------- case construction
 @dataclass
 class Case:
@@ -172,12 +210,14 @@ def dt_xi(config):
     return s.CFL * min(ds[i] / U[i] for i in range(dim))
 
 
-def build_case(config, ps_refine=None, vs_grid_fn=None, ps_max_level=None):
+def build_case(config, ps_refine=None, vs_grid_fn=None, ps_max_level=None, vs_grids_fn=None):
     """The flattened state initialize(config) would hand to the device.
 
     ps_refine(mid, ds, level) -> bool[] marks leaves to refine (static pre-refinement, then
     FULL 2:1 balance); vs_grid_fn(cell, prim, base_grid) -> VelocityGrid gives adapted
-    per-cell velocity grids (None: every cell carries the base grid).
+    per-cell velocity grids (None: every cell carries the base grid); vs_grids_fn(prim [n,
+    DIM+2], mid [n, DIM], base_grid) -> list of VelocityGrid does the same for all cells at
+    once (for large meshes).
     """
     s = config.solver
     dim = s.DIM
@@ -208,13 +248,15 @@ def build_case(config, ps_refine=None, vs_grid_fn=None, ps_max_level=None):
     # initial primitive state, initialize_ps! (src/Solver/Initialize.jl:343-380)
     if isinstance(config.IC, Uniform):
         prim = np.tile(np.asarray(config.IC.ic, dtype=np.float64), (n, 1))
+    elif isinstance(config.IC, PCoordField):
+        prim = np.ascontiguousarray(config.IC.fn(np.asarray(mid)), dtype=np.float64)
     else:
         prim = np.array([config.IC.fn(mid[c]) for c in range(n)], dtype=np.float64)
     w = get_conserved(prim, config.gas.gamma)
     base = VelocityGrid(dim, config.quadrature, config.vs_trees_num, s.AMR_VS_MAXLEVEL)
     ndf = s.NDF
     grids = []
-    if vs_grid_fn is None:
+    if vs_grid_fn is None and vs_grids_fn is None:
         n0 = base.n
         topo["vs_uniform"] = 1
         topo["vs_offset"] = np.arange(n + 1, dtype=np.int64) * n0
@@ -227,6 +269,15 @@ def build_case(config, ps_refine=None, vs_grid_fn=None, ps_max_level=None):
         for k, p in enumerate(uniq):
             df[inv.reshape(-1) == k] = discrete_maxwell(base.midpoint, p, config.gas.K)
         df = df.reshape(-1)
+    elif vs_grids_fn is not None:
+        grids = vs_grids_fn(prim, np.asarray(mid), base)
+        sizes = np.array([g.n for g in grids], dtype=np.int64)
+        topo["vs_uniform"] = 0
+        topo["vs_offset"] = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        topo["vs_level"] = np.concatenate([g.level for g in grids]).astype(np.int8)
+        topo["vs_mid"] = np.concatenate([g.mid_colmajor() for g in grids])
+        topo["vs_weight"] = np.concatenate([g.weight for g in grids])
+        df = discrete_maxwell_cells(grids, prim, config.gas.K, ndf)
     else:
         off = [0]
         lev, mids, wts, dfs = [], [], [], []

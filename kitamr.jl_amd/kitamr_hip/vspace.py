@@ -92,23 +92,18 @@ class VelocityGrid:
         if (self.level[flags] >= self.maxlevel).any():
             raise ValueError("refinement beyond AMR_VS_MAXLEVEL")
         nc = 1 << self.dim
-        lv, mid, wt = [], [], []
-        for i in range(self.n):
-            if flags[i]:
-                L = int(self.level[i])
-                wi = self.weight[i] / nc
-                for c in range(nc):
-                    lv.append(L + 1)
-                    wt.append(wi)
-                    mid.append([self.midpoint[i, d] + 0.5 * (self.ds[d] / 2 ** (L + 1)) * RMT[self.dim][c, d]
-                                for d in range(self.dim)])
-            else:
-                lv.append(self.level[i])
-                wt.append(self.weight[i])
-                mid.append(list(self.midpoint[i]))
-        self.level = np.asarray(lv, dtype=np.int8)
-        self.weight = np.asarray(wt, dtype=np.float64)
-        self.midpoint = np.asarray(mid, dtype=np.float64).reshape(-1, self.dim)
+        counts = np.where(flags, nc, 1)
+        src = np.repeat(np.arange(self.n), counts)
+        first = np.repeat(np.cumsum(counts) - counts, counts)
+        child = np.arange(len(src)) - first
+        ref = flags[src]
+        L = self.level[src].astype(np.int64)
+        # child midpoint = parent + 0.5*(ds/2^(L+1))*RMT[child], rounded in that order
+        step = 0.5 * (self.ds[None, :] / (2.0 ** (L + 1))[:, None])
+        mid = self.midpoint[src]
+        self.midpoint = np.where(ref[:, None], mid + step * RMT[self.dim][child % nc], mid)
+        self.weight = np.where(ref, self.weight[src] / nc, self.weight[src])
+        self.level = (L + ref).astype(np.int8)
         return self
 
     def keys(self):
@@ -148,5 +143,103 @@ def refine_toward(grid, prim, rounds, frac=0.3, rng=None):
         flags &= g.level < g.maxlevel
         if not flags.any():
             break
+        g.refine(flags)
+    return g
+
+
+# ---------------------------------------------------------------- initial adapted grids
+MAXWELLIAN_INIT_FLOOR = 1e-3   # src/Velocity_space/Criteria.jl:123
+VS_LOHNER_EPS_ABS = 1e-3       # src/Velocity_space/Criteria.jl:214
+
+
+def _maxwellian_1d(c, du, lam):
+    """_maxwellian_1d (src/Velocity_space/Criteria.jl:126-132): the integrals of exp(-lam s^2)
+    and s^2 exp(-lam s^2) over [c - du/2, c + du/2], vectorised over c and du."""
+    from scipy.special import erf
+    sq = np.sqrt(lam)
+    a = c - 0.5 * du
+    b = c + 0.5 * du
+    I0 = 0.5 * np.sqrt(np.pi / lam) * (erf(sq * b) - erf(sq * a))
+    I2 = (a * np.exp(-lam * a * a) - b * np.exp(-lam * b * b)) / (2 * lam) + I0 / (2 * lam)
+    return I0, I2
+
+
+def maxwellian_refine_flags(grid, prim, K, mode="lohner", coeffi_lohner=0.6):
+    """maxwellian_refine_flag (src/Velocity_space/Criteria.jl:165-198) of every cell of the
+    grid, level < maxlevel: refine where the exact cell mass or energy of the Maxwellian of
+    prim is above MAXWELLIAN_INIT_FLOOR of the total, or (lohner mode) where the midpoint rule
+    misses the cell mass by more than coeffi_lohner (maxwellian_quad_error, :144-157)."""
+    dim = grid.dim
+    lam, rho = float(prim[-1]), float(prim[0])
+    U = np.asarray(prim[1:1 + dim], dtype=np.float64)
+    du = grid.ds[None, :] / (2.0 ** grid.level.astype(np.float64))[:, None]
+    I0 = np.empty((grid.n, dim))
+    I2 = np.empty((grid.n, dim))
+    for d in range(dim):
+        I0[:, d], I2[:, d] = _maxwellian_1d(grid.midpoint[:, d] - U[d], du[:, d], lam)
+    pref = rho * (lam / np.pi) ** (dim / 2)
+    prod0 = I0[:, 0].copy()
+    for d in range(1, dim):
+        prod0 *= I0[:, d]
+    drho = pref * prod0
+    dc2 = np.zeros(grid.n)
+    for d in range(dim):
+        p = np.ones(grid.n)
+        for e in range(dim):
+            if e != d:
+                p *= I0[:, e]
+        dc2 += I2[:, d] * p
+    dc2 *= pref
+    if K is not None:    # 2D2F: the b population carries K/(2 lam) of internal energy
+        dE = 0.5 * (dc2 + (K / (2 * lam)) * drho)
+        eint = rho * (dim + K) / (4 * lam)
+    else:
+        dE = 0.5 * dc2
+        eint = rho * dim / (4 * lam)
+    flags = np.maximum(drho / rho, dE / eint) > MAXWELLIAN_INIT_FLOOR
+    if mode == "lohner":
+        V = np.prod(du, axis=1)
+        c2 = np.sum((grid.midpoint - U[None, :]) ** 2, axis=1)
+        mmid = pref * np.exp(-lam * c2)
+        err = np.abs(drho - mmid * V) / np.maximum(drho, VS_LOHNER_EPS_ABS * pref * V)
+        flags |= err > coeffi_lohner
+    return flags & (grid.level < grid.maxlevel)
+
+
+def initial_vs_grid(base, prim, K, mode="lohner", coeffi_lohner=0.6):
+    """initialize_vs_data's adapted grid (src/Velocity_space/Initialize.jl:31-34): maxlevel
+    rounds of initial_vs_adaptive_mesh_refinement! (src/Velocity_space/AMR.jl:183-200) on the
+    Maxwellian of prim. K is the gas's internal degrees of freedom for 2D2F, None for 3D1F."""
+    g = base.copy()
+    for _ in range(g.maxlevel):
+        fl = maxwellian_refine_flags(g, prim, K, mode, coeffi_lohner)
+        if not fl.any():
+            break
+        g.refine(fl)
+    return g
+
+
+def refine_to_target(grid, prim, n_target, rng):
+    """Seeded refinement of a grid toward the Maxwellian peak of prim until it holds at least
+    n_target cells (or no cell can be refined): round by round the unrefined cells carrying
+    the largest Maxwellian mass (with a seeded 25 % jitter) are split, as many as the missing
+    count needs. A workload generator, not a reference criterion."""
+    rng = np.random.default_rng(rng)
+    g = grid.copy()
+    dim = g.dim
+    nch = (1 << dim) - 1
+    U = np.asarray(prim[1:1 + dim], dtype=np.float64)
+    lam = float(prim[-1])
+    while g.n < n_target:
+        cand = g.level < g.maxlevel
+        if not cand.any():
+            break
+        c2 = np.sum((g.midpoint - U[None, :]) ** 2, axis=1)
+        pri = np.exp(-lam * c2) * g.weight * (1.0 + 0.25 * rng.random(g.n))
+        pri[~cand] = -1.0
+        k = min(int(cand.sum()), -(-(n_target - g.n) // nch))
+        sel = np.argpartition(-pri, k - 1)[:k]
+        flags = np.zeros(g.n, dtype=bool)
+        flags[sel] = True
         g.refine(flags)
     return g

@@ -1,0 +1,132 @@
+"""The stated workloads on the device.
+
+- C1 (the CPU-runnable configuration): the 2D four-quadrant Riemann problem on exactly 64x64
+  leaves with a 28x28 velocity grid on [-5,5]^2, Kn = 1e-2, omega = 0.5, CFL 0.8, FP64,
+  CAIDVM + CAIDVM_Marching, 100 fused steps against the oracle at 1e-12 (relative, per
+  array / per cell row), with the residual sums of the check steps.
+- C3 (the velocity-adaptive cylinder, 200,704 leaves, grids of 568-4000 cells): one step of
+  the FP64 form against the oracle at full size (w, prim per cell at 1e-12); the FP32 form's
+  moments within 1e-5 (per cell, relative) of the FP64 form's after 3 steps; and, with the
+  box closed by Maxwellian walls, the total mass kept to 1e-12 in both forms over 3 steps
+  (every face's mass flux enters its two cells with opposite signs and the wall densities
+  zero the walls' mass fluxes, so the total changes by round-off only).
+"""
+import numpy as np
+import pytest
+
+from _util import pair, rel_err, rel_err_rows, say
+from kitamr_hip import abi, cases
+from kitamr_hip.solver import kah_params
+
+TOL = 1e-12
+
+
+@pytest.mark.gpu
+def test_c1_riemann_64x64_28x28_100_steps():
+    case = cases.riemann_2d(n=64, nv=28)
+    assert case.n_cells == 4096 and case.phase_cells == 4096 * 784
+    say(f"[C1] cells={case.n_cells} P={case.phase_cells} dt={case.dt_xi:.6e}")
+    assert abs(case.dt_xi - 0.8 * (1 / 64) / (5 - 10 / 56)) < 1e-15
+    dev, orc = pair(case)
+    dt = case.dt_xi
+    for s in range(100):
+        check = (s + 1) % 20 == 0
+        rg = dev.step(dt, s, residual=check)
+        ro = orc.step(dt, s, residual=check)
+        if check:
+            assert rel_err(rg[0], ro[0]) <= 1e-10 and rel_err(rg[1], ro[1]) <= TOL, s
+            g, o = dev.get_state(), orc.get_state()
+            e = {k: rel_err(g[k], o[k]) if k == "df" else rel_err_rows(g[k], o[k], 4)
+                 for k in ("df", "w", "prim")}
+            say(f"[C1] step {s + 1}: {e}")
+            assert max(e.values()) <= TOL, (s, e)
+    dev.close()
+    orc.close()
+
+
+_C3 = {}
+
+
+def _c3(precision="fp64", walls=False):
+    """The C3 case (built once per boundary kind; the storage form only changes the
+    parameters)."""
+    import dataclasses
+    if walls not in _C3:
+        _C3.clear()    # one full-size case held at a time
+        _C3[walls] = cases.cylinder_c3(n=448, walls=walls)
+    c = _C3[walls]
+    return dataclasses.replace(c, config=dataclasses.replace(c.config, precision=precision))
+
+
+def _dev(case):
+    d = abi.HipContext(kah_params(case.config))
+    d.set_topology(case.topo)
+    d.set_state(case.w, case.prim, case.df)
+    return d
+
+
+@pytest.mark.gpu
+def test_c3_full_size_step_fp64_equals_oracle():
+    case = _c3()
+    sizes = np.diff(case.topo["vs_offset"])
+    say(f"[C3] cells={case.n_cells} P={case.phase_cells} grids {sizes.min()}-{sizes.max()}")
+    assert sizes.min() >= 400 and sizes.max() <= 4000 and case.n_cells == 448 * 448
+    dev, orc = pair(case)
+    dev.step(case.dt_xi, 0, residual=False)
+    orc.step(case.dt_xi, 0, residual=False)
+    g, o = dev.get_state(qf=False), orc.get_state()
+    fluid = np.asarray(case.topo["bound_enc"]) >= 0
+    for k in ("w", "prim"):
+        e = rel_err_rows(np.asarray(g[k]).reshape(-1, 4)[fluid],
+                         np.asarray(o[k]).reshape(-1, 4)[fluid], 4)
+        say(f"[C3] one step {k}: {e:.3g}")
+        assert e <= TOL, k
+    e = rel_err(g["df"], o["df"])
+    say(f"[C3] one step df: {e:.3g}")
+    assert e <= TOL
+    dev.close()
+    orc.close()
+
+
+@pytest.mark.gpu
+def test_c3_fp32_form_moments_track_fp64():
+    out = {}
+    for prec in ("fp64", "fp32f"):
+        case = _c3(prec)
+        say(f"[C3] {prec}: cells={case.n_cells} P={case.phase_cells}")
+        d = _dev(case)
+        for s in range(3):
+            d.step(case.dt_xi, s, residual=False)
+        out[prec] = d.get_state(df=False)
+        d.close()
+    fluid = np.asarray(case.topo["bound_enc"]) >= 0
+    for k in ("w", "prim"):
+        a = out["fp32f"][k].reshape(-1, 4)[fluid]
+        b = out["fp64"][k].reshape(-1, 4)[fluid]
+        e = rel_err_rows(a, b, 4)
+        say(f"[C3] fp32 vs fp64 {k}: {e:.3g}")
+        assert e <= 1e-5, k
+    q32 = out["fp32f"]["qf"].reshape(-1, 2)[fluid]
+    q64 = out["fp64"]["qf"].reshape(-1, 2)[fluid]
+    e = float(np.max(np.abs(q32 - q64)) / np.max(np.abs(q64)))
+    say(f"[C3] fp32 vs fp64 q: {e:.3g}")
+    assert e <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32f"])
+def test_c3_closed_box_keeps_mass(precision):
+    case = _c3(precision, walls=True)
+    say(f"[C3 closed] {precision}: cells={case.n_cells} P={case.phase_cells}")
+    n = case.n_cells
+    V = np.prod(np.asarray(case.topo["cell_ds"])[:n].reshape(n, 2), axis=1)
+    fluid = np.asarray(case.topo["bound_enc"]) >= 0
+    d = _dev(case)
+    m0 = float((np.asarray(case.w).reshape(n, 4)[fluid, 0] * V[fluid]).sum())
+    for s in range(3):
+        d.step(case.dt_xi, s, residual=False)
+        w = d.get_state(df=False, prim=False, qf=False)["w"].reshape(n, 4)
+        m = float((w[fluid, 0] * V[fluid]).sum())
+        say(f"[C3 closed] {precision} step {s + 1}: relative mass change {(m - m0) / m0:.3g}")
+        assert abs(m - m0) <= 1e-12 * m0, (s, (m - m0) / m0)
+    d.close()
