@@ -363,14 +363,17 @@ def target_grids(prim, s, base, K, n_min, n_max, seed, workers=0):
 
 def cylinder_c3(n=448, nv=20, vmax=15.0, L=8.0, radius=0.5, Ma=5.0, Kn=0.1, omega=0.81,
                 vs_levels=2, n_min=400, n_max=4000, precision="fp64", flux="CAIDVM",
-                marching="CAIDVM_Marching", CFL=0.8, seed=20261202, workers=0, walls=False):
+                marching="CAIDVM_Marching", CFL=0.4, seed=20261202, workers=0, walls=False,
+                immersed=True):
     """The velocity-adaptive cylinder (example/cylinder/cylinder.jl): Circle(Maxwellian,
     [0,0], radius, true, ., [1,0,0,1]) in [-L, L]^2 on n x n leaves, supersonic inflow at xmin,
     outflow elsewhere, a developed bow-shock state (bow_shock_state) and per-cell adapted
     velocity grids on a 20x20 base on [-vmax, vmax]^2 with vs_levels levels: each cell's
     initial reference grid, refined toward n_min..n_max cells with the local disturbance.
     walls=True closes the box with Maxwellian walls (at the freestream density and
-    temperature, at rest) instead of the inflow and outflow boundaries."""
+    temperature, at rest) instead of the inflow and outflow boundaries; immersed=False leaves
+    the circle out (same leaves, state and velocity grids, every cell fluid). CFL 0.4 is the
+    reference's default (src/Solver/Types.jl:122), which example/cylinder/cylinder.jl keeps."""
     from .ib import immerse_circle
     from .solver import PCoordField
     solver = Solver(DIM=2, NDF=2, AMR_PS_MAXLEVEL=0, AMR_VS_MAXLEVEL=vs_levels, flux=flux,
@@ -389,13 +392,14 @@ def cylinder_c3(n=448, nv=20, vmax=15.0, L=8.0, radius=0.5, Ma=5.0, Kn=0.1, omeg
         _, s = bow_shock_state(mid, Ma, radius)
         return target_grids(prim, s, base, gas.K, n_min, n_max, seed, workers)
     case = build_case(cfg, vs_grids_fn=grids_fn)
-    case.topo = immerse_circle(case.topo, (0.0, 0.0), radius, True, (1.0, 0.0, 0.0, 1.0))
+    if immersed:
+        case.topo = immerse_circle(case.topo, (0.0, 0.0), radius, True, (1.0, 0.0, 0.0, 1.0))
     return case
 
 
 def sphere_c4(n=48, nv=16, vmax=10.0, L=6.0, radius=0.5, Ma=3.0, Kn=0.05, omega=0.81,
               vs_levels=1, n_min=4096, n_max=30000, precision="fp32f", flux="CAIDVM",
-              marching="CAIDVM_Marching", CFL=0.5, seed=20261204, workers=0):
+              marching="CAIDVM_Marching", CFL=0.4, seed=20261204, workers=0):
     """The sphere (example/sphere/sphere.jl): Sphere(Maxwellian, [0,0,0], radius, true, .,
     [1,0,0,0,1]) in [-L, L]^3 on n^3 leaves, Mach-Ma inflow at xmin, outflow elsewhere, a
     developed bow-shock state and per-cell adapted grids on a 16^3 base on [-vmax, vmax]^3

@@ -164,10 +164,13 @@ class Oracle:
         self._ck(self.L.orc_iterate(self.h, float(dt), int(step), _p(r), _p(a)), "iterate")
         return r, a
 
-    def step(self, dt, step):
+    def step(self, dt, step, residual=True):
+        """slope!, flux!, iterate! of one step. The residual sums are always formed; with
+        residual=False they are dropped, as the device's step does not form them then."""
         self.slope()
         self.flux(dt)
-        return self.iterate(dt, step)
+        r = self.iterate(dt, step)
+        return r if residual else (None, None)
 
 
 def set_threads(n):

@@ -1,15 +1,28 @@
 """The stated workloads on the device.
 
 - C1 (the CPU-runnable configuration): the 2D four-quadrant Riemann problem on exactly 64x64
-  leaves with a 28x28 velocity grid on [-5,5]^2, Kn = 1e-2, omega = 0.5, CFL 0.8, FP64,
-  CAIDVM + CAIDVM_Marching, 100 fused steps against the oracle at 1e-12 (relative, per
-  array / per cell row), with the residual sums of the check steps.
+  leaves with a 28x28 velocity grid on [-5,5]^2, Kn = 1e-2, omega = 0.5, FP64, CAIDVM +
+  CAIDVM_Marching, 100 fused steps against the oracle at 1e-12 (relative, per array / per
+  cell row), with the residual sums of the check steps.
+  At the stated CFL 0.8 the reference's step is unstable: the time step is limited per
+  direction (Dt_xi = CFL min(ds/U), src/Solver/Types.jl:518), so at the corner velocities
+  (|u| = |v| = 4.82) the unsplit update has CFL_x + CFL_y = 1.6 > 1, and the negative
+  round-off values of f there grow about 2x per step from step ~50 (the limiter takes |f|,
+  src/Flux/CAIDVM.jl:127-141, so it does not stop them) until the state is not finite at step
+  ~63. The oracle and the device agree to 1e-12 up to step 60 and both turn non-finite at the
+  same step. At CFL 0.4 (example/Riemann_problem/rp_2D.jl:29 and the reference's default) all
+  100 steps stay finite and agree at 1e-12.
 - C3 (the velocity-adaptive cylinder, 200,704 leaves, grids of 568-4000 cells): one step of
-  the FP64 form against the oracle at full size (w, prim per cell at 1e-12); the FP32 form's
-  moments within 1e-5 (per cell, relative) of the FP64 form's after 3 steps; and, with the
-  box closed by Maxwellian walls, the total mass kept to 1e-12 in both forms over 3 steps
-  (every face's mass flux enters its two cells with opposite signs and the wall densities
-  zero the walls' mass fluxes, so the total changes by round-off only).
+  the FP64 form against the oracle at full size (w, prim, df of the live cells at 1e-12); the
+  FP32 form's moments within 1e-5 (per cell, relative) of the FP64 form's after 3 steps; and on
+  the same leaves, state and velocity grids closed by Maxwellian walls, the total mass kept to
+  1e-12 in both forms over 3 steps (every face's mass flux enters its two cells with opposite
+  signs and the wall densities zero the walls' mass fluxes, so the total changes by round-off
+  only). The circle is left out of that check: the reference's immersed boundary is a
+  ghost-cell method whose faces between donors and solid cells exchange mass with the solid
+  cells (the wall density zeroes the mass flux at the intersection point, not at the cell
+  face), so the fluid's mass is not an invariant with it (the oracle loses 2.5e-4 of it in one
+  step on a 32x32 closed box).
 """
 import numpy as np
 import pytest
@@ -21,15 +34,31 @@ from kitamr_hip.solver import kah_params
 TOL = 1e-12
 
 
+def _first_nonfinite(dev, orc, dt, s0, s1):
+    """Steps both sides from s0 to s1; the first step after which each state is not finite."""
+    first = {"dev": None, "orc": None}
+    for s in range(s0, s1):
+        dev.step(dt, s, residual=False)
+        orc.step(dt, s, residual=False)
+        for name, side in (("dev", dev), ("orc", orc)):
+            if first[name] is None and not np.isfinite(side.get_state()["w"]).all():
+                first[name] = s + 1
+        if first["dev"] is not None and first["orc"] is not None:
+            break
+    return first
+
+
 @pytest.mark.gpu
-def test_c1_riemann_64x64_28x28_100_steps():
-    case = cases.riemann_2d(n=64, nv=28)
+@pytest.mark.parametrize("CFL", [0.8, 0.4])
+def test_c1_riemann_64x64_28x28_100_steps(CFL):
+    case = cases.riemann_2d(n=64, nv=28, CFL=CFL)
     assert case.n_cells == 4096 and case.phase_cells == 4096 * 784
-    say(f"[C1] cells={case.n_cells} P={case.phase_cells} dt={case.dt_xi:.6e}")
-    assert abs(case.dt_xi - 0.8 * (1 / 64) / (5 - 10 / 56)) < 1e-15
+    say(f"[C1 CFL={CFL}] cells={case.n_cells} P={case.phase_cells} dt={case.dt_xi:.6e}")
+    assert abs(case.dt_xi - CFL * (1 / 64) / (5 - 10 / 56)) < 1e-15
     dev, orc = pair(case)
     dt = case.dt_xi
-    for s in range(100):
+    last = 60 if CFL > 0.5 else 100
+    for s in range(last):
         check = (s + 1) % 20 == 0
         rg = dev.step(dt, s, residual=check)
         ro = orc.step(dt, s, residual=check)
@@ -38,8 +67,13 @@ def test_c1_riemann_64x64_28x28_100_steps():
             g, o = dev.get_state(), orc.get_state()
             e = {k: rel_err(g[k], o[k]) if k == "df" else rel_err_rows(g[k], o[k], 4)
                  for k in ("df", "w", "prim")}
-            say(f"[C1] step {s + 1}: {e}")
+            say(f"[C1 CFL={CFL}] step {s + 1}: {e}")
             assert max(e.values()) <= TOL, (s, e)
+            assert np.isfinite(g["w"]).all()
+    if last < 100:
+        first = _first_nonfinite(dev, orc, dt, last, 100)
+        say(f"[C1 CFL={CFL}] first non-finite step: {first}")
+        assert first["orc"] is not None and first["dev"] == first["orc"], first
     dev.close()
     orc.close()
 
@@ -49,11 +83,12 @@ _C3 = {}
 
 def _c3(precision="fp64", walls=False):
     """The C3 case (built once per boundary kind; the storage form only changes the
-    parameters)."""
+    parameters). walls=True: the same leaves, state and grids closed by Maxwellian walls,
+    without the circle."""
     import dataclasses
     if walls not in _C3:
         _C3.clear()    # one full-size case held at a time
-        _C3[walls] = cases.cylinder_c3(n=448, walls=walls)
+        _C3[walls] = cases.cylinder_c3(n=448, walls=walls, immersed=not walls)
     c = _C3[walls]
     return dataclasses.replace(c, config=dataclasses.replace(c.config, precision=precision))
 
@@ -81,8 +116,12 @@ def test_c3_full_size_step_fp64_equals_oracle():
                          np.asarray(o[k]).reshape(-1, 4)[fluid], 4)
         say(f"[C3] one step {k}: {e:.3g}")
         assert e <= TOL, k
-    e = rel_err(g["df"], o["df"])
-    say(f"[C3] one step df: {e:.3g}")
+    live = fluid.copy()
+    live[np.asarray(case.topo["ib_solid"])] = True     # rebuilt by update_solid_cell!
+    off = np.asarray(case.topo["vs_offset"]) * 2
+    rows = np.repeat(live, np.diff(off))
+    e = rel_err(np.asarray(g["df"])[:off[-1]][rows], np.asarray(o["df"])[:off[-1]][rows])
+    say(f"[C3] one step df ({int(live.sum())} live cells): {e:.3g}")
     assert e <= TOL
     dev.close()
     orc.close()
@@ -119,7 +158,8 @@ def test_c3_closed_box_keeps_mass(precision):
     case = _c3(precision, walls=True)
     say(f"[C3 closed] {precision}: cells={case.n_cells} P={case.phase_cells}")
     n = case.n_cells
-    V = np.prod(np.asarray(case.topo["cell_ds"])[:n].reshape(n, 2), axis=1)
+    assert (np.asarray(case.topo["bound_enc"]) >= 0).all()
+    V = np.prod(np.asarray(case.topo["cell_ds"]).reshape(-1, 2)[:n], axis=1)
     fluid = np.asarray(case.topo["bound_enc"]) >= 0
     d = _dev(case)
     m0 = float((np.asarray(case.w).reshape(n, 4)[fluid, 0] * V[fluid]).sum())
