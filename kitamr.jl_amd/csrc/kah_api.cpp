@@ -17,6 +17,7 @@
 #include <cstring>
 #include <string>
 #include <limits>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/kitamr_hip.h"
@@ -73,7 +74,8 @@ struct kah_ctx {
   std::vector<int32_t> fluid;
   std::vector<int32_t> benc;
   std::vector<LevelBin*> bins;
-  DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey;
+  DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey, d_gof;
+  int64_t n_grids = 1, grid_cells = 0, map_entries = 0;   // distinct grids, their cells, map entries
   DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
   DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
   DBuf d_vtb, d_ovmap, d_moff;
@@ -134,7 +136,7 @@ struct kah_ctx {
   }
   // device memory held by the context
   int64_t device_bytes() const {
-    const DBuf* bufs[] = {&d_fluid, &d_voff, &d_gu, &d_gw, &d_glev, &d_gkey, &d_df[0], &d_df[1],
+    const DBuf* bufs[] = {&d_fluid, &d_voff, &d_gu, &d_gw, &d_glev, &d_gkey, &d_gof, &d_df[0], &d_df[1],
                           &d_sdf, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
                           &d_res, &d_scratch, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
                           &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
@@ -814,83 +816,152 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       }
     }
   }
-  // immersed boundary: on adapted grids each SolidNeighbor (cell nc + k) carries a copy of its
-  // donor's grid after the cells' grids, so kernels read it like any cell
+  // immersed boundary: on adapted grids each SolidNeighbor (cell nc + k) reads its donor's
+  // grid, so kernels read it like any cell
   KAH_TRY(setup_ib(C, T));
-  if (!C->uni && C->ib_nf > 0) {
-    std::vector<int64_t> voff(T->vs_offset, T->vs_offset + nc + 1);
-    std::vector<double> gu(T->vs_mid, T->vs_mid + ng * D), gw(T->vs_weight, T->vs_weight + ng);
-    std::vector<int8_t> gl(T->vs_level, T->vs_level + ng);
-    for (int64_t h : C->ib_donor) {
-      const int64_t o = T->vs_offset[h], n = T->vs_offset[h + 1] - o;
-      voff.push_back(voff.back() + n);
-      gu.insert(gu.end(), T->vs_mid + o * D, T->vs_mid + (o + n) * D);
-      gw.insert(gw.end(), T->vs_weight + o, T->vs_weight + o + n);
-      gl.insert(gl.end(), T->vs_level + o, T->vs_level + o + n);
-      keys.insert(keys.end(), keys.begin() + o, keys.begin() + o + n);
+  // Distinct velocity grids (adapted grids): cells whose grids are bit-identical (levels,
+  // midpoints, weights) share one copy of the geometry, and gof[c] is the first entry of cell
+  // c's grid in that table. The state (f, sdf) stays per cell at vs_offset. A face neighbour
+  // on the same grid then needs no overlap map (velocity cell i meets velocity cell i), and
+  // cells of one grid read its geometry from the caches.
+  std::vector<int64_t> gof;
+  std::vector<int64_t> voff(T->vs_offset, T->vs_offset + nc + 1);
+  for (int64_t h : C->ib_donor) voff.push_back(voff.back() + (T->vs_offset[h + 1] - T->vs_offset[h]));
+  C->n_grids = 1;
+  if (!C->uni) {
+    std::vector<double> gu, gw;
+    std::vector<int8_t> gl;
+    std::vector<uint32_t> gk;
+    gof.assign(size_t(nc) + C->ib_donor.size(), 0);
+    // hash of a grid's bytes -> first cell of each distinct grid with that hash
+    std::unordered_map<uint64_t, std::vector<int64_t>> seen;
+    seen.reserve(size_t(nc));
+    auto fnv = [](uint64_t h, const void* p, size_t n) {   // FNV-1a over 8-byte words
+      const unsigned char* b = static_cast<const unsigned char*>(p);
+      size_t x = 0;
+      for (; x + 8 <= n; x += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + x, 8);
+        h = (h ^ w) * 1099511628211ull;
+      }
+      for (; x < n; ++x) h = (h ^ b[x]) * 1099511628211ull;
+      return h;
+    };
+    auto same_grid = [&](int64_t a, int64_t b) {
+      const int64_t oa = T->vs_offset[a], ob = T->vs_offset[b];
+      const int64_t n = T->vs_offset[a + 1] - oa;
+      if (T->vs_offset[b + 1] - ob != n) return false;
+      return std::memcmp(T->vs_level + oa, T->vs_level + ob, size_t(n)) == 0 &&
+             std::memcmp(T->vs_mid + oa * D, T->vs_mid + ob * D, sizeof(double) * n * D) == 0 &&
+             std::memcmp(T->vs_weight + oa, T->vs_weight + ob, sizeof(double) * n) == 0;
+    };
+    std::vector<int64_t> rep_of(size_t(nc), -1);   // representative cell of each cell's grid
+    int64_t ndistinct = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+      const int64_t o = T->vs_offset[c], n = T->vs_offset[c + 1] - o;
+      uint64_t h = 1469598103934665603ull;
+      h = fnv(h, &n, sizeof(n));
+      h = fnv(h, T->vs_level + o, size_t(n));
+      h = fnv(h, keys.data() + o, sizeof(uint32_t) * size_t(n));
+      auto& lst = seen[h];
+      int64_t rep = -1;
+      for (int64_t r : lst)
+        if (same_grid(r, c)) {
+          rep = r;
+          break;
+        }
+      if (rep < 0) {
+        lst.push_back(c);
+        rep = c;
+        gof[size_t(c)] = int64_t(gw.size());
+        gu.insert(gu.end(), T->vs_mid + o * D, T->vs_mid + (o + n) * D);
+        gw.insert(gw.end(), T->vs_weight + o, T->vs_weight + o + n);
+        gl.insert(gl.end(), T->vs_level + o, T->vs_level + o + n);
+        gk.insert(gk.end(), keys.begin() + o, keys.begin() + o + n);
+        ++ndistinct;
+      } else {
+        gof[size_t(c)] = gof[size_t(rep)];
+      }
+      rep_of[size_t(c)] = rep;
     }
-    KAH_TRY(C->upload(C->d_voff, voff.data(), voff.size(), "vs_offset"));
+    for (size_t k = 0; k < C->ib_donor.size(); ++k)
+      gof[size_t(nc) + k] = gof[size_t(C->ib_donor[k])];
+    C->n_grids = ndistinct;
+    C->grid_cells = int64_t(gw.size());
     KAH_TRY(C->upload(C->d_gu, gu.data(), gu.size(), "vs_mid"));
     KAH_TRY(C->upload(C->d_gw, gw.data(), gw.size(), "vs_weight"));
     KAH_TRY(C->upload(C->d_glev, gl.data(), gl.size(), "vs_level"));
+    KAH_TRY(C->upload(C->d_gkey, gk.data(), gk.size(), "vs_key"));
+    KAH_TRY(C->upload(C->d_gof, gof.data(), gof.size(), "grid offsets"));
   } else {
-    KAH_TRY(C->upload(C->d_voff, T->vs_offset, size_t(nc + 1), "vs_offset"));
+    C->grid_cells = C->n0;
     KAH_TRY(C->upload(C->d_gu, T->vs_mid, size_t(ng * D), "vs_mid"));
     KAH_TRY(C->upload(C->d_gw, T->vs_weight, size_t(ng), "vs_weight"));
     KAH_TRY(C->upload(C->d_glev, T->vs_level, size_t(ng), "vs_level"));
+    C->d_gkey.release();
+    C->d_gof.release();
   }
-  if (!C->uni) KAH_TRY(C->upload(C->d_gkey, keys.data(), keys.size(), "vs_key"));
-  else C->d_gkey.release();
+  KAH_TRY(C->upload(C->d_voff, voff.data(), voff.size(), "vs_offset"));
 
   // descriptors
   Builder B(*C, *T);
   KAH_TRY(B.build());
 
-  // Overlap maps of adapted grids: one map slot per distinct neighbour cell that a fluid
-  // cell's slope stencil or face gather reads across grids (see OvJob).
-  std::vector<int64_t> moff(size_t(nc + 1), 0);
+  // Overlap maps of adapted grids: one map per distinct ordered pair (owner grid, neighbour
+  // grid) that a fluid cell's slope stencil or face gather reads across grids (see OvJob); a
+  // neighbour on the owner's own grid reads velocity cell i itself (MAP_SAME).
+  std::vector<int64_t> mbase;
   std::vector<OvJob> jobs;
   const bool maps = !C->uni && C->ov_enable && D * C->lmax <= 7;
-  for (auto& e : B.slnb) e.map = -1;
-  for (int64_t c = 0; c < nc; ++c) {
-    for (auto& ed : B.ent[c]) ed.map = -1;
-    moff[c + 1] = moff[c];
-    if (!maps || T->bound_enc[c] < 0) continue;
-    std::vector<int32_t> lst;
-    auto slot = [&](int32_t nbc) {
-      for (size_t k = 0; k < lst.size(); ++k)
-        if (lst[k] == nbc) return int32_t(k);
-      lst.push_back(nbc);
-      return int32_t(lst.size() - 1);
+  {
+    std::unordered_map<uint64_t, int32_t> map_id;   // (owner grid, neighbour grid) -> map
+    int64_t next = 0;
+    auto assign = [&](int64_t c, int32_t nbc) -> int32_t {
+      if (C->uni) return -1;
+      const int64_t go = gof[size_t(c)], gn = gof[size_t(nbc)];
+      if (go == gn) return MAP_SAME;
+      if (!maps) return -1;
+      const uint64_t key = uint64_t(go) * uint64_t(C->grid_cells + 1) + uint64_t(gn);
+      auto it = map_id.find(key);
+      if (it != map_id.end()) return it->second;
+      const int32_t id = int32_t(mbase.size());
+      const int32_t no = int32_t(voff[size_t(c) + 1] - voff[size_t(c)]);
+      const int32_t nn = int32_t(voff[size_t(nbc) + 1] - voff[size_t(nbc)]);
+      mbase.push_back(next);
+      jobs.push_back(OvJob{go, gn, no, nn, next});
+      next += no;
+      map_id.emplace(key, id);
+      return id;
     };
-    for (int d = 0; d < D; ++d) {
-      const SlopeDesc& sd = B.sl[size_t(c * D + d)];
-      const int sides = sd.mode == SL_INNER ? 2 : (sd.mode == SL_BOUND ? 1 : 0);
-      for (int sdx = 0; sdx < sides; ++sdx)
-        for (int e = 0; e < sd.nb_cnt[sdx]; ++e) {
-          SlopeNb& nbr = B.slnb[size_t(sd.nb_beg[sdx] + e)];
-          nbr.map = slot(nbr.cell);
-        }
+    for (auto& e : B.slnb) e.map = -1;
+    for (int64_t c = 0; c < nc; ++c) {
+      for (auto& ed : B.ent[c]) ed.map = -1;
+      if (T->bound_enc[c] < 0) continue;
+      for (int d = 0; d < D; ++d) {
+        const SlopeDesc& sd = B.sl[size_t(c * D + d)];
+        const int sides = sd.mode == SL_INNER ? 2 : (sd.mode == SL_BOUND ? 1 : 0);
+        for (int sdx = 0; sdx < sides; ++sdx)
+          for (int e = 0; e < sd.nb_cnt[sdx]; ++e) {
+            SlopeNb& nbr = B.slnb[size_t(sd.nb_beg[sdx] + e)];
+            nbr.map = assign(c, nbr.cell);
+          }
+      }
+      for (auto& ed : B.ent[c])
+        if (ed.oth_cell >= 0 &&
+            (ed.oth_mode == MI_LIMITED || ed.oth_mode == MI_UNLIMITED || ed.oth_mode == MI_PLAIN))
+          ed.map = assign(c, ed.oth_cell);
     }
-    for (auto& ed : B.ent[c])
-      if (ed.oth_cell >= 0 &&
-          (ed.oth_mode == MI_LIMITED || ed.oth_mode == MI_UNLIMITED || ed.oth_mode == MI_PLAIN))
-        ed.map = slot(ed.oth_cell);
-    const int64_t nvc = T->vs_offset[c + 1] - T->vs_offset[c];
-    for (size_t k = 0; k < lst.size(); ++k)
-      jobs.push_back(OvJob{int32_t(c), lst[k], moff[c] + int64_t(k) * nvc});
-    moff[c + 1] = moff[c] + int64_t(lst.size()) * nvc;
+    C->map_entries = next;
   }
   C->d_ovmap.release();
   C->d_moff.release();
-  if (maps && !jobs.empty()) {
-    KAH_TRY(C->upload(C->d_moff, moff.data(), moff.size(), "overlap map offsets"));
-    KAH_TRY(C->alloc(C->d_ovmap, sizeof(uint32_t) * size_t(moff[nc]), "overlap maps"));
+  if (!jobs.empty()) {
+    KAH_TRY(C->upload(C->d_moff, mbase.data(), mbase.size(), "overlap map bases"));
+    KAH_TRY(C->alloc(C->d_ovmap, sizeof(uint32_t) * size_t(C->map_entries), "overlap maps"));
     DBuf d_jobs;
     KAH_TRY(C->upload(d_jobs, jobs.data(), jobs.size(), "overlap jobs"));
     DevArgs A{};
     A.lmax = C->lmax;
-    A.voff = C->d_voff.as<int64_t>();
     A.glev = C->d_glev.as<int8_t>();
     A.gkey = C->d_gkey.as<uint32_t>();
     for (size_t j0 = 0; j0 < jobs.size(); j0 += (1u << 30)) {
@@ -1095,7 +1166,8 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.lmax = C->lmax;
   A.vt_base = C->d_vtb.as<int32_t>();
   A.ovmap = C->d_ovmap.as<uint32_t>();
-  A.map_off = C->d_moff.as<int64_t>();
+  A.map_base = C->d_moff.as<int64_t>();
+  A.gof = C->d_gof.as<int64_t>();
   A.vt_mode = C->vt_mode;
   A.vt_n0 = C->vt_n0;
   A.vt_n1 = C->vt_n1;

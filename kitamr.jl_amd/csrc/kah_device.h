@@ -164,7 +164,8 @@ struct Arith<float> {
 template <int D, int NDF, bool UNI, class FT>
 struct Grid {
   const DevArgs& A;
-  int64_t off;   // first velocity cell of the cell
+  int64_t off;   // first velocity cell of the cell (state arrays)
+  int64_t g;     // first entry of the cell's grid in the geometry table (adapted grids)
   int n;
   // per-cell base pointers (uniform over the workgroup): every access is base + a 32-bit
   // offset, which the loads take as a scalar base and one vector register
@@ -178,9 +179,11 @@ struct Grid {
     if constexpr (UNI) {
       n = a.uni_n;
       off = int64_t(c) * n;
+      g = 0;
     } else {
       off = a.voff[c];
       n = int(a.voff[c + 1] - off);
+      g = a.gof[c];
     }
     fb = reinterpret_cast<const FT*>(a.df) + off * NDF;
     sb = reinterpret_cast<const FT*>(a.sdf_r) + off * NDF * D;
@@ -189,7 +192,7 @@ struct Grid {
     gub = a.gu + goff() * D;
     gwb = a.gw + goff();
   }
-  __device__ int64_t goff() const { return UNI ? 0 : off; }
+  __device__ int64_t goff() const { return UNI ? 0 : g; }
   // velocity cell handled at thread position p: the identity, or the wave-tile order of a
   // shared tensor-product grid (the 64 positions of a wave form one tile). Any bijection of
   // [0, n) gives the same per-cell results; the order only makes the upwind branches
@@ -218,11 +221,11 @@ struct Grid {
   }
   __device__ double u(int i, int d) const { return at(gub, d * n + i); }
   __device__ double wt(int i) const { return at(gwb, i); }
-  __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[off + i]); }
-  __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[off + i]; }
+  __device__ int lev(int i) const { return UNI ? 0 : int(A.glev[g + i]); }
+  __device__ uint32_t key(int i) const { return UNI ? uint32_t(i) : A.gkey[g + i]; }
   // where key(i) is stored (adapted grids): the overlap walks load it only when a
   // neighbour has no overlap map
-  __device__ const uint32_t* keyp(int i) const { return UNI ? nullptr : A.gkey + off + i; }
+  __device__ const uint32_t* keyp(int i) const { return UNI ? nullptr : A.gkey + g + i; }
   // f, sdf and the next f are stored as FT (double, or float in the FP32 form); the
   // accessors widen to double, and the FP32 form's arithmetic narrows where it advects
   __device__ double f(int i, int k) const { return double(at(fb, k * n + i)); }
@@ -264,18 +267,22 @@ __device__ inline void for_overlap(const Grid<D, NDF, UNI, FT>& G, int i, int li
   }
 }
 
-// for_overlap through the overlap map of (owner cell c with n_own velocity cells, map slot):
-// one coalesced load instead of a binary search over the neighbour's keys; the same cells
-// and factors in the same order. Without a map (slot < 0) the search is done in place.
+// for_overlap through the overlap map `slot` of (owner grid, neighbour grid): one coalesced
+// load instead of a binary search over the neighbour's keys; the same cells and factors in
+// the same order. A neighbour on the owner's grid (MAP_SAME) meets cell i itself; without a
+// map (slot -1) the search is done in place.
 template <int D, int NDF, bool UNI, class FT, class Fn>
-__device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, FT>& N, int c,
-                                     int n_own, int slot, int i, int li,
-                                     const uint32_t* kp, Fn&& fn) {
+__device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, FT>& N, int slot,
+                                     int i, int li, const uint32_t* kp, Fn&& fn) {
   if constexpr (UNI) {
     fn(i, 1.0, 1.0);
   } else {
+    if (slot == MAP_SAME) {
+      fn(i, 1.0, 1.0);
+      return;
+    }
     if (slot >= 0) {
-      const uint32_t code = A.ovmap[A.map_off[c] + int64_t(slot) * n_own + i];
+      const uint32_t code = A.ovmap[A.map_base[slot] + i];
       const int j = int(code >> 8), cnt = int(code & 255u);
       if (cnt == 0) {
         fn(j, 1.0, 1.0);
@@ -291,14 +298,14 @@ __device__ inline void for_overlap_m(const DevArgs& A, const Grid<D, NDF, UNI, F
   }
 }
 
-// Overlap map of one (owner cell, neighbour) job, one velocity cell of the owner per thread
-// (the search of for_overlap, done once per topology).
+// Overlap map of one (owner grid, neighbour grid) job, one velocity cell of the owner per
+// thread (the search of for_overlap, done once per topology).
 template <int D, int NDF>
 __global__ __launch_bounds__(256) void k_overlap_map(DevArgs A, const OvJob* __restrict__ jobs,
                                                      int njobs, uint32_t* __restrict__ map) {
   const OvJob jb = jobs[blockIdx.x];
-  const int64_t go = A.voff[jb.cell], no = A.voff[jb.nb];
-  const int gn = int(A.voff[jb.cell + 1] - go), nn = int(A.voff[jb.nb + 1] - no);
+  const int64_t go = jb.own, no = jb.nb;
+  const int gn = jb.n_own, nn = jb.n_nb;
   const uint32_t* gk = A.gkey + go;
   const uint32_t* nk = A.gkey + no;
   const int8_t* gl = A.glev + go;
@@ -443,7 +450,7 @@ __device__ inline void side_sum(const DevArgs& A, int c, int n_own, int beg, int
   for (int e = 0; e < cnt; ++e) {
     const SlopeNb nb = A.slnb[beg + e];
     const Grid<D, NDF, UNI, FT> N(A, nb.cell);
-    for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, nb.map, i, li, kp,
+    for_overlap_m<D, NDF, UNI, FT>(A, N, nb.map, i, li, kp,
                                    [&](int j, double den, double rden) {
 #pragma unroll
       for (int k = 0; k < NDF; ++k) {
@@ -766,11 +773,12 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
       }
       default: {
         const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
-        for_overlap_m<D, NDF, UNI, FT>(A, N, c, n_own, ed.map, i, li, kp,
+        const bool same = UNI || ed.map == MAP_SAME;   // j == i: the own velocities
+        for_overlap_m<D, NDF, UNI, FT>(A, N, ed.map, i, li, kp,
                                        [&](int j, double den, double rden) {
           T uj[D], fj[NDF], sj[D][NDF];
 #pragma unroll
-          for (int t = 0; t < D; ++t) uj[t] = UNI ? u[t] : T(N.u(j, t));
+          for (int t = 0; t < D; ++t) uj[t] = same ? u[t] : T(N.u(j, t));
 #pragma unroll
           for (int k = 0; k < NDF; ++k) {
             fj[k] = T(N.f(j, k));

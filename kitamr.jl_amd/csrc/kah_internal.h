@@ -42,20 +42,24 @@ enum { VS_LOHNER = 0, VS_CONTRIBUTION = 1, VS_REFINE = 1, VS_COARSEN = 2 };
 struct SlopeNb {        // one neighbour in a slope stencil
   int32_t cell;         // data cell
   int32_t proj;         // 1: project with the neighbour's transverse slopes
-  int32_t map;          // overlap-map slot of (owner cell, this neighbour), -1: none
+  int32_t map;          // overlap map id of (owner grid, neighbour grid), MAP_SAME, or -1
   int32_t pad;
   double dm[3];         // transverse offsets (dm[dir] = 0)
   float dm_f[3];        // the same rounded to float (FP32 advection form)
   float pad_f;
 };
 
-// Overlap map of adapted velocity grids: for owner cell c, its map slot k (one per distinct
-// neighbour cell) and velocity cell i, entry map[map_off[c] + k*n_c + i] =
-// (j << 8) | cnt: j is the first overlapping velocity cell of the neighbour, cnt = 0 when
-// it is as coarse or coarser than i (one cell, factor 1), else the number of finer cells
-// j, j+1, ... inside i. One entry per (owner velocity cell, neighbour).
+// Overlap map of adapted velocity grids: one per distinct ordered pair (owner grid, neighbour
+// grid) of different grids; for velocity cell i of the owner grid, entry
+// map[map_base[id] + i] = (j << 8) | cnt: j is the first overlapping velocity cell of the
+// neighbour grid, cnt = 0 when it is as coarse or coarser than i (one cell, factor 1), else
+// the number of finer cells j, j+1, ... inside i. A descriptor's map field holds the id, or
+// MAP_SAME when the neighbour has the owner's grid (cell i meets cell i), or -1 (no map: the
+// key search is done in place). own/nb are the grids' first entries in the geometry table.
+constexpr int32_t MAP_SAME = -2;
 struct OvJob {
-  int32_t cell, nb;
+  int64_t own, nb;
+  int32_t n_own, n_nb;
   int64_t base;
 };
 
@@ -99,7 +103,7 @@ struct EntryDesc {      // one face side of a fluid cell, gathered in K2
   int32_t oth_mode;     // MI_* of the incoming set
   int32_t dir;
   int32_t bc;
-  int32_t map;          // overlap-map slot of the other cell (adapted grids), -1: none
+  int32_t map;          // overlap map id of (owner grid, other grid), MAP_SAME, or -1
   double rot;
   double area;          // signed face area, rot*A (or rot*A/2^(DIM-1))
   double fmid[3];

@@ -16,7 +16,7 @@ struct DevArgs {
   int32_t uni_n;            // cells per grid when all grids are the same one
   int32_t lmax;             // AMR_VS_MAXLEVEL (key width)
   const int64_t* voff;      // [nc+1]
-  const double* gu;         // midpoints, per-cell d-major blocks (one block if uniform)
+  const double* gu;         // midpoints, d-major blocks per distinct grid (one if uniform)
   const double* gw;         // weights
   const int8_t* glev;       // levels
   const uint32_t* gkey;     // cell keys (adapted grids)
@@ -52,7 +52,9 @@ struct DevArgs {
   // wave tiles of a shared tensor-product velocity grid (vt_mode 2: 8x8, 3: 4x4x4 blocks of
   // velocity cells per 64 consecutive thread positions; 0: identity order)
   const uint32_t* ovmap;    // overlap maps of adapted grids (see OvJob), or null
-  const int64_t* map_off;   // [nc] first map entry of each owner cell
+  const int64_t* map_base;  // first entry of each overlap map
+  const int64_t* gof;       // adapted grids: first geometry entry (gu/gw/glev/gkey) of the
+                            // grid of each cell (cells with identical grids share one)
   const int32_t* vt_base;   // first velocity cell of each tile
   int32_t vt_mode, vt_n0, vt_n1, vt_pad;
   // immersed boundary under CIP_Marching (positivity_preserving_ib!): the IB faces, whose
