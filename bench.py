@@ -292,6 +292,9 @@ def main():
                     help="tune the launch options on the workload first (coordinate descent)")
     ap.add_argument("--vel-tiles", type=int, default=-1,
                     help="1/0: wave tiles of the velocity grid on/off (-1: library default)")
+    ap.add_argument("--regular", type=int, default=-1,
+                    help="1/0: specialised slope and gather of regular cells on/off "
+                         "(-1: library default, on)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
@@ -325,6 +328,8 @@ def main():
         options["slope_vpt"] = args.slope_vpt
     if args.vel_tiles >= 0:
         options["vel_tiles"] = args.vel_tiles
+    if args.regular >= 0:
+        options["regular"] = args.regular
     r = timed_run(case, options, args.steps, args.warmup, bool(args.graphs), copy=True, log=log)
     copy_gbs, copy_ms = r["copy"]
     ms = r["ms"]

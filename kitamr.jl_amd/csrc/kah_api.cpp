@@ -103,10 +103,20 @@ struct kah_ctx {
   // workgroups (more of them resident: measured 1.4x on 2D grids of 1.6k cells); larger
   // grids, whose LDS already limits residency, and the other forms run 8 waves.
   int fr_waves = 0;
-  int flux_waves() const {
+  int flux_waves(size_t lds) const {
     if (fr_waves) return fr_waves;
-    return (f32 && !uni && lds_bytes > 0 && lds_bytes <= 40 * 1024) ? 4 : 8;
+    return (f32 && !uni && lds > 0 && lds <= 40 * 1024) ? 4 : 8;
   }
+  // launches of the flux/relaxation kernel: the fluid cells by grid size, each launch with
+  // the LDS its largest grid needs (0: the global scratch area)
+  struct FrBin {
+    DBuf cells;
+    int n = 0;
+    size_t lds = 0;
+    bool regular = false;   // fused step through the gather of regular cells
+  };
+  bool reg_enable = true;   // KAH_OPT_REGULAR
+  std::vector<FrBin*> frbins;
   int slope_vpt = 3;  // velocity cells per thread of the chunked slope kernel
   bool vt_enable = true;   // wave-tile order of shared tensor-product velocity grids
   int vt_mode = 0, vt_n0 = 0, vt_n1 = 0;
@@ -128,6 +138,7 @@ struct kah_ctx {
   ~kah_ctx() {
     drop_graphs();
     for (auto* b : bins) delete b;
+    for (auto* b : frbins) delete b;
     for (auto& e : events) {
       (void)hipEventDestroy(e.a);
       (void)hipEventDestroy(e.b);
@@ -146,6 +157,7 @@ struct kah_ctx {
     int64_t t = 0;
     for (const DBuf* b : bufs) t += int64_t(b->bytes);
     for (const LevelBin* b : bins) t += int64_t(b->items.bytes);
+    for (const FrBin* b : frbins) t += int64_t(b->cells.bytes);
     return t;
   }
   int fail(int code, const std::string& m) {
@@ -1065,12 +1077,32 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       for (int32_t c : C->fluid) depth[size_t(c)] = T->ps_level[c] - Lmin;
     for (int32_t c : C->fluid) Dmax = std::max(Dmax, depth[size_t(c)]);
   }
+  // Regular cells of the slope kernel: every direction SL_INNER with one neighbour per side,
+  // no transverse projection, both on the cell's velocity grid (or a shared grid), and 2*D
+  // half fluxes that are the cell's own limited or plain-slope reconstructions.
+  auto slope_regular = [&](int32_t c) {
+    if (!C->reg_enable || hoff[c + 1] - hoff[c] != 2 * D) return false;
+    for (int d = 0; d < D; ++d) {
+      const SlopeDesc& sd = B.sl[size_t(c) * D + d];
+      if (sd.mode != SL_INNER || sd.nb_cnt[0] != 1 || sd.nb_cnt[1] != 1) return false;
+      for (int sx = 0; sx < 2; ++sx) {
+        const SlopeNb& nbr = B.slnb[size_t(sd.nb_beg[sx])];
+        if (nbr.proj || (!C->uni && nbr.map != MAP_SAME)) return false;
+      }
+    }
+    for (int h = hoff[c]; h < hoff[c + 1]; ++h) {
+      const HalfDesc& hd = halves[size_t(h)];
+      if (hd.data_cell != c || (hd.mode != MI_LIMITED && hd.mode != MI_UNLIMITED)) return false;
+    }
+    return true;
+  };
   for (int L = 0; L <= Dmax; ++L) {
     // half-count bins: the per-thread accumulators of K1 are MAXH*(dim+2) doubles, so a bin
     // sized to the common counts (4: 2D interior, 6: 3D interior) keeps registers low
-    const int caps[5] = {4, 6, 8, 16, 24};
-    std::vector<int32_t> lists[5];
-    std::vector<int2> works[5];
+    // bin 5: regular cells (the slopes and half fluxes of slope_regular / half_regular)
+    const int caps[6] = {4, 6, 8, 16, 24, -1};
+    std::vector<int32_t> lists[6];
+    std::vector<int2> works[6];
     for (int32_t c : C->fluid) {
       if (depth[size_t(c)] != L) continue;
       int nh = hoff[c + 1] - hoff[c];
@@ -1078,10 +1110,11 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       if (walled[c]) {
         lists[b].push_back(c);
       } else {
+        if (slope_regular(c)) b = 5;
         for (int ch = 0; ch < nchunks(c); ++ch) works[b].push_back(make_int2(c, ch));
       }
     }
-    for (int b = 0; b < 5; ++b) {
+    for (int b = 0; b < 6; ++b) {
       for (int kind = 0; kind < 2; ++kind) {
         size_t cnt = kind ? works[b].size() : lists[b].size();
         if (cnt == 0) continue;
@@ -1140,11 +1173,56 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   const size_t sc_elem =
       (C->f32 && C->P.marching != KAH_MARCH_CIP) ? sizeof(float) : sizeof(double);
   C->lds_bytes = sc_elem * size_t(NDF + 1) * size_t(C->max_n);
-  if (C->lds_bytes > C->lds_limit) {
-    C->lds_bytes = 0;
-    KAH_TRY(C->alloc(C->d_scratch, sizeof(double) * C->np * (NDF + 1), "scratch"));
-  } else {
-    C->d_scratch.release();
+  if (C->lds_bytes > C->lds_limit) C->lds_bytes = 0;
+  // Flux/relaxation launches by grid size (adapted grids): grids up to 640, 1280, 2560, ...
+  // velocity cells, each launch with the LDS of its largest grid, so that the many small grids
+  // of an adapted mesh are not held to the LDS footprint of the largest one; and within a size,
+  // regular cells (2*DIM faces to fluid cells on the same velocity grid, limited or plain-slope
+  // reconstruction on both sides) apart from the others.
+  for (auto* b : C->frbins) delete b;
+  C->frbins.clear();
+  {
+    auto regular = [&](int32_t c) {
+      if (!C->reg_enable || eoff[c + 1] - eoff[c] != 2 * D) return false;
+      for (int e = eoff[c]; e < eoff[c + 1]; ++e) {
+        const EntryDesc& ed = ents[size_t(e)];
+        if (ed.oth_cell < 0 || ed.oth_cell >= nc || T->bound_enc[ed.oth_cell] < 0) return false;
+        if (!C->uni && ed.map != MAP_SAME) return false;
+        if ((ed.own_mode != MI_LIMITED && ed.own_mode != MI_UNLIMITED) ||
+            (ed.oth_mode != MI_LIMITED && ed.oth_mode != MI_UNLIMITED))
+          return false;
+      }
+      return true;
+    };
+    std::vector<std::vector<int32_t>> lists;
+    std::vector<int64_t> bmax;
+    for (int32_t c : C->fluid) {
+      const int64_t n = T->vs_offset[c + 1] - T->vs_offset[c];
+      size_t b = 0;
+      if (!C->uni)
+        while ((int64_t(640) << b) < n) ++b;
+      b = 2 * b + (regular(c) ? 1 : 0);
+      if (lists.size() <= b) {
+        lists.resize(b + 1);
+        bmax.resize(b + 1, 0);
+      }
+      lists[b].push_back(c);
+      bmax[b] = std::max(bmax[b], n);
+    }
+    bool need_scratch = false;
+    for (size_t b = 0; b < lists.size(); ++b) {
+      if (lists[b].empty()) continue;
+      auto* fb = new kah_ctx::FrBin();
+      C->frbins.push_back(fb);
+      fb->n = int(lists[b].size());
+      fb->regular = (b & 1) != 0;
+      fb->lds = sc_elem * size_t(NDF + 1) * size_t(bmax[b]);
+      if (fb->lds > C->lds_limit) fb->lds = 0;
+      need_scratch |= fb->lds == 0;
+      KAH_TRY(C->upload(fb->cells, lists[b].data(), lists[b].size(), "flux cells"));
+    }
+    if (need_scratch) KAH_TRY(C->alloc(C->d_scratch, sizeof(double) * C->np * (NDF + 1), "scratch"));
+    else C->d_scratch.release();
   }
   KAH_TRY(C->zero(C->d_sdf));
   KAH_TRY(C->zero(C->d_psflux));
@@ -1268,9 +1346,12 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   if (C->timing) KAH_TRY(ev_begin(C, e, 1));
   int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER
                : (C->P.marching == KAH_MARCH_CIP ? MARCH_CIP : MARCH_CAIDVM);
-  KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
-                                   mode, march, C->lds_bytes, C->flux_waves(), C->stream),
-                 "flux/relaxation kernel launch"));
+  for (const kah_ctx::FrBin* b : C->frbins) {
+    const int m = (mode == MODE_FUSED && b->regular) ? MODE_FUSED_REG : mode;
+    KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, b->cells.as<int32_t>(), b->n, m,
+                                     march, b->lds, C->flux_waves(b->lds), C->stream),
+                   "flux/relaxation kernel launch"));
+  }
   if (C->timing) KAH_TRY(ev_end(C, e));
   return 0;
 }
@@ -1859,6 +1940,7 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
       if (value != 4 && value != 8)
         return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4 or 8");
       C->fr_waves = int(value);
+      C->drop_graphs();   // captured graphs hold the launch shape
       return KAH_OK;
     case KAH_OPT_GRAPHS:
       if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "graphs must be 0 or 1");
@@ -1874,6 +1956,11 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
       if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "velocity tiles must be 0 or 1");
       C->vt_enable = value != 0;
       C->have_topo = false;   // the tile table is built by kah_set_topology
+      return KAH_OK;
+    case KAH_OPT_REGULAR:
+      if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "regular must be 0 or 1");
+      C->reg_enable = value != 0;
+      C->have_topo = false;   // the launch lists are built by kah_set_topology
       return KAH_OK;
     case KAH_OPT_SLOPE_VPT:
       if (value < 1 || value > SLOPE_MAX_VPT)

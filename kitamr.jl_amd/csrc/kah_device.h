@@ -564,6 +564,54 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
   moment_terms<D, NDF, T>(wt, u, mic, m);
 }
 
+// slope_dir of a regular cell (SL_INNER with one neighbour per side, no transverse
+// projection, both neighbours on the cell's velocity grid or a shared grid): velocity cell i
+// of each neighbour; the same operations in the same order as slope_dir.
+template <int D, int NDF, bool UNI, class FT, class T>
+__device__ inline void slope_regular(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c,
+                                     int d, int i, const T (&f)[NDF], T (&s)[NDF]) {
+  const SlopeDesc sd = A.sl[c * D + d];
+  T a[2][NDF];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int nbc = A.slnb[sd.nb_beg[side]].cell;
+    const int64_t no = UNI ? int64_t(nbc) * G.n : A.voff[nbc];
+    const FT* nf = reinterpret_cast<const FT*>(A.df) + no * NDF;
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      const T term = f[k] - T(double(Grid<D, NDF, UNI, FT>::at(nf, k * G.n + i)));
+      a[side][k] = T(0);
+      a[side][k] += div_ds(term, sd.ds[side], sd.rds[side], sd.rds_f[side]);
+    }
+  }
+  const T ic0 = desc_val<T>(sd.inv_cnt[0], sd.inv_cnt_f[0]);
+  const T ic1 = desc_val<T>(sd.inv_cnt[1], sd.inv_cnt_f[1]);
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) {
+    s[k] = minmod_sel(a[0][k] * ic0, a[1][k] * ic1);
+    s[k] = T(G.stored(s[k]));
+    G.set_s(i, k, d, s[k]);
+  }
+}
+
+// half_terms of a regular cell: every half flux it owns is its own limited or plain-slope
+// upwind reconstruction.
+template <int D, int NDF, class T>
+__device__ inline void half_regular(const DevArgs& A, const HalfDesc& hd, const T (&u)[D],
+                                    const T (&f)[NDF], const T (&s)[D][NDF], T wt,
+                                    double (&m)[D + 2]) {
+#pragma unroll
+  for (int q = 0; q < D + 2; ++q) m[q] = 0.0;
+  const T ud = pick<D>(u, hd.dir);
+  const T ru = desc_val<T>(hd.rot, hd.rot_f) * ud;
+  const bool in = hd.sel ? (ru > T(0)) : (ru <= T(0));
+  if (!in) return;
+  T mic[NDF];
+  micro_recon<D, NDF>(hd.mode, f, s, u, T(A.dt), hd.xrec, hd.dsrec, hd.fmid, hd.dx_f, hd.ds_f,
+                      hd.dir, mic);
+  moment_terms<D, NDF, T>(wt, u, mic, m);
+}
+
 template <int D, int NDF, bool UNI, class FT, int MAXH, int NWV>
 __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* __restrict__ cells,
                                                     int ncells, KAH_RO_PARAMS) {
@@ -668,7 +716,7 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
 // work as k_slope without the loop over the grid, for cells without wall faces. Each
 // workgroup writes its partial sums of the cell's half fluxes to row part_off[slot]+chunk;
 // k_flux_relax adds the rows in chunk order.
-template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT>
+template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT, bool REG = false>
 __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
                                                              const int2* __restrict__ work,
                                                              int nwork, KAH_RO_PARAMS) {
@@ -703,16 +751,22 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
       for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
       for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
-      const int li = G.lev(i);
-      const uint32_t* kp = G.keyp(i);
+      if constexpr (REG) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, kp, f, s[d]);
+        for (int d = 0; d < D; ++d) slope_regular<D, NDF, UNI, FT, CT>(A, G, c, d, i, f, s[d]);
+      } else {
+        const int li = G.lev(i);
+        const uint32_t* kp = G.keyp(i);
+#pragma unroll
+        for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, kp, f, s[d]);
+      }
       const CT wt = CT(G.wt(i));
 #pragma unroll
       for (int h = 0; h < MAXH; ++h) {
         if (h < nh) {
           double m[NW];
-          half_terms<D, NDF, UNI, FT, CT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
+          if constexpr (REG) half_regular<D, NDF, CT>(A, A.halves[hb + h], u, f, s, wt, m);
+          else half_terms<D, NDF, UNI, FT, CT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
 #pragma unroll
           for (int q = 0; q < NW; ++q) acc[h * NW + q] = (r == 0) ? m[q] : acc[h * NW + q] + m[q];
         }
@@ -797,6 +851,48 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
           }
         });
       }
+    }
+  }
+}
+
+// gather_flux for a regular cell: its 2*DIM entries are faces to fluid cells on the same
+// velocity grid (or a shared grid), so an incoming velocity cell i reads the neighbour's
+// cell i with the own velocities, and both sides use the limited or plain-slope
+// reconstruction. The same terms in the same order as gather_flux, so the same result.
+template <int D, int NDF, bool UNI, class FT, class T>
+__device__ inline void gather_regular(const DevArgs& A, int n, int eb, int i, const T (&u)[D],
+                                      const T (&f)[NDF], const T (&s)[D][NDF], T (&fl)[NDF]) {
+  const T dt = T(A.dt);
+#pragma unroll
+  for (int e = 0; e < 2 * D; ++e) {
+    const EntryDesc& ed = A.ents[eb + e];
+    const T ud = pick<D>(u, ed.dir);
+    const T ru = desc_val<T>(ed.rot, ed.rot_f) * ud;
+    const bool own = ed.side ? (ru > T(0)) : (ru <= T(0));
+    const T area = desc_val<T>(ed.area, ed.area_f);
+    T mic[NDF];
+    if (own) {
+      micro_recon<D, NDF>(ed.own_mode, f, s, u, dt, ed.xown, ed.dsown, ed.fmid, ed.dxo_f,
+                          ed.dso_f, ed.dir, mic);
+    } else {
+      // the neighbour's rows: same grid, so same size n and velocity cell i
+      const int64_t no = UNI ? int64_t(ed.oth_cell) * n : A.voff[ed.oth_cell];
+      const FT* nf = reinterpret_cast<const FT*>(A.df) + no * NDF;
+      const FT* ns = reinterpret_cast<const FT*>(A.sdf_r) + no * NDF * D;
+      T fj[NDF], sj[D][NDF];
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        fj[k] = T(Grid<D, NDF, UNI, FT>::at(nf, k * n + i));
+#pragma unroll
+        for (int t = 0; t < D; ++t) sj[t][k] = T(Grid<D, NDF, UNI, FT>::at(ns, (t * NDF + k) * n + i));
+      }
+      micro_recon<D, NDF>(ed.oth_mode, fj, sj, u, dt, ed.xoth, ed.dsoth, ed.fmid, ed.dxt_f,
+                          ed.dst_f, ed.dir, mic);
+    }
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      T t = area * mic[k];
+      fl[k] += ed.side ? -t : t;
     }
   }
 }
@@ -1174,6 +1270,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 
   // half flux = sum of its partial rows in chunk order (micro_to_macro's 1/2 on the energy
   // row applied after the sum), one thread per (entry, side, component)
+  constexpr bool FUSED = (MODE == MODE_FUSED || MODE == MODE_FUSED_REG);
   if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
     const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
     const int slot = A.ents[eb + e].pair * 2 + s;
@@ -1325,7 +1422,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
       for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
-      if (MODE == MODE_FUSED) {
+      if (FUSED) {
         CT s[D][NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -1333,7 +1430,10 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 #pragma unroll
           for (int d = 0; d < D; ++d) s[d][k] = CT(G.s(i, k, d));
         }
-        gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.keyp(i), fl);
+        if constexpr (MODE == MODE_FUSED_REG)
+          gather_regular<D, NDF, UNI, FT, CT>(A, n, eb, i, u, f, s, fl);
+        else
+          gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.keyp(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -2060,7 +2160,10 @@ inline hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, 
 template <int D, int NDF, bool UNI, class FT, int VPT>
 inline hipError_t launch_slope_chunk_v(const DevArgs& A, const int2* work, int n, int maxh,
                                        hipStream_t st) {
-  if (maxh <= 4)
+  if (maxh < 0)   // regular cells: 2*D half fluxes
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 2 * D, VPT, true>), dim3(n),
+                       dim3(SLOPE_CHUNK), 0, st, A, work, n, KAH_RO_ARGS(A));
+  else if (maxh <= 4)
     hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 6)
@@ -2117,6 +2220,13 @@ inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int
     if (march == MARCH_CIP)
       return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_CIP, NWV>(A, cells, n, lds, st);
     return launch_fr_t<D, NDF, UNI, FT, MODE_ITER, MARCH_EULER, NWV>(A, cells, n, lds, st);
+  }
+  if (mode == MODE_FUSED_REG) {
+    if (march == MARCH_CAIDVM)
+      return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REG, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+    if (march == MARCH_CIP)
+      return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REG, MARCH_CIP, NWV>(A, cells, n, lds, st);
+    return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REG, MARCH_EULER, NWV>(A, cells, n, lds, st);
   }
   if (march == MARCH_CAIDVM)
     return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);

@@ -7,7 +7,10 @@
 
 namespace kah {
 
-enum { MODE_FLUX = 0, MODE_ITER = 1, MODE_FUSED = 2 };
+// MODE_FUSED_REG: MODE_FUSED for regular cells (2*DIM faces to fluid cells on the same
+// velocity grid, reconstructed by the limited or plain-slope forms), gathered without the
+// per-face mode dispatch and overlap walks
+enum { MODE_FLUX = 0, MODE_ITER = 1, MODE_FUSED = 2, MODE_FUSED_REG = 3 };
 enum { MARCH_CAIDVM = 0, MARCH_EULER = 1, MARCH_CIP = 2 };
 
 // Everything a kernel reads, passed by value (device pointers + scalars).

@@ -71,20 +71,36 @@ hipError_t launch_flux_relax_3d_f32(const DevArgs&, bool, const int32_t*, int, i
                                   hipStream_t);
 
 // ------------------------------------------------------------------ copy reference
-// A plain streaming copy (16-byte vector loads/stores) used to measure attainable HBM
-// bandwidth and to calibrate hardware byte counters against a kernel of known traffic.
-__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
-                                              double2* __restrict__ dst, int64_t n) {
-  const int64_t stride = int64_t(gridDim.x) * 256;
-  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+// A plain streaming copy used to measure attainable HBM bandwidth and to calibrate hardware
+// byte counters against a kernel of known traffic: each thread has COPY_UNROLL 16-byte loads
+// in flight (coalesced: consecutive threads read consecutive 16-byte words) before it stores.
+constexpr int COPY_UNROLL = 4;
+typedef double copy_vec __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_copy(const copy_vec* __restrict__ src,
+                                              copy_vec* __restrict__ dst, int64_t n) {
+  const int64_t step = int64_t(gridDim.x) * 256 * COPY_UNROLL;
+  for (int64_t b = int64_t(blockIdx.x) * 256 * COPY_UNROLL + threadIdx.x; b < n; b += step) {
+    copy_vec v[COPY_UNROLL];
+#pragma unroll
+    for (int k = 0; k < COPY_UNROLL; ++k) {
+      const int64_t i = b + int64_t(k) * 256;
+      if (i < n) v[k] = __builtin_nontemporal_load(&src[i]);
+    }
+#pragma unroll
+    for (int k = 0; k < COPY_UNROLL; ++k) {
+      const int64_t i = b + int64_t(k) * 256;
+      if (i < n) __builtin_nontemporal_store(v[k], &dst[i]);
+    }
+  }
 }
 
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
   const int64_t n = bytes / 16;
-  int blocks = int(std::min<int64_t>((n + 255) / 256, 256 * 64));
+  const int64_t per_block = 256 * COPY_UNROLL;
+  int blocks = int(std::min<int64_t>((n + per_block - 1) / per_block, 256 * 16));
   if (blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, static_cast<const double2*>(src),
-                     static_cast<double2*>(dst), n);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, static_cast<const copy_vec*>(src),
+                     static_cast<copy_vec*>(dst), n);
   return hipGetLastError();
 }
 

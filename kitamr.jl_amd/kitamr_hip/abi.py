@@ -29,7 +29,8 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart", "kah_load_restart",
            "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap", "kah_vs_project", "kah_get_ib"]
 MIRROR = {"data": 0, "slope": 1}
-OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5}
+OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5,
+       "regular": 6}
 
 
 class KahParams(C.Structure):

@@ -155,6 +155,52 @@ def test_graph_replay_equals_launches(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["riemann8", "riemann8_euler", "riemann8_dvm", "riemann8_cip",
+                                  "riemann16_amr2", "box3d", "box3d_cip", "cylinder_c3_32"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_regular_cells_equal_general_gather(name, precision):
+    """The specialised slope and gather of regular cells (2*DIM faces to fluid cells on the
+    same velocity grid) give the same steps as the general ones, bit for bit, in both forms."""
+    if name == "cylinder_c3_32":
+        case = cases.cylinder_c3(n=32)
+    else:
+        case = CASES[name]()
+    say(f"[regular] {name} precision {precision}: cells={case.n_cells} P={case.phase_cells}")
+    a, _ = pair(case, options={"regular": 0}, precision=precision)
+    b, _ = pair(case, options={"regular": 1}, precision=precision)
+    for s in range(3):
+        a.step(case.dt_xi, s)
+        b.step(case.dt_xi, s)
+    ga, gb = a.get_state(sdf=True), b.get_state(sdf=True)
+    for k in ("df", "sdf", "w", "prim", "qf"):
+        assert np.array_equal(ga[k], gb[k]), k
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_graph_replay_follows_a_flux_waves_change():
+    """Changing the flux kernel's workgroup size after graph steps drops the captured graphs:
+    the replayed steps equal launches one by one with the same option change, bit for bit
+    (the workgroup size fixes the order of the per-cell sums, so a stale graph would show)."""
+    case = CASES["riemann16_amr_vsamr"]()
+    say(f"[graph waves] cells={case.n_cells} P={case.phase_cells}")
+    a, _ = pair(case, options={"flux_waves": 8})
+    b, _ = pair(case, options={"flux_waves": 8, "graphs": 1})
+    for s in range(6):
+        if s == 3:
+            a.set_option("flux_waves", 4)
+            b.set_option("flux_waves", 4)
+        a.step(case.dt_xi, s)
+        b.step(case.dt_xi, s)
+        ga, gb = a.get_state(), b.get_state()
+        for k in ("df", "w", "prim", "qf"):
+            assert np.array_equal(ga[k], gb[k]), (s, k)
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
 def test_autotune_on_device_picks_a_timed_setting():
     """The autotuner times real steps on the device and returns one of the settings it
     timed, the fastest of them."""
