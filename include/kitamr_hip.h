@@ -239,6 +239,13 @@ typedef struct kah_vs_adapt {
 /* vr[0..1] = Velocity_Resolution (density, energy) of the current state, the maxima over
  * the fluid cells of vs_resolution (src/Velocity_space/AMR.jl:133-167). */
 int kah_vs_resolution(kah_ctx* ctx, double* vr);
+/* kah_vs_resolution for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the call allocates nothing. No counterpart
+ * in the reference. Its scratch (the per-cell partial maxima) is a fixed device array of
+ * 16 MiB that each GPU the library runs on holds until the process ends, so it takes contexts
+ * of at most 1,048,576 cells (KAH_EUNSUPPORTED beyond). Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (from two threads or two contexts) race. */
+int kah_vs_resolution_noalloc(kah_ctx* ctx, double* vr);
 /* flags[P]: per velocity cell, KAH_VS_REFINE as vs_refine! decides it (level below
  * vs_maxlevel and the criterion, src/Velocity_space/AMR.jl:24-65) and KAH_VS_COARSEN as
  * vs_coarsen!'s per-cell willingness (:69-111), both on the current grids, state, slopes,

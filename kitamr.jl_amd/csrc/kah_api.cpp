@@ -17,6 +17,7 @@
 #include <cstring>
 #include <string>
 #include <limits>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -644,6 +645,35 @@ int setup_ib(kah_ctx* C, const kah_topology* T) {
 
 }  // namespace
 
+// ------------------------------------------------ scratch of the allocation-free entry point
+// kah_vs_resolution_noalloc keeps its per-cell partial maxima (two per cell) and its two
+// results in this fixed device array instead of memory it allocates: 16 MiB plus 16 bytes,
+// held by every GPU the library runs on until the process ends. In host code `scratch` only
+// names the symbol; its device address, one per GPU, comes from hipGetSymbolAddress in
+// kah_create (with that GPU current) and is the only form host code passes on.
+constexpr int64_t NOALLOC_MAX_CELLS = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_CELLS + 2];
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+hipError_t g_scratch_err[NOALLOC_MAX_DEVICES] = {};
+// once per device, with that device current
+void scratch_setup(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && p) {
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  } else {
+    g_scratch_err[dev] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+    (void)hipGetLastError();
+  }
+}
+}  // namespace
+
 // =========================================================================== C ABI
 extern "C" {
 
@@ -677,6 +707,7 @@ int kah_create(const kah_params* params, kah_ctx** out) {
     delete C;
     return KAH_EDEVICE;
   }
+  scratch_setup(params->device);   // the allocation-free entry point's scratch on this GPU
   // dynamic LDS per workgroup: the device's limit (160 KiB on gfx950) less the kernels'
   // static LDS
   int lds_max = 0;
@@ -1686,6 +1717,35 @@ int kah_vs_resolution(kah_ctx* C, double* vr) {
   vr[0] = dres;
   vr[1] = eres;
   return KAH_OK;
+}
+
+int kah_vs_resolution_noalloc(kah_ctx* C, double* vr) {
+  if (!C) return KAH_EINVAL;
+  if (!C->have_state) return C->fail(KAH_ESTATE, "no state: call kah_set_state first");
+  if (!vr) return C->fail(KAH_EINVAL, "null vr");
+  if (!C->vs_ok) return C->fail(KAH_EINVAL, "velocity root grid (vs_trees_num / vs_lo / vs_hi) not set");
+  if (C->nc > NOALLOC_MAX_CELLS)
+    return C->fail(KAH_EUNSUPPORTED, "kah_vs_resolution_noalloc: " + std::to_string(C->nc) +
+                                         " cells exceed its scratch limit of " +
+                                         std::to_string(NOALLOC_MAX_CELLS) + " cells");
+  const int dev = C->P.device;
+  double* scr = (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+  if (!scr) {
+    const hipError_t e = (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? g_scratch_err[dev]
+                                                                 : hipErrorInvalidDevice;
+    return C->fail(KAH_EDEVICE, std::string("kah_vs_resolution_noalloc: no scratch on device ") +
+                                    std::to_string(dev) + ": " + hipGetErrorString(e));
+  }
+  KAH_TRY(C->hip(hipSetDevice(dev), "hipSetDevice"));
+  double* res = scr + 2 * NOALLOC_MAX_CELLS;
+  DevArgs A = make_args(C, 0.0);
+  KAH_TRY(C->hip(launch_vs_res(A, C->D, C->uni, C->f32, C->vsa, int(C->nc), scr, C->stream),
+                 "vs resolution launch"));
+  KAH_TRY(C->hip(launch_vs_res_max(scr, C->d_benc.as<int32_t>(), int(C->nc), res, C->stream),
+                 "vs resolution maxima launch"));
+  KAH_TRY(C->hip(hipMemcpyAsync(vr, res, 2 * sizeof(double), hipMemcpyDeviceToHost, C->stream),
+                 "vs resolution download"));
+  return C->hip(hipStreamSynchronize(C->stream), "vs resolution");
 }
 
 int kah_vs_flags(kah_ctx* C, const kah_vs_adapt* p, const double* vr, uint8_t* flags) {

@@ -123,6 +123,9 @@ struct VsArgs {
 };
 hipError_t launch_vs_res(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P, int nc,
                          double* out, hipStream_t st);
+// out[0..1] = the maxima over the fluid cells of the per-cell pairs part[2c], part[2c+1]
+hipError_t launch_vs_res_max(const double* part, const int32_t* benc, int nc, double* out,
+                             hipStream_t st);
 hipError_t launch_vs_flags(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P,
                            int nc, uint8_t* flags, hipStream_t st);
 hipError_t launch_vs_remap(const DevArgs& A, int dim, bool f32, const int64_t* voff_old,

@@ -171,6 +171,45 @@ hipError_t launch_vs_res(const DevArgs& A, int dim, bool uni, bool f32, const Vs
   return dim == 2 ? launch_vs_2d(A, uni, 0, P, nc, out, nullptr, st)
                   : launch_vs_3d(A, uni, 0, P, nc, out, nullptr, st);
 }
+// Maxima over the fluid cells (benc >= 0) of the per-cell pairs part[2c], part[2c+1], each
+// started from 0 and taken with Julia's max (a NaN operand wins), as kah_vs_resolution takes
+// them on the host; one workgroup, the result in out[0..1].
+__device__ inline double vs_jmax(double a, double b) {
+  return (a != a) ? a : (b != b) ? b : (b > a ? b : a);
+}
+__global__ __launch_bounds__(1024) void k_vs_res_max(const double* __restrict__ part,
+                                                     const int32_t* __restrict__ benc, int nc,
+                                                     double* __restrict__ out) {
+  __shared__ double sh[2][1024];
+  const int t = threadIdx.x;
+  double d = 0.0, e = 0.0;
+  for (int c = t; c < nc; c += 1024)
+    if (benc[c] >= 0) {
+      d = vs_jmax(part[2 * int64_t(c)], d);
+      e = vs_jmax(part[2 * int64_t(c) + 1], e);
+    }
+  sh[0][t] = d;
+  sh[1][t] = e;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[0][t] = vs_jmax(sh[0][t + w], sh[0][t]);
+      sh[1][t] = vs_jmax(sh[1][t + w], sh[1][t]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = sh[0][0];
+    out[1] = sh[1][0];
+  }
+}
+
+hipError_t launch_vs_res_max(const double* part, const int32_t* benc, int nc, double* out,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_vs_res_max, dim3(1), dim3(1024), 0, st, part, benc, nc, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_vs_flags(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P,
                            int nc, uint8_t* flags, hipStream_t st) {
   if (f32) return dim == 2 ? launch_vs_2d_f32(A, uni, 1, P, nc, nullptr, flags, st)

@@ -27,7 +27,8 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_sync", "kah_phase_cells", "kah_device_bytes", "kah_kernel_times",
            "kah_copy_bandwidth", "kah_set_option", "kah_mirror_layout", "kah_pack_mirrors",
            "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart", "kah_load_restart",
-           "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap", "kah_vs_project", "kah_get_ib"]
+           "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap", "kah_vs_project", "kah_get_ib",
+           "kah_vs_resolution_noalloc"]
 MIRROR = {"data": 0, "slope": 1}
 OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5,
        "regular": 6}
@@ -206,6 +207,7 @@ def load_library(path=None):
     lib.kah_load_restart.argtypes = [C.c_void_p, lp, lp, dp, dp]
     u8p = C.POINTER(C.c_uint8)
     lib.kah_vs_resolution.argtypes = [C.c_void_p, dp]
+    lib.kah_vs_resolution_noalloc.argtypes = [C.c_void_p, dp]
     lib.kah_vs_flags.argtypes = [C.c_void_p, C.POINTER(KahVsAdapt), dp, u8p]
     lib.kah_vs_remap.argtypes = [C.c_void_p, C.POINTER(KahTopology), ip, u8p]
     lib.kah_vs_project.argtypes = [C.c_void_p, ip, i64]
@@ -214,7 +216,7 @@ def load_library(path=None):
                  "kah_kernel_times", "kah_mirror_layout", "kah_pack_mirrors",
                  "kah_unpack_mirrors", "kah_restart_sizes", "kah_save_restart",
                  "kah_load_restart", "kah_vs_resolution", "kah_vs_flags", "kah_vs_remap",
-                 "kah_vs_project", "kah_get_ib"]:
+                 "kah_vs_project", "kah_get_ib", "kah_vs_resolution_noalloc"]:
         getattr(lib, name).restype = C.c_int
     del i32
     if path is None:
@@ -382,6 +384,13 @@ class HipContext:
         """Velocity_Resolution (density, energy) of the current state."""
         vr = np.empty(2)
         self._check(self.lib.kah_vs_resolution(self.h, _ptr(vr)), "kah_vs_resolution")
+        return vr
+
+    def vs_resolution_noalloc(self):
+        """vs_resolution through the allocation-free entry point (fixed device scratch)."""
+        vr = np.empty(2)
+        self._check(self.lib.kah_vs_resolution_noalloc(self.h, _ptr(vr)),
+                    "kah_vs_resolution_noalloc")
         return vr
 
     def vs_flags(self, params, vr):
