@@ -128,50 +128,31 @@ def profiled_traffic(workload, precision):
     return None
 
 
-def serving_stream(n_req=16, seed=20261202, rate=40.0, budget_gb=96.0, slots=8):
-    """A short stream of the serving workload: 2D Riemann cases (outflow or Maxwellian-inflow
-    boundaries) and cylinders (an immersed circle, Mach uniform in [0.5, 5]), Kn log-uniform
-    in [1e-3, 1], 256-8.3k physical cells (16^2-48^2 base, 0-2 refinement levels for the
-    Riemann cases), 16^2-48^2 velocity grids, FP64 or FP32 form, Poisson arrivals; each runs
-    to the residual tolerance or its step budget.
-    Reports p50/p99 request latency (arrival to result) and the aggregate updates/s."""
-    import numpy as np
-    from kitamr_hip import cases, serve
-    rng = np.random.default_rng(seed)
+def serving_stream(n_req=16, seed=20261202, rate=2.0, budget_gb=200.0, slots=4, max_steps=100):
+    """A stream of the serving workload: requests drawn from its distributions (serve.c5_specs:
+    Riemann or cylinder, Kn log-uniform in [1e-3, 1], Ma in [0.5, 5], 4k-200k cells, 16^2-64^2
+    velocity grids, FP64 or the FP32 form), Poisson arrivals; each runs to the residual
+    tolerance or max_steps (reduced from 2000 to keep the run short). Four slots, one stream
+    each, as the GPU has four hardware queues per process. Reports p50/p99 request latency
+    (arrival to result) and the aggregate updates/s."""
+    from kitamr_hip import serve
+    specs = serve.c5_specs(n_req, seed)
     arrivals = serve.poisson_arrivals(n_req, rate, seed)
     reqs = []
     t0 = time.perf_counter()
-    for k in range(n_req):
-        n = int(rng.choice([16, 24, 32, 48]))
-        lv = int(rng.integers(0, 3))
-        nv = int(rng.choice([16, 24, 32, 48]))
-        Kn = float(10 ** rng.uniform(-3, 0))
-        prec = "fp32f" if rng.random() < 0.25 else "fp64"
-        kind = rng.random()
-        if kind < 1.0 / 3.0:
-            case = cases.riemann_2d(n=n, nv=nv, ps_levels=lv, halo=(3.0, 2.0, 1.5), Kn=Kn,
-                                    precision=prec)
-            tag = f"riemann n={n} L={lv} nv={nv}"
-        elif kind < 2.0 / 3.0:
-            case = cases.riemann_2d(n=n, nv=nv, ps_levels=lv, halo=(3.0, 2.0, 1.5), Kn=Kn,
-                                    bc="SuperSonicInflow", precision=prec)
-            tag = f"inflow n={n} L={lv} nv={nv}"
-        else:
-            Ma = float(rng.uniform(0.5, 5.0))
-            vmax = max(5.0, Ma * (5.0 / 6.0) ** 0.5 + 5.0)
-            case = cases.cylinder_2d(n=n, nv=nv, vmax=vmax, Ma=Ma, Kn=Kn, precision=prec)
-            tag = f"cylinder n={n} nv={nv} Ma={Ma:.2f}"
-        steps = int(rng.integers(40, 160))
-        reqs.append(serve.Request(rid=k, case=case, arrival=float(arrivals[k]), max_steps=steps,
-                                  tag=f"{tag} {prec} Kn={Kn:.2g}"))
+    for k, sp in enumerate(specs):
+        reqs.append(serve.Request(rid=k, case=serve.c5_case(sp), arrival=float(arrivals[k]),
+                                  max_steps=max_steps, tag=serve.c5_tag(sp)))
     log(f"[bench] serving stream of {n_req} requests built in {time.perf_counter() - t0:.1f}s")
     srv = serve.Server(budget_bytes=int(budget_gb * 1e9), max_slots=slots)
     rep = srv.run(reqs, keep_df=False)
-    rep["workload"] = (f"{n_req} requests, Poisson arrivals at {rate}/s, 2D Riemann cases "
-                       "(outflow or Maxwellian-inflow boundaries) and cylinders (immersed circle, "
-                       "Ma 0.5-5), 256-8.3k cells, 16^2-48^2 velocity grids, FP64 or FP32 form, "
-                       "40-160 steps each (step budget reduced from 2000 to keep the run short)")
+    rep["workload"] = (f"{n_req} requests, Poisson arrivals at {rate}/s, drawn from the serving "
+                       "distributions: 2D Riemann (outflow) or cylinder (immersed circle, Mach "
+                       "0.5-5 inflow), Kn log-uniform in [1e-3, 1], 4k-200k cells, 16^2-64^2 "
+                       f"velocity grids, FP64 or the FP32 form, each to the residual tolerance or "
+                       f"{max_steps} steps (step budget reduced from 2000 to keep the run short)")
     rep["slots"] = slots
+    rep["phase_cells_per_request"] = [int(r.case.phase_cells) for r in reqs]
     return rep
 
 

@@ -318,7 +318,7 @@ def bow_shock_state(mid, Ma, radius, gamma=5.0 / 3.0, Tw=1.0, delta=0.15, body="
 
 
 def _target_grids_chunk(args):
-    prims, targets, seeds, base, K = args
+    prims, targets, seeds, base, K, n_cap = args
     from .vspace import initial_vs_grid, refine_to_target
     out = []
     cache = {}
@@ -328,7 +328,7 @@ def _target_grids_chunk(args):
             cache[key] = initial_vs_grid(base, p, K)
         g = cache[key]
         if t > g.n:
-            g = refine_to_target(g, p, int(t), int(sd))
+            g = refine_to_target(g, p, int(t), int(sd), n_cap)
         out.append(g)
     return out
 
@@ -349,7 +349,7 @@ def target_grids(prim, s, base, K, n_min, n_max, seed, workers=0):
     step = max(1, -(-n // (workers * 4)))
     for c0 in range(0, n, step):
         c1 = min(n, c0 + step)
-        chunks.append((prim[c0:c1], targets[c0:c1], seeds[c0:c1], base, K))
+        chunks.append((prim[c0:c1], targets[c0:c1], seeds[c0:c1], base, K, n_max))
     if workers > 1 and n > 4096:
         import multiprocessing as mp
         with mp.get_context("fork").Pool(workers) as pool:
@@ -399,12 +399,14 @@ def cylinder_c3(n=448, nv=20, vmax=15.0, L=8.0, radius=0.5, Ma=5.0, Kn=0.1, omeg
 
 def sphere_c4(n=48, nv=16, vmax=10.0, L=6.0, radius=0.5, Ma=3.0, Kn=0.05, omega=0.81,
               vs_levels=1, n_min=4096, n_max=30000, precision="fp32f", flux="CAIDVM",
-              marching="CAIDVM_Marching", CFL=0.4, seed=20261204, workers=0):
+              marching="CAIDVM_Marching", CFL=0.4, seed=20261204, workers=0, walls=False,
+              immersed=True):
     """The sphere (example/sphere/sphere.jl): Sphere(Maxwellian, [0,0,0], radius, true, .,
     [1,0,0,0,1]) in [-L, L]^3 on n^3 leaves, Mach-Ma inflow at xmin, outflow elsewhere, a
     developed bow-shock state and per-cell adapted grids on a 16^3 base on [-vmax, vmax]^3
     (3D1F): each cell's initial reference grid refined toward n_min..n_max cells with the
-    local disturbance."""
+    local disturbance. walls=True closes the box with Maxwellian walls (freestream density and
+    temperature, at rest); immersed=False leaves the sphere out (same leaves, state, grids)."""
     from .ib import immerse_sphere
     from .solver import PCoordField
     solver = Solver(DIM=3, NDF=1, AMR_PS_MAXLEVEL=0, AMR_VS_MAXLEVEL=vs_levels, flux=flux,
@@ -414,6 +416,8 @@ def sphere_c4(n=48, nv=16, vmax=10.0, L=6.0, radius=0.5, Ma=3.0, Kn=0.05, omega=
     free = [1.0, U, 0.0, 0.0, 1.0]
     dom = [Domain("SuperSonicInflow", 1, free)] + [Domain("UniformOutflow", i)
                                                   for i in range(2, 7)]
+    if walls:
+        dom = [Domain("Maxwellian", i, [1.0, 0.0, 0.0, 0.0, 1.0]) for i in range(1, 7)]
     field = PCoordField(lambda m: bow_shock_state(m, Ma, radius, body="sphere")[0])
     cfg = Configure(solver, gas, [-L, L] * 3, [n, n, n], [-vmax, vmax] * 3, [nv] * 3, field,
                     dom, precision=precision)
@@ -422,6 +426,7 @@ def sphere_c4(n=48, nv=16, vmax=10.0, L=6.0, radius=0.5, Ma=3.0, Kn=0.05, omega=
         _, s = bow_shock_state(mid, Ma, radius, body="sphere")
         return target_grids(prim, s, base, None, n_min, n_max, seed, workers)
     case = build_case(cfg, vs_grids_fn=grids_fn)
-    case.topo = immerse_sphere(case.topo, (0.0, 0.0, 0.0), radius, True,
-                               (1.0, 0.0, 0.0, 0.0, 1.0))
+    if immersed:
+        case.topo = immerse_sphere(case.topo, (0.0, 0.0, 0.0), radius, True,
+                                   (1.0, 0.0, 0.0, 0.0, 1.0))
     return case

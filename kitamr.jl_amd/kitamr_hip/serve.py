@@ -30,7 +30,8 @@ class Request:
     arrival: float                # seconds after the start of the run
     max_steps: int = 2000
     tag: str = ""
-    nbytes: int = 0               # device bytes charged against the budget
+    nbytes: int = 0               # device bytes reserved for admission (an upper bound)
+    charged: int = 0              # device bytes the created context holds (kah_device_bytes)
     admitted: Optional[float] = None
     finished: Optional[float] = None
     steps: int = 0
@@ -115,18 +116,22 @@ class Server:
             now = self.clock() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            # admission: FIFO, while a slot and the memory for the head request are free
+            # admission: FIFO, while a slot and the memory for the head request are free;
+            # the request is admitted on its upper-bound estimate, then charged what its
+            # context actually holds on the device
             while (queue and len(active) < self.max_slots
                    and self.used + queue[0].nbytes <= self.budget):
                 r = queue.popleft()
                 r.ka = self.factory(r.case)
                 r.admitted = self.clock() - t0
-                self.used += r.nbytes
+                held = getattr(r.ka.ctx, "device_bytes", None)
+                r.charged = int(held()) if held else r.nbytes
+                self.used += r.charged
                 self.peak_used = max(self.peak_used, self.used)
                 active.append(r)
                 self.admit_order.append(r.rid)
                 self._say(f"[serve] t={r.admitted:.3f}s admit {r.rid} {r.tag} "
-                          f"({r.nbytes / 1e6:.1f} MB, {len(active)} active)")
+                          f"({r.charged / 1e6:.1f} MB, {len(active)} active)")
             self.peak_active = max(self.peak_active, len(active))
             if not active:
                 if pending:
@@ -153,7 +158,7 @@ class Server:
                 updates += int(r.case.phase_cells) * r.steps
                 r.ka.close()
                 r.ka = None
-                self.used -= r.nbytes
+                self.used -= r.charged
                 active.remove(r)
                 self._say(f"[serve] t={r.finished:.3f}s done {r.rid} {r.tag} steps={r.steps} "
                           f"latency={r.latency:.3f}s")
@@ -176,3 +181,41 @@ def poisson_arrivals(n, rate, seed):
     """Arrival times (s) of n requests of a Poisson process of the given rate (1/s)."""
     rng = np.random.default_rng(seed)
     return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def c5_specs(n_req, seed):
+    """Requests of the serving workload, drawn (seeded) from its stated distributions: case
+    type uniform over {Riemann, cylinder}; Kn = 10^U(-3, 0); Ma ~ U(0.5, 5); physical cells
+    log-uniform in [4k, 200k] (an n x n mesh, n = round(sqrt(cells)) rounded to even); velocity
+    grids m x m with m uniform over {16, 24, ..., 64} on [-V, V]^2, V = max(5, Ma sqrt(5/6) + 5)
+    (m even, so v = 0 lies on a cell corner); FP64 or (one in four) the FP32 form."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_req):
+        kind = "riemann" if rng.random() < 0.5 else "cylinder"
+        Kn = float(10.0 ** rng.uniform(-3.0, 0.0))
+        Ma = float(rng.uniform(0.5, 5.0))
+        cells = float(np.exp(rng.uniform(np.log(4e3), np.log(2e5))))
+        n = max(64, 2 * int(round(np.sqrt(cells) / 2.0)))
+        m = int(rng.choice(np.arange(16, 65, 8)))
+        vmax = max(5.0, Ma * np.sqrt(5.0 / 6.0) + 5.0)
+        prec = "fp32f" if rng.random() < 0.25 else "fp64"
+        out.append(dict(kind=kind, Kn=Kn, Ma=Ma, n=n, nv=m, vmax=vmax, precision=prec))
+    return out
+
+
+def c5_case(spec):
+    """The case of one serving request (c5_specs): the four-quadrant Riemann problem on
+    [0,1]^2 with outflow boundaries, or Circle(Maxwellian, r = 1) in [-4, 4]^2 with a Mach-Ma
+    inflow, at the reference's default CFL 0.4."""
+    from . import cases
+    if spec["kind"] == "riemann":
+        return cases.riemann_2d(n=spec["n"], nv=spec["nv"], vmax=spec["vmax"], Kn=spec["Kn"],
+                                CFL=0.4, precision=spec["precision"])
+    return cases.cylinder_2d(n=spec["n"], nv=spec["nv"], vmax=spec["vmax"], Ma=spec["Ma"],
+                             Kn=spec["Kn"], CFL=0.4, precision=spec["precision"])
+
+
+def c5_tag(spec):
+    return (f"{spec['kind']} {spec['n']}^2 cells {spec['nv']}^2 vel Kn={spec['Kn']:.2g} "
+            f"Ma={spec['Ma']:.2f} {spec['precision']}")

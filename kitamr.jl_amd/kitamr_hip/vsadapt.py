@@ -182,9 +182,13 @@ def balance_flags(topo, dim):
     :12-64, takes the maximum neighbour level over the cell's extent). Faces to a domain
     boundary or to a solid cell (SolidNeighbor) are skipped, as are cells inside a solid.
     Every flagged cell has to be refined in any balanced refinement of the current grids, so
-    repeating rounds reaches the same grids as the reference's cell-by-cell passes."""
+    repeating rounds reaches the same grids as the reference's cell-by-cell passes. A face
+    across a periodic boundary names the image record (n_cells + k) of its other cell; the
+    balance reads the image's source cell, which p4est would return as the neighbour."""
     off = np.asarray(topo["vs_offset"])
     benc = np.asarray(topo["bound_enc"])
+    n = len(off) - 1
+    img = np.asarray(topo.get("image_src", []), dtype=np.int64)
     L = int(topo["vs_maxlevel"])
     grids = cell_grids(topo, dim)
     keys = [grid_keys(topo, g[0], g[1], dim) for g in grids]
@@ -192,7 +196,13 @@ def balance_flags(topo, dim):
     here = np.asarray(topo["pair_here"])
     there = np.asarray(topo["pair_there"])
     for h, t in zip(here, there):
-        if t < 0 or benc[h] < 0 or benc[t] < 0:
+        if t < 0:
+            continue
+        if t >= n:
+            t = img[t - n]
+        if h >= n:
+            h = img[h - n]
+        if benc[h] < 0 or benc[t] < 0:
             continue
         for a, b in ((int(h), int(t)), (int(t), int(h))):
             ka, la = keys[a], grids[a][0].astype(np.int64)

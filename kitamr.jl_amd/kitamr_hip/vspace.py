@@ -219,11 +219,12 @@ def initial_vs_grid(base, prim, K, mode="lohner", coeffi_lohner=0.6):
     return g
 
 
-def refine_to_target(grid, prim, n_target, rng):
+def refine_to_target(grid, prim, n_target, rng, n_cap=None):
     """Seeded refinement of a grid toward the Maxwellian peak of prim until it holds at least
-    n_target cells (or no cell can be refined): round by round the unrefined cells carrying
-    the largest Maxwellian mass (with a seeded 25 % jitter) are split, as many as the missing
-    count needs. A workload generator, not a reference criterion."""
+    n_target cells (or no cell can be refined, or one more split would pass n_cap): round by
+    round the unrefined cells carrying the largest Maxwellian mass (with a seeded 25 % jitter)
+    are split, as many as the missing count needs. A workload generator, not a reference
+    criterion."""
     rng = np.random.default_rng(rng)
     g = grid.copy()
     dim = g.dim
@@ -238,6 +239,10 @@ def refine_to_target(grid, prim, n_target, rng):
         pri = np.exp(-lam * c2) * g.weight * (1.0 + 0.25 * rng.random(g.n))
         pri[~cand] = -1.0
         k = min(int(cand.sum()), -(-(n_target - g.n) // nch))
+        if n_cap is not None:
+            k = min(k, (int(n_cap) - g.n) // nch)
+        if k <= 0:
+            break
         sel = np.argpartition(-pri, k - 1)[:k]
         flags = np.zeros(g.n, dtype=bool)
         flags[sel] = True

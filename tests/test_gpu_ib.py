@@ -241,3 +241,39 @@ def test_velocity_adaptation_with_immersed_boundary():
     assert rel_err(dev.get_ib(flat=True), o2.get_ib(flat=True)) <= TOL
     dev.close()
     o2.close()
+
+
+@pytest.mark.gpu
+def test_balance_with_immersed_boundary():
+    """vs_balance! next to the immersed circle (grids of three levels that break the balance
+    as built): the rounds skip solid and inside-solid cells, whose grids stay as they were;
+    the cut cells follow the donors' refined grids; and the balanced state steps in parity
+    with the oracle built on the new topology."""
+    from kitamr_hip import ib, vsadapt
+    case = cases.cylinder_2d(n=16, nv=8, vs_levels=3)
+    say("[ib balance]")
+    params = kah_params(case.config)
+    dev, _ = pair(case)
+    for s in range(2):
+        dev.step(case.dt_xi, s)
+    topo2, rounds = vsadapt.balance_rounds(dev, case.topo, 2)
+    assert rounds >= 1
+    g0, g1 = vsadapt.cell_grids(case.topo, 2), vsadapt.cell_grids(topo2, 2)
+    benc = np.asarray(case.topo["bound_enc"])
+    for c in np.flatnonzero(benc < 0):
+        assert np.array_equal(g0[c][0], g1[c][0]) and np.array_equal(g0[c][1], g1[c][1]), c
+    assert any(len(g1[c][0]) > len(g0[c][0]) for c in np.flatnonzero(benc >= 0))
+    cut = ib.cut_lists(topo2, 2)
+    for key, arr in zip(("ib_cut_off", "ib_cut_index", "ib_cut_gas", "ib_cut_solid"), cut):
+        assert np.array_equal(topo2[key], arr), key
+    got = dev.get_state()
+    o2 = O.Oracle(params, topo2)
+    o2.set_state(got["w"], got["prim"], got["df"])
+    case2 = type(case)(case.config, topo2, got["w"], got["prim"], got["df"], case.dt_xi)
+    for s in range(2):
+        dev.step(case.dt_xi, 2 + s)
+        o2.step(case.dt_xi, 2 + s)
+    check_state(case2, dev.get_state(), o2.get_state(), TOL, "after balance")
+    assert rel_err(dev.get_ib(flat=True), o2.get_ib(flat=True)) <= TOL
+    dev.close()
+    o2.close()

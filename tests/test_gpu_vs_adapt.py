@@ -25,6 +25,8 @@ CASES = {
 # grids whose neighbours break the 2:1 velocity-level balance as built
 BALANCE_CASES = {
     "riemann8_vsamr3": lambda: cases.riemann_2d(n=8, nv=8, vs_levels=3),
+    # faces across the periodic boundaries name image records of their cells
+    "periodic8_vsamr3": lambda: cases.periodic_box_2d(n=8, nv=8, vs_levels=3),
     "box3d_vsamr2": lambda: cases.box_3d(n=4, nv=6, vs_levels=2),
 }
 

@@ -8,8 +8,11 @@ from kitamr_hip.solver import Residual, Status
 
 
 class FakeCtx:
-    def __init__(self, log, rid):
-        self.log, self.rid = log, rid
+    def __init__(self, log, rid, held=None):
+        self.log, self.rid, self.held = log, rid, held
+
+    def device_bytes(self):
+        return self.held
 
     def step(self, dt, step, residual=True):
         self.log.append(("step", self.rid))
@@ -26,10 +29,10 @@ class FakeCtx:
 
 
 class FakeKA:
-    def __init__(self, case, log):
+    def __init__(self, case, log, held=None):
         self.case = case
         self.config = case.config
-        self.ctx = FakeCtx(log, case.rid)
+        self.ctx = FakeCtx(log, case.rid, held if held is not None else case.nbytes)
         self.status = Status(1e-3, 1e-3, residual=Residual(np.zeros(4), np.zeros(4), np.ones(4)))
 
     def close(self):
@@ -43,6 +46,7 @@ def _requests(sizes, steps, arrivals):
         c = type("C", (), {})()
         c.config, c.topo, c.phase_cells, c.n_cells, c.rid = (base.config, base.topo,
                                                              base.phase_cells, base.n_cells, i)
+        c.nbytes = nb
         out.append(serve.Request(rid=i, case=c, arrival=t, max_steps=st, nbytes=nb))
     return out
 
@@ -71,6 +75,32 @@ def test_head_of_line_waits_for_memory():
     # request 1 (60 B) cannot join request 0 (70 B); request 2 queues behind it (FIFO)
     assert srv.admit_order == [0, 1, 2]
     assert reqs[1].admitted >= reqs[0].finished
+
+
+def test_charged_the_bytes_the_context_holds():
+    """Admission reserves the estimate, then the budget is charged what the created context
+    holds: two 60 B requests whose contexts hold 30 B each run together in a 100 B budget."""
+    log = []
+    srv = serve.Server(budget_bytes=100, max_slots=4, factory=lambda c: FakeKA(c, log, held=30))
+    reqs = _requests([60, 60], [5, 5], [0.0, 0.0])
+    srv.run(reqs)
+    assert srv.peak_active == 2 and srv.peak_used == 60
+    assert [r.charged for r in reqs] == [30, 30] and srv.used == 0
+
+
+def test_c5_specs_follow_the_stated_distributions():
+    sp = serve.c5_specs(400, 7)
+    n = np.array([s["n"] for s in sp])
+    m = np.array([s["nv"] for s in sp])
+    kn = np.array([s["Kn"] for s in sp])
+    ma = np.array([s["Ma"] for s in sp])
+    assert np.all(n % 2 == 0) and np.all(n * n >= 4000) and np.all(n * n <= 2.1e5)
+    assert set(m.tolist()) <= set(range(16, 65, 8)) and len(set(m.tolist())) == 7
+    assert np.all((kn >= 1e-3) & (kn <= 1.0)) and np.all((ma >= 0.5) & (ma <= 5.0))
+    kinds = [s["kind"] for s in sp]
+    assert 150 < kinds.count("riemann") < 250
+    assert np.all([s["vmax"] >= max(5.0, s["Ma"] * (5 / 6) ** 0.5 + 5.0) - 1e-12 for s in sp])
+    assert serve.c5_specs(5, 3) == serve.c5_specs(5, 3)
 
 
 def test_request_over_budget_is_rejected():

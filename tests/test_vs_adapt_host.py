@@ -192,13 +192,29 @@ def _boxes(topo, c, dim):
 
 
 def _fluid_pairs(topo):
+    """Face pairs of fluid cells; a periodic face's image record is read as its source."""
     benc = np.asarray(topo["bound_enc"])
-    return [(int(h), int(t)) for h, t in zip(topo["pair_here"], topo["pair_there"])
-            if t >= 0 and benc[h] >= 0 and benc[t] >= 0]
+    n = len(benc)
+    img = np.asarray(topo.get("image_src", []), dtype=np.int64)
+    src = lambda r: int(r) if r < n else int(img[r - n])
+    out = []
+    for h, t in zip(topo["pair_here"], topo["pair_there"]):
+        if t < 0:
+            continue
+        h, t = src(h), src(t)
+        if benc[h] >= 0 and benc[t] >= 0:
+            out.append((h, t))
+    return out
 
 
-@pytest.mark.parametrize("levels", [2, 3])
-def test_balance_is_a_forced_two_to_one_refinement(levels):
+def _balance_case(kind, levels):
+    if kind == "periodic":
+        return cases.periodic_box_2d(n=8, nv=8, vs_levels=levels)
+    return cases.riemann_2d(n=8, nv=8, vs_levels=levels)
+
+
+@pytest.mark.parametrize("kind,levels", [("riemann", 2), ("riemann", 3), ("periodic", 3)])
+def test_balance_is_a_forced_two_to_one_refinement(kind, levels):
     """vs_balance! (src/Velocity_space/Balance.jl): pinned by geometry, with overlaps found by
     brute force over velocity-cell boxes. After the rounds (a) no two overlapping velocity
     cells of face-neighbouring fluid cells differ by more than one level; (b) every grid
@@ -207,7 +223,9 @@ def test_balance_is_a_forced_two_to_one_refinement(levels):
     (c) every refinement was forced: the parent box of each cell made by balance overlaps a
     neighbour's cell at least two levels finer than that parent."""
     dim = 2
-    case = cases.riemann_2d(n=8, nv=8, vs_levels=levels)
+    case = _balance_case(kind, levels)
+    if kind == "periodic":   # faces across the periodic boundaries name image records
+        assert np.asarray(case.topo["pair_there"]).max() >= case.n_cells
     t0 = case.topo
     t, rounds = t0, 0
     while True:
