@@ -311,9 +311,10 @@ int64_t kah_phase_cells(const kah_ctx* ctx);
 /* Device bytes held by the context (state + topology). */
 int64_t kah_device_bytes(const kah_ctx* ctx);
 /* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
- * the flux/relaxation kernel (4 or 8; default 4 for the FP32 form on adapted velocity
- * grids whose per-workgroup LDS copy of f* is at most 40 KiB, else 8). KAH_OPT_SLOPE_VPT: velocity cells per thread
- * of the slope kernel (1..4). KAH_OPT_VEL_TILES: 1 (default) processes a shared
+ * the flux/relaxation kernel (3, 4 or 8; 0, the default, chooses by shape: on adapted velocity
+ * grids per launch the count that leaves the fewest idle thread slots over its grids, on
+ * shared grids 8). KAH_OPT_SLOPE_VPT: velocity cells per thread of the slope kernel (1..4;
+ * its workgroups have 3 or 4 waves, per cell whichever pads the grid least). KAH_OPT_VEL_TILES: 1 (default) processes a shared
  * tensor-product velocity grid in 8x8 / 4x4x4 tiles per wave, 0 in storage order.
  * KAH_OPT_OVERLAP_MAPS: 1 (default) precomputes, per fluid cell and neighbour with another
  * adapted velocity grid, the overlapping velocity cells (4 bytes per velocity cell and

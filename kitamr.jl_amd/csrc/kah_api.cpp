@@ -45,6 +45,7 @@ struct DBuf {
 struct LevelBin {
   int level;
   int maxh;
+  int wpi = 4;    // chunked kernel: waves per workgroup (3 or 4)
   int n;          // work items: cells (per-cell kernel) or (cell, chunk) pairs
   bool chunked;
   DBuf items;
@@ -77,7 +78,7 @@ struct kah_ctx {
   std::vector<LevelBin*> bins;
   DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey, d_gof;
   int64_t n_grids = 1, grid_cells = 0, map_entries = 0;   // distinct grids, their cells, map entries
-  DBuf d_df[2], d_sdf, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
+  DBuf d_df[2], d_sdf, d_oflux, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
   DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
   DBuf d_vtb, d_ovmap, d_moff;
   // ghost-exchange buffers: cell midpoints and bound_enc on the device, and the staging of
@@ -108,12 +109,31 @@ struct kah_ctx {
     if (fr_waves) return fr_waves;
     return (f32 && !uni && lds > 0 && lds <= 40 * 1024) ? 4 : 8;
   }
+  // Adapted grids: the waves per workgroup (3, 4 or 8) of one launch of cells whose grids
+  // have the sizes ns, as the fewest idle thread slots over the cells (each thread takes
+  // velocity cells p, p + 64*waves, ...); ties go to more waves. A 568-cell grid fills 3-wave
+  // workgroups to 99 % (3 x 192 slots), 4-wave ones to 74 % (3 x 256).
+  int flux_waves_for(const std::vector<int64_t>& ns, size_t lds) const {
+    if (fr_waves || uni) return flux_waves(lds);
+    int best = 8;
+    int64_t best_pad = -1;
+    for (int w : {8, 4, 3}) {
+      int64_t pad = 0;
+      for (int64_t n : ns) pad += (n + 64 * w - 1) / (64 * w) * (64 * w) - n;
+      if (best_pad < 0 || pad < best_pad) {
+        best = w;
+        best_pad = pad;
+      }
+    }
+    return best;
+  }
   // launches of the flux/relaxation kernel: the fluid cells by grid size, each launch with
   // the LDS its largest grid needs (0: the global scratch area)
   struct FrBin {
     DBuf cells;
     int n = 0;
     size_t lds = 0;
+    int waves = 8;          // waves per workgroup
     bool regular = false;   // fused step through the gather of regular cells
   };
   bool reg_enable = true;   // KAH_OPT_REGULAR
@@ -149,7 +169,7 @@ struct kah_ctx {
   // device memory held by the context
   int64_t device_bytes() const {
     const DBuf* bufs[] = {&d_fluid, &d_voff, &d_gu, &d_gw, &d_glev, &d_gkey, &d_gof, &d_df[0], &d_df[1],
-                          &d_sdf, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
+                          &d_sdf, &d_oflux, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
                           &d_res, &d_scratch, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
                           &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
                           &d_moff, &d_cmid, &d_benc, &d_xcells, &d_xoff, &d_xbuf, &d_xfluid,
@@ -1064,7 +1084,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   }
   // Per level and half-flux count: cells owning a wall half need the whole-cell wall density
   // and run the per-cell slope kernel (one partial row per half flux); all others run the
-  // chunked kernel (one partial row per slope_vpt*SLOPE_CHUNK velocity cells).
+  // chunked kernel (one partial row per chunk of 64*VPT*(3 or 4) velocity cells).
   std::vector<char> walled(size_t(nc), 0);
   for (int64_t c = 0; c < nc; ++c)
     for (int h = hoff[c]; h < hoff[c + 1]; ++h) walled[c] |= (halves[h].mode == MI_WALL);
@@ -1072,8 +1092,19 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     if (eoff[c + 1] - eoff[c] > MAX_ENTRIES)
       return C->fail(KAH_EUNSUPPORTED, "cell " + std::to_string(c) + " has more than " +
                                            std::to_string(MAX_ENTRIES) + " face sides");
+  // Chunks of the slope kernel: 3-wave or 4-wave workgroups (64*VPT velocity cells per wave),
+  // whichever pads the cell's grid least (a 568-cell grid fills one 3-wave chunk of 576 at
+  // VPT 3; ties go to 4 waves).
+  auto wpi_of = [&](int64_t c) {
+    const int64_t n = T->vs_offset[c + 1] - T->vs_offset[c];
+    auto padded = [&](int64_t w) {
+      const int64_t cv = w * 64 * C->slope_vpt;
+      return (n + cv - 1) / cv * cv;
+    };
+    return padded(3) < padded(4) ? 3 : 4;
+  };
   auto nchunks = [&](int64_t c) {
-    const int64_t cv = int64_t(C->slope_vpt) * SLOPE_CHUNK;
+    const int64_t cv = int64_t(C->slope_vpt) * 64 * wpi_of(c);
     return int((T->vs_offset[c + 1] - T->vs_offset[c] + cv - 1) / cv);
   };
   // Launch order of the slopes: the reference sweeps coarse to fine levels because the
@@ -1133,7 +1164,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     // bin 5: regular cells (the slopes and half fluxes of slope_regular / half_regular)
     const int caps[6] = {4, 6, 8, 16, 24, -1};
     std::vector<int32_t> lists[6];
-    std::vector<int2> works[6];
+    std::vector<int2> works[6][2];   // [bin][3-wave items, 4-wave items]
     for (int32_t c : C->fluid) {
       if (depth[size_t(c)] != L) continue;
       int nh = hoff[c + 1] - hoff[c];
@@ -1142,20 +1173,22 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
         lists[b].push_back(c);
       } else {
         if (slope_regular(c)) b = 5;
-        for (int ch = 0; ch < nchunks(c); ++ch) works[b].push_back(make_int2(c, ch));
+        auto& wl = works[b][wpi_of(c) == 3 ? 0 : 1];
+        for (int ch = 0; ch < nchunks(c); ++ch) wl.push_back(make_int2(c, ch));
       }
     }
     for (int b = 0; b < 6; ++b) {
-      for (int kind = 0; kind < 2; ++kind) {
-        size_t cnt = kind ? works[b].size() : lists[b].size();
+      for (int kind = 0; kind < 3; ++kind) {   // per-cell kernel, 3-wave items, 4-wave items
+        const size_t cnt = kind ? works[b][kind - 1].size() : lists[b].size();
         if (cnt == 0) continue;
         LevelBin* lb = new LevelBin();
         lb->level = L;
         lb->maxh = caps[b];
         lb->n = int(cnt);
-        lb->chunked = kind == 1;
+        lb->chunked = kind > 0;
+        lb->wpi = kind == 1 ? 3 : 4;
         C->bins.push_back(lb);
-        if (kind) KAH_TRY(C->upload(lb->items, works[b].data(), cnt, "slope work items"));
+        if (kind) KAH_TRY(C->upload(lb->items, works[b][kind - 1].data(), cnt, "slope work items"));
         else KAH_TRY(C->upload(lb->items, lists[b].data(), cnt, "level cells"));
       }
     }
@@ -1191,6 +1224,7 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     KAH_TRY(C->zero(C->d_df[1]));
   }
   KAH_TRY(C->alloc(C->d_sdf, fs * prow * NDF * D, "sdf"));   // + the SolidNeighbors' slopes
+  KAH_TRY(C->alloc(C->d_oflux, fs * prow * NDF * D, "outgoing micro fluxes"));
   KAH_TRY(C->alloc(C->d_psflux, sizeof(double) * nc * NW, "flux"));
   KAH_TRY(C->alloc(C->d_w, sizeof(double) * nc * NW, "w"));
   KAH_TRY(C->alloc(C->d_prim, sizeof(double) * nc * NW, "prim"));
@@ -1213,11 +1247,21 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   for (auto* b : C->frbins) delete b;
   C->frbins.clear();
   {
+    // the regular gather reads the outgoing micro fluxes the slope kernel stores while it
+    // forms the half fluxes (CAIDVM only), so neither side may be a cell whose slope kernel
+    // forms them in another arithmetic (the per-cell kernel of walled cells; in the FP32 form
+    // CIP_Marching gathers in FP64 while the slope kernel reconstructs in FP32)
+    const bool caidvm = C->P.flux_kind == KAH_FLUX_CAIDVM;
+    const bool same_arith = !(C->f32 && C->P.marching == KAH_MARCH_CIP);
     auto regular = [&](int32_t c) {
-      if (!C->reg_enable || eoff[c + 1] - eoff[c] != 2 * D) return false;
+      if (!C->reg_enable || !caidvm || !same_arith || walled[c] || eoff[c + 1] - eoff[c] != 2 * D)
+        return false;
       for (int e = eoff[c]; e < eoff[c + 1]; ++e) {
         const EntryDesc& ed = ents[size_t(e)];
         if (ed.oth_cell < 0 || ed.oth_cell >= nc || T->bound_enc[ed.oth_cell] < 0) return false;
+        // a neighbour on another level shares a hanging face: its side holds more than one
+        // face, each with its own reconstruction, so one stored outgoing flux cannot serve
+        if (walled[ed.oth_cell] || T->ps_level[ed.oth_cell] != T->ps_level[c]) return false;
         if (!C->uni && ed.map != MAP_SAME) return false;
         if ((ed.own_mode != MI_LIMITED && ed.own_mode != MI_UNLIMITED) ||
             (ed.oth_mode != MI_LIMITED && ed.oth_mode != MI_UNLIMITED))
@@ -1249,6 +1293,10 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       fb->regular = (b & 1) != 0;
       fb->lds = sc_elem * size_t(NDF + 1) * size_t(bmax[b]);
       if (fb->lds > C->lds_limit) fb->lds = 0;
+      std::vector<int64_t> ns;
+      ns.reserve(lists[b].size());
+      for (int32_t c : lists[b]) ns.push_back(T->vs_offset[c + 1] - T->vs_offset[c]);
+      fb->waves = C->flux_waves_for(ns, fb->lds);
       need_scratch |= fb->lds == 0;
       KAH_TRY(C->upload(fb->cells, lists[b].data(), lists[b].size(), "flux cells"));
     }
@@ -1288,6 +1336,7 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.df = C->d_df[C->cur].as<double>();
   A.df_out = C->d_df[1 - C->cur].as<double>();
   A.sdf = C->d_sdf.as<double>();
+  A.oflux = C->d_oflux.as<double>();
   A.vflux = C->d_vflux.as<double>();
   A.psflux = C->d_psflux.as<double>();
   A.w = C->d_w.as<double>();
@@ -1349,7 +1398,7 @@ int run_slope(kah_ctx* C, const DevArgs& A) {
   for (LevelBin* b : C->bins) {
     if (b->chunked)
       KAH_TRY(C->hip(launch_slope_chunk(A, C->D, C->uni, C->f32, b->items.as<int2>(), b->n, b->maxh,
-                                        C->slope_vpt, C->stream), "slope kernel launch"));
+                                        C->slope_vpt, b->wpi, C->stream), "slope kernel launch"));
     else
       KAH_TRY(C->hip(launch_slope(A, C->D, C->uni, C->f32, b->items.as<int32_t>(), b->n, b->maxh,
                                   C->stream), "slope kernel launch"));
@@ -1380,7 +1429,8 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   for (const kah_ctx::FrBin* b : C->frbins) {
     const int m = (mode == MODE_FUSED && b->regular) ? MODE_FUSED_REG : mode;
     KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, b->cells.as<int32_t>(), b->n, m,
-                                     march, b->lds, C->flux_waves(b->lds), C->stream),
+                                     march, b->lds, C->fr_waves ? C->fr_waves : b->waves,
+                                     C->stream),
                    "flux/relaxation kernel launch"));
   }
   if (C->timing) KAH_TRY(ev_end(C, e));
@@ -1997,8 +2047,8 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
   if (!C) return KAH_EINVAL;
   switch (option) {
     case KAH_OPT_FLUX_WAVES:
-      if (value != 4 && value != 8)
-        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 4 or 8");
+      if (value != 0 && value != 3 && value != 4 && value != 8)
+        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 3, 4 or 8 (0: by shape)");
       C->fr_waves = int(value);
       C->drop_graphs();   // captured graphs hold the launch shape
       return KAH_OK;

@@ -35,73 +35,136 @@ __device__ inline double wave_sum(double v) {
 }
 
 // Workgroup sum of the first cnt of CNT per-thread partials. out[q] (LDS) receives the
-// total of value q; red needs NWV*CNT doubles of LDS. Two barriers, fixed order.
+// total of value q; red needs 2*NWV*red_stride(CNT) doubles of LDS. Three barriers, fixed
+// order; the sum is compensated (block_totals).
+template <int CNT, int NWV>
+__device__ inline void block_reduce(const double (&vals)[CNT], int cnt, double* red,
+                                    double* out);
+
+// Compensated sums. two_sum(a, b) returns a + b rounded and the exact rounding error
+// (Knuth's TwoSum, six additions, no branch); it is symmetric in a and b, so two lanes that add
+// each other's values get bit-identical results.
+__device__ inline double two_sum(double a, double b, double& err) {
+  const double s = a + b;
+  const double bb = s - a;
+  err = (a - (s - bb)) + (b - bb);
+  return s;
+}
+
+// Sum over the 64 lanes of a wave of (s, e) pairs (a value and its running error): an xor
+// butterfly whose every level adds with two_sum and carries the errors, so every lane ends
+// with the same compensated total s + e.
+__device__ inline void wave_sum2(double& s, double& e) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double so = __shfl_xor(s, o, 64);
+    const double eo = __shfl_xor(e, o, 64);
+    double err;
+    s = two_sum(s, so, err);
+    e = (e + eo) + err;
+  }
+}
+
+constexpr int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+// per-wave stride of block_totals' LDS (in (s, e) pairs)
+constexpr int red_stride(int cnt) { return pow2_at_least(cnt) <= 64 ? pow2_at_least(cnt) : cnt; }
+
+// Compensated reduce-scatter of P (a power of two, at most 64) values over the 64 lanes of a
+// wave. At the level with xor offset o (32, 16, ..., 64/P) every lane keeps one half of its
+// current values (the upper half when its lane bit o is set) and adds the partner's copy of
+// that half with two_sum; after log2(P) levels lane L holds, for value (L >> log2(64/P)), the
+// sum over the lanes that share its high bits, and a plain two_sum butterfly over the low
+// bits finishes it. P/2 + P/4 + ... + 1 + log2(64/P) exchanges instead of P*6.
+template <int P>
+__device__ inline void wave_scatter2(double (&v)[P], double (&e)[P]) {
+  static_assert(P >= 1 && P <= 64 && (P & (P - 1)) == 0, "power of two up to 64");
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int m = P, o = 32; m > 1; m >>= 1, o >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int j = 0; j < m / 2; ++j) {
+      const double keep = up ? v[m / 2 + j] : v[j], ke = up ? e[m / 2 + j] : e[j];
+      const double send = up ? v[j] : v[m / 2 + j], se = up ? e[j] : e[m / 2 + j];
+      const double recv = __shfl_xor(send, o, 64), re = __shfl_xor(se, o, 64);
+      double err;
+      v[j] = two_sum(keep, recv, err);
+      e[j] = (ke + re) + err;
+    }
+  }
+#pragma unroll
+  for (int o = 32 / P; o > 0; o >>= 1) {
+    const double so = __shfl_xor(v[0], o, 64), eo = __shfl_xor(e[0], o, 64);
+    double err;
+    v[0] = two_sum(v[0], so, err);
+    e[0] = (e[0] + eo) + err;
+  }
+}
+
+// Compensated sum of the first cnt of CNT per-thread values over a workgroup of NWV waves:
+// per wave a compensated reduce-scatter (or, for more than 64 values, one two_sum butterfly
+// per value), then the waves' (s, e) pairs added in wave order with two_sum; thread q < cnt
+// returns the total of value q. red needs 2*NWV*RED_STRIDE(CNT) doubles of LDS (the power of
+// two at least CNT, or CNT above 64). Two barriers, fixed order.
+template <int CNT, int NWV>
+__device__ inline double block_totals(const double (&vals)[CNT], int cnt, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int P = pow2_at_least(CNT);
+  if constexpr (P <= 64) {
+    double v[P], e[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      v[q] = q < CNT ? vals[q] : 0.0;
+      e[q] = 0.0;
+    }
+    wave_scatter2<P>(v, e);
+    constexpr int SH = (P == 1) ? 6 : (P == 2) ? 5 : (P == 4) ? 4 : (P == 8) ? 3 : (P == 16) ? 2 : (P == 32) ? 1 : 0;
+    if ((lane & ((1 << SH) - 1)) == 0) {
+      const int q = lane >> SH;
+      red[(wv * P + q) * 2] = v[0];
+      red[(wv * P + q) * 2 + 1] = e[0];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < CNT; ++q) {
+      if (q < cnt) {
+        double v = vals[q], e = 0.0;
+        wave_sum2(v, e);
+        if (lane == 0) {
+          red[(wv * CNT + q) * 2] = v;
+          red[(wv * CNT + q) * 2 + 1] = e;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int STRIDE = red_stride(CNT);
+  double out = 0.0;
+  if (int(threadIdx.x) < cnt) {
+    const int q = threadIdx.x;
+    double S = red[q * 2], E = red[q * 2 + 1];
+#pragma unroll
+    for (int w = 1; w < NWV; ++w) {
+      double err;
+      S = two_sum(S, red[(w * STRIDE + q) * 2], err);
+      E = (E + red[(w * STRIDE + q) * 2 + 1]) + err;
+    }
+    out = S + E;
+  }
+  __syncthreads();
+  return out;
+}
+
 template <int CNT, int NWV>
 __device__ inline void block_reduce(const double (&vals)[CNT], int cnt, double* red,
                                     double* out) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < CNT; ++q) {
-    if (q < cnt) {
-      double s = wave_sum(vals[q]);
-      if (lane == 0) red[wv * CNT + q] = s;
-    }
-  }
+  const double t = block_totals<CNT, NWV>(vals, cnt, red);
+  if (int(threadIdx.x) < cnt) out[threadIdx.x] = t;
   __syncthreads();
-  if (int(threadIdx.x) < cnt) {
-    double s = red[threadIdx.x];
-#pragma unroll
-    for (int w = 1; w < NWV; ++w) s += red[w * CNT + threadIdx.x];
-    out[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
-
-// Workgroup sum (SLOPE_CHUNK = 256 threads) of the first cnt of CNT per-thread values,
-// through LDS, in batches of CHUNK_QB values: the batch is written transposed, 32 threads
-// per value each add a strided set of 8 lanes, 4 threads per value add 8 of those
-// partials, and one thread adds the last 4; fixed order throughout. buf needs
-// CHUNK_QB*SLOPE_CHUNK doubles, part CHUNK_QB*36, out cnt.
-constexpr int CHUNK_QB = 8;
-template <int CNT, int B0 = 0, class V>
-__device__ inline void chunk_sum(const V (&vals)[CNT], int cnt, double* buf, double* part,
-                                 double* out) {
-  static_assert(SLOPE_CHUNK == 256, "chunk_sum assumes 256 threads");
-  constexpr int NT = SLOPE_CHUNK, QB = CHUNK_QB, TPV = NT / QB, RPT = NT / TPV;
-  double* part2 = part + QB * TPV;
-  const int t = threadIdx.x;
-  if (B0 >= cnt) return;   // cnt is uniform over the workgroup
-#pragma unroll
-  for (int q = 0; q < QB; ++q) {
-    if constexpr (B0 + QB <= CNT) {
-      if (B0 + q < cnt) buf[q * NT + t] = vals[B0 + q];
-    } else {
-      if (B0 + q < cnt) buf[q * NT + t] = vals[B0 + q < CNT ? B0 + q : CNT - 1];
-    }
-  }
-  __syncthreads();
-  {
-    const int q = t / TPV, j = t % TPV;
-    if (B0 + q < cnt) {
-      double s = buf[q * NT + j];
-#pragma unroll
-      for (int r = 1; r < RPT; ++r) s += buf[q * NT + r * TPV + j];
-      part[q * TPV + j] = s;
-    }
-  }
-  __syncthreads();
-  if (t < QB * 4 && B0 + t / 4 < cnt) {
-    const double* p = &part[(t / 4) * TPV + (t % 4) * (TPV / 4)];
-    double s = p[0];
-#pragma unroll
-    for (int j = 1; j < TPV / 4; ++j) s += p[j];
-    part2[t] = s;
-  }
-  __syncthreads();
-  if (t < QB && B0 + t < cnt)
-    out[B0 + t] = ((part2[4 * t] + part2[4 * t + 1]) + part2[4 * t + 2]) + part2[4 * t + 3];
-  __syncthreads();
-  if constexpr (B0 + QB < CNT) chunk_sum<CNT, B0 + QB, V>(vals, cnt, buf, part, out);
 }
 
 // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs; give every XCD
@@ -172,6 +235,7 @@ struct Grid {
   const FT* fb;
   const FT* sb;
   FT* swb;
+  FT* ofb;
   FT* ob;
   const double* gub;
   const double* gwb;
@@ -188,6 +252,7 @@ struct Grid {
     fb = reinterpret_cast<const FT*>(a.df) + off * NDF;
     sb = reinterpret_cast<const FT*>(a.sdf_r) + off * NDF * D;
     swb = reinterpret_cast<FT*>(a.sdf) + off * NDF * D;
+    ofb = reinterpret_cast<FT*>(a.oflux) + off * NDF * D;
     ob = reinterpret_cast<FT*>(a.df_out) + off * NDF;
     gub = a.gu + goff() * D;
     gwb = a.gw + goff();
@@ -230,6 +295,10 @@ struct Grid {
   // accessors widen to double, and the FP32 form's arithmetic narrows where it advects
   __device__ double f(int i, int k) const { return double(at(fb, k * n + i)); }
   __device__ double s(int i, int k, int d) const { return double(at(sb, (d * NDF + k) * n + i)); }
+  // the outgoing micro flux of velocity cell i through the cell's face in direction d
+  __device__ void set_of(int i, int k, int d, double v) const {
+    at(ofb, (d * NDF + k) * n + i) = FT(v);
+  }
   __device__ void set_s(int i, int k, int d, double v) const {
     at(swb, (d * NDF + k) * n + i) = FT(v);
   }
@@ -554,6 +623,12 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
   } else if (hd.data_cell == c) {
     micro_recon<D, NDF>(hd.mode, f, s, u, T(A.dt), hd.xrec, hd.dsrec, hd.fmid, hd.dx_f, hd.ds_f,
                         hd.dir, mic);
+    // the cell's own outgoing flux through an inner face (a domain face's second half is the
+    // ghost side: the cell's data flowing in)
+    if (hd.bc < 0) {
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) G.set_of(i, k, hd.dir, mic[k]);
+    }
   } else {
     // the there side of a face whose other cell is solid: f of that cell * vn (shared
     // velocity grids only; checked on the host)
@@ -596,8 +671,9 @@ __device__ inline void slope_regular(const DevArgs& A, const Grid<D, NDF, UNI, F
 
 // half_terms of a regular cell: every half flux it owns is its own limited or plain-slope
 // upwind reconstruction.
-template <int D, int NDF, class T>
-__device__ inline void half_regular(const DevArgs& A, const HalfDesc& hd, const T (&u)[D],
+template <int D, int NDF, bool UNI, class FT, class T>
+__device__ inline void half_regular(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G,
+                                    const HalfDesc& hd, int i, const T (&u)[D],
                                     const T (&f)[NDF], const T (&s)[D][NDF], T wt,
                                     double (&m)[D + 2]) {
 #pragma unroll
@@ -609,6 +685,8 @@ __device__ inline void half_regular(const DevArgs& A, const HalfDesc& hd, const 
   T mic[NDF];
   micro_recon<D, NDF>(hd.mode, f, s, u, T(A.dt), hd.xrec, hd.dsrec, hd.fmid, hd.dx_f, hd.ds_f,
                       hd.dir, mic);
+#pragma unroll
+  for (int k = 0; k < NDF; ++k) G.set_of(i, k, hd.dir, mic[k]);
   moment_terms<D, NDF, T>(wt, u, mic, m);
 }
 
@@ -624,7 +702,7 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   const Grid<D, NDF, UNI, FT> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
-  __shared__ double red[NWV * CNT];
+  __shared__ double red[2 * NWV * red_stride(CNT)];
   __shared__ double tot[CNT];
   __shared__ double rho_sh[MAXH];
 
@@ -712,37 +790,40 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
   }
 }
 
-// One velocity cell per thread, SLOPE_CHUNK velocity cells per workgroup: the same per-cell
-// work as k_slope without the loop over the grid, for cells without wall faces. Each
-// workgroup writes its partial sums of the cell's half fluxes to row part_off[slot]+chunk;
-// k_flux_relax adds the rows in chunk order.
-template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT, bool REG = false>
-__global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
-                                                             const int2* __restrict__ work,
-                                                             int nwork, KAH_RO_PARAMS) {
+// VPT velocity cells per thread, one (cell, chunk) work item per workgroup: the same
+// per-cell work as k_slope without the loop over the grid, for cells without wall faces.
+// Each workgroup writes its partial sums of the cell's half fluxes to row
+// part_off[slot]+chunk; k_flux_relax adds the rows in chunk order.
+template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT, bool REG = false, int WPI = 4>
+__global__ __launch_bounds__(WPI * 64) void k_slope_chunk(DevArgs A0,
+                                                          const int2* __restrict__ work,
+                                                          int nwork, KAH_RO_PARAMS) {
+  // WPI waves (3 or 4) per workgroup, 64*WPI*VPT velocity cells per chunk; the host picks
+  // WPI per cell so that the chunks pad its grid least (a 568-cell grid fills one 3-wave
+  // chunk of 576 where a 4-wave chunk would hold 768).
+  static_assert(WPI == 3 || WPI == 4, "three or four waves per work item");
   constexpr int NW = D + 2;
   constexpr int CNT = MAXH * NW;
+  constexpr int CH = 64 * WPI;   // threads per item
   DevArgs A = A0;
   KAH_RO_BIND(A);
   const int2 wk = work[xcd_remap(blockIdx.x, nwork)];
   const int c = wk.x, ch = wk.y;
+  const int tid = threadIdx.x;
   const Grid<D, NDF, UNI, FT> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
-  __shared__ double buf[CHUNK_QB * SLOPE_CHUNK];
-  __shared__ double part[CHUNK_QB * 36];
-  __shared__ double tot[CNT];
+  __shared__ double red[2 * WPI * red_stride(CNT)];
 
-  // Per-thread partials (VPT <= 3 terms each) in the arithmetic type: FP32 in the FP32 form,
-  // which halves their registers; the chunk sum over threads is FP64 in both forms.
-  using AT = typename Arith<FT>::type;
-  AT acc[CNT];
+  // Per-thread partials (VPT <= 4 terms each) in FP64; the sum over the item's threads is
+  // compensated (two_sum butterflies, then the waves in order).
+  double acc[CNT];
 #pragma unroll
   for (int q = 0; q < CNT; ++q) acc[q] = 0.0;
-  // velocity cells ch*VPT*SLOPE_CHUNK + r*SLOPE_CHUNK + threadIdx.x, r < VPT
+  // velocity cells (ch*VPT + r)*CH + tid, r < VPT
 #pragma unroll
   for (int r = 0; r < VPT; ++r) {
-    const int p = (ch * VPT + r) * SLOPE_CHUNK + int(threadIdx.x);
+    const int p = (ch * VPT + r) * CH + tid;
     if (p < G.n) {
       using CT = typename Arith<FT>::type;
       const int i = G.vi(p);
@@ -765,7 +846,7 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
       for (int h = 0; h < MAXH; ++h) {
         if (h < nh) {
           double m[NW];
-          if constexpr (REG) half_regular<D, NDF, CT>(A, A.halves[hb + h], u, f, s, wt, m);
+          if constexpr (REG) half_regular<D, NDF, UNI, FT, CT>(A, G, A.halves[hb + h], i, u, f, s, wt, m);
           else half_terms<D, NDF, UNI, FT, CT>(A, G, c, A.halves[hb + h], i, u, f, s, wt, m);
 #pragma unroll
           for (int q = 0; q < NW; ++q) acc[h * NW + q] = (r == 0) ? m[q] : acc[h * NW + q] + m[q];
@@ -773,10 +854,10 @@ __global__ __launch_bounds__(SLOPE_CHUNK) void k_slope_chunk(DevArgs A0,
       }
     }
   }
-  chunk_sum<CNT>(acc, nh * NW, buf, part, tot);
-  if (int(threadIdx.x) < nh * NW) {
-    const int h = threadIdx.x / NW, q = threadIdx.x % NW;
-    A.half_fw[(A.part_off[A.halves[hb + h].slot] + ch) * NW + q] = tot[threadIdx.x];
+  const double tot = block_totals<CNT, WPI>(acc, nh * NW, red);
+  if (tid < nh * NW) {
+    const int h = tid / NW, q = tid % NW;
+    A.half_fw[(A.part_off[A.halves[hb + h].slot] + ch) * NW + q] = tot;
   }
 }
 
@@ -856,13 +937,16 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
 }
 
 // gather_flux for a regular cell: its 2*DIM entries are faces to fluid cells on the same
-// velocity grid (or a shared grid), so an incoming velocity cell i reads the neighbour's
-// cell i with the own velocities, and both sides use the limited or plain-slope
-// reconstruction. The same terms in the same order as gather_flux, so the same result.
+// velocity grid (or a shared grid), whose micro flux through the face, in each direction,
+// the slope kernel stored as the outgoing micro flux of the upwind cell (its own
+// reconstruction with the same face parameters and arithmetic as gather_flux's). Velocity
+// cell i of the cell reads its own outgoing flux or the neighbour's cell i: the same terms in
+// the same order as gather_flux, so the same result, without slopes or reconstruction.
 template <int D, int NDF, bool UNI, class FT, class T>
-__device__ inline void gather_regular(const DevArgs& A, int n, int eb, int i, const T (&u)[D],
-                                      const T (&f)[NDF], const T (&s)[D][NDF], T (&fl)[NDF]) {
-  const T dt = T(A.dt);
+__device__ inline void gather_regular(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int eb,
+                                      int i, const T (&u)[D], T (&fl)[NDF]) {
+  const int n = G.n;
+  const FT* of = reinterpret_cast<const FT*>(A.oflux);
 #pragma unroll
   for (int e = 0; e < 2 * D; ++e) {
     const EntryDesc& ed = A.ents[eb + e];
@@ -870,28 +954,14 @@ __device__ inline void gather_regular(const DevArgs& A, int n, int eb, int i, co
     const T ru = desc_val<T>(ed.rot, ed.rot_f) * ud;
     const bool own = ed.side ? (ru > T(0)) : (ru <= T(0));
     const T area = desc_val<T>(ed.area, ed.area_f);
-    T mic[NDF];
-    if (own) {
-      micro_recon<D, NDF>(ed.own_mode, f, s, u, dt, ed.xown, ed.dsown, ed.fmid, ed.dxo_f,
-                          ed.dso_f, ed.dir, mic);
-    } else {
-      // the neighbour's rows: same grid, so same size n and velocity cell i
-      const int64_t no = UNI ? int64_t(ed.oth_cell) * n : A.voff[ed.oth_cell];
-      const FT* nf = reinterpret_cast<const FT*>(A.df) + no * NDF;
-      const FT* ns = reinterpret_cast<const FT*>(A.sdf_r) + no * NDF * D;
-      T fj[NDF], sj[D][NDF];
-#pragma unroll
-      for (int k = 0; k < NDF; ++k) {
-        fj[k] = T(Grid<D, NDF, UNI, FT>::at(nf, k * n + i));
-#pragma unroll
-        for (int t = 0; t < D; ++t) sj[t][k] = T(Grid<D, NDF, UNI, FT>::at(ns, (t * NDF + k) * n + i));
-      }
-      micro_recon<D, NDF>(ed.oth_mode, fj, sj, u, dt, ed.xoth, ed.dsoth, ed.fmid, ed.dxt_f,
-                          ed.dst_f, ed.dir, mic);
-    }
+    // rows of the upwind cell: the cell's own, or the neighbour's (same grid, same size n)
+    const int64_t no = UNI ? int64_t(ed.oth_cell) * n : A.voff[ed.oth_cell];
+    const int64_t ro = own ? G.off : no;
+    const FT* b = of + ro * (NDF * D);
 #pragma unroll
     for (int k = 0; k < NDF; ++k) {
-      T t = area * mic[k];
+      const T mic = T(Grid<D, NDF, UNI, FT>::at(b, (ed.dir * NDF + k) * n + i));
+      const T t = area * mic;
       fl[k] += ed.side ? -t : t;
     }
   }
@@ -1251,9 +1321,9 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   const int eb = A.ent_off[c];
   const int ne = A.ent_off[c + 1] - eb;
   const double dt = A.dt;
-  constexpr int NRED = (MARCH == MARCH_CIP) ? (D + 2) * (D + 5) / 2 : 8;   // G and J of the
-                                                                         // I-projection
-  __shared__ double red[NWV * NRED];
+  // the largest block sum: G and J of the I-projection (CIP_Marching), else D+2 values
+  constexpr int NRED = red_stride((MARCH == MARCH_CIP) ? (D + 2) * (D + 5) / 2 : 8);
+  __shared__ double red[2 * NWV * NRED];
   __shared__ double res[8];
   __shared__ double cell_sh[4 * NW];   // wnew | prim_c | prim | q
   __shared__ double hsum[MAX_ENTRIES * 2 * NW];   // the half fluxes of the cell's faces
@@ -1276,8 +1346,13 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     const int slot = A.ents[eb + e].pair * 2 + s;
     const int64_t off = A.part_off[slot];
     const int cnt = A.part_cnt[slot];
-    double sum = 0.0;
-    for (int k = 0; k < cnt; ++k) sum += A.half_fw_r[(off + k) * NW + q];
+    double sum = 0.0, err = 0.0;   // the rows in chunk order, compensated
+    for (int k = 0; k < cnt; ++k) {
+      double e;
+      sum = two_sum(sum, A.half_fw_r[(off + k) * NW + q], e);
+      err += e;
+    }
+    sum = sum + err;
     hsum[threadIdx.x] = (q == NW - 1) ? 0.5 * sum : sum;
   }
   __syncthreads();
@@ -1422,7 +1497,11 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
       for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
-      if (FUSED) {
+      if constexpr (MODE == MODE_FUSED_REG) {
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) fl[k] = CT(0);
+        gather_regular<D, NDF, UNI, FT, CT>(A, G, eb, i, u, fl);
+      } else if (FUSED) {
         CT s[D][NDF];
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -1430,10 +1509,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 #pragma unroll
           for (int d = 0; d < D; ++d) s[d][k] = CT(G.s(i, k, d));
         }
-        if constexpr (MODE == MODE_FUSED_REG)
-          gather_regular<D, NDF, UNI, FT, CT>(A, n, eb, i, u, f, s, fl);
-        else
-          gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.keyp(i), fl);
+        gather_flux<D, NDF, UNI, FT, CT>(A, c, n, eb, ne, i, u, f, s, G.lev(i), G.keyp(i), fl);
       } else {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) {
@@ -2017,7 +2093,7 @@ __global__ __launch_bounds__(NWV * 64) void k_vs_project(DevArgs A, const int32_
                                                          double* __restrict__ scratch,
                                                          FT* __restrict__ dfw) {
   constexpr int M = D + 2, BS = NWV * 64;
-  __shared__ double red[NWV * (D + 2) * (D + 5) / 2];   // G and J of the Newton step
+  __shared__ double red[2 * NWV * red_stride((D + 2) * (D + 5) / 2)];   // G and J of the Newton step
   __shared__ double st[64];
   extern __shared__ double lds[];
   const int c = cells[blockIdx.x];
@@ -2114,7 +2190,7 @@ __global__ __launch_bounds__(256) void k_macro_slope(DevArgs A, const int32_t* _
   constexpr int NW = D + 2;
   const int c = cells[blockIdx.x];
   const Grid<D, NDF, UNI, FT> G(A, c);
-  __shared__ double red[4 * NW];
+  __shared__ double red[2 * 4 * red_stride(NW)];
   __shared__ double tot[NW];
   for (int d = 0; d < D; ++d) {
     double acc[NW];
@@ -2157,39 +2233,47 @@ inline hipError_t launch_slope_t(const DevArgs& A, const int32_t* cells, int n, 
   return hipGetLastError();
 }
 
-template <int D, int NDF, bool UNI, class FT, int VPT>
-inline hipError_t launch_slope_chunk_v(const DevArgs& A, const int2* work, int n, int maxh,
+template <int D, int NDF, bool UNI, class FT, int VPT, int WPI>
+inline hipError_t launch_slope_chunk_w(const DevArgs& A, const int2* work, int n, int maxh,
                                        hipStream_t st) {
+  const dim3 grid(n), block(WPI * 64);
   if (maxh < 0)   // regular cells: 2*D half fluxes
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 2 * D, VPT, true>), dim3(n),
-                       dim3(SLOPE_CHUNK), 0, st, A, work, n, KAH_RO_ARGS(A));
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 2 * D, VPT, true, WPI>), grid, block, 0,
+                       st, A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 4)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT, false, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 6)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 6, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 6, VPT, false, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 8)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 8, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 8, VPT, false, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 16)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 16, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 16, VPT, false, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 24, VPT>), dim3(n), dim3(SLOPE_CHUNK), 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 24, VPT, false, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   return hipGetLastError();
 }
 
+template <int D, int NDF, bool UNI, class FT, int VPT>
+inline hipError_t launch_slope_chunk_v(const DevArgs& A, const int2* work, int n, int maxh,
+                                       int wpi, hipStream_t st) {
+  return wpi == 3 ? launch_slope_chunk_w<D, NDF, UNI, FT, VPT, 3>(A, work, n, maxh, st)
+                  : launch_slope_chunk_w<D, NDF, UNI, FT, VPT, 4>(A, work, n, maxh, st);
+}
+
 template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_slope_chunk_t(const DevArgs& A, const int2* work, int n, int maxh,
-                                       int vpt, hipStream_t st) {
+                                       int vpt, int wpi, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   switch (vpt) {
-    case 1: return launch_slope_chunk_v<D, NDF, UNI, FT, 1>(A, work, n, maxh, st);
-    case 2: return launch_slope_chunk_v<D, NDF, UNI, FT, 2>(A, work, n, maxh, st);
-    case 3: return launch_slope_chunk_v<D, NDF, UNI, FT, 3>(A, work, n, maxh, st);
-    case 4: return launch_slope_chunk_v<D, NDF, UNI, FT, 4>(A, work, n, maxh, st);
+    case 1: return launch_slope_chunk_v<D, NDF, UNI, FT, 1>(A, work, n, maxh, wpi, st);
+    case 2: return launch_slope_chunk_v<D, NDF, UNI, FT, 2>(A, work, n, maxh, wpi, st);
+    case 3: return launch_slope_chunk_v<D, NDF, UNI, FT, 3>(A, work, n, maxh, wpi, st);
+    case 4: return launch_slope_chunk_v<D, NDF, UNI, FT, 4>(A, work, n, maxh, wpi, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2238,6 +2322,7 @@ inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int
 template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int mode,
                               int march, size_t lds, int nwv, hipStream_t st) {
+  if (nwv == 3) return launch_fr_w<D, NDF, UNI, FT, 3>(A, cells, n, mode, march, lds, st);
   if (nwv == 4) return launch_fr_w<D, NDF, UNI, FT, 4>(A, cells, n, mode, march, lds, st);
   return launch_fr_w<D, NDF, UNI, FT, 8>(A, cells, n, mode, march, lds, st);
 }

@@ -27,6 +27,10 @@ struct DevArgs {
   const double* df;         // f of this step
   double* df_out;           // f of the next step
   double* sdf;
+  // outgoing micro flux of every velocity cell per direction, written by K1 for the face of
+  // the cell's outgoing side (the cell's own reconstruction times u[dir]); the gather of
+  // regular cells reads it instead of reconstructing (layout of sdf, storage type of f)
+  double* oflux;
   double* vflux;            // VsData.flux accumulators (split flux/iterate calls)
   double* psflux;           // PsData.flux accumulators
   double* w;
@@ -91,16 +95,16 @@ struct DevArgs {
     (B).rec_ds = ro_recds;                                                                   \
   } while (0)
 
-constexpr int SLOPE_CHUNK = 256;   // threads per workgroup of the chunked slope kernel
 constexpr int SLOPE_MAX_VPT = 4;   // velocity cells per thread of the chunked slope kernel: 1..4
 constexpr int MAX_ENTRIES = 24;    // face sides per cell (2*DIM sides, up to 2^(DIM-1) each)
 
 // f32: f, sdf and the next f stored as float (FP32 form), else double
 hipError_t launch_slope(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
                         int n, int maxh, hipStream_t st);
-// work = (cell, chunk) pairs, chunk = vpt*SLOPE_CHUNK velocity cells
+// work = (cell, chunk) pairs, one per workgroup of wpi (3 or 4) waves; chunk = vpt*64*wpi
+// velocity cells
 hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, bool f32, const int2* work,
-                              int n, int maxh, int vpt, hipStream_t st);
+                              int n, int maxh, int vpt, int wpi, hipStream_t st);
 hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,
                              int n, int mode, int march, size_t lds, int waves_per_block,
                              hipStream_t st);

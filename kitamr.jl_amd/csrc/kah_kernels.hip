@@ -50,22 +50,22 @@ KAH_VS_DECL(3d_f32)
 #undef KAH_VS_DECL
 hipError_t launch_overlap_map_3d(const DevArgs&, const OvJob*, int, uint32_t*, hipStream_t);
 hipError_t launch_slope_2d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
-hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_slope_chunk_2d(const DevArgs&, bool, const int2*, int, int, int, int, hipStream_t);
 hipError_t launch_macro_slope_2d(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
 hipError_t launch_flux_relax_2d(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
                                   hipStream_t);
 hipError_t launch_slope_3d(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
-hipError_t launch_slope_chunk_3d(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_slope_chunk_3d(const DevArgs&, bool, const int2*, int, int, int, int, hipStream_t);
 hipError_t launch_macro_slope_3d(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
 hipError_t launch_flux_relax_3d(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
                                   hipStream_t);
 hipError_t launch_slope_2d_f32(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
-hipError_t launch_slope_chunk_2d_f32(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_slope_chunk_2d_f32(const DevArgs&, bool, const int2*, int, int, int, int, hipStream_t);
 hipError_t launch_macro_slope_2d_f32(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
 hipError_t launch_flux_relax_2d_f32(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
                                   hipStream_t);
 hipError_t launch_slope_3d_f32(const DevArgs&, bool, const int32_t*, int, int, hipStream_t);
-hipError_t launch_slope_chunk_3d_f32(const DevArgs&, bool, const int2*, int, int, int, hipStream_t);
+hipError_t launch_slope_chunk_3d_f32(const DevArgs&, bool, const int2*, int, int, int, int, hipStream_t);
 hipError_t launch_macro_slope_3d_f32(const DevArgs&, bool, const int32_t*, int, double*, hipStream_t);
 hipError_t launch_flux_relax_3d_f32(const DevArgs&, bool, const int32_t*, int, int, int, size_t, int,
                                   hipStream_t);
@@ -114,12 +114,12 @@ hipError_t launch_slope(const DevArgs& A, int dim, bool uni, bool f32, const int
 }
 
 hipError_t launch_slope_chunk(const DevArgs& A, int dim, bool uni, bool f32, const int2* work,
-                              int n, int maxh, int vpt, hipStream_t st) {
+                              int n, int maxh, int vpt, int wpi, hipStream_t st) {
   if (f32)
-    return dim == 2 ? launch_slope_chunk_2d_f32(A, uni, work, n, maxh, vpt, st)
-                    : launch_slope_chunk_3d_f32(A, uni, work, n, maxh, vpt, st);
-  return dim == 2 ? launch_slope_chunk_2d(A, uni, work, n, maxh, vpt, st)
-                  : launch_slope_chunk_3d(A, uni, work, n, maxh, vpt, st);
+    return dim == 2 ? launch_slope_chunk_2d_f32(A, uni, work, n, maxh, vpt, wpi, st)
+                    : launch_slope_chunk_3d_f32(A, uni, work, n, maxh, vpt, wpi, st);
+  return dim == 2 ? launch_slope_chunk_2d(A, uni, work, n, maxh, vpt, wpi, st)
+                  : launch_slope_chunk_3d(A, uni, work, n, maxh, vpt, wpi, st);
 }
 
 hipError_t launch_flux_relax(const DevArgs& A, int dim, bool uni, bool f32, const int32_t* cells,

@@ -10,9 +10,9 @@ hipError_t launch_slope_2d_f32(const DevArgs& A, bool uni, const int32_t* cells,
 }
 
 hipError_t launch_slope_chunk_2d_f32(const DevArgs& A, bool uni, const int2* work, int n,
-                                int maxh, int vpt, hipStream_t st) {
-  return uni ? launch_slope_chunk_t<2, 2, true, float>(A, work, n, maxh, vpt, st)
-             : launch_slope_chunk_t<2, 2, false, float>(A, work, n, maxh, vpt, st);
+                                int maxh, int vpt, int wpi, hipStream_t st) {
+  return uni ? launch_slope_chunk_t<2, 2, true, float>(A, work, n, maxh, vpt, wpi, st)
+             : launch_slope_chunk_t<2, 2, false, float>(A, work, n, maxh, vpt, wpi, st);
 }
 
 hipError_t launch_macro_slope_2d_f32(const DevArgs& A, bool uni, const int32_t* cells, int n,

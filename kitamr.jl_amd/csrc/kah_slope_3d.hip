@@ -10,9 +10,9 @@ hipError_t launch_slope_3d(const DevArgs& A, bool uni, const int32_t* cells, int
 }
 
 hipError_t launch_slope_chunk_3d(const DevArgs& A, bool uni, const int2* work, int n,
-                                int maxh, int vpt, hipStream_t st) {
-  return uni ? launch_slope_chunk_t<3, 1, true, double>(A, work, n, maxh, vpt, st)
-             : launch_slope_chunk_t<3, 1, false, double>(A, work, n, maxh, vpt, st);
+                                int maxh, int vpt, int wpi, hipStream_t st) {
+  return uni ? launch_slope_chunk_t<3, 1, true, double>(A, work, n, maxh, vpt, wpi, st)
+             : launch_slope_chunk_t<3, 1, false, double>(A, work, n, maxh, vpt, wpi, st);
 }
 
 hipError_t launch_macro_slope_3d(const DevArgs& A, bool uni, const int32_t* cells, int n,

@@ -20,7 +20,7 @@ import numpy as np
 
 SPACE = {
     "slope_vpt": [3, 1, 2, 4],
-    "flux_waves": [8, 4],
+    "flux_waves": [0, 8, 4, 3],      # 0: the library's choice by shape
     "vel_tiles": [1, 0],
     "overlap_maps": [1, 0],
 }

@@ -13,13 +13,13 @@ def _timer(costs, calls):
 
 def test_coordinate_descent_finds_the_separable_minimum():
     case = cases.riemann_2d(n=4, nv=16)
-    costs = {"slope_vpt": {1: 3.0, 2: 1.0, 3: 2.0, 4: 4.0}, "flux_waves": {4: 0.5, 8: 1.0},
+    costs = {"slope_vpt": {1: 3.0, 2: 1.0, 3: 2.0, 4: 4.0}, "flux_waves": {0: 0.8, 3: 0.9, 4: 0.5, 8: 1.0},
              "vel_tiles": {0: 2.0, 1: 1.0}}
     calls = []
     best, ms, trials = autotune.autotune(case, timer=_timer(costs, calls))
     assert best == {"slope_vpt": 2, "flux_waves": 4, "vel_tiles": 1}
     assert ms == 1.0 + 0.5 + 1.0
-    assert len(trials) == len(calls) == 1 + 3 + 1 + 1     # start + the other values
+    assert len(trials) == len(calls) == 1 + 3 + 3 + 1     # start + the other values
     assert all("overlap_maps" not in c for c in calls)    # shared grid: no maps option
 
 
@@ -31,7 +31,7 @@ def test_adapted_grids_tune_maps_not_tiles():
 
 def test_cache_round_trip(tmp_path):
     case = cases.riemann_2d(n=4, nv=16)
-    costs = {"slope_vpt": {1: 1.0, 2: 2.0, 3: 3.0, 4: 4.0}, "flux_waves": {4: 1.0, 8: 2.0},
+    costs = {"slope_vpt": {1: 1.0, 2: 2.0, 3: 3.0, 4: 4.0}, "flux_waves": {0: 1.5, 3: 1.2, 4: 1.0, 8: 2.0},
              "vel_tiles": {0: 1.0, 1: 2.0}}
     calls = []
     path = tmp_path / "tune.json"

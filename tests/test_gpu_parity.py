@@ -41,6 +41,7 @@ OPTION_SETS = {
     "vpt1": {"slope_vpt": 1},
     "vpt4": {"slope_vpt": 4},
     "waves4": {"flux_waves": 4},
+    "waves3": {"flux_waves": 3},
     "no_maps": {"overlap_maps": 0},
 }
 
