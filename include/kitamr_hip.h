@@ -321,11 +321,13 @@ int64_t kah_device_bytes(const kah_ctx* ctx);
  * neighbour), 0 searches the neighbour's keys in every step. These three take effect at the
  * next kah_set_topology. KAH_OPT_GRAPHS: 1 replays the launches of kah_step as one captured
  * HIP graph per buffer parity and dt (recaptured when dt changes; not while kernel timing is
- * on), 0 (default) launches them one by one. KAH_OPT_REGULAR: 1 (default) runs the fused
- * flux/relaxation of regular cells (2*DIM faces to fluid cells on the same velocity grid,
- * limited or plain-slope reconstruction on both sides) with a gather specialised to them,
- * bit-identical to the general one; 0 runs every cell through the general gather (takes
- * effect at the next kah_set_topology). No counterpart in the reference. */
+ * on), 0 (default) launches them one by one. KAH_OPT_REGULAR: 1 (default) gives regular cells
+ * a slope and a flux gather specialised to them, bit-identical to the general ones: for the
+ * slope, cells with one neighbour per side and no transverse projection; for the gather
+ * (CAIDVM flux), cells whose 2*DIM faces go to fluid cells on the same level, which read the
+ * outgoing micro fluxes the slope kernel stores instead of reconstructing; 0 runs every cell
+ * through the general slope and gather. Takes effect at the next kah_set_topology. No
+ * counterpart in the reference. */
 enum { KAH_OPT_FLUX_WAVES = 1, KAH_OPT_SLOPE_VPT = 2, KAH_OPT_VEL_TILES = 3,
        KAH_OPT_OVERLAP_MAPS = 4, KAH_OPT_GRAPHS = 5, KAH_OPT_REGULAR = 6 };
 int kah_set_option(kah_ctx* ctx, int option, int64_t value);

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <string>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -109,24 +110,6 @@ struct kah_ctx {
     if (fr_waves) return fr_waves;
     return (f32 && !uni && lds > 0 && lds <= 40 * 1024) ? 4 : 8;
   }
-  // Adapted grids: the waves per workgroup (3, 4 or 8) of one launch of cells whose grids
-  // have the sizes ns, as the fewest idle thread slots over the cells (each thread takes
-  // velocity cells p, p + 64*waves, ...); ties go to more waves. A 568-cell grid fills 3-wave
-  // workgroups to 99 % (3 x 192 slots), 4-wave ones to 74 % (3 x 256).
-  int flux_waves_for(const std::vector<int64_t>& ns, size_t lds) const {
-    if (fr_waves || uni) return flux_waves(lds);
-    int best = 8;
-    int64_t best_pad = -1;
-    for (int w : {8, 4, 3}) {
-      int64_t pad = 0;
-      for (int64_t n : ns) pad += (n + 64 * w - 1) / (64 * w) * (64 * w) - n;
-      if (best_pad < 0 || pad < best_pad) {
-        best = w;
-        best_pad = pad;
-      }
-    }
-    return best;
-  }
   // launches of the flux/relaxation kernel: the fluid cells by grid size, each launch with
   // the LDS its largest grid needs (0: the global scratch area)
   struct FrBin {
@@ -134,9 +117,9 @@ struct kah_ctx {
     int n = 0;
     size_t lds = 0;
     int waves = 8;          // waves per workgroup
-    bool regular = false;   // fused step through the gather of regular cells
+    int reg_class = 0;      // fused step through the gather of regular cells (1, 2) or not
   };
-  bool reg_enable = true;   // KAH_OPT_REGULAR
+  int reg_mode = 1;   // KAH_OPT_REGULAR: 0 every cell through the general slope and gather
   std::vector<FrBin*> frbins;
   int slope_vpt = 3;  // velocity cells per thread of the chunked slope kernel
   bool vt_enable = true;   // wave-tile order of shared tensor-product velocity grids
@@ -1140,31 +1123,36 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     for (int32_t c : C->fluid) Dmax = std::max(Dmax, depth[size_t(c)]);
   }
   // Regular cells of the slope kernel: every direction SL_INNER with one neighbour per side,
-  // no transverse projection, both on the cell's velocity grid (or a shared grid), and 2*D
-  // half fluxes that are the cell's own limited or plain-slope reconstructions.
+  // no transverse projection, and 2*D half fluxes that are the cell's own limited or
+  // plain-slope reconstructions.
+  // Returns 0 (not regular), 1 (both neighbours of every direction on the cell's velocity
+  // grid) or 2 (some on other grids).
   auto slope_regular = [&](int32_t c) {
-    if (!C->reg_enable || hoff[c + 1] - hoff[c] != 2 * D) return false;
+    if (!C->reg_mode || hoff[c + 1] - hoff[c] != 2 * D) return 0;
+    int cls = 1;
     for (int d = 0; d < D; ++d) {
       const SlopeDesc& sd = B.sl[size_t(c) * D + d];
-      if (sd.mode != SL_INNER || sd.nb_cnt[0] != 1 || sd.nb_cnt[1] != 1) return false;
+      if (sd.mode != SL_INNER || sd.nb_cnt[0] != 1 || sd.nb_cnt[1] != 1) return 0;
       for (int sx = 0; sx < 2; ++sx) {
         const SlopeNb& nbr = B.slnb[size_t(sd.nb_beg[sx])];
-        if (nbr.proj || (!C->uni && nbr.map != MAP_SAME)) return false;
+        if (nbr.proj) return 0;
+        if (!C->uni && nbr.map != MAP_SAME) cls = 2;
       }
     }
     for (int h = hoff[c]; h < hoff[c + 1]; ++h) {
       const HalfDesc& hd = halves[size_t(h)];
-      if (hd.data_cell != c || (hd.mode != MI_LIMITED && hd.mode != MI_UNLIMITED)) return false;
+      if (hd.data_cell != c || (hd.mode != MI_LIMITED && hd.mode != MI_UNLIMITED)) return 0;
     }
-    return true;
+    return cls;
   };
   for (int L = 0; L <= Dmax; ++L) {
     // half-count bins: the per-thread accumulators of K1 are MAXH*(dim+2) doubles, so a bin
     // sized to the common counts (4: 2D interior, 6: 3D interior) keeps registers low
-    // bin 5: regular cells (the slopes and half fluxes of slope_regular / half_regular)
-    const int caps[6] = {4, 6, 8, 16, 24, -1};
-    std::vector<int32_t> lists[6];
-    std::vector<int2> works[6][2];   // [bin][3-wave items, 4-wave items]
+    // bins 5 and 6: regular cells (the slopes and half fluxes of slope_regular /
+    // half_regular), with every neighbour on the cell's grid, or some on other grids
+    const int caps[7] = {4, 6, 8, 16, 24, -1, -2};
+    std::vector<int32_t> lists[7];
+    std::vector<int2> works[7][2];   // [bin][3-wave items, 4-wave items]
     for (int32_t c : C->fluid) {
       if (depth[size_t(c)] != L) continue;
       int nh = hoff[c + 1] - hoff[c];
@@ -1172,12 +1160,13 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
       if (walled[c]) {
         lists[b].push_back(c);
       } else {
-        if (slope_regular(c)) b = 5;
+        const int rc = slope_regular(c);
+        if (rc) b = 4 + rc;
         auto& wl = works[b][wpi_of(c) == 3 ? 0 : 1];
         for (int ch = 0; ch < nchunks(c); ++ch) wl.push_back(make_int2(c, ch));
       }
     }
-    for (int b = 0; b < 6; ++b) {
+    for (int b = 0; b < 7; ++b) {
       for (int kind = 0; kind < 3; ++kind) {   // per-cell kernel, 3-wave items, 4-wave items
         const size_t cnt = kind ? works[b][kind - 1].size() : lists[b].size();
         if (cnt == 0) continue;
@@ -1253,52 +1242,52 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
     // CIP_Marching gathers in FP64 while the slope kernel reconstructs in FP32)
     const bool caidvm = C->P.flux_kind == KAH_FLUX_CAIDVM;
     const bool same_arith = !(C->f32 && C->P.marching == KAH_MARCH_CIP);
+    // 0: not regular; 1: every neighbour on the cell's velocity grid; 2: some on other grids
     auto regular = [&](int32_t c) {
-      if (!C->reg_enable || !caidvm || !same_arith || walled[c] || eoff[c + 1] - eoff[c] != 2 * D)
-        return false;
+      if (!C->reg_mode || !caidvm || !same_arith || walled[c] || eoff[c + 1] - eoff[c] != 2 * D)
+        return 0;
+      int cls = 1;
       for (int e = eoff[c]; e < eoff[c + 1]; ++e) {
         const EntryDesc& ed = ents[size_t(e)];
-        if (ed.oth_cell < 0 || ed.oth_cell >= nc || T->bound_enc[ed.oth_cell] < 0) return false;
+        if (ed.oth_cell < 0 || ed.oth_cell >= nc || T->bound_enc[ed.oth_cell] < 0) return 0;
         // a neighbour on another level shares a hanging face: its side holds more than one
         // face, each with its own reconstruction, so one stored outgoing flux cannot serve
-        if (walled[ed.oth_cell] || T->ps_level[ed.oth_cell] != T->ps_level[c]) return false;
-        if (!C->uni && ed.map != MAP_SAME) return false;
+        if (walled[ed.oth_cell] || T->ps_level[ed.oth_cell] != T->ps_level[c]) return 0;
+        if (!C->uni && ed.map != MAP_SAME) cls = 2;
         if ((ed.own_mode != MI_LIMITED && ed.own_mode != MI_UNLIMITED) ||
             (ed.oth_mode != MI_LIMITED && ed.oth_mode != MI_UNLIMITED))
-          return false;
+          return 0;
       }
-      return true;
+      return cls;
     };
-    std::vector<std::vector<int32_t>> lists;
-    std::vector<int64_t> bmax;
+    // keys: 3*size bin + regular class
+    std::map<int, std::vector<int32_t>> lists;
+    std::map<int, int64_t> bmax;
     for (int32_t c : C->fluid) {
       const int64_t n = T->vs_offset[c + 1] - T->vs_offset[c];
-      size_t b = 0;
+      int b = 0;
       if (!C->uni)
         while ((int64_t(640) << b) < n) ++b;
-      b = 2 * b + (regular(c) ? 1 : 0);
-      if (lists.size() <= b) {
-        lists.resize(b + 1);
-        bmax.resize(b + 1, 0);
-      }
-      lists[b].push_back(c);
-      bmax[b] = std::max(bmax[b], n);
+      const int key = 3 * b + regular(c);
+      lists[key].push_back(c);
+      bmax[key] = std::max(bmax[key], n);
     }
     bool need_scratch = false;
-    for (size_t b = 0; b < lists.size(); ++b) {
-      if (lists[b].empty()) continue;
+    for (auto& kv : lists) {
+      const int key = kv.first;
       auto* fb = new kah_ctx::FrBin();
       C->frbins.push_back(fb);
-      fb->n = int(lists[b].size());
-      fb->regular = (b & 1) != 0;
-      fb->lds = sc_elem * size_t(NDF + 1) * size_t(bmax[b]);
+      fb->n = int(kv.second.size());
+      fb->reg_class = key % 3;
+      // LDS and workgroup size from the size class, so that a cell gets the same launch shape
+      // (and the same order of its sums) whether it is regular or not
+      const int64_t cap = C->uni ? int64_t(C->n0) : std::min<int64_t>(int64_t(640) << (key / 3),
+                                                                       int64_t(C->max_n));
+      fb->lds = sc_elem * size_t(NDF + 1) * size_t(std::max(cap, bmax[key]));
       if (fb->lds > C->lds_limit) fb->lds = 0;
-      std::vector<int64_t> ns;
-      ns.reserve(lists[b].size());
-      for (int32_t c : lists[b]) ns.push_back(T->vs_offset[c + 1] - T->vs_offset[c]);
-      fb->waves = C->flux_waves_for(ns, fb->lds);
+      fb->waves = C->flux_waves(fb->lds);
       need_scratch |= fb->lds == 0;
-      KAH_TRY(C->upload(fb->cells, lists[b].data(), lists[b].size(), "flux cells"));
+      KAH_TRY(C->upload(fb->cells, kv.second.data(), kv.second.size(), "flux cells"));
     }
     if (need_scratch) KAH_TRY(C->alloc(C->d_scratch, sizeof(double) * C->np * (NDF + 1), "scratch"));
     else C->d_scratch.release();
@@ -1427,7 +1416,8 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER
                : (C->P.marching == KAH_MARCH_CIP ? MARCH_CIP : MARCH_CAIDVM);
   for (const kah_ctx::FrBin* b : C->frbins) {
-    const int m = (mode == MODE_FUSED && b->regular) ? MODE_FUSED_REG : mode;
+    const int m = (mode != MODE_FUSED || b->reg_class == 0) ? mode
+                  : (b->reg_class == 1 ? MODE_FUSED_REG : MODE_FUSED_REGX);
     KAH_TRY(C->hip(launch_flux_relax(A, C->D, C->uni, C->f32, b->cells.as<int32_t>(), b->n, m,
                                      march, b->lds, C->fr_waves ? C->fr_waves : b->waves,
                                      C->stream),
@@ -2069,7 +2059,7 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
       return KAH_OK;
     case KAH_OPT_REGULAR:
       if (value != 0 && value != 1) return C->fail(KAH_EINVAL, "regular must be 0 or 1");
-      C->reg_enable = value != 0;
+      C->reg_mode = int(value);
       C->have_topo = false;   // the launch lists are built by kah_set_topology
       return KAH_OK;
     case KAH_OPT_SLOPE_VPT:

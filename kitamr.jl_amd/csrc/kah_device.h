@@ -640,23 +640,38 @@ __device__ inline void half_terms(const DevArgs& A, const Grid<D, NDF, UNI, FT>&
 }
 
 // slope_dir of a regular cell (SL_INNER with one neighbour per side, no transverse
-// projection, both neighbours on the cell's velocity grid or a shared grid): velocity cell i
-// of each neighbour; the same operations in the same order as slope_dir.
-template <int D, int NDF, bool UNI, class FT, class T>
+// projection): the neighbours' velocity cells that overlap i (cell i itself on the same grid);
+// the same operations in the same order as slope_dir.
+// SAME: both neighbours on the cell's velocity grid (or a shared grid): velocity cell i.
+template <int D, int NDF, bool UNI, class FT, class T, bool SAME>
 __device__ inline void slope_regular(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int c,
-                                     int d, int i, const T (&f)[NDF], T (&s)[NDF]) {
+                                     int d, int i, int li, const uint32_t* kp,
+                                     const T (&f)[NDF], T (&s)[NDF]) {
   const SlopeDesc sd = A.sl[c * D + d];
   T a[2][NDF];
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
-    const int nbc = A.slnb[sd.nb_beg[side]].cell;
-    const int64_t no = UNI ? int64_t(nbc) * G.n : A.voff[nbc];
-    const FT* nf = reinterpret_cast<const FT*>(A.df) + no * NDF;
+    const SlopeNb nb = A.slnb[sd.nb_beg[side]];
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) {
-      const T term = f[k] - T(double(Grid<D, NDF, UNI, FT>::at(nf, k * G.n + i)));
-      a[side][k] = T(0);
-      a[side][k] += div_ds(term, sd.ds[side], sd.rds[side], sd.rds_f[side]);
+    for (int k = 0; k < NDF; ++k) a[side][k] = T(0);
+    if constexpr (SAME) {
+      const int64_t no = UNI ? int64_t(nb.cell) * G.n : A.voff[nb.cell];
+      const FT* nf = reinterpret_cast<const FT*>(A.df) + no * NDF;
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        const T term = f[k] - T(double(Grid<D, NDF, UNI, FT>::at(nf, k * G.n + i)));
+        a[side][k] += div_ds(term, sd.ds[side], sd.rds[side], sd.rds_f[side]);
+      }
+    } else {
+      const Grid<D, NDF, UNI, FT> N(A, nb.cell);
+      for_overlap_m<D, NDF, UNI, FT>(A, N, nb.map, i, li, kp, [&](int j, double den, double rden) {
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          T term = f[k] - T(N.f(j, k));
+          if (den != 1.0) term = term * T(rden);
+          a[side][k] += div_ds(term, sd.ds[side], sd.rds[side], sd.rds_f[side]);
+        }
+      });
     }
   }
   const T ic0 = desc_val<T>(sd.inv_cnt[0], sd.inv_cnt_f[0]);
@@ -794,7 +809,7 @@ __global__ __launch_bounds__(NWV * 64) void k_slope(DevArgs A0, const int32_t* _
 // per-cell work as k_slope without the loop over the grid, for cells without wall faces.
 // Each workgroup writes its partial sums of the cell's half fluxes to row
 // part_off[slot]+chunk; k_flux_relax adds the rows in chunk order.
-template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT, bool REG = false, int WPI = 4>
+template <int D, int NDF, bool UNI, class FT, int MAXH, int VPT, int REG = 0, int WPI = 4>
 __global__ __launch_bounds__(WPI * 64) void k_slope_chunk(DevArgs A0,
                                                           const int2* __restrict__ work,
                                                           int nwork, KAH_RO_PARAMS) {
@@ -832,12 +847,14 @@ __global__ __launch_bounds__(WPI * 64) void k_slope_chunk(DevArgs A0,
       for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
       for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
-      if constexpr (REG) {
+      // REG 1: regular cell, neighbours on its grid; 2: regular cell, neighbours on any grid
+      const int li = REG == 1 ? 0 : G.lev(i);
+      const uint32_t* kp = REG == 1 ? nullptr : G.keyp(i);
+      if constexpr (REG != 0) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) slope_regular<D, NDF, UNI, FT, CT>(A, G, c, d, i, f, s[d]);
+        for (int d = 0; d < D; ++d)
+          slope_regular<D, NDF, UNI, FT, CT, REG == 1>(A, G, c, d, i, li, kp, f, s[d]);
       } else {
-        const int li = G.lev(i);
-        const uint32_t* kp = G.keyp(i);
 #pragma unroll
         for (int d = 0; d < D; ++d) slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, kp, f, s[d]);
       }
@@ -937,15 +954,19 @@ __device__ inline void gather_flux(const DevArgs& A, int c, int n_own, int eb, i
 }
 
 // gather_flux for a regular cell: its 2*DIM entries are faces to fluid cells on the same
-// velocity grid (or a shared grid), whose micro flux through the face, in each direction,
-// the slope kernel stored as the outgoing micro flux of the upwind cell (its own
-// reconstruction with the same face parameters and arithmetic as gather_flux's). Velocity
-// cell i of the cell reads its own outgoing flux or the neighbour's cell i: the same terms in
-// the same order as gather_flux, so the same result, without slopes or reconstruction.
-template <int D, int NDF, bool UNI, class FT, class T>
+// level (one face per side on both), reconstructed by the limited or plain-slope forms, whose
+// micro flux through the face the slope kernel stored as the outgoing micro flux of the
+// upwind cell per direction (its own reconstruction with the same face parameters and
+// arithmetic as gather_flux's). Velocity cell i reads the cell's own outgoing flux, or the
+// neighbour's at the velocity cells that overlap i (cell i itself on the same grid; across
+// grids through the overlap map, with the 2^(-DIM*dl) factors): the same terms in the same
+// order as gather_flux, so the same result, without slopes, neighbour velocities or
+// reconstruction.
+// SAME: every neighbour on the cell's velocity grid (or a shared grid): velocity cell i.
+template <int D, int NDF, bool UNI, class FT, class T, bool SAME>
 __device__ inline void gather_regular(const DevArgs& A, const Grid<D, NDF, UNI, FT>& G, int eb,
-                                      int i, const T (&u)[D], T (&fl)[NDF]) {
-  const int n = G.n;
+                                      int i, int li, const uint32_t* kp, const T (&u)[D],
+                                      T (&fl)[NDF]) {
   const FT* of = reinterpret_cast<const FT*>(A.oflux);
 #pragma unroll
   for (int e = 0; e < 2 * D; ++e) {
@@ -954,15 +975,33 @@ __device__ inline void gather_regular(const DevArgs& A, const Grid<D, NDF, UNI, 
     const T ru = desc_val<T>(ed.rot, ed.rot_f) * ud;
     const bool own = ed.side ? (ru > T(0)) : (ru <= T(0));
     const T area = desc_val<T>(ed.area, ed.area_f);
-    // rows of the upwind cell: the cell's own, or the neighbour's (same grid, same size n)
-    const int64_t no = UNI ? int64_t(ed.oth_cell) * n : A.voff[ed.oth_cell];
-    const int64_t ro = own ? G.off : no;
-    const FT* b = of + ro * (NDF * D);
+    if (own) {
+      const FT* b = of + G.off * (NDF * D);
 #pragma unroll
-    for (int k = 0; k < NDF; ++k) {
-      const T mic = T(Grid<D, NDF, UNI, FT>::at(b, (ed.dir * NDF + k) * n + i));
-      const T t = area * mic;
-      fl[k] += ed.side ? -t : t;
+      for (int k = 0; k < NDF; ++k) {
+        const T t = area * T(Grid<D, NDF, UNI, FT>::at(b, (ed.dir * NDF + k) * G.n + i));
+        fl[k] += ed.side ? -t : t;
+      }
+    } else if constexpr (SAME) {
+      const int64_t no = UNI ? int64_t(ed.oth_cell) * G.n : A.voff[ed.oth_cell];
+      const FT* b = of + no * (NDF * D);
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) {
+        const T t = area * T(Grid<D, NDF, UNI, FT>::at(b, (ed.dir * NDF + k) * G.n + i));
+        fl[k] += ed.side ? -t : t;
+      }
+    } else {
+      const Grid<D, NDF, UNI, FT> N(A, ed.oth_cell);
+      const FT* b = of + N.off * (NDF * D);
+      for_overlap_m<D, NDF, UNI, FT>(A, N, ed.map, i, li, kp,
+                                     [&](int j, double den, double rden) {
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) {
+          T t = area * T(Grid<D, NDF, UNI, FT>::at(b, (ed.dir * NDF + k) * N.n + j));
+          if (den != 1.0) t = t * T(rden);   // exact: den is a power of two
+          fl[k] += ed.side ? -t : t;
+        }
+      });
     }
   }
 }
@@ -1340,7 +1379,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
 
   // half flux = sum of its partial rows in chunk order (micro_to_macro's 1/2 on the energy
   // row applied after the sum), one thread per (entry, side, component)
-  constexpr bool FUSED = (MODE == MODE_FUSED || MODE == MODE_FUSED_REG);
+  constexpr bool FUSED = (MODE == MODE_FUSED || MODE == MODE_FUSED_REG || MODE == MODE_FUSED_REGX);
   if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
     const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
     const int slot = A.ents[eb + e].pair * 2 + s;
@@ -1497,10 +1536,13 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       for (int d = 0; d < D; ++d) u[d] = CT(G.u(i, d));
 #pragma unroll
       for (int k = 0; k < NDF; ++k) f[k] = CT(G.f(i, k));
-      if constexpr (MODE == MODE_FUSED_REG) {
+      if constexpr (MODE == MODE_FUSED_REG || MODE == MODE_FUSED_REGX) {
 #pragma unroll
         for (int k = 0; k < NDF; ++k) fl[k] = CT(0);
-        gather_regular<D, NDF, UNI, FT, CT>(A, G, eb, i, u, fl);
+        if constexpr (MODE == MODE_FUSED_REG)
+          gather_regular<D, NDF, UNI, FT, CT, true>(A, G, eb, i, 0, nullptr, u, fl);
+        else
+          gather_regular<D, NDF, UNI, FT, CT, false>(A, G, eb, i, G.lev(i), G.keyp(i), u, fl);
       } else if (FUSED) {
         CT s[D][NDF];
 #pragma unroll
@@ -2237,23 +2279,26 @@ template <int D, int NDF, bool UNI, class FT, int VPT, int WPI>
 inline hipError_t launch_slope_chunk_w(const DevArgs& A, const int2* work, int n, int maxh,
                                        hipStream_t st) {
   const dim3 grid(n), block(WPI * 64);
-  if (maxh < 0)   // regular cells: 2*D half fluxes
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 2 * D, VPT, true, WPI>), grid, block, 0,
+  if (maxh == -1)   // regular cells, neighbours on the cell's grid: 2*D half fluxes
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 2 * D, VPT, 1, WPI>), grid, block, 0,
+                       st, A, work, n, KAH_RO_ARGS(A));
+  else if (maxh == -2)   // regular cells, neighbours on other grids
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 2 * D, VPT, 2, WPI>), grid, block, 0,
                        st, A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 4)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT, false, WPI>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 4, VPT, 0, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 6)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 6, VPT, false, WPI>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 6, VPT, 0, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 8)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 8, VPT, false, WPI>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 8, VPT, 0, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else if (maxh <= 16)
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 16, VPT, false, WPI>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 16, VPT, 0, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   else
-    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 24, VPT, false, WPI>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_slope_chunk<D, NDF, UNI, FT, 24, VPT, 0, WPI>), grid, block, 0, st,
                        A, work, n, KAH_RO_ARGS(A));
   return hipGetLastError();
 }
@@ -2311,6 +2356,15 @@ inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int
     if (march == MARCH_CIP)
       return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REG, MARCH_CIP, NWV>(A, cells, n, lds, st);
     return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REG, MARCH_EULER, NWV>(A, cells, n, lds, st);
+  }
+  if constexpr (!UNI) {   // on a shared grid every neighbour is on the cell's grid
+    if (mode == MODE_FUSED_REGX) {
+      if (march == MARCH_CAIDVM)
+        return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REGX, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);
+      if (march == MARCH_CIP)
+        return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REGX, MARCH_CIP, NWV>(A, cells, n, lds, st);
+      return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED_REGX, MARCH_EULER, NWV>(A, cells, n, lds, st);
+    }
   }
   if (march == MARCH_CAIDVM)
     return launch_fr_t<D, NDF, UNI, FT, MODE_FUSED, MARCH_CAIDVM, NWV>(A, cells, n, lds, st);

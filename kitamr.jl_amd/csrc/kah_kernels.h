@@ -7,10 +7,10 @@
 
 namespace kah {
 
-// MODE_FUSED_REG: MODE_FUSED for regular cells (2*DIM faces to fluid cells on the same
-// velocity grid, reconstructed by the limited or plain-slope forms), gathered without the
-// per-face mode dispatch and overlap walks
-enum { MODE_FLUX = 0, MODE_ITER = 1, MODE_FUSED = 2, MODE_FUSED_REG = 3 };
+// MODE_FUSED_REG / MODE_FUSED_REGX: MODE_FUSED for regular cells (2*DIM faces to fluid cells
+// on the same level, limited or plain-slope reconstruction), gathered from the stored outgoing
+// micro fluxes; REG: every neighbour on the cell's velocity grid, REGX: on any grid
+enum { MODE_FLUX = 0, MODE_ITER = 1, MODE_FUSED = 2, MODE_FUSED_REG = 3, MODE_FUSED_REGX = 4 };
 enum { MARCH_CAIDVM = 0, MARCH_EULER = 1, MARCH_CIP = 2 };
 
 // Everything a kernel reads, passed by value (device pointers + scalars).

@@ -157,11 +157,13 @@ def test_graph_replay_equals_launches(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["riemann8", "riemann8_euler", "riemann8_dvm", "riemann8_cip",
-                                  "riemann16_amr2", "box3d", "box3d_cip", "cylinder_c3_32"])
+                                  "riemann16_amr2", "box3d", "box3d_cip", "cylinder_c3_32",
+                                  "riemann8_vsamr", "box3d_amr_vsamr"])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_regular_cells_equal_general_gather(name, precision):
-    """The specialised slope and gather of regular cells (2*DIM faces to fluid cells on the
-    same velocity grid) give the same steps as the general ones, bit for bit, in both forms."""
+    """The specialised slope and gather of regular cells (one neighbour per side, faces to
+    fluid cells on the same level, on the same or another velocity grid) give the same steps as
+    the general ones, bit for bit, in both forms."""
     if name == "cylinder_c3_32":
         case = cases.cylinder_c3(n=32)
     else:
