@@ -256,6 +256,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving stream and print its report")
     ap.add_argument("--no-other-form", action="store_true",
                     help="skip the run of the other storage form (FP64 beside FP32 and back)")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -284,6 +286,13 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    if args.serve_only:
+        rep = serving_stream()
+        log(f"[bench] serving: p50 {rep['p50_latency_s']:.3f}s p99 {rep['p99_latency_s']:.3f}s "
+            f"{rep['updates_per_s'] / 1e9:.2f} G updates/s")
+        print(json.dumps({"serving": rep}), flush=True)
+        return
 
     t0 = time.perf_counter()
     case = make_case(args.workload, args.precision)
