@@ -239,14 +239,34 @@ def timed_run(case, options, steps, warmup, graphs, copy=False, log=None):
     ms, _ = dev.kernel_times()
     log(f"[bench] timed {steps} steps ({'graph replay' if graphs else 'eager'}): "
         f"{elapsed:.3f}s; kernel ms slope={ms[0]:.3f} flux+relax={ms[1]:.3f}")
-    st = dev.get_state(df=False, qf=False)
+    st = dev.get_state(df=False)
     finite = bool(np.isfinite(st["w"]).all())
     cp = None
     if copy:
         cp = dev.copy_bandwidth(1 << 30, 10)
         log(f"[bench] plain copy of 1 GiB: {cp[1]:.3f} ms -> {cp[0]:.0f} GB/s")
     dev.close()
-    return {"elapsed": elapsed, "ms": ms, "finite": finite, "copy": cp, "setup_s": t_setup}
+    return {"elapsed": elapsed, "ms": ms, "finite": finite, "copy": cp, "setup_s": t_setup,
+            "state": st, "steps_run": step}
+
+
+def form_errors(a, b, case):
+    """Largest per-cell relative difference of rho, rho U, rho E (each cell's w scaled by its
+    own max |w|) and of q (scaled by max |q| over the cells) between two runs of the same
+    steps, over the fluid cells."""
+    import numpy as np
+    dim = case.config.solver.DIM
+    nw = dim + 2
+    fluid = np.asarray(case.topo["bound_enc"]) >= 0
+    wa = np.asarray(a["w"]).reshape(-1, nw)[fluid]
+    wb = np.asarray(b["w"]).reshape(-1, nw)[fluid]
+    scale = np.max(np.abs(wb), axis=1)
+    d = np.abs(wa - wb) / scale[:, None]
+    qa = np.asarray(a["qf"]).reshape(-1, dim)[fluid]
+    qb = np.asarray(b["qf"]).reshape(-1, dim)[fluid]
+    return {"rho": float(d[:, 0].max()), "rhoU": float(d[:, 1:1 + dim].max()),
+            "rhoE": float(d[:, nw - 1].max()),
+            "q": float(np.abs(qa - qb).max() / np.abs(qb).max())}
 
 
 def main():
@@ -389,6 +409,13 @@ def main():
             "state_finite": ro["finite"],
         }
         out["speedup_fp32_vs_fp64"] = (value / vo) if other == "fp64" else (vo / value)
+        if ro["steps_run"] == r["steps_run"]:
+            e32, e64 = (r, ro) if other == "fp64" else (ro, r)
+            out["fp32_form_vs_fp64_form"] = dict(
+                form_errors(e32["state"], e64["state"], case),
+                steps=r["steps_run"],
+                note="per-cell relative difference after the same steps from the same state; "
+                     "the FP64 form agrees with the FP64 oracle to 1e-12 (GPU parity tests)")
         log(f"[bench] {other} form: {vo / 1e9:.2f} G updates/s")
     if not args.no_serve:
         rep = serving_stream()
