@@ -27,6 +27,13 @@ Löhner velocity-space criterion from what the mathematics fixes (no retyped for
   one-cell spike of height A has eta = 2A / (2A (1 + 1e-2) + 2 A 1e-3) at the spike and
   A / (A (1 + 1e-2) + 2 A 1e-3) at its face neighbours, which thresholds just above and below
   each value separate exactly.
+- the contribution criteria (contribution_refine_flag, Criteria.jl:4-6, 17-24, 50-55): with
+  h constant on a uniform grid, b = 0 and U = 0, a cell's share of the mass is 1/N and its share
+  of the internal energy |u|^2 / sum |u|^2; the local criterion flags exactly the cells whose
+  larger share exceeds ADAPT_COEFFI_VS_LOCAL, and the global one the cells whose mass or
+  energy content exceeds ADAPT_COEFFI_VS_GLOBAL times Velocity_Resolution, so thresholds
+  between the distinct shares of an 8x8 grid select all cells, the 32 outer ones or the 4
+  corners.
 """
 import math
 
@@ -357,3 +364,45 @@ def test_lohner_spike_values_separate_at_their_thresholds():
                       (eta_nb * (1 + 1e-9), {i0})]:
         fl = _flags(case, np.stack([h, b]), tau)
         assert set(np.flatnonzero(fl).tolist()) == want, (tau, np.flatnonzero(fl))
+
+
+# ------------------------------------------------------------------ contribution criteria
+def _flags_contribution(case, df, cl, cg, vr):
+    g = case.grids[0]
+    w = _moments(g.midpoint, g.weight, df[0], df[1])
+    prim = O.get_prim(2, w, case.config.gas.gamma)
+    o = O.Oracle(kah_params(case.config), case.topo)
+    o.set_state(w, prim, df.reshape(-1))
+    params = abi.vs_adapt_params("contribution", coeff_local=cl, coeff_global=cg)
+    fl = o.vs_flags(params, np.asarray(vr, dtype=np.float64))
+    o.close()
+    return fl & 1
+
+
+def test_contribution_criterion_flags_the_cells_holding_a_share():
+    case = _vs_case(lambda base: base.copy(), nv=8, vmax=4.0)
+    g = case.grids[0]
+    n = g.n
+    assert n == 64 and np.all(g.weight == 1.0)
+    h, b = np.ones(n), np.zeros(n)
+    u2 = np.sum(g.midpoint ** 2, axis=1)                  # U = 0 by symmetry
+    share_e = u2 / u2.sum()                               # share of the internal energy
+    corners = set(np.flatnonzero(u2 == u2.max()).tolist())
+    outer = set(np.flatnonzero(share_e > 1.0 / n).tolist())
+    assert len(corners) == 4 and len(outer) == 32
+    top2 = np.unique(share_e)[-2:]
+    df = np.stack([h, b])
+    for cl, want in [(1.0 / n * (1 - 1e-6), set(range(n))),          # every cell's mass share
+                     (1.0 / n * (1 + 1e-6), outer),                   # energy shares above 1/N
+                     (0.5 * (top2[0] + top2[1]), corners),
+                     (top2[1] * (1 + 1e-6), set())]:
+        fl = _flags_contribution(case, df, cl, 1e30, [1.0, 1.0])
+        assert set(np.flatnonzero(fl).tolist()) == want, (cl, np.flatnonzero(fl))
+    # global: mass content h w = 1 everywhere, energy content |u|^2/2: with vr = (1, 1) the
+    # threshold 10 sits between the corners' 12.25 and the next 9.25
+    e = 0.5 * u2
+    assert e.max() == 12.25 and np.unique(e)[-2] == 9.25
+    fl = _flags_contribution(case, df, 1e30, 10.0, [1.0, 1.0])
+    assert set(np.flatnonzero(fl).tolist()) == corners
+    fl = _flags_contribution(case, df, 1e30, 0.999, [1.0, 1.0])      # mass content above
+    assert fl.all()
