@@ -79,30 +79,47 @@ constexpr int red_stride(int cnt) { return pow2_at_least(cnt) <= 64 ? pow2_at_le
 // that half with two_sum; after log2(P) levels lane L holds, for value (L >> log2(64/P)), the
 // sum over the lanes that share its high bits, and a plain two_sum butterfly over the low
 // bits finishes it. P/2 + P/4 + ... + 1 + log2(64/P) exchanges instead of P*6.
+// The levels are one template instance per value count M, each writing fresh arrays of M/2,
+// so that every index is a compile-time constant: a loop over the levels, left to the
+// unroller, stays rolled for 32 values and puts the arrays in scratch memory.
+template <int M>
+__device__ inline void scatter_levels(const double (&v)[M], const double (&e)[M], double& s0,
+                                      double& e0, int lane, int o) {
+  if constexpr (M == 1) {
+    s0 = v[0];
+    e0 = e[0];
+  } else {
+    const bool up = (lane & o) != 0;
+    double nv[M / 2], ne[M / 2];
+#pragma unroll
+    for (int j = 0; j < M / 2; ++j) {
+      const double lo = v[j], hi = v[M / 2 + j], loe = e[j], hie = e[M / 2 + j];
+      const double keep = up ? hi : lo, ke = up ? hie : loe;
+      const double send = up ? lo : hi, se = up ? loe : hie;
+      const double recv = __shfl_xor(send, o, 64), re = __shfl_xor(se, o, 64);
+      double err;
+      nv[j] = two_sum(keep, recv, err);
+      ne[j] = (ke + re) + err;
+    }
+    scatter_levels<M / 2>(nv, ne, s0, e0, lane, o >> 1);
+  }
+}
+
 template <int P>
 __device__ inline void wave_scatter2(double (&v)[P], double (&e)[P]) {
   static_assert(P >= 1 && P <= 64 && (P & (P - 1)) == 0, "power of two up to 64");
   const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int m = P, o = 32; m > 1; m >>= 1, o >>= 1) {
-    const bool up = (lane & o) != 0;
-#pragma unroll
-    for (int j = 0; j < m / 2; ++j) {
-      const double keep = up ? v[m / 2 + j] : v[j], ke = up ? e[m / 2 + j] : e[j];
-      const double send = up ? v[j] : v[m / 2 + j], se = up ? e[j] : e[m / 2 + j];
-      const double recv = __shfl_xor(send, o, 64), re = __shfl_xor(se, o, 64);
-      double err;
-      v[j] = two_sum(keep, recv, err);
-      e[j] = (ke + re) + err;
-    }
-  }
+  double s0, e0;
+  scatter_levels<P>(v, e, s0, e0, lane, 32);
 #pragma unroll
   for (int o = 32 / P; o > 0; o >>= 1) {
-    const double so = __shfl_xor(v[0], o, 64), eo = __shfl_xor(e[0], o, 64);
+    const double so = __shfl_xor(s0, o, 64), eo = __shfl_xor(e0, o, 64);
     double err;
-    v[0] = two_sum(v[0], so, err);
-    e[0] = (e[0] + eo) + err;
+    s0 = two_sum(s0, so, err);
+    e0 = (e0 + eo) + err;
   }
+  v[0] = s0;
+  e[0] = e0;
 }
 
 // Compensated sum of the first cnt of CNT per-thread values over a workgroup of NWV waves:
@@ -828,6 +845,70 @@ __global__ __launch_bounds__(WPI * 64) void k_slope_chunk(DevArgs A0,
   const Grid<D, NDF, UNI, FT> G(A, c);
   const int hb = A.half_off[c];
   const int nh = A.half_off[c + 1] - hb;
+  if constexpr (MAXH > 8) {
+    // Cells with many half fluxes (hanging faces): the velocity cells' slopes stay in
+    // registers and the half fluxes are summed one at a time (NW values per thread instead
+    // of MAXH*NW), each with a per-wave reduce-scatter; one barrier, then the waves in order.
+    using CT = typename Arith<FT>::type;
+    constexpr int P = pow2_at_least(NW);
+    constexpr int SH = (P == 4) ? 4 : 3;   // lanes per value after the reduce-scatter
+    __shared__ double redh[2 * WPI * MAXH * P];
+    CT u[VPT][D], f[VPT][NDF], s[VPT][D][NDF], wt[VPT];
+    int iv[VPT];
+#pragma unroll
+    for (int r = 0; r < VPT; ++r) {
+      const int p = (ch * VPT + r) * CH + tid;
+      iv[r] = -1;
+      if (p < G.n) {
+        const int i = G.vi(p);
+        iv[r] = i;
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[r][d] = CT(G.u(i, d));
+#pragma unroll
+        for (int k = 0; k < NDF; ++k) f[r][k] = CT(G.f(i, k));
+        const int li = G.lev(i);
+        const uint32_t* kp = G.keyp(i);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          slope_dir<D, NDF, UNI, FT, CT>(A, G, c, d, i, li, kp, f[r], s[r][d]);
+        wt[r] = CT(G.wt(i));
+      }
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int h = 0; h < nh; ++h) {
+      const HalfDesc hd = A.halves[hb + h];
+      double v[P], e[P];
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = e[q] = 0.0;
+#pragma unroll
+      for (int r = 0; r < VPT; ++r) {
+        if (iv[r] >= 0) {
+          double m[NW];
+          half_terms<D, NDF, UNI, FT, CT>(A, G, c, hd, iv[r], u[r], f[r], s[r], wt[r], m);
+#pragma unroll
+          for (int q = 0; q < NW; ++q) v[q] = (r == 0) ? m[q] : v[q] + m[q];
+        }
+      }
+      wave_scatter2<P>(v, e);
+      if ((lane & ((1 << SH) - 1)) == 0) {
+        const int q = lane >> SH;
+        redh[((wv * MAXH + h) * P + q) * 2] = v[0];
+        redh[((wv * MAXH + h) * P + q) * 2 + 1] = e[0];
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < nh * NW; t += CH) {
+      const int h = t / NW, q = t % NW;
+      double S = redh[(h * P + q) * 2], E = redh[(h * P + q) * 2 + 1];
+      for (int w = 1; w < WPI; ++w) {
+        double err;
+        S = two_sum(S, redh[((w * MAXH + h) * P + q) * 2], err);
+        E = (E + redh[((w * MAXH + h) * P + q) * 2 + 1]) + err;
+      }
+      A.half_fw[(A.part_off[A.halves[hb + h].slot] + ch) * NW + q] = S + E;
+    }
+    return;
+  }
   __shared__ double red[2 * WPI * red_stride(CNT)];
 
   // Per-thread partials (VPT <= 4 terms each) in FP64; the sum over the item's threads is

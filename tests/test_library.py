@@ -59,3 +59,24 @@ def test_struct_layout_matches_header(tmp_path):
         assert got[(st, "sizeof")] == __import__("ctypes").sizeof(cls)
         for f in fields[st]:
             assert getattr(cls, f).offset == got[(st, f)], (st, f)
+
+
+def test_per_step_kernels_use_no_scratch_memory():
+    """The compiler's resource report of the build (kernel_resources.json beside the library):
+    no instantiation of the per-step kernels (slopes, flux/relaxation, immersed boundary)
+    keeps arrays in scratch memory, which costs several times their run time. The walled
+    slope kernel of cells with more than 8 half fluxes is the known exception (a handful of
+    cells at wall corners of refined meshes)."""
+    import json
+    from kitamr_hip import build as B
+    if not B.RESOURCES.exists():
+        pytest.skip("library built without the resource report")
+    res = json.loads(B.RESOURCES.read_text())
+    hot = [k for k in res if any(s in k for s in ("k_slope_chunk", "k_flux_relax", "k_ib_solid",
+                                                   "k_ib_face", "k_overlap_map"))]
+    assert len(hot) > 100
+    spilled = sorted(k for k in hot if res[k].get("ScratchSize", 0) > 0)
+    assert not spilled, spilled[:5]
+    walled = [k for k in res if "7k_slopeI" in k and res[k].get("ScratchSize", 0) > 0]
+    for k in walled:
+        assert re.search(r"Li(16|24)ELi4EEEv", k), k
