@@ -311,15 +311,16 @@ int64_t kah_phase_cells(const kah_ctx* ctx);
 /* Device bytes held by the context (state + topology). */
 int64_t kah_device_bytes(const kah_ctx* ctx);
 /* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
- * the flux/relaxation kernel (3, 4 or 8; 0, the default, chooses by shape: on adapted velocity
- * grids per launch the count that leaves the fewest idle thread slots over its grids, on
- * shared grids 8). KAH_OPT_SLOPE_VPT: velocity cells per thread of the slope kernel (1..4;
- * its workgroups have 3 or 4 waves, per cell whichever pads the grid least). KAH_OPT_VEL_TILES: 1 (default) processes a shared
- * tensor-product velocity grid in 8x8 / 4x4x4 tiles per wave, 0 in storage order.
- * KAH_OPT_OVERLAP_MAPS: 1 (default) precomputes, per fluid cell and neighbour with another
- * adapted velocity grid, the overlapping velocity cells (4 bytes per velocity cell and
- * neighbour), 0 searches the neighbour's keys in every step. These three take effect at the
- * next kah_set_topology. KAH_OPT_GRAPHS: 1 replays the launches of kah_step as one captured
+ * the flux/relaxation kernel (3, 4 or 8; 0, the default, chooses per size-class launch: 4 for
+ * the FP32 form on adapted velocity grids whose LDS copy of f* takes at most 40 KiB, else 8);
+ * a change drops captured graphs. KAH_OPT_SLOPE_VPT: velocity cells per thread of the slope
+ * kernel (1..4, default 3; its workgroups have 3 or 4 waves, per cell whichever pads the grid
+ * least). KAH_OPT_VEL_TILES: 1 (default) processes a shared tensor-product velocity grid in
+ * 8x8 / 4x4x4 tiles per wave, 0 in storage order. KAH_OPT_OVERLAP_MAPS: 1 (default)
+ * precomputes, per distinct pair of adapted velocity grids that face each other, the
+ * overlapping velocity cells of the other grid (4 bytes per velocity cell of the first), 0
+ * searches the neighbour's keys in every step. These three take effect at the next
+ * kah_set_topology. KAH_OPT_GRAPHS: 1 replays the launches of kah_step as one captured
  * HIP graph per buffer parity and dt (recaptured when dt changes; not while kernel timing is
  * on), 0 (default) launches them one by one. KAH_OPT_REGULAR: 1 (default) gives regular cells
  * a slope and a flux gather specialised to them, bit-identical to the general ones: for the
