@@ -80,6 +80,7 @@ struct kah_ctx {
   DBuf d_fluid, d_voff, d_gu, d_gw, d_glev, d_gkey, d_gof;
   int64_t n_grids = 1, grid_cells = 0, map_entries = 0;   // distinct grids, their cells, map entries
   DBuf d_df[2], d_sdf, d_oflux, d_vflux, d_psflux, d_w, d_prim, d_qf, d_half, d_rho, d_res, d_scratch;
+  DBuf d_cpre;   // per-cell prologue records of the fused step (k_cell_pre)
   DBuf d_sl, d_slnb, d_hoff, d_halves, d_eoff, d_ents, d_recds, d_bc, d_sw, d_poff, d_pcnt;
   DBuf d_vtb, d_ovmap, d_moff;
   // ghost-exchange buffers: cell midpoints and bound_enc on the device, and the staging of
@@ -153,7 +154,7 @@ struct kah_ctx {
   int64_t device_bytes() const {
     const DBuf* bufs[] = {&d_fluid, &d_voff, &d_gu, &d_gw, &d_glev, &d_gkey, &d_gof, &d_df[0], &d_df[1],
                           &d_sdf, &d_oflux, &d_vflux, &d_psflux, &d_w, &d_prim, &d_qf, &d_half, &d_rho,
-                          &d_res, &d_scratch, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
+                          &d_res, &d_scratch, &d_cpre, &d_sl, &d_slnb, &d_hoff, &d_halves, &d_eoff,
                           &d_ents, &d_recds, &d_bc, &d_sw, &d_poff, &d_pcnt, &d_vtb, &d_ovmap,
                           &d_moff, &d_cmid, &d_benc, &d_xcells, &d_xoff, &d_xbuf, &d_xfluid,
                           &d_ib_solid, &d_ib_snoff, &d_ib_snb, &d_ib_faces, &d_ib_img,
@@ -1221,6 +1222,10 @@ int kah_set_topology(kah_ctx* C, const kah_topology* T) {
   KAH_TRY(C->alloc(C->d_half, sizeof(double) * std::max<int64_t>(C->part_rows, 1) * NW, "half fluxes"));
   KAH_TRY(C->alloc(C->d_rho, sizeof(double) * std::max<int64_t>(C->npairs, 1), "wall density"));
   KAH_TRY(C->alloc(C->d_res, sizeof(double) * nc * 2 * NW, "residual"));
+  if (C->P.marching != KAH_MARCH_CIP)
+    KAH_TRY(C->alloc(C->d_cpre, sizeof(double) * nc * cpre_len(D), "cell prologue records"));
+  else
+    C->d_cpre.release();
   C->d_vflux.release();
   // f* and M(prim_c) of the cell in LDS, in the per-velocity arithmetic type (float in the
   // FP32 form, except under CIP_Marching)
@@ -1336,6 +1341,7 @@ DevArgs make_args(kah_ctx* C, double dt) {
   A.part_cnt = C->d_pcnt.as<int32_t>();
   A.rho_w = C->d_rho.as<double>();
   A.res = C->d_res.as<double>();
+  A.cpre = C->d_cpre.p ? C->d_cpre.as<double>() : nullptr;
   A.scratch = C->d_scratch.as<double>();
   A.sl = C->d_sl.as<SlopeDesc>();
   A.slnb = C->d_slnb.as<SlopeNb>();
@@ -1415,6 +1421,10 @@ int run_flux_relax(kah_ctx* C, const DevArgs& A, int mode) {
   if (C->timing) KAH_TRY(ev_begin(C, e, 1));
   int march = C->P.marching == KAH_MARCH_EULER ? MARCH_EULER
                : (C->P.marching == KAH_MARCH_CIP ? MARCH_CIP : MARCH_CAIDVM);
+  if (mode == MODE_FUSED && march != MARCH_CIP)
+    KAH_TRY(C->hip(launch_cell_pre(A, C->D, C->d_fluid.as<int32_t>(), int(C->fluid.size()),
+                                   C->stream),
+                   "cell prologue launch"));
   for (const kah_ctx::FrBin* b : C->frbins) {
     const int m = (mode != MODE_FUSED || b->reg_class == 0) ? mode
                   : (b->reg_class == 1 ? MODE_FUSED_REG : MODE_FUSED_REGX);

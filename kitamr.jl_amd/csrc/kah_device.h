@@ -1420,6 +1420,93 @@ __device__ inline void relax_cip(const DevArgs& A, const Grid<D, NDF, UNI, FT>& 
 // iterate!(CAIDVM_Marching) (src/Theory/Iterate.jl:96-130) or iterate!(Euler) (:131-162)
 // for one cell, fed by the gathered flux (MODE_FUSED), by the flux accumulators
 // (MODE_ITER), or only accumulating the flux (MODE_FLUX).
+// Half flux of entry e's side sd, component q, of a cell: the sum of its partial rows in chunk
+// order, compensated (micro_to_macro's 1/2 on the energy row applied after the sum).
+template <int D>
+__device__ inline double half_total(const DevArgs& A, const EntryDesc& ed, int sd, int q) {
+  constexpr int NW = D + 2;
+  const int slot = ed.pair * 2 + sd;
+  const int64_t off = A.part_off[slot];
+  const int cnt = A.part_cnt[slot];
+  double sum = 0.0, err = 0.0;
+  for (int k = 0; k < cnt; ++k) {
+    double e;
+    sum = two_sum(sum, A.half_fw_r[(off + k) * NW + q], e);
+    err += e;
+  }
+  sum = sum + err;
+  return (q == NW - 1) ? 0.5 * sum : sum;
+}
+
+// Macroscopic face flux of cell c in face order (flux += fw*area / flux -= fw*area) and
+// w + dt/V*flux (both marchings: src/Theory/Iterate.jl:108, :144-149); returns dt/V.
+template <int D, class HF>
+__device__ inline double cell_update(const DevArgs& A, int c, int eb, int ne, bool faces,
+                                     HF half, double (&wn)[D + 2]) {
+  constexpr int NW = D + 2;
+  const double* dsc = &A.rec_ds[int64_t(c) * D];
+  double V = dsc[0];
+#pragma unroll
+  for (int d = 1; d < D; ++d) V = V * dsc[d];
+  double psf[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) psf[q] = A.psflux[int64_t(c) * NW + q];
+  if (faces) {
+    for (int e = 0; e < ne; ++e) {
+      const EntryDesc& ed = A.ents[eb + e];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        const double h0 = half(e, 0, q), h1 = half(e, 1, q);
+        double t;
+        if (ed.oth_cell < 0) t = ed.area * (h0 + h1);    // update_domain_flux!
+        else t = (h0 + h1) * ed.area;                     // update_macro_flux!
+        psf[q] = ed.side ? psf[q] - t : psf[q] + t;
+      }
+    }
+  }
+  const double* wold = &A.w[int64_t(c) * NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) wn[q] = wold[q] + psf[q] * A.dt / V;
+  return A.dt / V;
+}
+
+// Per-cell prologue of the fused step under CAIDVM_Marching and Euler, one thread per cell:
+// the half fluxes of the cell's faces, w + dt/V*flux, prim_c, the Maxwellian pre-factors and
+// tau of prim_c and the relaxation weights, into the cell's record (cpre_len(D) doubles).
+// k_flux_relax reads the record instead of forming it in one thread of each workgroup while
+// the others wait at a barrier; the operations and their order are the same.
+template <int D>
+__global__ __launch_bounds__(256) void k_cell_pre(DevArgs A, const int32_t* __restrict__ cells,
+                                                  int ncells) {
+  constexpr int NW = D + 2;
+  static_assert(sizeof(MaxPre<D>) == sizeof(double) * (NW + 4), "MaxPre layout");
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ncells) return;
+  const int c = cells[t];
+  const int eb = A.ent_off[c];
+  const int ne = A.ent_off[c + 1] - eb;
+  double wn[NW], pc[NW];
+  const double dtV = cell_update<D>(
+      A, c, eb, ne, A.flux_kind == 0,
+      [&](int e, int sd, int q) { return half_total<D>(A, A.ents[eb + e], sd, q); }, wn);
+  get_prim<D>(wn, A.gamma, pc);
+  const MaxPre<D> pre = max_pre<D>(pc, A.Pr);
+  const double tau = get_tau<D>(pc, A.mu, A.omega);
+  double* r = A.cpre + int64_t(c) * cpre_len(D);
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    r[q] = wn[q];
+    r[NW + q] = pc[q];
+  }
+  const double* pp = reinterpret_cast<const double*>(&pre);
+#pragma unroll
+  for (int k = 0; k < NW + 4; ++k) r[2 * NW + k] = pp[k];
+  r[3 * NW + 4] = tau;
+  r[3 * NW + 5] = tau / (tau + A.dt);
+  r[3 * NW + 6] = A.dt / (tau + A.dt);
+  r[3 * NW + 7] = dtV;
+}
+
 template <int D, int NDF, bool UNI, class FT, int MODE, int MARCH, bool LDS, int NWV>
 __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
                                                          const int32_t* __restrict__ cells,
@@ -1461,61 +1548,54 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   // half flux = sum of its partial rows in chunk order (micro_to_macro's 1/2 on the energy
   // row applied after the sum), one thread per (entry, side, component)
   constexpr bool FUSED = (MODE == MODE_FUSED || MODE == MODE_FUSED_REG || MODE == MODE_FUSED_REGX);
-  if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
-    const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
-    const int slot = A.ents[eb + e].pair * 2 + s;
-    const int64_t off = A.part_off[slot];
-    const int cnt = A.part_cnt[slot];
-    double sum = 0.0, err = 0.0;   // the rows in chunk order, compensated
-    for (int k = 0; k < cnt; ++k) {
-      double e;
-      sum = two_sum(sum, A.half_fw_r[(off + k) * NW + q], e);
-      err += e;
+  // the record of k_cell_pre (fused CAIDVM_Marching / Euler steps, which the host always
+  // precedes by it): copied to LDS below
+  constexpr bool pre = FUSED && MARCH != MARCH_CIP;
+  if constexpr (pre) {
+    const double* r = A.cpre + int64_t(c) * cpre_len(D);
+    const int j = threadIdx.x;
+    if (j < 2 * NW) cell_sh[j] = r[j];
+    else if (j < 3 * NW + 4) reinterpret_cast<double*>(&pre_c)[j - 2 * NW] = r[j];
+    else if (j < 3 * NW + 7) pre_t[j - (3 * NW + 4)] = r[j];
+    else if (j == 3 * NW + 7) res[0] = r[j];
+  } else {
+    if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
+      const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
+      hsum[threadIdx.x] = half_total<D>(A, A.ents[eb + e], s, q);
     }
-    sum = sum + err;
-    hsum[threadIdx.x] = (q == NW - 1) ? 0.5 * sum : sum;
+    __syncthreads();
   }
-  __syncthreads();
 
-  if (threadIdx.x == 0) {
-    // macroscopic face flux, in face order: flux += fw*area / flux -= fw*area, then
-    // w += flux*dt/V (both marchings: src/Theory/Iterate.jl:108, :144-149)
-    const double* dsc = &A.rec_ds[int64_t(c) * D];
-    double V = dsc[0];
+  if (!pre && threadIdx.x == 0) {
+    const bool faces = MODE != MODE_ITER && A.flux_kind == 0;
+    auto half = [&](int e, int sd, int q) { return hsum[(e * 2 + sd) * NW + q]; };
+    if (MODE == MODE_FLUX) {
+      // the split flux! call: the macroscopic face flux into PsData.flux
+      double psf[NW];
 #pragma unroll
-    for (int d = 1; d < D; ++d) V = V * dsc[d];
-    double psf[NW];
-#pragma unroll
-    for (int q = 0; q < NW; ++q) psf[q] = A.psflux[int64_t(c) * NW + q];
-    if (MODE != MODE_ITER && A.flux_kind == 0) {
-      for (int e = 0; e < ne; ++e) {
+      for (int q = 0; q < NW; ++q) psf[q] = A.psflux[int64_t(c) * NW + q];
+      for (int e = 0; faces && e < ne; ++e) {
         const EntryDesc& ed = A.ents[eb + e];
-        const double* h0 = &hsum[e * 2 * NW];
-        const double* h1 = h0 + NW;
 #pragma unroll
         for (int q = 0; q < NW; ++q) {
+          const double h0 = half(e, 0, q), h1 = half(e, 1, q);
           double t;
-          if (ed.oth_cell < 0) t = ed.area * (h0[q] + h1[q]);    // update_domain_flux!
-          else t = (h0[q] + h1[q]) * ed.area;                     // update_macro_flux!
+          if (ed.oth_cell < 0) t = ed.area * (h0 + h1);    // update_domain_flux!
+          else t = (h0 + h1) * ed.area;                     // update_macro_flux!
           psf[q] = ed.side ? psf[q] - t : psf[q] + t;
         }
       }
-    }
-    if (MODE == MODE_FLUX) {
 #pragma unroll
       for (int q = 0; q < NW; ++q) A.psflux[int64_t(c) * NW + q] = psf[q];
     } else {
-      const double* wold = &A.w[int64_t(c) * NW];
       double wn[NW], pc[NW];
-#pragma unroll
-      for (int q = 0; q < NW; ++q) wn[q] = wold[q] + psf[q] * dt / V;
+      res[0] = cell_update<D>(A, c, eb, ne, faces, half, wn);
       get_prim<D>(wn, A.gamma, pc);
 #pragma unroll
       for (int q = 0; q < NW; ++q) {
         cell_sh[q] = wn[q];
         cell_sh[NW + q] = pc[q];
       }
-      res[0] = dt / V;
     }
   }
   if (MODE == MODE_FLUX) {
@@ -1585,7 +1665,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
       }
     }
   }
-  if (MARCH != MARCH_CIP && threadIdx.x == 0) {
+  if (MARCH != MARCH_CIP && !pre && threadIdx.x == 0) {
     // needed after the first reduction; wave 0 computes them before its share of pass 1
     double pc[NW];
 #pragma unroll

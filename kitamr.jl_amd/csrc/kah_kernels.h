@@ -12,6 +12,7 @@ namespace kah {
 // micro fluxes; REG: every neighbour on the cell's velocity grid, REGX: on any grid
 enum { MODE_FLUX = 0, MODE_ITER = 1, MODE_FUSED = 2, MODE_FUSED_REG = 3, MODE_FUSED_REGX = 4 };
 enum { MARCH_CAIDVM = 0, MARCH_EULER = 1, MARCH_CIP = 2 };
+constexpr int cpre_len(int dim) { return 3 * (dim + 2) + 8; }
 
 // Everything a kernel reads, passed by value (device pointers + scalars).
 struct DevArgs {
@@ -41,6 +42,10 @@ struct DevArgs {
   const int32_t* part_cnt;  // [npairs*2] number of partial rows (velocity chunks)
   double* rho_w;            // [npairs]
   double* res;              // [nc*2*(D+2)] residual terms of the last iterate
+  // per-cell prologue of the fused step (k_cell_pre), cpre_len(D) doubles per cell:
+  // w + dt/V*flux | prim_c | MaxPre of prim_c | tau, tau/(tau+dt), dt/(tau+dt) | dt/V;
+  // null: k_flux_relax forms them itself (CIP_Marching, split calls)
+  double* cpre;
   double* scratch;          // global scratch when the per-cell state exceeds LDS
   // topology
   const SlopeDesc* sl;      // [nc*D]
@@ -128,6 +133,8 @@ struct VsArgs {
 hipError_t launch_vs_res(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P, int nc,
                          double* out, hipStream_t st);
 // out[0..1] = the maxima over the fluid cells of the per-cell pairs part[2c], part[2c+1]
+hipError_t launch_cell_pre(const DevArgs& A, int dim, const int32_t* cells, int n,
+                           hipStream_t st);
 hipError_t launch_vs_res_max(const double* part, const int32_t* benc, int nc, double* out,
                              hipStream_t st);
 hipError_t launch_vs_flags(const DevArgs& A, int dim, bool uni, bool f32, const VsArgs& P,

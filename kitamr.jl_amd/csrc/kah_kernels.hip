@@ -204,6 +204,13 @@ __global__ __launch_bounds__(1024) void k_vs_res_max(const double* __restrict__ 
   }
 }
 
+hipError_t launch_cell_pre_2d(const DevArgs&, const int32_t*, int, hipStream_t);
+hipError_t launch_cell_pre_3d(const DevArgs&, const int32_t*, int, hipStream_t);
+hipError_t launch_cell_pre(const DevArgs& A, int dim, const int32_t* cells, int n,
+                           hipStream_t st) {
+  return dim == 2 ? launch_cell_pre_2d(A, cells, n, st) : launch_cell_pre_3d(A, cells, n, st);
+}
+
 hipError_t launch_vs_res_max(const double* part, const int32_t* benc, int nc, double* out,
                              hipStream_t st) {
   hipLaunchKernelGGL(k_vs_res_max, dim3(1), dim3(1024), 0, st, part, benc, nc, out);

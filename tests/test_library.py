@@ -73,7 +73,7 @@ def test_per_step_kernels_use_no_scratch_memory():
         pytest.skip("library built without the resource report")
     res = json.loads(B.RESOURCES.read_text())
     hot = [k for k in res if any(s in k for s in ("k_slope_chunk", "k_flux_relax", "k_ib_solid",
-                                                   "k_ib_face", "k_overlap_map"))]
+                                                   "k_ib_face", "k_overlap_map", "k_cell_pre"))]
     assert len(hot) > 100
     spilled = sorted(k for k in hot if res[k].get("ScratchSize", 0) > 0)
     assert not spilled, spilled[:5]
