@@ -291,8 +291,9 @@ def main():
                          "fp64 (the parity form)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: replay each step as one captured HIP graph (default); 0: eager")
-    ap.add_argument("--autotune", action="store_true",
-                    help="tune the launch options on the workload first (coordinate descent)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: tune the launch options on the workload first (coordinate descent "
+                         "over timed steps; the default), 0: library defaults")
     ap.add_argument("--vel-tiles", type=int, default=-1,
                     help="1/0: wave tiles of the velocity grid on/off (-1: library default)")
     ap.add_argument("--regular", type=int, default=-1,
@@ -325,12 +326,18 @@ def main():
     log(f"[bench] case built in {time.perf_counter() - t0:.1f}s: leaves={case.n_cells} "
         f"pairs={case.topo['n_pairs']} P={P} grid sizes {sizes.min()}-{sizes.max()}")
     tuned = None
+    tune_rep = None
     if args.autotune:
         from kitamr_hip import autotune
         t1 = time.perf_counter()
         tuned, tms, trials = autotune.autotune(case, steps=3, warmup=1, log=log)
-        log(f"[bench] autotune ({len(trials)} trials, {time.perf_counter() - t1:.1f}s): {tuned} "
+        tune_s = time.perf_counter() - t1
+        log(f"[bench] autotune ({len(trials)} trials, {tune_s:.1f}s): {tuned} "
             f"{tms:.3f} ms/step")
+        tune_rep = {"trials": [{"options": o, "ms_per_step_eager": m} for o, m in trials],
+                    "seconds": tune_s,
+                    "search": "coordinate descent over kitamr_hip.autotune.SPACE, 3 eager steps "
+                              "per trial on a fresh context of the workload"}
     options = dict(tuned or {})
     if args.flux_waves:
         options["flux_waves"] = args.flux_waves
@@ -392,6 +399,8 @@ def main():
         "state_finite": r["finite"],
         "setup_s": r["setup_s"],
     }
+    if tune_rep:
+        out["autotune"] = tune_rep
     if not args.no_other_form:
         other = "fp64" if args.precision == "fp32f" else "fp32f"
         co = with_precision(case, other)
