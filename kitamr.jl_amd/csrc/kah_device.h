@@ -1536,7 +1536,7 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   __shared__ double hsum[MAX_ENTRIES * 2 * NW];   // the half fluxes of the cell's faces
   // per-cell constants of the relaxation, computed once: the Maxwellian pre-factors of
   // prim_c and of the moments of f*, tau(prim_c) and the relaxation weights
-  __shared__ MaxPre<D> pre_c, pre_r;
+  __shared__ MaxPre<D> pre_c;
   __shared__ double pre_t[3];
   extern __shared__ double lds[];
   // per-velocity arithmetic type: FP32 in the FP32 form, except for CIP_Marching, whose
@@ -1786,21 +1786,21 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
   } else {
   if (MARCH == MARCH_CAIDVM) {
     block_reduce<NW, NWV>(mw, NW, red, &cell_sh[2 * NW]);
-    if (threadIdx.x == 0) {
+    // prim of f* and its Maxwellian pre-factors, formed by every wave from the reduced
+    // moments (the same operations in each; no further barrier)
+    MaxPre<D> mpr;
+    {
       double wf[NW], pr[NW];
 #pragma unroll
-      for (int q = 0; q < NW; ++q) wf[q] = cell_sh[2 * NW + q];
+      for (int q = 0; q < NW; ++q) wf[q] = uniform(cell_sh[2 * NW + q]);
       wf[NW - 1] = 0.5 * wf[NW - 1];
       get_prim<D>(wf, A.gamma, pr);
-#pragma unroll
-      for (int q = 0; q < NW; ++q) cell_sh[2 * NW + q] = pr[q];
-      pre_r = max_pre<D>(pr, A.Pr);
+      mpr = uniform<D>(max_pre<D>(pr, A.Pr));
     }
-    __syncthreads();
     // pass 2: conservation correction f += M(prim_c) - M(prim); heat flux of the result
     {
       double pc[NW];
-      const MaxPre<D> mpc = uniform<D>(pre_c), mpr = uniform<D>(pre_r);
+      const MaxPre<D> mpc = uniform<D>(pre_c);
 #pragma unroll
       for (int q = 0; q < NW; ++q) pc[q] = mpc.p[q];
       const CT K = CT(A.K);
