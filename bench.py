@@ -296,6 +296,8 @@ def main():
                          "over timed steps; the default), 0: library defaults")
     ap.add_argument("--vel-tiles", type=int, default=-1,
                     help="1/0: wave tiles of the velocity grid on/off (-1: library default)")
+    ap.add_argument("--branches", type=int, default=0,
+                    help="streams for the independent launches of a phase (0: library default)")
     ap.add_argument("--regular", type=int, default=-1,
                     help="1/0: specialised slope and gather of regular cells on/off "
                          "(-1: library default, on)")
@@ -347,6 +349,8 @@ def main():
         options["vel_tiles"] = args.vel_tiles
     if args.regular >= 0:
         options["regular"] = args.regular
+    if args.branches:
+        options["branches"] = args.branches
     r = timed_run(case, options, args.steps, args.warmup, bool(args.graphs), copy=True, log=log)
     copy_gbs, copy_ms = r["copy"]
     ms = r["ms"]

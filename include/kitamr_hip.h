@@ -327,10 +327,14 @@ int64_t kah_device_bytes(const kah_ctx* ctx);
  * slope, cells with one neighbour per side and no transverse projection; for the gather
  * (CAIDVM flux), cells whose 2*DIM faces go to fluid cells on the same level, which read the
  * outgoing micro fluxes the slope kernel stores instead of reconstructing; 0 runs every cell
- * through the general slope and gather. Takes effect at the next kah_set_topology. No
- * counterpart in the reference. */
+ * through the general slope and gather. Takes effect at the next kah_set_topology.
+ * KAH_OPT_BRANCHES: 1..4 (default 4) streams over which the independent launches of a phase
+ * (slope launches of one projection depth, flux/relaxation launches) are spread, forked from
+ * and joined back to the context's stream by events (parallel branches of a captured graph).
+ * No counterpart in the reference. */
 enum { KAH_OPT_FLUX_WAVES = 1, KAH_OPT_SLOPE_VPT = 2, KAH_OPT_VEL_TILES = 3,
-       KAH_OPT_OVERLAP_MAPS = 4, KAH_OPT_GRAPHS = 5, KAH_OPT_REGULAR = 6 };
+       KAH_OPT_OVERLAP_MAPS = 4, KAH_OPT_GRAPHS = 5, KAH_OPT_REGULAR = 6,
+       KAH_OPT_BRANCHES = 7 };
 int kah_set_option(kah_ctx* ctx, int option, int64_t value);
 /* Times reps device-to-device copies of `bytes` bytes with a plain streaming copy kernel
  * (attainable HBM bandwidth = 2*bytes / ms_per_copy); the reference point for the

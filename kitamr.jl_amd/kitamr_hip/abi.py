@@ -31,7 +31,7 @@ EXPORTS = ["kah_create", "kah_destroy", "kah_last_error", "kah_set_topology", "k
            "kah_vs_resolution_noalloc"]
 MIRROR = {"data": 0, "slope": 1}
 OPT = {"flux_waves": 1, "slope_vpt": 2, "vel_tiles": 3, "overlap_maps": 4, "graphs": 5,
-       "regular": 6}
+       "regular": 6, "branches": 7}
 
 
 class KahParams(C.Structure):

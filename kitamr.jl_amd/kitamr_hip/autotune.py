@@ -3,7 +3,8 @@
 The schedule choices of the device path do not change the arithmetic of an update (results
 agree with the oracle to 1e-12 under every setting); they change how the work is mapped onto
 the GPU: velocity cells per thread of the slope kernel, waves per workgroup of the flux /
-relaxation kernel, wave tiles of shared velocity grids, overlap maps of adapted grids. The
+relaxation kernel, wave tiles of shared velocity grids, overlap maps of adapted grids, streams
+for the independent launches of a phase. The
 tuner times ``kah_step`` for candidate settings by coordinate descent (one option at a time,
 keeping the best value found so far) and caches the winner per case shape (dimension, grid
 kind, storage type, marching, cell count and velocity-grid size buckets).
@@ -23,6 +24,7 @@ SPACE = {
     "flux_waves": [0, 8, 4, 3],      # 0: the library's choice by shape
     "vel_tiles": [1, 0],
     "overlap_maps": [1, 0],
+    "branches": [4, 1, 2],           # streams for the independent launches of a phase
 }
 
 

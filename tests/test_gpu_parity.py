@@ -182,6 +182,28 @@ def test_regular_cells_equal_general_gather(name, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["riemann16_amr2", "riemann16_amr_vsamr", "box3d_amr_vsamr",
+                                  "cylinder_c3_32"])
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_branches_equal_one_stream(name, graphs):
+    """The independent launches of a phase spread over four streams (forked and joined by
+    events; parallel branches of the captured graph) give the same state as one stream, bit
+    for bit, step after step: every launch writes its own cells."""
+    case = cases.cylinder_c3(n=32) if name == "cylinder_c3_32" else CASES[name]()
+    say(f"[branches] {name} graphs={graphs}: cells={case.n_cells} P={case.phase_cells}")
+    a, _ = pair(case, options={"graphs": graphs})
+    b, _ = pair(case, options={"graphs": graphs, "branches": 4})
+    for s in range(4):
+        a.step(case.dt_xi, s)
+        b.step(case.dt_xi, s)
+        ga, gb = a.get_state(sdf=True), b.get_state(sdf=True)
+        for k in ("df", "sdf", "w", "prim", "qf"):
+            assert np.array_equal(ga[k], gb[k]), (s, k)
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
 def test_graph_replay_follows_a_flux_waves_change():
     """Changing the flux kernel's workgroup size after graph steps drops the captured graphs:
     the replayed steps equal launches one by one with the same option change, bit for bit
