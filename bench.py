@@ -132,8 +132,8 @@ def serving_stream(n_req=16, seed=20261202, rate=2.0, budget_gb=200.0, slots=4, 
     """A stream of the serving workload: requests drawn from its distributions (serve.c5_specs:
     Riemann or cylinder, Kn log-uniform in [1e-3, 1], Ma in [0.5, 5], 4k-200k cells, 16^2-64^2
     velocity grids, FP64 or the FP32 form), Poisson arrivals; each runs to the residual
-    tolerance or max_steps (reduced from 2000 to keep the run short). Four slots, one stream
-    each, as the GPU has four hardware queues per process. Reports p50/p99 request latency
+    tolerance or max_steps (reduced from 2000 to keep the run short). Four slots (one context
+    each), as the GPU has four hardware queues per process. Reports p50/p99 request latency
     (arrival to result) and the aggregate updates/s."""
     from kitamr_hip import serve
     specs = serve.c5_specs(n_req, seed)
