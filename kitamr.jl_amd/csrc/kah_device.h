@@ -2486,6 +2486,8 @@ inline hipError_t launch_slope_chunk_t(const DevArgs& A, const int2* work, int n
     case 2: return launch_slope_chunk_v<D, NDF, UNI, FT, 2>(A, work, n, maxh, wpi, st);
     case 3: return launch_slope_chunk_v<D, NDF, UNI, FT, 3>(A, work, n, maxh, wpi, st);
     case 4: return launch_slope_chunk_v<D, NDF, UNI, FT, 4>(A, work, n, maxh, wpi, st);
+    case 5: return launch_slope_chunk_v<D, NDF, UNI, FT, 5>(A, work, n, maxh, wpi, st);
+    case 6: return launch_slope_chunk_v<D, NDF, UNI, FT, 6>(A, work, n, maxh, wpi, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -100,7 +100,7 @@ struct DevArgs {
     (B).rec_ds = ro_recds;                                                                   \
   } while (0)
 
-constexpr int SLOPE_MAX_VPT = 4;   // velocity cells per thread of the chunked slope kernel: 1..4
+constexpr int SLOPE_MAX_VPT = 6;   // velocity cells per thread of the chunked slope kernel: 1..6
 constexpr int MAX_ENTRIES = 24;    // face sides per cell (2*DIM sides, up to 2^(DIM-1) each)
 
 // f32: f, sdf and the next f stored as float (FP32 form), else double
