@@ -175,6 +175,42 @@ def grid_keys(topo, lev, mid, dim):
     return (root_lin << (dim * L)) | morton
 
 
+def grid_ids(topo, dim):
+    """Id of each cell's velocity grid: cells with identical grids (levels and midpoints, byte
+    for byte) share one id. Returns (ids, representative cell of each id)."""
+    off = np.asarray(topo["vs_offset"])
+    n = len(off) - 1
+    if int(topo.get("vs_uniform", 0)):
+        return np.zeros(n, dtype=np.int64), np.zeros(1, dtype=np.int64)
+    lev = np.ascontiguousarray(np.asarray(topo["vs_level"]))
+    mid = np.ascontiguousarray(np.asarray(topo["vs_mid"], dtype=np.float64))
+    try:
+        import xxhash
+        digest = lambda b: xxhash.xxh3_128_digest(b)
+    except ImportError:   # pragma: no cover - xxhash is part of the image
+        import hashlib
+        digest = lambda b: hashlib.blake2b(b, digest_size=16).digest()
+    ids = np.empty(n, dtype=np.int64)
+    seen, reps = {}, []
+    for c in range(n):
+        a, b = int(off[c]), int(off[c + 1])
+        lv, md = lev[a:b], mid[a * dim:b * dim]
+        key = digest(lv.tobytes()) + digest(md.tobytes())
+        cand = seen.setdefault(key, [])
+        for g in cand:   # equal digests are checked byte for byte
+            r = reps[g]
+            ra, rb = int(off[r]), int(off[r + 1])
+            if rb - ra == b - a and np.array_equal(lev[ra:rb], lv) and \
+                    np.array_equal(mid[ra * dim:rb * dim], md):
+                ids[c] = g
+                break
+        else:
+            ids[c] = len(reps)
+            cand.append(len(reps))
+            reps.append(c)
+    return ids, np.asarray(reps, dtype=np.int64)
+
+
 def balance_flags(topo, dim):
     """Refine flags of one balance round (vs_balance_local!, src/Velocity_space/Balance.jl:
     72-104): a velocity cell of a fluid cell is flagged when a velocity cell of one of its
@@ -184,39 +220,70 @@ def balance_flags(topo, dim):
     Every flagged cell has to be refined in any balanced refinement of the current grids, so
     repeating rounds reaches the same grids as the reference's cell-by-cell passes. A face
     across a periodic boundary names the image record (n_cells + k) of its other cell; the
-    balance reads the image's source cell, which p4est would return as the neighbour."""
+    balance reads the image's source cell, which p4est would return as the neighbour.
+    The finest neighbour level inside each velocity cell depends only on the two grids, so it
+    is computed once per distinct ordered pair of grids; neighbours on the cell's own grid
+    are skipped (the cell inside each velocity cell is that cell itself, one level apart at
+    most)."""
     off = np.asarray(topo["vs_offset"])
     benc = np.asarray(topo["bound_enc"])
     n = len(off) - 1
     img = np.asarray(topo.get("image_src", []), dtype=np.int64)
     L = int(topo["vs_maxlevel"])
-    grids = cell_grids(topo, dim)
-    keys = [grid_keys(topo, g[0], g[1], dim) for g in grids]
-    need = [np.full(len(g[0]), -1, dtype=np.int64) for g in grids]
-    here = np.asarray(topo["pair_here"])
-    there = np.asarray(topo["pair_there"])
-    for h, t in zip(here, there):
-        if t < 0:
-            continue
-        if t >= n:
-            t = img[t - n]
-        if h >= n:
-            h = img[h - n]
-        if benc[h] < 0 or benc[t] < 0:
-            continue
-        for a, b in ((int(h), int(t)), (int(t), int(h))):
-            ka, la = keys[a], grids[a][0].astype(np.int64)
-            kb, lb = keys[b], grids[b][0].astype(np.int64)
+    flags = np.zeros(int(off[-1]), dtype=np.uint8)
+    if int(topo.get("vs_uniform", 0)):
+        return flags
+    gid, reps = grid_ids(topo, dim)
+    lev_all = np.asarray(topo["vs_level"])
+    mid_all = np.asarray(topo["vs_mid"], dtype=np.float64)
+
+    def grid(g):
+        c = int(reps[g])
+        a, b = int(off[c]), int(off[c + 1])
+        lv = lev_all[a:b]
+        return lv.astype(np.int64), grid_keys(topo, lv, mid_all[a * dim:b * dim].reshape(dim, b - a).T, dim)
+
+    here = np.asarray(topo["pair_here"], dtype=np.int64)
+    there = np.asarray(topo["pair_there"], dtype=np.int64)
+    keep = there >= 0
+    here, there = here[keep], there[keep]
+    if len(img):
+        here = np.where(here >= n, img[np.clip(here - n, 0, len(img) - 1)], here)
+        there = np.where(there >= n, img[np.clip(there - n, 0, len(img) - 1)], there)
+    fluid = (benc[here] >= 0) & (benc[there] >= 0)
+    here, there = here[fluid], there[fluid]
+    cross = gid[here] != gid[there]
+    # (cell, neighbour grid) of every cross-grid face side, both directions
+    cells = np.concatenate([here[cross], there[cross]])
+    nbg = np.concatenate([gid[there[cross]], gid[here[cross]]])
+    cache, gcache = {}, {}
+
+    def g_of(g):
+        if g not in gcache:
+            gcache[g] = grid(g)
+        return gcache[g]
+
+    need = {}
+    for c, gb in sorted(set(zip(cells.tolist(), nbg.tolist()))):
+        ga = int(gid[c])
+        key = (ga, gb)
+        if key not in cache:
+            la, ka = g_of(ga)
+            lb, kb = g_of(gb)
             own = np.searchsorted(ka, kb, side="right") - 1
-            inside = (own >= 0)
+            inside = own >= 0
             own_c = np.where(inside, own, 0)
             inside &= kb < ka[own_c] + (np.int64(1) << (dim * (L - la[own_c])))
-            np.maximum.at(need[a], own_c[inside], lb[inside])
-    flags = np.zeros(int(off[-1]), dtype=np.uint8)
-    for c, g in enumerate(grids):
-        if benc[c] < 0:
-            continue
-        flags[off[c]:off[c + 1]] = need[c] > g[0].astype(np.int64) + 1
+            nd = np.full(len(la), -1, dtype=np.int64)
+            np.maximum.at(nd, own_c[inside], lb[inside])
+            cache[key] = nd
+        if c in need:
+            np.maximum(need[c], cache[key], out=need[c])
+        else:
+            need[c] = cache[key].copy()
+    for c, nd in need.items():
+        la = g_of(int(gid[c]))[0]
+        flags[off[c]:off[c + 1]] = nd > la + 1
     return flags
 
 
