@@ -288,3 +288,44 @@ def test_balance_is_a_forced_two_to_one_refinement(kind, levels):
             assert forced, (c, i)
             made += 1
     assert made > 0
+
+
+def _balance_flags_every_pair(topo, dim):
+    """The balance flags by the direct loop over every fluid face pair and both directions
+    (no grouping by grid): the finest neighbour cell inside each velocity cell."""
+    off = np.asarray(topo["vs_offset"])
+    L = int(topo["vs_maxlevel"])
+    grids = vsadapt.cell_grids(topo, dim)
+    keys = [vsadapt.grid_keys(topo, g[0], g[1], dim) for g in grids]
+    need = [np.full(len(g[0]), -1, dtype=np.int64) for g in grids]
+    for h, t in _fluid_pairs(topo):
+        for a, b in ((h, t), (t, h)):
+            ka, la = keys[a], grids[a][0].astype(np.int64)
+            kb, lb = keys[b], grids[b][0].astype(np.int64)
+            for j in range(len(kb)):
+                i = np.searchsorted(ka, kb[j], side="right") - 1
+                if i >= 0 and kb[j] < ka[i] + (1 << (dim * (L - la[i]))):
+                    need[a][i] = max(need[a][i], lb[j])
+    flags = np.zeros(int(off[-1]), dtype=np.uint8)
+    benc = np.asarray(topo["bound_enc"])
+    for c, g in enumerate(grids):
+        if benc[c] >= 0:
+            flags[off[c]:off[c + 1]] = need[c] > g[0].astype(np.int64) + 1
+    return flags
+
+
+@pytest.mark.parametrize("make", [
+    lambda: cases.riemann_2d(n=8, nv=8, vs_levels=3),
+    lambda: cases.periodic_box_2d(n=8, nv=8, vs_levels=3),
+    lambda: cases.cylinder_2d(n=16, nv=8, vs_levels=2),
+])
+def test_balance_flags_grouped_by_grid_equal_every_pair(make):
+    """The balance planning computes each distinct ordered pair of grids once and skips faces
+    between cells on one grid; its flags equal the direct loop over every face pair."""
+    case = make()
+    dim = case.config.solver.DIM
+    a = vsadapt.balance_flags(case.topo, dim)
+    b = _balance_flags_every_pair(case.topo, dim)
+    assert np.array_equal(a, b)
+    ids, reps = vsadapt.grid_ids(case.topo, dim)
+    assert len(reps) <= case.n_cells and ids.max() == len(reps) - 1
