@@ -104,11 +104,14 @@ struct kah_ctx {
   bool timing = false;
   // waves per workgroup of the flux/relaxation kernel; 0: by shape. The FP32 form on adapted
   // grids whose LDS copy of f* leaves room for four or more workgroups per CU runs 4-wave
-  // workgroups (more of them resident: measured 1.4x on 2D grids of 1.6k cells); larger
-  // grids, whose LDS already limits residency, and the other forms run 8 waves.
+  // workgroups (more of them resident: measured 1.4x on 2D grids of 1.6k cells), and 2-wave
+  // workgroups for the smallest size class (grids of up to 640 velocity cells: -7 % on the
+  // flux phase of the adapted cylinder, whose free stream has 568-cell grids); larger grids,
+  // whose LDS already limits residency, and the other forms run 8 waves.
   int fr_waves = 0;
   int flux_waves(size_t lds) const {
     if (fr_waves) return fr_waves;
+    if (f32 && !uni && lds > 0 && lds <= 8 * 1024) return 2;
     return (f32 && !uni && lds > 0 && lds <= 40 * 1024) ? 4 : 8;
   }
   // launches of the flux/relaxation kernel: the fluid cells by grid size, each launch with
@@ -2119,8 +2122,8 @@ int kah_set_option(kah_ctx* C, int option, int64_t value) {
   if (!C) return KAH_EINVAL;
   switch (option) {
     case KAH_OPT_FLUX_WAVES:
-      if (value != 0 && value != 3 && value != 4 && value != 8)
-        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 3, 4 or 8 (0: by shape)");
+      if (value != 0 && value != 2 && value != 3 && value != 4 && value != 8)
+        return C->fail(KAH_EINVAL, "flux waves per workgroup must be 2, 3, 4 or 8 (0: by shape)");
       C->fr_waves = int(value);
       C->drop_graphs();   // captured graphs hold the launch shape
       return KAH_OK;

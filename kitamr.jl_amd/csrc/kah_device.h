@@ -1562,10 +1562,11 @@ __global__ __launch_bounds__(NWV * 64) void k_flux_relax(DevArgs A0,
     else if (j < 3 * NW + 7) pre_t[j - (3 * NW + 4)] = r[j];
     else if (j == 3 * NW + 7) res[0] = r[j];
   } else {
-    if (MODE != MODE_ITER && A.flux_kind == 0 && int(threadIdx.x) < ne * 2 * NW) {
-      const int e = threadIdx.x / (2 * NW), s = (threadIdx.x / NW) % 2, q = threadIdx.x % NW;
-      hsum[threadIdx.x] = half_total<D>(A, A.ents[eb + e], s, q);
-    }
+    if (MODE != MODE_ITER && A.flux_kind == 0)
+      for (int x = threadIdx.x; x < ne * 2 * NW; x += BS) {
+        const int e = x / (2 * NW), s = (x / NW) % 2, q = x % NW;
+        hsum[x] = half_total<D>(A, A.ents[eb + e], s, q);
+      }
     __syncthreads();
   }
 
@@ -2545,6 +2546,7 @@ inline hipError_t launch_fr_w(const DevArgs& A, const int32_t* cells, int n, int
 template <int D, int NDF, bool UNI, class FT>
 inline hipError_t launch_fr_d(const DevArgs& A, const int32_t* cells, int n, int mode,
                               int march, size_t lds, int nwv, hipStream_t st) {
+  if (nwv == 2) return launch_fr_w<D, NDF, UNI, FT, 2>(A, cells, n, mode, march, lds, st);
   if (nwv == 3) return launch_fr_w<D, NDF, UNI, FT, 3>(A, cells, n, mode, march, lds, st);
   if (nwv == 4) return launch_fr_w<D, NDF, UNI, FT, 4>(A, cells, n, mode, march, lds, st);
   return launch_fr_w<D, NDF, UNI, FT, 8>(A, cells, n, mode, march, lds, st);

@@ -21,7 +21,7 @@ import numpy as np
 
 SPACE = {
     "slope_vpt": [4, 3, 5, 6],
-    "flux_waves": [0, 8, 4, 3],      # 0: the library's choice by shape
+    "flux_waves": [0, 8, 4, 2],      # 0: the library's choice by shape
     "vel_tiles": [1, 0],
     "overlap_maps": [1, 0],
     "branches": [4, 1, 2],           # streams for the independent launches of a phase

@@ -13,7 +13,7 @@ def _timer(costs, calls):
 
 def test_coordinate_descent_finds_the_separable_minimum():
     case = cases.riemann_2d(n=4, nv=16)
-    costs = {"slope_vpt": {3: 3.0, 5: 1.0, 4: 2.0, 6: 4.0}, "flux_waves": {0: 0.8, 3: 0.9, 4: 0.5, 8: 1.0},
+    costs = {"slope_vpt": {3: 3.0, 5: 1.0, 4: 2.0, 6: 4.0}, "flux_waves": {0: 0.8, 2: 0.9, 4: 0.5, 8: 1.0},
              "vel_tiles": {0: 2.0, 1: 1.0}, "branches": {1: 0.5, 2: 0.25, 4: 0.0}}
     calls = []
     best, ms, trials = autotune.autotune(case, timer=_timer(costs, calls))
@@ -31,7 +31,7 @@ def test_adapted_grids_tune_maps_not_tiles():
 
 def test_cache_round_trip(tmp_path):
     case = cases.riemann_2d(n=4, nv=16)
-    costs = {"slope_vpt": {3: 1.0, 4: 2.0, 5: 3.0, 6: 4.0}, "flux_waves": {0: 1.5, 3: 1.2, 4: 1.0, 8: 2.0},
+    costs = {"slope_vpt": {3: 1.0, 4: 2.0, 5: 3.0, 6: 4.0}, "flux_waves": {0: 1.5, 2: 1.2, 4: 1.0, 8: 2.0},
              "vel_tiles": {0: 1.0, 1: 2.0}, "branches": {1: 1.0, 2: 0.5, 4: 0.75}}
     calls = []
     path = tmp_path / "tune.json"

@@ -59,7 +59,7 @@ def test_invalid_options():
     case = cases.riemann_2d(n=4, nv=8)
     say("[errors] options")
     dev = _ctx(case)
-    for name, value in (("flux_waves", 6), ("slope_vpt", 0), ("slope_vpt", 7), ("vel_tiles", 2),
+    for name, value in (("flux_waves", 6), ("flux_waves", 1), ("slope_vpt", 0), ("slope_vpt", 7), ("vel_tiles", 2),
                         ("overlap_maps", -1), ("graphs", 3), ("branches", 0), ("branches", 5),
                         ("regular", 2)):
         with pytest.raises(abi.KahError):

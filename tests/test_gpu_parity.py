@@ -44,6 +44,7 @@ OPTION_SETS = {
     "vpt6": {"slope_vpt": 6},
     "waves4": {"flux_waves": 4},
     "waves3": {"flux_waves": 3},
+    "waves2": {"flux_waves": 2},
     "no_maps": {"overlap_maps": 0},
 }
 
