@@ -315,7 +315,7 @@ int64_t kah_device_bytes(const kah_ctx* ctx);
  * the FP32 form on adapted velocity grids 2 when the LDS copy of f* takes at most 8 KiB (grids
  * of up to 640 cells), 4 up to 40 KiB, else 8; 8 for the other forms);
  * a change drops captured graphs. KAH_OPT_SLOPE_VPT: velocity cells per thread of the slope
- * kernel (1..6, default 4 in 2D and 6 in 3D; its workgroups have 3 or 4 waves, per cell
+ * kernel (1..6, default 3 in 2D and 6 in 3D; its workgroups have 3 or 4 waves, per cell
  * whichever pads the grid least). KAH_OPT_VEL_TILES: 1 (default) processes a shared tensor-product velocity grid in
  * 8x8 / 4x4x4 tiles per wave, 0 in storage order. KAH_OPT_OVERLAP_MAPS: 1 (default)
  * precomputes, per distinct pair of adapted velocity grids that face each other, the

@@ -135,9 +135,9 @@ struct kah_ctx {
   size_t ev_next = 0;
   std::vector<FrBin*> frbins;
   // velocity cells per thread of the chunked slope kernel; the default by dimension (set in
-  // kah_create): 4 in 2D, 6 in 3D (measured best on the adapted cylinder and sphere: 2D kernels
-  // above 4 need more registers, the 3D ones do not)
-  int slope_vpt = 4;
+  // kah_create): 3 in 2D, 6 in 3D (measured on the adapted cylinder and sphere and the refined
+  // Riemann mesh: 2D kernels above 4 need more registers, the 3D ones do not)
+  int slope_vpt = 3;
   bool vt_enable = true;   // wave-tile order of shared tensor-product velocity grids
   int vt_mode = 0, vt_n0 = 0, vt_n1 = 0;
   std::vector<EvPair> events;
@@ -718,7 +718,7 @@ int kah_create(const kah_params* params, kah_ctx** out) {
   C->D = params->dim;
   C->NDF = params->ndf;
   C->NW = params->dim + 2;
-  C->slope_vpt = params->dim == 3 ? 6 : 4;
+  C->slope_vpt = params->dim == 3 ? 6 : 3;
   C->f32 = params->precision == KAH_PREC_FP32F;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {

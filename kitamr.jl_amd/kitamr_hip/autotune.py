@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 
 SPACE = {
-    "slope_vpt": [4, 3, 5, 6],
+    "slope_vpt": [3, 4, 5, 6],
     "flux_waves": [0, 8, 4, 2],      # 0: the library's choice by shape
     "vel_tiles": [1, 0],
     "overlap_maps": [1, 0],

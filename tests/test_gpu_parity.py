@@ -40,6 +40,7 @@ OPTION_SETS = {
     "no_tiles": {"vel_tiles": 0},
     "vpt1": {"slope_vpt": 1},
     "vpt3": {"slope_vpt": 3},
+    "vpt4": {"slope_vpt": 4},
     "vpt5": {"slope_vpt": 5},
     "vpt6": {"slope_vpt": 6},
     "waves4": {"flux_waves": 4},
