@@ -312,8 +312,8 @@ int64_t kah_phase_cells(const kah_ctx* ctx);
 int64_t kah_device_bytes(const kah_ctx* ctx);
 /* Launch options (autotuned schedule choices). KAH_OPT_FLUX_WAVES: waves per workgroup of
  * the flux/relaxation kernel (2, 3, 4 or 8; 0, the default, chooses per size-class launch: for
- * the FP32 form on adapted velocity grids 2 when the LDS copy of f* takes at most 8 KiB (grids
- * of up to 640 cells), 4 up to 40 KiB, else 8; 8 for the other forms);
+ * the 2D FP32 form on adapted velocity grids 2 when the LDS copy of f* takes at most 8 KiB
+ * (grids of up to 640 cells), 4 up to 40 KiB, else 8; 8 for 3D and the other forms);
  * a change drops captured graphs. KAH_OPT_SLOPE_VPT: velocity cells per thread of the slope
  * kernel (1..6, default 3 in 2D and 6 in 3D; its workgroups have 3 or 4 waves, per cell
  * whichever pads the grid least). KAH_OPT_VEL_TILES: 1 (default) processes a shared tensor-product velocity grid in

@@ -107,12 +107,14 @@ struct kah_ctx {
   // workgroups (more of them resident: measured 1.4x on 2D grids of 1.6k cells), and 2-wave
   // workgroups for the smallest size class (grids of up to 640 velocity cells: -7 % on the
   // flux phase of the adapted cylinder, whose free stream has 568-cell grids); larger grids,
-  // whose LDS already limits residency, and the other forms run 8 waves.
+  // whose LDS already limits residency, the 3D grids (8 waves: -14 % on the sphere's flux
+  // phase against 4) and the other forms run 8 waves.
   int fr_waves = 0;
   int flux_waves(size_t lds) const {
     if (fr_waves) return fr_waves;
-    if (f32 && !uni && lds > 0 && lds <= 8 * 1024) return 2;
-    return (f32 && !uni && lds > 0 && lds <= 40 * 1024) ? 4 : 8;
+    if (D != 2 || !f32 || uni || lds == 0) return 8;   // 3D grids (4.5k cells and up): 8 waves
+    if (lds <= 8 * 1024) return 2;
+    return lds <= 40 * 1024 ? 4 : 8;
   }
   // launches of the flux/relaxation kernel: the fluid cells by grid size, each launch with
   // the LDS its largest grid needs (0: the global scratch area)
