@@ -293,16 +293,26 @@ def plan_balance(topo, dim):
     return plan_refine(topo, balance_flags(topo, dim), dim)
 
 
-def balance_rounds(dev, topo, dim):
-    """vs_balance! on a context: balance rounds until one flags nothing, each refining on the
-    host and injecting f on the device (``kah_vs_remap``). Returns (topology, rounds)."""
-    rounds = 0
+def plan_balance_rounds(topo, dim):
+    """The balance rounds of vs_balance! planned on the host alone (they depend on the grids,
+    not on f): a list of (topology, sources, changed) per round, until a round flags nothing."""
+    plans = []
     while True:
         t, src, ch = plan_balance(topo, dim)
         if not ch.any():
-            return topo, rounds
+            return plans
+        plans.append((t, src, ch))
+        topo = t
+
+
+def balance_rounds(dev, topo, dim):
+    """vs_balance! on a context: every round planned on the host first, then each injected on
+    the device (``kah_vs_remap``), so that a host error leaves the device untouched. Returns
+    (topology, rounds)."""
+    plans = plan_balance_rounds(topo, dim)
+    for t, src, ch in plans:
         dev.vs_remap(t, src, ch)
-        topo, rounds = t, rounds + 1
+    return (plans[-1][0] if plans else topo), len(plans)
 
 
 def _inside_rows(topo):
@@ -341,10 +351,15 @@ def vs_adapt(dev, topo, params, dim, balance=False):
     if rows is not None:
         flags[rows] = 0
     t2, src2, ch2 = plan_coarsen(t1, flags, dim)
+    # the balance rounds depend on the grids alone: planned before the coarsen remap, so that
+    # a host error there leaves the device as the refine pass left it
+    plans = plan_balance_rounds(t2, dim) if balance else []
     # coarsen_grid_stream! rebuilds every grid, so every cell's slopes are zero afterwards
     dev.vs_remap(t2, src2, np.ones(len(ch2), dtype=np.uint8))
-    if balance:
-        t2, _ = balance_rounds(dev, t2, dim)
+    for tb, srcb, chb in plans:
+        dev.vs_remap(tb, srcb, chb)
+    if plans:
+        t2 = plans[-1][0]
     va = (ch1 | ch2).astype(bool)
     dev.vs_project(np.flatnonzero(va).astype(np.int32))
     return t2, va

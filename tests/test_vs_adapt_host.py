@@ -329,3 +329,35 @@ def test_balance_flags_grouped_by_grid_equal_every_pair(make):
     assert np.array_equal(a, b)
     ids, reps = vsadapt.grid_ids(case.topo, dim)
     assert len(reps) <= case.n_cells and ids.max() == len(reps) - 1
+
+
+def test_balance_rounds_are_planned_before_any_device_call(monkeypatch):
+    """vs_balance! plans every round on the host before the first injection on the device: a
+    host error in a later round leaves the device untouched (no remap was issued)."""
+    case = cases.riemann_2d(n=8, nv=8, vs_levels=3)
+    calls = []
+
+    class Dev:
+        def vs_remap(self, t, src, ch):
+            calls.append(int(np.asarray(ch).sum()))
+
+    plans = vsadapt.plan_balance_rounds(case.topo, 2)
+    assert len(plans) >= 2
+    topo, rounds = vsadapt.balance_rounds(Dev(), case.topo, 2)
+    assert rounds == len(plans) and len(calls) == rounds
+    for k in ("vs_offset", "vs_level", "vs_mid"):
+        assert np.array_equal(np.asarray(topo[k]), np.asarray(plans[-1][0][k])), k
+    real = vsadapt.plan_balance
+    count = {"n": 0}
+
+    def failing(topo, dim):
+        count["n"] += 1
+        if count["n"] == 2:
+            raise RuntimeError("planning failed")
+        return real(topo, dim)
+
+    monkeypatch.setattr(vsadapt, "plan_balance", failing)
+    calls.clear()
+    with pytest.raises(RuntimeError, match="planning failed"):
+        vsadapt.balance_rounds(Dev(), case.topo, 2)
+    assert calls == []
