@@ -6,9 +6,11 @@ omega = 0.81, a developed bow-shock state, per-cell adapted velocity grids on a 
 on [-15,15]^2 with 2 levels (568-4000 cells per grid: the reference's initial grid of each
 cell's Maxwellian, refined toward 4000 cells with the local disturbance), CAIDVM flux +
 CAIDVM_Marching with the immersed boundary, in the FP32 form (f stored and advected in FP32,
-moments and Maxwellian parameters in FP64), the step's launches replayed as one HIP graph. The
-FP64 form runs beside it on the same workload. One step = slope + IB + flux + relaxation over
-all fluid phase-space cells.
+compensated moment sums and the Maxwellian parameters in FP64), each step replayed as one HIP
+graph whose independent launches run as parallel branches on four streams, with the launch
+options first tuned on the workload (coordinate descent; the choice is reported). The FP64
+form runs beside it on the same workload. One step = slope + IB + flux + relaxation over all
+fluid phase-space cells.
 
 Prints one JSON line (rank 0). The CPU baseline is the oracle (a C++ FP64 restatement of the
 reference algorithm, OpenMP over cells and faces) timed on this host on whole steps of the
